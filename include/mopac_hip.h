@@ -1,0 +1,155 @@
+/*
+ * mopac_hip.h — C ABI of libmopachip, a gfx950 (MI355X) implementation of
+ * MOPAC's SCF inner loop.
+ *
+ * Conventions (the reference's Fortran conventions, so a Fortran host can bind
+ * every entry point with ISO_C_BINDING):
+ *   - dense matrices are column-major with leading dimension n;
+ *   - symmetric matrices are MOPAC "packed": the lower triangle by rows,
+ *     element (i,j), i>=j, 1-based, at i(i-1)/2 + j (mpack = n(n+1)/2);
+ *   - orbital ranges nfirst/nlast are 1-based;
+ *   - every function returns 0 on success and a nonzero status on error;
+ *     mhip_last_error() returns a message for the last failure on this thread.
+ *   - pointers are host memory unless the name ends in _dev.
+ *
+ * One GPU per context.  A context is not thread-safe (like the reference's
+ * global handles); distinct contexts may be driven from distinct threads.
+ *
+ * Citations name the reference interface each entry point replaces
+ * (paths relative to the reference repository root).
+ */
+#ifndef MOPAC_HIP_H
+#define MOPAC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mhip_ctx mhip_ctx;
+
+/* ---- lifecycle --------------------------------------------------------- */
+
+/* Open device `device` and create a context.
+ * Replaces getGPUInfo / setDevice / create_handle
+ * (src/gpu/cuda_wrappers.cu:71-118, :158-187). */
+int mhip_init(int device, mhip_ctx** ctx);
+
+/* Release every device buffer of the context.  Idempotent for NULL.
+ * Replaces mopac_cuda_destroy_resources (src/gpu/cuda_wrappers.cu:830-856). */
+int mhip_finalize(mhip_ctx* ctx);
+
+const char* mhip_last_error(void);
+
+/* Device name and HBM size, for logs (getGPUInfo, cuda_wrappers.cu:71-112). */
+int mhip_device_info(mhip_ctx* ctx, char* name, int name_len, double* hbm_gib, int* n_cu);
+
+/* ---- system upload (once per geometry) --------------------------------- */
+
+/* Make H (mpack) and W (n2elec, hcore order) resident on the device and build
+ * the atom-pair tables the Fock kernels walk.  Data produced by hcore
+ * (src/integrals/hcore.F90:157-288) crosses PCIe once, not once per call. */
+int mhip_upload_system(mhip_ctx* ctx, int numat, const int* nfirst, const int* nlast, int norbs,
+                       long long n2elec, const double* h, const double* w);
+
+/* ---- the hot path, one routine at a time -------------------------------- */
+
+/* P = occ * C(:,1:ndubl) * C(:,1:ndubl)^T, packed into p (mpack).
+ * c == NULL uses the context's resident eigenvectors; p == NULL leaves the
+ * result resident.  Semantics of density_for_GPU case 4/5 for fract < 1e-2
+ * (src/matrix/density_for_GPU.F90:92-108); replaces call_syrk_cublas
+ * (src/gpu/cuda_wrappers.cu:229-254) on that path. */
+int mhip_density(mhip_ctx* ctx, const double* c, int ndubl, double occ, double* p);
+
+/* F += G(P): two-centre Coulomb/exchange over atom pairs plus (mode == 2) the
+ * one-centre terms.  f holds H (or the shifted H) on entry.  ptot = total
+ * density, p = alpha density.  Semantics of fock2(f,ptot,p,w,w,w,numat,
+ * nfirst,nlast,mode) for molecules (src/SCF/fock2.F90:16-374, fock1.F90). */
+int mhip_fock2(mhip_ctx* ctx, double* f, const double* ptot, const double* p, int mode);
+
+/* Electronic energy E = sum_{i>j} P(H+F) + 1/2 sum_i P(H+F)
+ * (src/integrals/helect.F90:16-66). */
+int mhip_helect(mhip_ctx* ctx, int norbs, const double* p, const double* h, const double* f,
+                double* e);
+
+/* All eigenpairs of the packed Fock matrix.  Adds i*1e-10 to F(i,i) IN PLACE
+ * (as the reference does), returns eigenvectors as columns of c (n x n) in
+ * ascending eigenvalue order.  Semantics of eigenvectors_LAPACK(c,f,eig,n)
+ * (src/matrix/eigenvectors_LAPACK.F90:16-127); replaces mopac_cuda_dsyevd
+ * (src/gpu/cuda_wrappers.cu:469-505). */
+int mhip_eigen(mhip_ctx* ctx, double* c, double* fpacked, double* eig, int ndim);
+
+/* Pseudo-diagonalization: FMO = C_virt^T F C_occ, then the ordered 2x2
+ * occupied-virtual rotations of `vector`.  Semantics of
+ * diag(fao,vector,nocc,eig,mdim,n) (src/matrix/diag.F90:16-163); replaces
+ * call_rot_cuda_gpu (src/gpu/cuda_wrappers.cu:547-623).  *nrot receives the
+ * number of rotations applied (may be NULL). */
+int mhip_pseudo_diag(mhip_ctx* ctx, const double* fao, double* vector, int nocc,
+                     const double* eig, int mdim, int n, long long* nrot);
+
+/* ---- the whole SCF, resident on the device ------------------------------ */
+
+typedef struct {
+  int nclose;          /* doubly occupied levels */
+  int pulay;           /* PULAY keyword (iter.F90:131) */
+  int itrmax;          /* ITRY= (default 2000) */
+  double scfcrt;       /* SCFCRT= (default 1e-4) */
+  int last;            /* 1 for a 1SCF job (phase_lock at exit) */
+  int precision;       /* 0 = FP64; 1 = split-BF16 MFMA density with FP64 correction */
+  int pseudo_diag;     /* 1 = pseudo-diagonalize once newdg holds (iter.F90:843-861) */
+  double enuclr;       /* eV */
+  double atheat;       /* kcal/mol */
+  const double* pdiag; /* starting diagonal (moldat.F90:731-790), n */
+  const double* p_init;/* optional starting total density (mpack), overrides pdiag */
+} mhip_scf_opts;
+
+typedef struct {
+  double escf;         /* heat of formation, kcal/mol (without geometry-only terms) */
+  double ee;           /* electronic energy, eV */
+  int niter;
+  int converged;
+  int n_full_diag;
+  int n_pseudo_diag;
+  double seconds;      /* wall time of guess + iterations + final diag */
+} mhip_scf_result;
+
+/* Run iter() (src/SCF/iter.F90:16-1107, RHF closed shell) with every matrix
+ * resident in HBM.  Outputs (any may be NULL): p (mpack total density),
+ * f (mpack final Fock), c (n x n eigenvectors), eig (n). */
+int mhip_scf(mhip_ctx* ctx, const mhip_scf_opts* opts, mhip_scf_result* res, double* p,
+             double* f, double* c, double* eig);
+
+/* ---- reference-boundary drop-ins -----------------------------------------
+ * Same C signatures as the reference's exported symbols, so its Fortran
+ * interfaces (src/gpu/cublas_interfaces.F90, src/matrix/eigenvectors_cuda.F90)
+ * bind by changing only the bind(C, name=...) strings.  They use an implicit
+ * context on device 0. */
+
+/* call_gemm_cublas (src/gpu/cuda_wrappers.cu:192-226) */
+void mhip_call_gemm(char tra, char trb, int m, int n, int k, double alpha, const double* a,
+                    int lda, const double* b, int ldb, double beta, double* c, int ldc);
+/* call_syrk_cublas (src/gpu/cuda_wrappers.cu:229-254); uplo 'U' or 'L' */
+void mhip_call_syrk(char uplo, char tra, int n, int k, double alpha, const double* a, int lda,
+                    double beta, double* c, int ldc);
+/* mopac_cuda_dsyevd (src/gpu/cuda_wrappers.cu:469-505): A (upper) overwritten
+ * by eigenvectors, W eigenvalues ascending, *info = 0 on success. */
+void mhip_dsyevd(int n, double* a, int lda, double* w, int* info);
+/* call_rot_cuda_gpu (src/gpu/cuda_wrappers.cu:547-623): fmo is the
+ * (n-nocc) x nocc occupied-virtual block, virtual index slow. */
+void mhip_call_rot(const double* fmo, const double* eig, double* v, const double* ci0,
+                   const double* ca0, int nocc, int lumo, int n, double bigeps, double tiny);
+/* mopac_cuda_destroy_resources (src/gpu/cuda_wrappers.cu:830-856) */
+void mhip_destroy_resources(void);
+
+/* ---- PM7 Hamiltonian (host, once per geometry) ---------------------------
+ * The producer of the loop's inputs (hcore, src/integrals/hcore.F90), for
+ * the elements H, C, N, O.  Not on the per-iteration path. */
+int mhip_pm7_sizes(int numat, const int* nat, int* norbs, long long* n2elec);
+int mhip_pm7_hamiltonian(int numat, const int* nat, const double* coord, int charge,
+                         int* nfirst, int* nlast, double* h, double* w, double* enuclr,
+                         double* atheat, double* pdiag);
+double mhip_ev_to_kcal(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOPAC_HIP_H */

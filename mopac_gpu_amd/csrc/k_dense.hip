@@ -1,0 +1,281 @@
+// Dense FP64 building blocks: tiled GEMM, SYRK into MOPAC packed storage,
+// packed <-> dense conversion, the packed Pulay commutator, linear
+// combinations and deterministic dot products.
+#include "common.h"
+
+namespace mhip {
+namespace {
+
+constexpr int TB = 64;    // output tile edge
+constexpr int TK = 16;    // k chunk
+constexpr int NT = 256;   // threads per workgroup (16 x 16, 4x4 outputs each)
+
+// op(A)(m,k), op(B)(k,n) loaders for column-major storage.
+template <bool TA>
+__device__ __forceinline__ double ld_a(const double* a, int lda, int m, int k, int M, int K) {
+  if (m >= M || k >= K) return 0.0;
+  return TA ? a[(size_t)k + (size_t)m * lda] : a[(size_t)m + (size_t)k * lda];
+}
+template <bool TBT>
+__device__ __forceinline__ double ld_b(const double* b, int ldb, int k, int n, int K, int N) {
+  if (k >= K || n >= N) return 0.0;
+  return TBT ? b[(size_t)n + (size_t)k * ldb] : b[(size_t)k + (size_t)n * ldb];
+}
+
+// One 64x64 tile of op(A) op(B); acc[i][j] is row ty*4+i, col tx*4+j.
+template <bool TA, bool TBT>
+__device__ __forceinline__ void tile_mm(const double* a, int lda, const double* b, int ldb, int M,
+                                        int N, int K, int row0, int col0, double acc[4][4],
+                                        double (*As)[TB + 1], double (*Bs)[TB + 1]) {
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  for (int k0 = 0; k0 < K; k0 += TK) {
+    // A tile: 64 (m) x 16 (k)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + r * NT;
+      int mm, kk;
+      if (TA) { kk = e & 15; mm = e >> 4; } else { mm = e & 63; kk = e >> 6; }
+      As[kk][mm] = ld_a<TA>(a, lda, row0 + mm, k0 + kk, M, K);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + r * NT;
+      int kk, nn;
+      if (TBT) { nn = e & 63; kk = e >> 6; } else { kk = e & 15; nn = e >> 4; }
+      Bs[kk][nn] = ld_b<TBT>(b, ldb, k0 + kk, col0 + nn, K, N);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; ++kk) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = As[kk][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+}
+
+template <bool TA, bool TBT>
+__global__ __launch_bounds__(NT) void k_gemm(int M, int N, int K, double alpha, const double* a,
+                                             int lda, const double* b, int ldb, double beta,
+                                             double* c, int ldc) {
+  __shared__ double As[TK][TB + 1];
+  __shared__ double Bs[TK][TB + 1];
+  const int row0 = blockIdx.y * TB, col0 = blockIdx.x * TB;
+  double acc[4][4] = {};
+  tile_mm<TA, TBT>(a, lda, b, ldb, M, N, K, row0, col0, acc, As, Bs);
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = row0 + ty * 4 + i;
+    if (r >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int cc = col0 + tx * 4 + j;
+      if (cc >= N) continue;
+      double* dst = c + (size_t)r + (size_t)cc * ldc;
+      const double v = alpha * acc[i][j];
+      *dst = (beta == 0.0) ? v : fma(beta, *dst, v);
+    }
+  }
+}
+
+// Lower-triangle tiles of occ * A A^T written straight into packed storage.
+__global__ __launch_bounds__(NT) void k_syrk_packed(int n, int k, double occ, const double* a,
+                                                    int lda, double* p) {
+  __shared__ double As[TK][TB + 1];
+  __shared__ double Bs[TK][TB + 1];
+  // blockIdx.x enumerates (bi >= bj) tiles
+  const int t = blockIdx.x;
+  int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+  while (bi * (bi + 1) / 2 > t) --bi;
+  const int bj = t - bi * (bi + 1) / 2;
+  const int row0 = bi * TB, col0 = bj * TB;
+  double acc[4][4] = {};
+  tile_mm<false, true>(a, lda, a, lda, n, n, k, row0, col0, acc, As, Bs);
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const long long r = row0 + ty * 4 + i;
+    if (r >= n) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long long cc = col0 + tx * 4 + j;
+      if (cc > r) continue;
+      p[r * (r + 1) / 2 + cc] = occ * acc[i][j];
+    }
+  }
+}
+
+__global__ void k_unpack_sym(int n, const double* __restrict__ p, double* __restrict__ a, int lda,
+                             int npad, double pad_base) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= npad || j >= npad) return;
+  double v;
+  if (i < n && j < n) {
+    const long long r = i > j ? i : j, c = i > j ? j : i;
+    v = p[r * (r + 1) / 2 + c];
+  } else {
+    v = (i == j) ? pad_base + (double)i : 0.0;
+  }
+  a[(size_t)i + (size_t)j * lda] = v;
+}
+
+__global__ void k_pack_lower(int n, const double* __restrict__ a, int lda, double* __restrict__ p) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (r >= n || c > r) return;
+  p[(long long)r * (r + 1) / 2 + c] = a[(size_t)r + (size_t)c * lda];
+}
+
+__global__ void k_pack_antisym(int n, const double* __restrict__ x, int ldx,
+                               double* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (r >= n || c > r) return;
+  out[(long long)r * (r + 1) / 2 + c] = x[(size_t)c + (size_t)r * ldx] - x[(size_t)r + (size_t)c * ldx];
+}
+
+__global__ void k_lincomb(long long len, int nk, const double* __restrict__ coef,
+                          const double* __restrict__ x, long long stride, double* __restrict__ y) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < len;
+       i += (long long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int k = 0; k < nk; ++k) s += coef[k] * x[(size_t)k * stride + i];
+    y[i] = s;
+  }
+}
+
+__global__ void k_fill(double* p, long long n, double v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+constexpr int DOT_BLOCKS = 512;
+
+__global__ __launch_bounds__(256) void k_dots_partial(long long len, int nvec, const double* a,
+                                                      long long stride, const double* b,
+                                                      double* partial) {
+  __shared__ double red[256];
+  const int v = blockIdx.y;
+  const double* av = a + (size_t)v * stride;
+  double s = 0.0;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < len; i += (long long)gridDim.x * 256)
+    s = fma(av[i], b[i], s);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(size_t)v * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_dots_final(int nblocks, const double* partial,
+                                                    double* out) {
+  __shared__ double red[256];
+  const int v = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nblocks; i += 256) s += partial[(size_t)v * nblocks + i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[v] = red[0];
+}
+
+inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+
+void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double* a,
+          int lda, const double* b, int ldb, double beta, double* c, int ldc) {
+  if (m <= 0 || n <= 0) return;
+  dim3 grid(cdiv(n, TB), cdiv(m, TB));
+  const bool TA = (ta == 'T' || ta == 't'), TBT = (tb == 'T' || tb == 't');
+  if (!TA && !TBT) hipLaunchKernelGGL((k_gemm<false, false>), grid, dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  else if (TA && !TBT) hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  else if (!TA && TBT) hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  else hipLaunchKernelGGL((k_gemm<true, true>), grid, dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  MHIP_LAUNCH_CHECK();
+}
+
+void syrk_packed(hipStream_t s, int n, int k, double occ, const double* a, int lda, double* p) {
+  if (n <= 0) return;
+  const int nt = cdiv(n, TB);
+  hipLaunchKernelGGL(k_syrk_packed, dim3(nt * (nt + 1) / 2), dim3(NT), 0, s, n, k, occ, a, lda, p);
+  MHIP_LAUNCH_CHECK();
+}
+
+void unpack_sym(hipStream_t s, int n, const double* p, double* a, int lda, int npad,
+                double pad_base) {
+  hipLaunchKernelGGL(k_unpack_sym, dim3(cdiv(npad, 256), npad), dim3(256), 0, s, n, p, a, lda, npad,
+                     pad_base);
+  MHIP_LAUNCH_CHECK();
+}
+
+void pack_lower(hipStream_t s, int n, const double* a, int lda, double* p) {
+  hipLaunchKernelGGL(k_pack_lower, dim3(cdiv(n, 256), n), dim3(256), 0, s, n, a, lda, p);
+  MHIP_LAUNCH_CHECK();
+}
+
+void pack_antisym(hipStream_t s, int n, const double* x, int ldx, double* out) {
+  hipLaunchKernelGGL(k_pack_antisym, dim3(cdiv(n, 256), n), dim3(256), 0, s, n, x, ldx, out);
+  MHIP_LAUNCH_CHECK();
+}
+
+void lincomb(hipStream_t s, long long len, int nk, const double* coef_dev, const double* x,
+             long long stride, double* y) {
+  const int blocks = std::min(4096, cdiv(len, 256));
+  hipLaunchKernelGGL(k_lincomb, dim3(blocks), dim3(256), 0, s, len, nk, coef_dev, x, stride, y);
+  MHIP_LAUNCH_CHECK();
+}
+
+void fill(hipStream_t s, double* p, long long n, double v) {
+  if (n <= 0) return;
+  const int blocks = std::min(4096, cdiv(n, 256));
+  hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, s, p, n, v);
+  MHIP_LAUNCH_CHECK();
+}
+
+size_t dots_partial_doubles(long long len, int nvec) {
+  (void)len;
+  return (size_t)DOT_BLOCKS * (size_t)nvec;
+}
+
+void dots(hipStream_t s, long long len, int nvec, const double* a, long long stride,
+          const double* b, double* out, double* partial) {
+  if (nvec <= 0) return;
+  hipLaunchKernelGGL(k_dots_partial, dim3(DOT_BLOCKS, nvec), dim3(256), 0, s, len, nvec, a, stride,
+                     b, partial);
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_dots_final, dim3(nvec), dim3(256), 0, s, DOT_BLOCKS, partial, out);
+  MHIP_LAUNCH_CHECK();
+}
+
+}  // namespace mhip
+
+namespace mhip {
+size_t syrk_split_scratch_bytes(int n, int k) {
+  (void)n;
+  (void)k;
+  return 8;
+}
+void syrk_packed_split_bf16(hipStream_t s, int n, int k, double occ, const double* a, int lda,
+                            double* p, void* scratch) {
+  (void)s; (void)n; (void)k; (void)occ; (void)a; (void)lda; (void)p; (void)scratch;
+  throw HipError{"split-BF16 density kernel is not available in this build"};
+}
+}  // namespace mhip

@@ -1,0 +1,27 @@
+// Block-Jacobi symmetric eigensolver (k_eigen.hip).
+#pragma once
+#include "common.h"
+
+namespace mhip {
+
+struct JacobiPlan {
+  int n = 0, npad = 0, nb = 0, npairs = 0;
+  DevBuf pairs;  // (nb-1) steps x npairs x 2 block indices
+  DevBuf qbuf;   // npairs x 64 x 64 rotations of the current step
+  DevBuf part;   // reduction scratch
+  DevBuf perm;   // sort permutation
+  void build(int n);
+  void release() { pairs.release(); qbuf.release(); part.release(); perm.release(); }
+};
+
+void jacobi_set_identity(hipStream_t s, int npad, double* C, int ldc);
+// Sweeps until max|offdiag| <= tol * max(1, max|diag|).  Returns 0 when
+// converged, 1 when max_sweeps ran out.
+int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double tol, int max_sweeps,
+                  double* host_scal, double* dev_scal, int* sweeps_done);
+// eig_dev[k] = k-th smallest diagonal entry, Cout[:,k] = C[:,perm[k]].
+void jacobi_sort(hipStream_t s, JacobiPlan& plan, const double* A, const double* C, double* Cout,
+                 double* eig_dev);
+void phase_lock(hipStream_t s, int n, double* C, int ldc);
+
+}  // namespace mhip

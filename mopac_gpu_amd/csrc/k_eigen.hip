@@ -1,0 +1,430 @@
+// FP64 symmetric eigensolver: parallel two-sided block Jacobi.
+//
+// The matrix (padded to a multiple of 64 with decoupled large diagonal
+// entries) is split into 32-wide blocks.  A sweep visits every block pair once
+// in round-robin order; each step
+//   1. diagonalizes the 64x64 submatrix of every pair in LDS (inner cyclic
+//      Jacobi, one workgroup per pair) giving an orthogonal Q_pair,
+//   2. applies A <- Qt^T A Qt tile by tile (two 64^3 products per tile),
+//   3. applies C <- C Qt to the eigenvector accumulator.
+// In the SCF the solver starts from the previous eigenvectors (A = C^T F C is
+// then nearly diagonal and one to three sweeps converge it); the first call
+// starts from C = I.  Stopping test: max |A_ij| (i != j) <= tol * max|A_ii|.
+#include "common.h"
+#include "k_eigen.h"
+
+namespace mhip {
+namespace {
+
+constexpr int BS = 32;  // block width
+constexpr int SB = 64;  // pair width
+constexpr int LDSW = SB + 1;
+
+__device__ __forceinline__ int gidx_dev(int pairI, int pairJ, int local) {
+  return (local < BS) ? pairI * BS + local : pairJ * BS + (local - BS);
+}
+
+__device__ __forceinline__ void rr_pair(int round, int k, int& p, int& q) {
+  // circle method on 64 indices: index 0 fixed, 1..63 rotate
+  auto pos = [&](int slot) -> int { return slot == 0 ? 0 : 1 + ((slot - 1 + round) % 63); };
+  p = pos(k);
+  q = pos(63 - k);
+}
+
+__global__ __launch_bounds__(256) void k_jac_solve(const double* __restrict__ A, int lda, int nreal,
+                                                   const int* __restrict__ pairs,
+                                                   double* __restrict__ Qbuf, double tol,
+                                                   int max_inner) {
+  __shared__ double S[SB][LDSW];
+  __shared__ double Q[SB][LDSW];
+  __shared__ double rc[32], rs[32];
+  __shared__ int rp[32], rq[32];
+  __shared__ double red[256];
+  const int pidx = blockIdx.x;
+  const int I = pairs[2 * pidx], J = pairs[2 * pidx + 1];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < SB * SB; e += 256) {
+    const int r = e % SB, c = e / SB;
+    const int gr = (r < BS) ? I * BS + r : J * BS + (r - BS);
+    const int gc = (c < BS) ? I * BS + c : J * BS + (c - BS);
+    S[r][c] = A[(size_t)gr + (size_t)gc * lda];
+    Q[r][c] = (r == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int sw = 0; sw < max_inner; ++sw) {
+    // convergence of the subproblem
+    double off = 0.0, dg = 0.0;
+    for (int e = tid; e < SB * SB; e += 256) {
+      const int r = e % SB, c = e / SB;
+      if (r == c) {
+        if (gidx_dev(I, J, r) < nreal) dg = fmax(dg, fabs(S[r][c]));
+      } else {
+        off = fmax(off, fabs(S[r][c]));
+      }
+    }
+    red[tid] = off;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+      __syncthreads();
+    }
+    off = red[0];
+    __syncthreads();
+    red[tid] = dg;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+      __syncthreads();
+    }
+    dg = red[0];
+    __syncthreads();
+    if (off <= tol * fmax(dg, 1.0)) break;
+    for (int round = 0; round < 63; ++round) {
+      if (tid < 32) {
+        int p, q;
+        rr_pair(round, tid, p, q);
+        if (p > q) { const int t = p; p = q; q = t; }
+        const double app = S[p][p], aqq = S[q][q], apq = S[p][q];
+        double c = 1.0, s = 0.0;
+        if (fabs(apq) > 1.0e-300 && fabs(apq) > 1.0e-18 * sqrt(fabs(app * aqq))) {
+          const double theta = (aqq - app) / (2.0 * apq);
+          double t;
+          if (fabs(theta) > 1.0e150) t = 0.5 / theta;
+          else t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+          c = 1.0 / sqrt(t * t + 1.0);
+          s = t * c;
+        }
+        rc[tid] = c; rs[tid] = s; rp[tid] = p; rq[tid] = q;
+      }
+      __syncthreads();
+      // rows
+      for (int e = tid; e < 32 * SB; e += 256) {
+        const int k = e / SB, col = e % SB;
+        const int p = rp[k], q = rq[k];
+        const double c = rc[k], s = rs[k];
+        const double x = S[p][col], y = S[q][col];
+        S[p][col] = c * x - s * y;
+        S[q][col] = s * x + c * y;
+      }
+      __syncthreads();
+      // columns of S and Q
+      for (int e = tid; e < 32 * SB; e += 256) {
+        const int k = e / SB, row = e % SB;
+        const int p = rp[k], q = rq[k];
+        const double c = rc[k], s = rs[k];
+        const double x = S[row][p], y = S[row][q];
+        S[row][p] = c * x - s * y;
+        S[row][q] = s * x + c * y;
+        const double u = Q[row][p], v = Q[row][q];
+        Q[row][p] = c * u - s * v;
+        Q[row][q] = s * u + c * v;
+      }
+      __syncthreads();
+    }
+  }
+  double* Qo = Qbuf + (size_t)pidx * SB * SB;
+  for (int e = tid; e < SB * SB; e += 256) {
+    const int r = e % SB, c = e / SB;
+    Qo[e] = Q[r][c];
+  }
+}
+
+__device__ __forceinline__ int gidx(int pairI, int pairJ, int local) {
+  return (local < BS) ? pairI * BS + local : pairJ * BS + (local - BS);
+}
+
+// A(rows of pair P1, cols of pair P2) <- Q1^T A Q2
+__global__ __launch_bounds__(256) void k_jac_apply_a(double* __restrict__ A, int lda,
+                                                     const int* __restrict__ pairs,
+                                                     const double* __restrict__ Qbuf) {
+  __shared__ double At[SB][LDSW];
+  __shared__ double Q1[SB][LDSW];
+  __shared__ double Q2[SB][LDSW];
+  const int P1 = blockIdx.y, P2 = blockIdx.x;
+  const int I1 = pairs[2 * P1], J1 = pairs[2 * P1 + 1];
+  const int I2 = pairs[2 * P2], J2 = pairs[2 * P2 + 1];
+  const int tid = threadIdx.x;
+  const double* q1 = Qbuf + (size_t)P1 * SB * SB;
+  const double* q2 = Qbuf + (size_t)P2 * SB * SB;
+  for (int e = tid; e < SB * SB; e += 256) {
+    const int r = e % SB, c = e / SB;
+    At[r][c] = A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda];
+    Q1[r][c] = q1[e];
+    Q2[r][c] = q2[e];
+  }
+  __syncthreads();
+  const int tx = tid & 15, ty = tid >> 4;
+  double acc[4][4] = {};
+  // T = Q1^T At : T[r][c] = sum_k Q1[k][r] At[k][c]
+  for (int k = 0; k < SB; ++k) {
+    double a4[4], b4[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a4[i] = Q1[k][ty * 4 + i];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b4[j] = At[k][tx * 4 + j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = fma(a4[i], b4[j], acc[i][j]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) At[ty * 4 + i][tx * 4 + j] = acc[i][j];
+  __syncthreads();
+  double out[4][4] = {};
+  // Out = T Q2 : Out[r][c] = sum_k T[r][k] Q2[k][c]
+  for (int k = 0; k < SB; ++k) {
+    double a4[4], b4[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a4[i] = At[ty * 4 + i][k];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b4[j] = Q2[k][tx * 4 + j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) out[i][j] = fma(a4[i], b4[j], out[i][j]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = ty * 4 + i, c = tx * 4 + j;
+      A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda] = out[i][j];
+    }
+}
+
+// C(rows tile, cols of pair P) <- C(rows tile, cols of P) Q_P
+__global__ __launch_bounds__(256) void k_jac_apply_c(double* __restrict__ C, int ldc, int nrows,
+                                                     const int* __restrict__ pairs,
+                                                     const double* __restrict__ Qbuf) {
+  __shared__ double Ct[SB][LDSW];
+  __shared__ double Qs[SB][LDSW];
+  const int P = blockIdx.y;
+  const int row0 = blockIdx.x * SB;
+  const int I = pairs[2 * P], J = pairs[2 * P + 1];
+  const double* q = Qbuf + (size_t)P * SB * SB;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < SB * SB; e += 256) {
+    const int r = e % SB, c = e / SB;
+    const int gr = row0 + r;
+    Ct[r][c] = (gr < nrows) ? C[(size_t)gr + (size_t)gidx(I, J, c) * ldc] : 0.0;
+    Qs[r][c] = q[e];
+  }
+  __syncthreads();
+  const int tx = tid & 15, ty = tid >> 4;
+  double out[4][4] = {};
+  for (int k = 0; k < SB; ++k) {
+    double a4[4], b4[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a4[i] = Ct[ty * 4 + i][k];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b4[j] = Qs[k][tx * 4 + j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) out[i][j] = fma(a4[i], b4[j], out[i][j]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gr = row0 + ty * 4 + i;
+    if (gr >= nrows) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) C[(size_t)gr + (size_t)gidx(I, J, tx * 4 + j) * ldc] = out[i][j];
+  }
+}
+
+// max |A_ij| (i != j) and max |A_ii| over the leading n x n block.
+__global__ __launch_bounds__(256) void k_offmax_partial(int n, const double* __restrict__ A,
+                                                        int lda, double* __restrict__ part) {
+  const int j = blockIdx.x;
+  double off = 0.0, dg = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double v = fabs(A[(size_t)i + (size_t)j * lda]);
+    if (i == j) dg = v;
+    else off = fmax(off, v);
+  }
+  __shared__ double r1[256], r2[256];
+  r1[threadIdx.x] = off;
+  r2[threadIdx.x] = dg;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + w]);
+      r2[threadIdx.x] = fmax(r2[threadIdx.x], r2[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * j] = r1[0];
+    part[2 * j + 1] = r2[0];
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_offmax_final(int n, const double* __restrict__ part,
+                                                       double* __restrict__ out) {
+  double off = 0.0, dg = 0.0;
+  for (int j = threadIdx.x; j < n; j += 1024) {
+    off = fmax(off, part[2 * j]);
+    dg = fmax(dg, part[2 * j + 1]);
+  }
+  __shared__ double r1[1024], r2[1024];
+  r1[threadIdx.x] = off;
+  r2[threadIdx.x] = dg;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + w]);
+      r2[threadIdx.x] = fmax(r2[threadIdx.x], r2[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = r1[0];
+    out[1] = r2[0];
+  }
+}
+
+// Stable ascending rank sort of the diagonal; perm[rank] = index.
+__global__ __launch_bounds__(1024) void k_sort_diag(int npad, const double* __restrict__ A, int lda,
+                                                    int* __restrict__ perm,
+                                                    double* __restrict__ eig) {
+  for (int i = threadIdx.x; i < npad; i += 1024) {
+    const double di = A[(size_t)i + (size_t)i * lda];
+    int rank = 0;
+    for (int j = 0; j < npad; ++j) {
+      const double dj = A[(size_t)j + (size_t)j * lda];
+      rank += (dj < di) || (dj == di && j < i);
+    }
+    perm[rank] = i;
+    eig[rank] = di;
+  }
+}
+
+__global__ void k_permute_cols(int nrows, int ncols, const double* __restrict__ src, int lds,
+                               const int* __restrict__ perm, double* __restrict__ dst, int ldd) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (r >= nrows || k >= ncols) return;
+  dst[(size_t)r + (size_t)k * ldd] = src[(size_t)r + (size_t)perm[k] * lds];
+}
+
+__global__ void k_set_identity(int npad, double* __restrict__ C, int ldc) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= npad) return;
+  C[(size_t)r + (size_t)c * ldc] = (r == c) ? 1.0 : 0.0;
+}
+
+// Largest-|c| coefficient of each column made positive (first occurrence wins).
+__global__ __launch_bounds__(256) void k_phase_lock(int n, double* __restrict__ C, int ldc) {
+  const int col = blockIdx.x;
+  double* v = C + (size_t)col * ldc;
+  double best = 0.0, sgn = 0.0;
+  int bi = n;
+  for (int r = threadIdx.x; r < n; r += 256) {
+    const double a = fabs(v[r]);
+    if (a > best) { best = a; sgn = v[r]; bi = r; }
+  }
+  __shared__ double sb[256], ss[256];
+  __shared__ int si[256];
+  sb[threadIdx.x] = best; ss[threadIdx.x] = sgn; si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      const double b2 = sb[threadIdx.x + w];
+      const int i2 = si[threadIdx.x + w];
+      if (b2 > sb[threadIdx.x] || (b2 == sb[threadIdx.x] && i2 < si[threadIdx.x])) {
+        sb[threadIdx.x] = b2; ss[threadIdx.x] = ss[threadIdx.x + w]; si[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  const bool flip = ss[0] < 0.0;
+  if (flip)
+    for (int r = threadIdx.x; r < n; r += 256) v[r] = -v[r];
+}
+
+}  // namespace
+
+void JacobiPlan::build(int n_in) {
+  n = n_in;
+  npad = ((n + SB - 1) / SB) * SB;
+  nb = npad / BS;
+  npairs = nb / 2;
+  std::vector<int> h((size_t)(nb - 1) * npairs * 2);
+  // circle method on nb blocks
+  for (int step = 0; step < nb - 1; ++step)
+    for (int k = 0; k < npairs; ++k) {
+      auto pos = [&](int slot) { return slot == 0 ? 0 : 1 + ((slot - 1 + step) % (nb - 1)); };
+      int a = pos(k), b = pos(nb - 1 - k);
+      if (a > b) std::swap(a, b);
+      h[((size_t)step * npairs + k) * 2] = a;
+      h[((size_t)step * npairs + k) * 2 + 1] = b;
+    }
+  pairs.ensure(h.size() * sizeof(int));
+  MHIP_CHECK(hipMemcpy(pairs.ptr, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+  qbuf.ensure((size_t)npairs * SB * SB * sizeof(double));
+  part.ensure((size_t)2 * npad * sizeof(double) + 64);
+  perm.ensure((size_t)npad * sizeof(int));
+}
+
+void jacobi_set_identity(hipStream_t s, int npad, double* C, int ldc) {
+  hipLaunchKernelGGL(k_set_identity, dim3((npad + 255) / 256, npad), dim3(256), 0, s, npad, C, ldc);
+  MHIP_LAUNCH_CHECK();
+}
+
+int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double tol, int max_sweeps,
+                  double* host_scal, double* dev_scal, int* sweeps_done) {
+  const int lda = plan.npad;
+  int done = 0;
+  for (int sw = 0; sw <= max_sweeps; ++sw) {
+    hipLaunchKernelGGL(k_offmax_partial, dim3(plan.n), dim3(256), 0, s, plan.n, A, lda,
+                       plan.part.as<double>());
+    MHIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_offmax_final, dim3(1), dim3(1024), 0, s, plan.n, plan.part.as<double>(),
+                       dev_scal);
+    MHIP_LAUNCH_CHECK();
+    MHIP_CHECK(hipMemcpyAsync(host_scal, dev_scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    MHIP_CHECK(hipStreamSynchronize(s));
+    if (host_scal[0] <= tol * std::max(1.0, host_scal[1])) break;
+    if (sw == max_sweeps) {
+      if (sweeps_done) *sweeps_done = done;
+      return 1;
+    }
+    for (int step = 0; step < plan.nb - 1; ++step) {
+      const int* pr = plan.pairs.as<int>() + (size_t)step * plan.npairs * 2;
+      hipLaunchKernelGGL(k_jac_solve, dim3(plan.npairs), dim3(256), 0, s, A, lda, plan.n, pr,
+                         plan.qbuf.as<double>(), 1.0e-15, 12);
+      MHIP_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_jac_apply_a, dim3(plan.npairs, plan.npairs), dim3(256), 0, s, A, lda, pr,
+                         plan.qbuf.as<double>());
+      MHIP_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_jac_apply_c, dim3(plan.npad / SB, plan.npairs), dim3(256), 0, s, C,
+                         lda, plan.npad, pr, plan.qbuf.as<double>());
+      MHIP_LAUNCH_CHECK();
+    }
+    ++done;
+  }
+  if (sweeps_done) *sweeps_done = done;
+  return 0;
+}
+
+void jacobi_sort(hipStream_t s, JacobiPlan& plan, const double* A, const double* C, double* Cout,
+                 double* eig_dev) {
+  const int npad = plan.npad;
+  hipLaunchKernelGGL(k_sort_diag, dim3(1), dim3(1024), 0, s, npad, A, npad, plan.perm.as<int>(),
+                     eig_dev);
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_permute_cols, dim3((npad + 255) / 256, npad), dim3(256), 0, s, npad, npad, C,
+                     npad, plan.perm.as<int>(), Cout, npad);
+  MHIP_LAUNCH_CHECK();
+}
+
+void phase_lock(hipStream_t s, int n, double* C, int ldc) {
+  hipLaunchKernelGGL(k_phase_lock, dim3(n), dim3(256), 0, s, n, C, ldc);
+  MHIP_LAUNCH_CHECK();
+}
+
+}  // namespace mhip

@@ -1,0 +1,112 @@
+"""ctypes binding of libmopachip.so (the C ABI declared in include/mopac_hip.h).
+
+The library is built in-tree by mopac_gpu_amd.build; nothing here falls back
+to a CPU implementation: if the shared object is missing, loading fails.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmopachip.so")
+_lib = None
+
+c_int = ctypes.c_int
+c_ll = ctypes.c_longlong
+c_double = ctypes.c_double
+c_void_p = ctypes.c_void_p
+c_char = ctypes.c_char
+
+
+class ScfOpts(ctypes.Structure):
+    _fields_ = [
+        ("nclose", c_int),
+        ("pulay", c_int),
+        ("itrmax", c_int),
+        ("scfcrt", c_double),
+        ("last", c_int),
+        ("precision", c_int),
+        ("pseudo_diag", c_int),
+        ("enuclr", c_double),
+        ("atheat", c_double),
+        ("pdiag", c_void_p),
+        ("p_init", c_void_p),
+    ]
+
+
+class ScfResult(ctypes.Structure):
+    _fields_ = [
+        ("escf", c_double),
+        ("ee", c_double),
+        ("niter", c_int),
+        ("converged", c_int),
+        ("n_full_diag", c_int),
+        ("n_pseudo_diag", c_int),
+        ("seconds", c_double),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "mhip_init": (c_int, [c_int, ctypes.POINTER(c_void_p)]),
+    "mhip_finalize": (c_int, [c_void_p]),
+    "mhip_last_error": (ctypes.c_char_p, []),
+    "mhip_device_info": (c_int, [c_void_p, ctypes.c_char_p, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_int)]),
+    "mhip_upload_system": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_ll, c_void_p, c_void_p]),
+    "mhip_density": (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p]),
+    "mhip_fock2": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "mhip_helect": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double)]),
+    "mhip_eigen": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "mhip_pseudo_diag": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, ctypes.POINTER(c_ll)]),
+    "mhip_scf": (c_int, [c_void_p, ctypes.POINTER(ScfOpts), ctypes.POINTER(ScfResult), c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mhip_call_gemm": (None, [c_char, c_char, c_int, c_int, c_int, c_double, c_void_p, c_int, c_void_p, c_int, c_double, c_void_p, c_int]),
+    "mhip_call_syrk": (None, [c_char, c_char, c_int, c_int, c_double, c_void_p, c_int, c_double, c_void_p, c_int]),
+    "mhip_dsyevd": (None, [c_int, c_void_p, c_int, c_void_p, ctypes.POINTER(c_int)]),
+    "mhip_call_rot": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double]),
+    "mhip_destroy_resources": (None, []),
+    "mhip_pm7_sizes": (c_int, [c_int, c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_ll)]),
+    "mhip_pm7_hamiltonian": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double), ctypes.POINTER(c_double), c_void_p]),
+    "mhip_ev_to_kcal": (c_double, []),
+}
+
+
+def load():
+    """Load libmopachip.so and attach signatures; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m mopac_gpu_amd.build` "
+            "(there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names declared in include/mopac_hip.h that the ABI must export."""
+    hdr = os.path.join(os.path.dirname(_HERE), "include", "mopac_hip.h")
+    import re
+    names = []
+    with open(hdr) as fh:
+        for line in fh:
+            m = re.match(r"^\s*(?:const\s+)?(?:int|void|double|char)\s*\*?\s*(mhip_\w+)\s*\(", line)
+            if m:
+                names.append(m.group(1))
+    return names
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().mhip_last_error()
+        raise RuntimeError((msg or b"unknown error").decode())
+
+
+def ptr(a):
+    return a.ctypes.data_as(c_void_p) if a is not None else None
