@@ -140,6 +140,15 @@ void mhip_call_rot(const double* fmo, const double* eig, double* v, const double
 /* mopac_cuda_destroy_resources (src/gpu/cuda_wrappers.cu:830-856) */
 void mhip_destroy_resources(void);
 
+/* ---- diagnostics / tuning --------------------------------------------------
+ * No counterpart in the reference.  kernel_variant: 1 = FP64 MFMA tiles
+ * (default), 0 = vector-FMA tiles, <0 unchanged; max_inner: inner Jacobi
+ * sweeps per 64x64 pair problem (<=0 unchanged); tol_iter: eigensolver
+ * stopping tolerance inside the SCF loop (<=0 unchanged). */
+int mhip_set_tuning(mhip_ctx* ctx, int kernel_variant, int max_inner, double tol_iter);
+/* Block-Jacobi sweeps used by the last eigensolve of this context. */
+int mhip_last_sweeps(mhip_ctx* ctx);
+
 /* ---- PM7 Hamiltonian (host, once per geometry) ---------------------------
  * The producer of the loop's inputs (hcore, src/integrals/hcore.F90), for
  * the elements H, C, N, O.  Not on the per-iteration path. */

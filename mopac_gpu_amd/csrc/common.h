@@ -62,6 +62,8 @@ struct DevBuf {
 inline long long mpack_of(long long n) { return n * (n + 1) / 2; }
 
 // ---- dense kernels (k_dense.hip) ------------------------------------------
+// 1: FP64 MFMA tiles (default), 0: vector-FMA tiles (kept as a cross-check).
+extern int g_kernel_variant;
 // C = alpha * op(A) * op(B) + beta * C, column-major FP64.
 void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double* a,
           int lda, const double* b, int ldb, double beta, double* c, int ldc);

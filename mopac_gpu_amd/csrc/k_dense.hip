@@ -2,6 +2,7 @@
 // packed <-> dense conversion, the packed Pulay commutator, linear
 // combinations and deterministic dot products.
 #include "common.h"
+#include "mfma.h"
 
 namespace mhip {
 namespace {
@@ -196,15 +197,98 @@ __global__ __launch_bounds__(256) void k_dots_final(int nblocks, const double* p
   if (threadIdx.x == 0) out[v] = red[0];
 }
 
+
+// Same tiling on the FP64 matrix cores (see mfma.h for the lane layout).
+template <bool TA, bool TBT>
+__global__ __launch_bounds__(NT) void k_gemm_mfma(int M, int N, int K, double alpha,
+                                                  const double* a, int lda, const double* b,
+                                                  int ldb, double beta, double* c, int ldc) {
+  __shared__ double As[TK][TB + 1];
+  __shared__ double Bs[TK][TB + 1];
+  const int row0 = blockIdx.y * TB, col0 = blockIdx.x * TB;
+  const int tid = threadIdx.x;
+  d4_t acc[4];
+  for (int i = 0; i < 4; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < K; k0 += TK) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + r * NT;
+      int mm, kk;
+      if (TA) { kk = e & 15; mm = e >> 4; } else { mm = e & 63; kk = e >> 6; }
+      As[kk][mm] = ld_a<TA>(a, lda, row0 + mm, k0 + kk, M, K);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + r * NT;
+      int kk, nn;
+      if (TBT) { nn = e & 63; kk = e >> 6; } else { kk = e & 15; nn = e >> 4; }
+      Bs[kk][nn] = ld_b<TBT>(b, ldb, k0 + kk, col0 + nn, K, N);
+    }
+    __syncthreads();
+    mfma_tile64([&](int r, int k) { return As[k][r]; }, [&](int k, int cc) { return Bs[k][cc]; },
+                TK, acc);
+    __syncthreads();
+  }
+  mfma_tile64_store(acc, [&](int r, int cc, double v) {
+    const int gr = row0 + r, gc = col0 + cc;
+    if (gr >= M || gc >= N) return;
+    double* dst = c + (size_t)gr + (size_t)gc * ldc;
+    *dst = (beta == 0.0) ? alpha * v : fma(beta, *dst, alpha * v);
+  });
+}
+
+__global__ __launch_bounds__(NT) void k_syrk_packed_mfma(int n, int k, double occ, const double* a,
+                                                         int lda, double* p) {
+  __shared__ double As[TK][TB + 1];
+  __shared__ double Bs[TK][TB + 1];
+  const int t = blockIdx.x;
+  int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+  while (bi * (bi + 1) / 2 > t) --bi;
+  const int bj = t - bi * (bi + 1) / 2;
+  const int row0 = bi * TB, col0 = bj * TB;
+  const int tid = threadIdx.x;
+  d4_t acc[4];
+  for (int i = 0; i < 4; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < k; k0 += TK) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + r * NT;
+      const int mm = e & 63, kk = e >> 6;
+      As[kk][mm] = ld_a<false>(a, lda, row0 + mm, k0 + kk, n, k);
+      Bs[kk][mm] = ld_a<false>(a, lda, col0 + mm, k0 + kk, n, k);
+    }
+    __syncthreads();
+    mfma_tile64([&](int r, int kk) { return As[kk][r]; }, [&](int kk, int cc) { return Bs[kk][cc]; },
+                TK, acc);
+    __syncthreads();
+  }
+  mfma_tile64_store(acc, [&](int r, int cc, double v) {
+    const long long gr = row0 + r, gc = col0 + cc;
+    if (gr >= n || gc > gr) return;
+    p[gr * (gr + 1) / 2 + gc] = occ * v;
+  });
+}
+
 inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace
+
+int g_kernel_variant = 1;  // 1: FP64 MFMA tiles, 0: vector-FMA tiles
 
 void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double* a,
           int lda, const double* b, int ldb, double beta, double* c, int ldc) {
   if (m <= 0 || n <= 0) return;
   dim3 grid(cdiv(n, TB), cdiv(m, TB));
   const bool TA = (ta == 'T' || ta == 't'), TBT = (tb == 'T' || tb == 't');
+  if (g_kernel_variant == 1) {
+    if (!TA && !TBT) hipLaunchKernelGGL((k_gemm_mfma<false, false>), grid, dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+    else if (TA && !TBT) hipLaunchKernelGGL((k_gemm_mfma<true, false>), grid, dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+    else if (!TA && TBT) hipLaunchKernelGGL((k_gemm_mfma<false, true>), grid, dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+    else hipLaunchKernelGGL((k_gemm_mfma<true, true>), grid, dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+    MHIP_LAUNCH_CHECK();
+    return;
+  }
   if (!TA && !TBT) hipLaunchKernelGGL((k_gemm<false, false>), grid, dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
   else if (TA && !TBT) hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
   else if (!TA && TBT) hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
@@ -215,7 +299,10 @@ void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
 void syrk_packed(hipStream_t s, int n, int k, double occ, const double* a, int lda, double* p) {
   if (n <= 0) return;
   const int nt = cdiv(n, TB);
-  hipLaunchKernelGGL(k_syrk_packed, dim3(nt * (nt + 1) / 2), dim3(NT), 0, s, n, k, occ, a, lda, p);
+  if (g_kernel_variant == 1)
+    hipLaunchKernelGGL(k_syrk_packed_mfma, dim3(nt * (nt + 1) / 2), dim3(NT), 0, s, n, k, occ, a, lda, p);
+  else
+    hipLaunchKernelGGL(k_syrk_packed, dim3(nt * (nt + 1) / 2), dim3(NT), 0, s, n, k, occ, a, lda, p);
   MHIP_LAUNCH_CHECK();
 }
 

@@ -6,12 +6,14 @@ namespace mhip {
 
 struct JacobiPlan {
   int n = 0, npad = 0, nb = 0, npairs = 0;
+  int max_inner = 2;  // inner Jacobi sweeps per 64x64 pair problem
   DevBuf pairs;  // (nb-1) steps x npairs x 2 block indices
   DevBuf qbuf;   // npairs x 64 x 64 rotations of the current step
   DevBuf part;   // reduction scratch
   DevBuf perm;   // sort permutation
+  DevBuf active; // per pair: 1 when Q != I this step
   void build(int n);
-  void release() { pairs.release(); qbuf.release(); part.release(); perm.release(); }
+  void release() { pairs.release(); qbuf.release(); part.release(); perm.release(); active.release(); }
 };
 
 void jacobi_set_identity(hipStream_t s, int npad, double* C, int ldc);

@@ -12,6 +12,7 @@
 // starts from C = I.  Stopping test: max |A_ij| (i != j) <= tol * max|A_ii|.
 #include "common.h"
 #include "k_eigen.h"
+#include "mfma.h"
 
 namespace mhip {
 namespace {
@@ -33,8 +34,9 @@ __device__ __forceinline__ void rr_pair(int round, int k, int& p, int& q) {
 
 __global__ __launch_bounds__(256) void k_jac_solve(const double* __restrict__ A, int lda, int nreal,
                                                    const int* __restrict__ pairs,
-                                                   double* __restrict__ Qbuf, double tol,
-                                                   int max_inner) {
+                                                   double* __restrict__ Qbuf,
+                                                   int* __restrict__ active, double tol,
+                                                   double skip_abs, int max_inner) {
   __shared__ double S[SB][LDSW];
   __shared__ double Q[SB][LDSW];
   __shared__ double rc[32], rs[32];
@@ -78,6 +80,11 @@ __global__ __launch_bounds__(256) void k_jac_solve(const double* __restrict__ A,
     }
     dg = red[0];
     __syncthreads();
+    if (sw == 0 && off <= skip_abs) {
+      // already diagonal to the outer tolerance: Q = I, nothing to apply
+      if (tid == 0) active[pidx] = 0;
+      return;
+    }
     if (off <= tol * fmax(dg, 1.0)) break;
     for (int round = 0; round < 63; ++round) {
       if (tid < 32) {
@@ -127,20 +134,30 @@ __global__ __launch_bounds__(256) void k_jac_solve(const double* __restrict__ A,
     const int r = e % SB, c = e / SB;
     Qo[e] = Q[r][c];
   }
+  if (tid == 0) active[pidx] = 1;
 }
 
 __device__ __forceinline__ int gidx(int pairI, int pairJ, int local) {
   return (local < BS) ? pairI * BS + local : pairJ * BS + (local - BS);
 }
 
-// A(rows of pair P1, cols of pair P2) <- Q1^T A Q2
-__global__ __launch_bounds__(256) void k_jac_apply_a(double* __restrict__ A, int lda,
+// A(rows of pair P1, cols of pair P2) <- Q1^T A Q2 for P1 <= P2 (the mirror
+// tile is written as the transpose, keeping A exactly symmetric).
+__global__ __launch_bounds__(256) void k_jac_apply_a(double* __restrict__ A, int lda, int npairs,
                                                      const int* __restrict__ pairs,
-                                                     const double* __restrict__ Qbuf) {
+                                                     const double* __restrict__ Qbuf,
+                                                     const int* __restrict__ active) {
+  // blockIdx.x enumerates (P1 <= P2)
+  const int t = blockIdx.x;
+  int P2 = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((P2 + 1) * (P2 + 2) / 2 <= t) ++P2;
+  while (P2 * (P2 + 1) / 2 > t) --P2;
+  const int P1 = t - P2 * (P2 + 1) / 2;
+  const bool a1 = active[P1] != 0, a2 = active[P2] != 0;
+  if (!a1 && !a2) return;
   __shared__ double At[SB][LDSW];
   __shared__ double Q1[SB][LDSW];
   __shared__ double Q2[SB][LDSW];
-  const int P1 = blockIdx.y, P2 = blockIdx.x;
   const int I1 = pairs[2 * P1], J1 = pairs[2 * P1 + 1];
   const int I2 = pairs[2 * P2], J2 = pairs[2 * P2 + 1];
   const int tid = threadIdx.x;
@@ -149,8 +166,8 @@ __global__ __launch_bounds__(256) void k_jac_apply_a(double* __restrict__ A, int
   for (int e = tid; e < SB * SB; e += 256) {
     const int r = e % SB, c = e / SB;
     At[r][c] = A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda];
-    Q1[r][c] = q1[e];
-    Q2[r][c] = q2[e];
+    Q1[r][c] = a1 ? q1[e] : (r == c ? 1.0 : 0.0);
+    Q2[r][c] = a2 ? q2[e] : (r == c ? 1.0 : 0.0);
   }
   __syncthreads();
   const int tx = tid & 15, ty = tid >> 4;
@@ -186,22 +203,44 @@ __global__ __launch_bounds__(256) void k_jac_apply_a(double* __restrict__ A, int
 #pragma unroll
       for (int j = 0; j < 4; ++j) out[i][j] = fma(a4[i], b4[j], out[i][j]);
   }
+  if (P1 == P2) {
+    // diagonal tile: symmetrize exactly
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) At[ty * 4 + i][tx * 4 + j] = out[i][j];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = ty * 4 + i, c = tx * 4 + j;
+        const double v = (r == c) ? out[i][j] : 0.5 * (out[i][j] + At[c][r]);
+        A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda] = v;
+      }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int r = ty * 4 + i, c = tx * 4 + j;
-      A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda] = out[i][j];
+      const size_t gr = gidx(I1, J1, r), gc = gidx(I2, J2, c);
+      A[gr + gc * lda] = out[i][j];
+      A[gc + gr * lda] = out[i][j];
     }
 }
 
 // C(rows tile, cols of pair P) <- C(rows tile, cols of P) Q_P
 __global__ __launch_bounds__(256) void k_jac_apply_c(double* __restrict__ C, int ldc, int nrows,
                                                      const int* __restrict__ pairs,
-                                                     const double* __restrict__ Qbuf) {
+                                                     const double* __restrict__ Qbuf,
+                                                     const int* __restrict__ active) {
+  const int P = blockIdx.y;
+  if (!active[P]) return;
   __shared__ double Ct[SB][LDSW];
   __shared__ double Qs[SB][LDSW];
-  const int P = blockIdx.y;
   const int row0 = blockIdx.x * SB;
   const int I = pairs[2 * P], J = pairs[2 * P + 1];
   const double* q = Qbuf + (size_t)P * SB * SB;
@@ -233,6 +272,88 @@ __global__ __launch_bounds__(256) void k_jac_apply_c(double* __restrict__ C, int
 #pragma unroll
     for (int j = 0; j < 4; ++j) C[(size_t)gr + (size_t)gidx(I, J, tx * 4 + j) * ldc] = out[i][j];
   }
+}
+
+
+// MFMA versions of the two update kernels (same data flow as above).
+__global__ __launch_bounds__(256) void k_jac_apply_a_mfma(double* __restrict__ A, int lda,
+                                                          int npairs, const int* __restrict__ pairs,
+                                                          const double* __restrict__ Qbuf,
+                                                          const int* __restrict__ active) {
+  const int t = blockIdx.x;
+  int P2 = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((P2 + 1) * (P2 + 2) / 2 <= t) ++P2;
+  while (P2 * (P2 + 1) / 2 > t) --P2;
+  const int P1 = t - P2 * (P2 + 1) / 2;
+  const bool a1 = active[P1] != 0, a2 = active[P2] != 0;
+  if (!a1 && !a2) return;
+  __shared__ double At[SB][LDSW];
+  __shared__ double Q1[SB][LDSW];
+  __shared__ double Q2[SB][LDSW];
+  const int I1 = pairs[2 * P1], J1 = pairs[2 * P1 + 1];
+  const int I2 = pairs[2 * P2], J2 = pairs[2 * P2 + 1];
+  const int tid = threadIdx.x;
+  const double* q1 = Qbuf + (size_t)P1 * SB * SB;
+  const double* q2 = Qbuf + (size_t)P2 * SB * SB;
+  for (int e = tid; e < SB * SB; e += 256) {
+    const int r = e % SB, c = e / SB;
+    At[r][c] = A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda];
+    Q1[r][c] = a1 ? q1[e] : (r == c ? 1.0 : 0.0);
+    Q2[r][c] = a2 ? q2[e] : (r == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  d4_t acc[4];
+  for (int i = 0; i < 4; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
+  mfma_tile64([&](int r, int k) { return Q1[k][r]; }, [&](int k, int c) { return At[k][c]; }, SB, acc);
+  __syncthreads();
+  mfma_tile64_store(acc, [&](int r, int c, double v) { At[r][c] = v; });
+  __syncthreads();
+  for (int i = 0; i < 4; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
+  mfma_tile64([&](int r, int k) { return At[r][k]; }, [&](int k, int c) { return Q2[k][c]; }, SB, acc);
+  if (P1 == P2) {
+    __syncthreads();
+    mfma_tile64_store(acc, [&](int r, int c, double v) { Q1[r][c] = v; });
+    __syncthreads();
+    for (int e = tid; e < SB * SB; e += 256) {
+      const int r = e % SB, c = e / SB;
+      const double v = (r == c) ? Q1[r][c] : 0.5 * (Q1[r][c] + Q1[c][r]);
+      A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda] = v;
+    }
+    return;
+  }
+  mfma_tile64_store(acc, [&](int r, int c, double v) {
+    const size_t gr = gidx(I1, J1, r), gc = gidx(I2, J2, c);
+    A[gr + gc * lda] = v;
+    A[gc + gr * lda] = v;
+  });
+}
+
+__global__ __launch_bounds__(256) void k_jac_apply_c_mfma(double* __restrict__ C, int ldc, int nrows,
+                                                          const int* __restrict__ pairs,
+                                                          const double* __restrict__ Qbuf,
+                                                          const int* __restrict__ active) {
+  const int P = blockIdx.y;
+  if (!active[P]) return;
+  __shared__ double Ct[SB][LDSW];
+  __shared__ double Qs[SB][LDSW];
+  const int row0 = blockIdx.x * SB;
+  const int I = pairs[2 * P], J = pairs[2 * P + 1];
+  const double* q = Qbuf + (size_t)P * SB * SB;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < SB * SB; e += 256) {
+    const int r = e % SB, c = e / SB;
+    const int gr = row0 + r;
+    Ct[r][c] = (gr < nrows) ? C[(size_t)gr + (size_t)gidx(I, J, c) * ldc] : 0.0;
+    Qs[r][c] = q[e];
+  }
+  __syncthreads();
+  d4_t acc[4];
+  for (int i = 0; i < 4; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
+  mfma_tile64([&](int r, int k) { return Ct[r][k]; }, [&](int k, int c) { return Qs[k][c]; }, SB, acc);
+  mfma_tile64_store(acc, [&](int r, int c, double v) {
+    const int gr = row0 + r;
+    if (gr < nrows) C[(size_t)gr + (size_t)gidx(I, J, c) * ldc] = v;
+  });
 }
 
 // max |A_ij| (i != j) and max |A_ii| over the leading n x n block.
@@ -368,6 +489,7 @@ void JacobiPlan::build(int n_in) {
   qbuf.ensure((size_t)npairs * SB * SB * sizeof(double));
   part.ensure((size_t)2 * npad * sizeof(double) + 64);
   perm.ensure((size_t)npad * sizeof(int));
+  active.ensure((size_t)npairs * sizeof(int));
 }
 
 void jacobi_set_identity(hipStream_t s, int npad, double* C, int ldc) {
@@ -388,7 +510,10 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
     MHIP_LAUNCH_CHECK();
     MHIP_CHECK(hipMemcpyAsync(host_scal, dev_scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     MHIP_CHECK(hipStreamSynchronize(s));
-    if (host_scal[0] <= tol * std::max(1.0, host_scal[1])) break;
+    const double thr = tol * std::max(1.0, host_scal[1]);
+    if (host_scal[0] <= thr) break;
+    // pairs whose whole 64x64 block is already this diagonal are left alone
+    const double skip_abs = 0.1 * thr;
     if (sw == max_sweeps) {
       if (sweeps_done) *sweeps_done = done;
       return 1;
@@ -396,13 +521,23 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
     for (int step = 0; step < plan.nb - 1; ++step) {
       const int* pr = plan.pairs.as<int>() + (size_t)step * plan.npairs * 2;
       hipLaunchKernelGGL(k_jac_solve, dim3(plan.npairs), dim3(256), 0, s, A, lda, plan.n, pr,
-                         plan.qbuf.as<double>(), 1.0e-15, 12);
+                         plan.qbuf.as<double>(), plan.active.as<int>(), 1.0e-14, skip_abs,
+                         plan.max_inner);
       MHIP_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_jac_apply_a, dim3(plan.npairs, plan.npairs), dim3(256), 0, s, A, lda, pr,
-                         plan.qbuf.as<double>());
-      MHIP_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_jac_apply_c, dim3(plan.npad / SB, plan.npairs), dim3(256), 0, s, C,
-                         lda, plan.npad, pr, plan.qbuf.as<double>());
+      if (g_kernel_variant == 1) {
+        hipLaunchKernelGGL(k_jac_apply_a_mfma, dim3(plan.npairs * (plan.npairs + 1) / 2), dim3(256),
+                           0, s, A, lda, plan.npairs, pr, plan.qbuf.as<double>(),
+                           plan.active.as<int>());
+        MHIP_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_jac_apply_c_mfma, dim3(plan.npad / SB, plan.npairs), dim3(256), 0, s,
+                           C, lda, plan.npad, pr, plan.qbuf.as<double>(), plan.active.as<int>());
+      } else {
+        hipLaunchKernelGGL(k_jac_apply_a, dim3(plan.npairs * (plan.npairs + 1) / 2), dim3(256), 0,
+                           s, A, lda, plan.npairs, pr, plan.qbuf.as<double>(), plan.active.as<int>());
+        MHIP_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_jac_apply_c, dim3(plan.npad / SB, plan.npairs), dim3(256), 0, s, C,
+                           lda, plan.npad, pr, plan.qbuf.as<double>(), plan.active.as<int>());
+      }
       MHIP_LAUNCH_CHECK();
     }
     ++done;

@@ -37,6 +37,9 @@ struct mhip_ctx {
   mhip::PseudoWork pseudo;
   // statistics of the last eigensolve
   int last_sweeps = 0;
+  // Jacobi stopping tolerance (relative to max|diag|) inside the SCF loop;
+  // the exit diagonalization always uses 1e-12.
+  double tol_iter = 1.0e-10;
 };
 
 namespace mhip {
@@ -184,7 +187,7 @@ FockArgs fock_args(mhip_ctx* c, int mode, double shift) {
 
 // eigenvectors_LAPACK on the device copy of a packed matrix: adds i*1e-10 to
 // the diagonal in place, then solves; ctx C/eig hold the result (sorted).
-static void eigen_packed(mhip_ctx* c, double* fpacked, bool perturb) {
+static void eigen_packed(mhip_ctx* c, double* fpacked, bool perturb, double tol = 1.0e-12) {
   hipStream_t s = c->stream;
   const int n = c->norbs, np = c->npad;
   const double pad_base = 1.0e4;
@@ -203,7 +206,7 @@ static void eigen_packed(mhip_ctx* c, double* fpacked, bool perturb) {
     jacobi_set_identity(s, np, C, np);
   }
   int sweeps = 0;
-  const int rc = jacobi_sweeps(s, c->plan, A, C, 1.0e-12, 40, c->hscal + 32,
+  const int rc = jacobi_sweeps(s, c->plan, A, C, tol, 40, c->hscal + 32,
                                c->scal.as<double>() + 32, &sweeps);
   MHIP_REQUIRE(rc == 0, "block-Jacobi eigensolver did not converge in 40 sweeps");
   c->last_sweeps = sweeps;
@@ -494,7 +497,7 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
                          c->eig.as<double>(), nullptr);
       ++npseudo;
     } else {
-      eigen_packed(c, c->f.as<double>(), true);
+      eigen_packed(c, c->f.as<double>(), true, c->tol_iter);
       ++nfull;
     }
     density(c, nclose, o->precision == 1);
@@ -706,5 +709,14 @@ int mhip_scf(mhip_ctx* c, const mhip_scf_opts* opts, mhip_scf_result* res, doubl
 }
 
 int mhip_last_sweeps(mhip_ctx* c) { return c ? c->last_sweeps : 0; }
+
+// Diagnostics/tuning (not part of the reference boundary): kernel family
+// (1 = FP64 MFMA, 0 = vector FMA), inner Jacobi sweeps, in-loop tolerance.
+int mhip_set_tuning(mhip_ctx* c, int kernel_variant, int max_inner, double tol_iter) {
+  if (kernel_variant >= 0) mhip::g_kernel_variant = kernel_variant;
+  if (c && max_inner > 0) c->plan.max_inner = max_inner;
+  if (c && tol_iter > 0) c->tol_iter = tol_iter;
+  return 0;
+}
 
 }  // extern "C"

@@ -63,6 +63,8 @@ SIGNATURES = {
     "mhip_dsyevd": (None, [c_int, c_void_p, c_int, c_void_p, ctypes.POINTER(c_int)]),
     "mhip_call_rot": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double]),
     "mhip_destroy_resources": (None, []),
+    "mhip_set_tuning": (c_int, [c_void_p, c_int, c_int, c_double]),
+    "mhip_last_sweeps": (c_int, [c_void_p]),
     "mhip_pm7_sizes": (c_int, [c_int, c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_ll)]),
     "mhip_pm7_hamiltonian": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double), ctypes.POINTER(c_double), c_void_p]),
     "mhip_ev_to_kcal": (c_double, []),

@@ -169,3 +169,22 @@ def test_reference_boundary_dropins(gpu_ctx):
     ref = 1.5 * x @ b + 0.5 * cg
     lib.mhip_call_gemm(b"N", b"N", n, n, k, 1.5, L.ptr(x), n, L.ptr(b), k, 0.5, L.ptr(cg), n)
     assert np.max(np.abs(cg - ref)) < 1e-12
+
+
+def test_mfma_tiles_match_vector_tiles(gpu_ctx):
+    """Checks the FP64 MFMA lane layout: both kernel families against numpy."""
+    lib = L.load()
+    rng = np.random.default_rng(8)
+    m, n, k = 150, 130, 70
+    a = np.asfortranarray(rng.standard_normal((k, m)))
+    b = np.asfortranarray(rng.standard_normal((k, n)))
+    ref = a.T @ b
+    outs = []
+    for variant in (0, 1):
+        lib.mhip_set_tuning(gpu_ctx.h, variant, -1, -1.0)
+        c = np.zeros((m, n), order="F")
+        lib.mhip_call_gemm(b"T", b"N", m, n, k, 1.0, L.ptr(a), k, L.ptr(b), k, 0.0, L.ptr(c), m)
+        outs.append(c)
+    lib.mhip_set_tuning(gpu_ctx.h, 1, -1, -1.0)
+    assert np.max(np.abs(outs[0] - ref)) < 1e-12
+    assert np.max(np.abs(outs[1] - ref)) < 1e-12
