@@ -96,6 +96,7 @@ typedef struct {
   int last;            /* 1 for a 1SCF job (phase_lock at exit) */
   int precision;       /* 0 = FP64; 1 = split-BF16 MFMA density with FP64 correction */
   int pseudo_diag;     /* 1 = pseudo-diagonalize once newdg holds (iter.F90:843-861) */
+  int adaptive_eigen;  /* 1 = loosen the eigensolver tolerance while PL is large */
   double enuclr;       /* eV */
   double atheat;       /* kcal/mol */
   const double* pdiag; /* starting diagonal (moldat.F90:731-790), n */
@@ -110,6 +111,7 @@ typedef struct {
   int n_full_diag;
   int n_pseudo_diag;
   double seconds;      /* wall time of guess + iterations + final diag */
+  int jacobi_sweeps;   /* block-Jacobi sweeps summed over all diagonalizations */
 } mhip_scf_result;
 
 /* Run iter() (src/SCF/iter.F90:16-1107, RHF closed shell) with every matrix
@@ -148,6 +150,14 @@ void mhip_destroy_resources(void);
 int mhip_set_tuning(mhip_ctx* ctx, int kernel_variant, int max_inner, double tol_iter);
 /* Block-Jacobi sweeps used by the last eigensolve of this context. */
 int mhip_last_sweeps(mhip_ctx* ctx);
+/* HIP-event timing of the block-Jacobi solve / A-update / C-update launches
+ * on the context's stream.  Returns the accumulated milliseconds and launch
+ * counts, and the 64x64 tiles / block pairs the timed launches actually
+ * processed; enable = 1 or 0 resets and switches timing on or off, -1 only
+ * reads. */
+int mhip_kernel_timing(mhip_ctx* ctx, int enable, double* solve_ms, long long* n_solve,
+                       double* apply_ms, long long* n_apply, double* applyc_ms,
+                       long long* n_applyc, long long* active_tiles, long long* active_pairs);
 
 /* ---- PM7 Hamiltonian (host, once per geometry) ---------------------------
  * The producer of the loop's inputs (hcore, src/integrals/hcore.F90), for

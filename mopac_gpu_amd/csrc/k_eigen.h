@@ -7,13 +7,25 @@ namespace mhip {
 struct JacobiPlan {
   int n = 0, npad = 0, nb = 0, npairs = 0;
   int max_inner = 2;  // inner Jacobi sweeps per 64x64 pair problem
+  // Optional HIP-event timing of the solve and A-update launches.
+  bool timing = false;
+  double t_solve_ms = 0.0, t_apply_ms = 0.0, t_applyc_ms = 0.0;
+  long long n_solve = 0, n_apply = 0, n_applyc = 0;
+  long long active_tiles = 0, active_pairs = 0;  // summed over timed launches
+  std::vector<hipEvent_t> ev;
   DevBuf pairs;  // (nb-1) steps x npairs x 2 block indices
   DevBuf qbuf;   // npairs x 64 x 64 rotations of the current step
   DevBuf part;   // reduction scratch
   DevBuf perm;   // sort permutation
   DevBuf active; // per pair: 1 when Q != I this step
+  DevBuf counts; // active pairs per step (timing mode)
   void build(int n);
-  void release() { pairs.release(); qbuf.release(); part.release(); perm.release(); active.release(); }
+  void release() {
+    pairs.release(); qbuf.release(); part.release(); perm.release(); active.release();
+    counts.release();
+    for (auto e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+  }
 };
 
 void jacobi_set_identity(hipStream_t s, int npad, double* C, int ldc);

@@ -32,31 +32,39 @@ __device__ __forceinline__ void rr_pair(int round, int k, int& p, int& q) {
   q = pos(63 - k);
 }
 
-__global__ __launch_bounds__(256) void k_jac_solve(const double* __restrict__ A, int lda, int nreal,
-                                                   const int* __restrict__ pairs,
-                                                   double* __restrict__ Qbuf,
-                                                   int* __restrict__ active, double tol,
-                                                   double skip_abs, int max_inner) {
+constexpr int SOLVE_THREADS = 1024;
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// One workgroup per block pair: load the 64x64 submatrix, run cyclic Jacobi
+// in parallel (round-robin) order until converged or max_inner sweeps, and
+// write the accumulated rotation Q.  A pair already diagonal to `skip_abs`
+// is marked inactive (Q = I) and skipped by the update kernels.
+__global__ __launch_bounds__(SOLVE_THREADS) void k_jac_solve(
+    const double* __restrict__ A, int lda, int nreal, const int* __restrict__ pairs,
+    double* __restrict__ Qbuf, int* __restrict__ active, double tol, double skip_abs,
+    int max_inner) {
   __shared__ double S[SB][LDSW];
   __shared__ double Q[SB][LDSW];
   __shared__ double rc[32], rs[32];
   __shared__ int rp[32], rq[32];
-  __shared__ double red[256];
+  __shared__ double red_off[SOLVE_THREADS / 64], red_dg[SOLVE_THREADS / 64];
   const int pidx = blockIdx.x;
   const int I = pairs[2 * pidx], J = pairs[2 * pidx + 1];
   const int tid = threadIdx.x;
-  for (int e = tid; e < SB * SB; e += 256) {
+  for (int e = tid; e < SB * SB; e += SOLVE_THREADS) {
     const int r = e % SB, c = e / SB;
-    const int gr = (r < BS) ? I * BS + r : J * BS + (r - BS);
-    const int gc = (c < BS) ? I * BS + c : J * BS + (c - BS);
-    S[r][c] = A[(size_t)gr + (size_t)gc * lda];
+    S[r][c] = A[(size_t)gidx_dev(I, J, r) + (size_t)gidx_dev(I, J, c) * lda];
     Q[r][c] = (r == c) ? 1.0 : 0.0;
   }
   __syncthreads();
   for (int sw = 0; sw < max_inner; ++sw) {
-    // convergence of the subproblem
     double off = 0.0, dg = 0.0;
-    for (int e = tid; e < SB * SB; e += 256) {
+    for (int e = tid; e < SB * SB; e += SOLVE_THREADS) {
       const int r = e % SB, c = e / SB;
       if (r == c) {
         if (gidx_dev(I, J, r) < nreal) dg = fmax(dg, fabs(S[r][c]));
@@ -64,24 +72,22 @@ __global__ __launch_bounds__(256) void k_jac_solve(const double* __restrict__ A,
         off = fmax(off, fabs(S[r][c]));
       }
     }
-    red[tid] = off;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
-      __syncthreads();
+    off = wave_max(off);
+    dg = wave_max(dg);
+    if ((tid & 63) == 0) {
+      red_off[tid >> 6] = off;
+      red_dg[tid >> 6] = dg;
     }
-    off = red[0];
     __syncthreads();
-    red[tid] = dg;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
-      __syncthreads();
+    off = 0.0;
+    dg = 0.0;
+#pragma unroll
+    for (int w = 0; w < SOLVE_THREADS / 64; ++w) {
+      off = fmax(off, red_off[w]);
+      dg = fmax(dg, red_dg[w]);
     }
-    dg = red[0];
     __syncthreads();
     if (sw == 0 && off <= skip_abs) {
-      // already diagonal to the outer tolerance: Q = I, nothing to apply
       if (tid == 0) active[pidx] = 0;
       return;
     }
@@ -92,45 +98,45 @@ __global__ __launch_bounds__(256) void k_jac_solve(const double* __restrict__ A,
         rr_pair(round, tid, p, q);
         if (p > q) { const int t = p; p = q; q = t; }
         const double app = S[p][p], aqq = S[q][q], apq = S[p][q];
-        double c = 1.0, s = 0.0;
+        double c = 1.0, sn = 0.0;
         if (fabs(apq) > 1.0e-300 && fabs(apq) > 1.0e-18 * sqrt(fabs(app * aqq))) {
           const double theta = (aqq - app) / (2.0 * apq);
           double t;
           if (fabs(theta) > 1.0e150) t = 0.5 / theta;
           else t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
           c = 1.0 / sqrt(t * t + 1.0);
-          s = t * c;
+          sn = t * c;
         }
-        rc[tid] = c; rs[tid] = s; rp[tid] = p; rq[tid] = q;
+        rc[tid] = c; rs[tid] = sn; rp[tid] = p; rq[tid] = q;
       }
       __syncthreads();
-      // rows
-      for (int e = tid; e < 32 * SB; e += 256) {
+      // rows of S: 32 rotations x 64 columns
+      for (int e = tid; e < 32 * SB; e += SOLVE_THREADS) {
         const int k = e / SB, col = e % SB;
         const int p = rp[k], q = rq[k];
-        const double c = rc[k], s = rs[k];
+        const double c = rc[k], sn = rs[k];
         const double x = S[p][col], y = S[q][col];
-        S[p][col] = c * x - s * y;
-        S[q][col] = s * x + c * y;
+        S[p][col] = c * x - sn * y;
+        S[q][col] = sn * x + c * y;
       }
       __syncthreads();
-      // columns of S and Q
-      for (int e = tid; e < 32 * SB; e += 256) {
+      // columns of S and of Q
+      for (int e = tid; e < 32 * SB; e += SOLVE_THREADS) {
         const int k = e / SB, row = e % SB;
         const int p = rp[k], q = rq[k];
-        const double c = rc[k], s = rs[k];
+        const double c = rc[k], sn = rs[k];
         const double x = S[row][p], y = S[row][q];
-        S[row][p] = c * x - s * y;
-        S[row][q] = s * x + c * y;
+        S[row][p] = c * x - sn * y;
+        S[row][q] = sn * x + c * y;
         const double u = Q[row][p], v = Q[row][q];
-        Q[row][p] = c * u - s * v;
-        Q[row][q] = s * u + c * v;
+        Q[row][p] = c * u - sn * v;
+        Q[row][q] = sn * u + c * v;
       }
       __syncthreads();
     }
   }
   double* Qo = Qbuf + (size_t)pidx * SB * SB;
-  for (int e = tid; e < SB * SB; e += 256) {
+  for (int e = tid; e < SB * SB; e += SOLVE_THREADS) {
     const int r = e % SB, c = e / SB;
     Qo[e] = Q[r][c];
   }
@@ -407,6 +413,14 @@ __global__ __launch_bounds__(1024) void k_offmax_final(int n, const double* __re
   }
 }
 
+__global__ void k_count_active(int npairs, const int* __restrict__ active, int* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    int a = 0;
+    for (int i = 0; i < npairs; ++i) a += active[i] != 0;
+    *out = a;
+  }
+}
+
 // Stable ascending rank sort of the diagonal; perm[rank] = index.
 __global__ __launch_bounds__(1024) void k_sort_diag(int npad, const double* __restrict__ A, int lda,
                                                     int* __restrict__ perm,
@@ -490,6 +504,7 @@ void JacobiPlan::build(int n_in) {
   part.ensure((size_t)2 * npad * sizeof(double) + 64);
   perm.ensure((size_t)npad * sizeof(int));
   active.ensure((size_t)npairs * sizeof(int));
+  counts.ensure((size_t)nb * sizeof(int));
 }
 
 void jacobi_set_identity(hipStream_t s, int npad, double* C, int ldc) {
@@ -518,19 +533,35 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
       if (sweeps_done) *sweeps_done = done;
       return 1;
     }
-    for (int step = 0; step < plan.nb - 1; ++step) {
+    const int nsteps = plan.nb - 1;
+    if (plan.timing && plan.ev.size() < (size_t)(6 * nsteps)) {
+      for (auto e : plan.ev) (void)hipEventDestroy(e);
+      plan.ev.assign(6 * nsteps, nullptr);
+      for (auto& e : plan.ev) MHIP_CHECK(hipEventCreate(&e));
+    }
+    for (int step = 0; step < nsteps; ++step) {
       const int* pr = plan.pairs.as<int>() + (size_t)step * plan.npairs * 2;
-      hipLaunchKernelGGL(k_jac_solve, dim3(plan.npairs), dim3(256), 0, s, A, lda, plan.n, pr,
+      if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 0], s));
+      hipLaunchKernelGGL(k_jac_solve, dim3(plan.npairs), dim3(SOLVE_THREADS), 0, s, A, lda, plan.n, pr,
                          plan.qbuf.as<double>(), plan.active.as<int>(), 1.0e-14, skip_abs,
                          plan.max_inner);
       MHIP_LAUNCH_CHECK();
+      if (plan.timing) {
+        MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 1], s));
+        hipLaunchKernelGGL(k_count_active, dim3(1), dim3(64), 0, s, plan.npairs,
+                           plan.active.as<int>(), plan.counts.as<int>() + step);
+      }
       if (g_kernel_variant == 1) {
+        if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 2], s));
         hipLaunchKernelGGL(k_jac_apply_a_mfma, dim3(plan.npairs * (plan.npairs + 1) / 2), dim3(256),
                            0, s, A, lda, plan.npairs, pr, plan.qbuf.as<double>(),
                            plan.active.as<int>());
         MHIP_LAUNCH_CHECK();
+        if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 3], s));
+        if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 4], s));
         hipLaunchKernelGGL(k_jac_apply_c_mfma, dim3(plan.npad / SB, plan.npairs), dim3(256), 0, s,
                            C, lda, plan.npad, pr, plan.qbuf.as<double>(), plan.active.as<int>());
+        if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 5], s));
       } else {
         hipLaunchKernelGGL(k_jac_apply_a, dim3(plan.npairs * (plan.npairs + 1) / 2), dim3(256), 0,
                            s, A, lda, plan.npairs, pr, plan.qbuf.as<double>(), plan.active.as<int>());
@@ -539,6 +570,26 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
                            lda, plan.npad, pr, plan.qbuf.as<double>(), plan.active.as<int>());
       }
       MHIP_LAUNCH_CHECK();
+    }
+    if (plan.timing && g_kernel_variant == 1) {
+      MHIP_CHECK(hipStreamSynchronize(s));
+      std::vector<int> act(nsteps);
+      MHIP_CHECK(hipMemcpy(act.data(), plan.counts.ptr, sizeof(int) * nsteps, hipMemcpyDeviceToHost));
+      for (int step = 0; step < nsteps; ++step) {
+        const long long u = plan.npairs - act[step];  // inactive pairs
+        plan.active_pairs += act[step];
+        plan.active_tiles += (long long)plan.npairs * (plan.npairs + 1) / 2 - u * (u + 1) / 2;
+        float a = 0.f, b = 0.f, c2 = 0.f;
+        MHIP_CHECK(hipEventElapsedTime(&a, plan.ev[6 * step + 0], plan.ev[6 * step + 1]));
+        MHIP_CHECK(hipEventElapsedTime(&b, plan.ev[6 * step + 2], plan.ev[6 * step + 3]));
+        MHIP_CHECK(hipEventElapsedTime(&c2, plan.ev[6 * step + 4], plan.ev[6 * step + 5]));
+        plan.t_solve_ms += a;
+        plan.t_apply_ms += b;
+        plan.t_applyc_ms += c2;
+        plan.n_solve++;
+        plan.n_apply++;
+        plan.n_applyc++;
+      }
     }
     ++done;
   }

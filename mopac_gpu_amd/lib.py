@@ -26,6 +26,7 @@ class ScfOpts(ctypes.Structure):
         ("last", c_int),
         ("precision", c_int),
         ("pseudo_diag", c_int),
+        ("adaptive_eigen", c_int),
         ("enuclr", c_double),
         ("atheat", c_double),
         ("pdiag", c_void_p),
@@ -42,6 +43,7 @@ class ScfResult(ctypes.Structure):
         ("n_full_diag", c_int),
         ("n_pseudo_diag", c_int),
         ("seconds", c_double),
+        ("jacobi_sweeps", c_int),
     ]
 
 
@@ -65,6 +67,10 @@ SIGNATURES = {
     "mhip_destroy_resources": (None, []),
     "mhip_set_tuning": (c_int, [c_void_p, c_int, c_int, c_double]),
     "mhip_last_sweeps": (c_int, [c_void_p]),
+    "mhip_kernel_timing": (c_int, [c_void_p, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_ll),
+                                   ctypes.POINTER(c_double), ctypes.POINTER(c_ll),
+                                   ctypes.POINTER(c_double), ctypes.POINTER(c_ll),
+                                   ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)]),
     "mhip_pm7_sizes": (c_int, [c_int, c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_ll)]),
     "mhip_pm7_hamiltonian": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double), ctypes.POINTER(c_double), c_void_p]),
     "mhip_ev_to_kcal": (c_double, []),

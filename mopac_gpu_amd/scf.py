@@ -87,7 +87,7 @@ class Context:
 
     # -- the whole loop ---------------------------------------------------
     def iter(self, pulay=False, scfcrt=1e-4, itrmax=2000, last=1, precision=0, pseudo_diag=False,
-             p_init=None, want=("p", "eig")):
+             adaptive_eigen=False, p_init=None, want=("p", "eig")):
         s = self.system
         n = s.norbs
         o = L.ScfOpts()
@@ -98,6 +98,7 @@ class Context:
         o.last = last
         o.precision = precision
         o.pseudo_diag = int(bool(pseudo_diag))
+        o.adaptive_eigen = int(bool(adaptive_eigen))
         o.enuclr = s.enuclr
         o.atheat = s.atheat
         pd = np.ascontiguousarray(s.pdiag)
@@ -116,5 +117,6 @@ class Context:
                                   L.ptr(c), L.ptr(e)))
         out.update(escf=r.escf, ee=r.ee, niter=r.niter, converged=bool(r.converged),
                    n_full_diag=r.n_full_diag, n_pseudo_diag=r.n_pseudo_diag, seconds=r.seconds,
+                   jacobi_sweeps=r.jacobi_sweeps,
                    p=p, f=f, c=c, eig=e)
         return out
