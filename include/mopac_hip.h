@@ -142,6 +142,24 @@ void mhip_call_rot(const double* fmo, const double* eig, double* v, const double
 /* mopac_cuda_destroy_resources (src/gpu/cuda_wrappers.cu:830-856) */
 void mhip_destroy_resources(void);
 
+/* ---- learned starting density: the device MLP ------------------------------
+ * No counterpart in the reference (it replaces nothing; it feeds the SCF a
+ * starting density through mhip_scf_opts.p_init, as OLDENS feeds iter).
+ * Row-major activations; layer l has dims[l] inputs and dims[l+1] outputs;
+ * the flat parameter vector holds, per layer, W (dims[l] x dims[l+1],
+ * row-major) then b.  BF16 weights and layer inputs, FP32 accumulation, tanh
+ * hidden activations, linear output, mean-squared-error loss, Adam. */
+typedef struct mhip_mlp mhip_mlp;
+int mhip_mlp_create(int nlayers, const int* dims, mhip_mlp** out);
+void mhip_mlp_destroy(mhip_mlp* mlp);
+int mhip_mlp_set_params(mhip_mlp* mlp, const float* params);
+int mhip_mlp_get_params(mhip_mlp* mlp, float* params);
+int mhip_mlp_forward(mhip_mlp* mlp, int batch, const float* x, float* y);
+int mhip_mlp_gradients(mhip_mlp* mlp, int batch, const float* x, const float* t, float* grad,
+                       float* loss);
+int mhip_mlp_train_step(mhip_mlp* mlp, int batch, const float* x, const float* t, float lr,
+                        float* loss);
+
 /* ---- diagnostics / tuning --------------------------------------------------
  * No counterpart in the reference.  kernel_variant: 1 = FP64 MFMA tiles
  * (default), 0 = vector-FMA tiles, <0 unchanged; max_inner: inner Jacobi

@@ -71,6 +71,13 @@ SIGNATURES = {
                                    ctypes.POINTER(c_double), ctypes.POINTER(c_ll),
                                    ctypes.POINTER(c_double), ctypes.POINTER(c_ll),
                                    ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)]),
+    "mhip_mlp_create": (c_int, [c_int, c_void_p, ctypes.POINTER(c_void_p)]),
+    "mhip_mlp_destroy": (None, [c_void_p]),
+    "mhip_mlp_set_params": (c_int, [c_void_p, c_void_p]),
+    "mhip_mlp_get_params": (c_int, [c_void_p, c_void_p]),
+    "mhip_mlp_forward": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "mhip_mlp_gradients": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "mhip_mlp_train_step": (c_int, [c_void_p, c_int, c_void_p, c_void_p, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
     "mhip_pm7_sizes": (c_int, [c_int, c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_ll)]),
     "mhip_pm7_hamiltonian": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double), ctypes.POINTER(c_double), c_void_p]),
     "mhip_ev_to_kcal": (c_double, []),
