@@ -1,0 +1,63 @@
+"""Compile MOPAC's own SCF routines from the reference sources into
+oracle/_ref/libmopac_ref.so — TEST INFRASTRUCTURE (a checker for the C
+restatement, never part of the product).
+
+    python oracle/build_ref.py [--src /root/reference/src]
+
+Only the routines on the SCF inner loop and the small modules they read are
+compiled, with amdflang, straight from where they lie.  The reference's own
+CMake build is not used.  Symbols those files reference but never reach on
+this path (the COSMO Fock term addfck, guarded by useps = .false.) stay
+unresolved, so the library is opened lazily (RTLD_LAZY); the BLAS/LAPACK
+routines pulay calls resolve from MKL, which the oracle loader makes global.
+"""
+import argparse
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "_ref")
+SO = os.path.join(OUT, "libmopac_ref.so")
+FC = os.environ.get("MOPAC_REF_FC", "/opt/rocm/bin/amdflang")
+
+# order matters: modules before their users
+SOURCES = [
+    "molkst_C.F90",
+    "solvation/cosmo_C.F90",
+    "chanel_C.F90",
+    "SCF/iter_C.F90",
+    "SCF/fock1.F90",
+    "integrals/jab.F90",
+    "integrals/kab.F90",
+    "SCF/fock2.F90",
+    "integrals/helect.F90",
+    "SCF/cnvg.F90",
+    "SCF/pulay.F90",
+    "matrix/diag.F90",
+    "matrix/osinv.F90",
+]
+
+
+def build(src="/root/reference/src"):
+    if not os.path.isdir(src):
+        raise FileNotFoundError(f"reference sources not found at {src}")
+    mod = os.path.join(OUT, "mod")
+    os.makedirs(mod, exist_ok=True)
+    objs = []
+    flags = ["-O2", "-fPIC", "-module-dir", mod, "-I", mod]
+    for rel in SOURCES + [os.path.join(HERE, "ref_harness.F90")]:
+        path = rel if os.path.isabs(rel) else os.path.join(src, rel)
+        obj = os.path.join(OUT, os.path.basename(path).replace(".F90", ".o"))
+        subprocess.run([FC] + flags + ["-c", path, "-o", obj], check=True, cwd=OUT)
+        objs.append(obj)
+    subprocess.run([FC, "-shared", "-o", SO] + objs, check=True, cwd=OUT)
+    return SO
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--src", default="/root/reference/src")
+    a = ap.parse_args()
+    print(build(a.src))
+    sys.exit(0)

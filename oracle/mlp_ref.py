@@ -28,18 +28,22 @@ def unpack(params, dims):
     return out
 
 
-def forward(params, dims, x):
-    layers = unpack(np.asarray(params, dtype=np.float32), dims)
+def forward(params, dims, x, rnd=None, pdtype=np.float32):
+    """Activations of every layer.  rnd (default bf16) rounds weights and layer
+    inputs; pdtype is the storage type of the parameters (float32 masters)."""
+    rnd = bf16 if rnd is None else rnd
+    layers = unpack(np.asarray(params, dtype=pdtype), dims)
     acts = [np.asarray(x, dtype=np.float64)]
     for li, (w, b) in enumerate(layers):
-        z = bf16(acts[-1]) @ bf16(w) + b.astype(np.float64)
+        z = rnd(acts[-1]) @ rnd(w) + b.astype(np.float64)
         acts.append(np.tanh(z) if li + 1 < len(layers) else z)
     return acts
 
 
-def loss_and_grad(params, dims, x, t):
-    layers = unpack(np.asarray(params, dtype=np.float32), dims)
-    acts = forward(params, dims, x)
+def loss_and_grad(params, dims, x, t, rnd=None, pdtype=np.float32):
+    rnd = bf16 if rnd is None else rnd
+    layers = unpack(np.asarray(params, dtype=pdtype), dims)
+    acts = forward(params, dims, x, rnd, pdtype)
     y = acts[-1]
     t = np.asarray(t, dtype=np.float64)
     ny = y.size
@@ -49,10 +53,10 @@ def loss_and_grad(params, dims, x, t):
     for li in range(len(layers) - 1, -1, -1):
         if li + 1 < len(layers):
             d = d * (1.0 - acts[li + 1] ** 2)
-        gw = bf16(acts[li]).T @ d
+        gw = rnd(acts[li]).T @ d
         gb = d.sum(axis=0)
         grads[li] = (gw, gb)
         if li > 0:
-            d = d @ bf16(layers[li][0]).T
+            d = d @ rnd(layers[li][0]).T
     flat = np.concatenate([np.concatenate([g[0].ravel(), g[1]]) for g in grads])
     return loss, flat

@@ -41,6 +41,7 @@ def load():
         lib.o_diag.argtypes = [V, V, I, V, I]
         lib.o_diag.restype = ctypes.c_longlong
         lib.o_osinv.argtypes = [V, I, V]
+        lib.o_pulay_seq.argtypes = [I, I, I, V, V, V]
         lib.o_lapack_available.restype = I
         lib.o_set_threads.argtypes = [I]
         _lib = lib
@@ -94,6 +95,21 @@ def diag(fao, vector, nocc, eig):
     n = v.shape[0]
     nr = load().o_diag(P(np.ascontiguousarray(fao)), P(v), nocc, P(np.ascontiguousarray(eig)), n)
     return v, nr
+
+
+def osinv(a):
+    a = np.array(a, dtype=np.float64, order="F", copy=True)
+    d = ctypes.c_double()
+    load().o_osinv(P(a), a.shape[0], ctypes.byref(d))
+    return a, d.value
+
+
+def pulay_seq(n, f_seq, p_seq, npulay=10):
+    """len(f_seq) Pulay steps from a fresh start; returns (extrapolated F's, PLs)."""
+    f = np.array(f_seq, dtype=np.float64, copy=True)
+    pl = np.zeros(f.shape[0])
+    load().o_pulay_seq(n, npulay, f.shape[0], P(f), P(np.ascontiguousarray(p_seq)), P(pl))
+    return f, pl
 
 
 def iterate(system, pulay=False, scfcrt=1e-4, itrmax=2000, last=1, p_init=None, trace_cap=2000):

@@ -626,6 +626,23 @@ void o_pulay(int n, int npulay, o_pulay_state* st, double* f, const double* p, d
   }
 }
 
+/* A sequence of `ncalls` Pulay steps from a fresh start, as iter makes them:
+ * call k extrapolates f_seq[k] (mpack) in place using p_seq[k], and pl_seq[k]
+ * receives its PL.  Flat arguments so a test can drive it call for call. */
+void o_pulay_seq(int n, int npulay, int ncalls, double* f_seq, const double* p_seq,
+                 double* pl_seq) {
+  long long mpack = (long long)n * (n + 1) / 2;
+  o_pulay_state st;
+  memset(&st, 0, sizeof(st));
+  st.start = 1;
+  st.fppf = (double*)calloc((size_t)npulay * mpack, sizeof(double));
+  st.fock = (double*)calloc((size_t)npulay * mpack, sizeof(double));
+  for (int k = 0; k < ncalls; ++k)
+    o_pulay(n, npulay, &st, f_seq + k * mpack, p_seq + k * mpack, pl_seq + k);
+  free(st.fppf);
+  free(st.fock);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Pseudo-diagonalization (src/matrix/diag.F90:16-163).                       */
 /* vector is n x n column-major (mdim == n); fao packed.                      */

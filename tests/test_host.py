@@ -1,0 +1,169 @@
+"""CPU tests of the host side: the C-ABI library loads and exports what the
+header declares, the product package never reaches for the oracle, the
+Hamiltonian producer's structure, the learned-guess host logic, the MLP
+reference's gradients, and the bench's one-GPU contract."""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from mopac_gpu_amd import geometry, guess, hamiltonian
+from mopac_gpu_amd import lib as L
+from oracle import mlp_ref, oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ---- the C ABI ----------------------------------------------------------------
+
+def test_library_exports_every_header_symbol():
+    names = L.exported_symbols()
+    assert len(names) >= 29
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    lib = L.load()  # loads without a GPU; every symbol resolves through ctypes
+    for n in names:
+        assert getattr(lib, n) is not None
+    assert set(names) <= set(L.SIGNATURES)
+
+
+def test_library_carries_gfx950_code_only():
+    blob = open(L.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
+    targets = set(re.findall(rb"amdgcn-amd-amdhsa--(gfx[0-9a-f]+)", blob))
+    assert targets == {b"gfx950"}, targets
+
+
+def test_product_package_never_imports_the_oracle():
+    pkg = os.path.join(ROOT, "mopac_gpu_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for fn in files:
+            if fn.endswith((".py", ".hip", ".cpp", ".h")):
+                text = open(os.path.join(dirpath, fn), encoding="utf-8").read()
+                assert not re.search(r"^\s*(from|import)\s+oracle", text, re.M), fn
+                assert "liboracle" not in text and "oracle_scf" not in text, fn
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stdout
+
+
+# ---- the Hamiltonian producer -----------------------------------------------
+
+def test_hamiltonian_sizes_and_translation_invariance():
+    nat, xyz = geometry.glycine_helix(2, 1)
+    s0 = hamiltonian.build(nat, xyz)
+    s1 = hamiltonian.build(nat, xyz + np.array([10.0, -3.0, 7.5]))
+    heavy = int(np.sum(np.asarray(nat) > 1))
+    assert s0.norbs == 4 * heavy + (len(nat) - heavy)
+    assert s0.mpack == s0.norbs * (s0.norbs + 1) // 2
+    assert np.max(np.abs(s0.h - s1.h)) < 1e-10
+    assert np.max(np.abs(s0.w - s1.w)) < 1e-10
+    assert abs(s0.enuclr - s1.enuclr) < 1e-9
+
+
+def test_water_cluster_geometry():
+    nat, xyz = geometry.water_cluster(64, 3)
+    assert len(nat) == 192 and np.sum(nat == 8) == 64
+    fw = guess.find_waters(nat, xyz)
+    assert fw is not None
+    _, waters = fw
+    w = np.array(waters)
+    roh = np.linalg.norm(xyz[w[:, 1:]] - xyz[w[:, :1]], axis=2)
+    assert np.all(np.abs(roh - 0.96) < 0.1)
+    o = xyz[w[:, 0]]
+    d = np.linalg.norm(o[:, None] - o[None], axis=2) + 1e3 * np.eye(64)
+    assert d.min() > 2.3
+
+
+# ---- learned-guess host logic -------------------------------------------------
+
+def _rotated(xyz, seed):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return xyz @ q.T + rng.standard_normal(3)
+
+
+def test_pair_list_matches_brute_force():
+    nat, xyz = geometry.water_cluster(27, 5)
+    mol, _ = guess.find_waters(nat, xyz)
+    a, b, _ = guess.pair_list(nat, xyz, mol)
+    got = set(zip(a.tolist(), b.tolist()))
+    n = len(nat)
+    want = {(i, i) for i in range(n)}
+    for i in range(n):
+        for j in range(i):
+            if np.linalg.norm(xyz[i] - xyz[j]) < guess.RCUT or mol[i] == mol[j]:
+                want.add((i, j))
+    assert got == want
+
+
+def test_features_are_rotation_invariant():
+    nat, xyz = geometry.water_cluster(16, 2)
+    x1 = _rotated(xyz, 9)
+    out = []
+    for c in (xyz, x1):
+        mol, waters = guess.find_waters(nat, c)
+        a, b, tree = guess.pair_list(nat, c, mol)
+        order = np.lexsort((b, a))
+        X, typ = guess.features(nat, c, mol, waters, a, b, tree)
+        out.append(X[order])
+    assert np.max(np.abs(out[0] - out[1])) < 2e-5
+
+
+def test_blocks_roundtrip_and_equivariance():
+    """Targets computed from a converged density rotate with the molecule, and
+    assembling them back reproduces the density on every covered element."""
+    nat, xyz = geometry.water_cluster(6, 4)
+    res = []
+    for c in (xyz, _rotated(xyz, 3)):
+        s = hamiltonian.build(nat, c)
+        o = oracle.iterate(s, scfcrt=1e-9)
+        mol, waters = guess.find_waters(nat, c)
+        a, b, _ = guess.pair_list(nat, c, mol)
+        order = np.lexsort((b, a))
+        y = guess.blocks_from_density(s, o["p"], a, b, mol, waters)
+        p = guess.assemble(s, y, a, b, mol, waters)
+        d = np.array([i * (i + 1) // 2 + i for i in range(s.norbs)])
+        covered = np.zeros(s.mpack, bool)
+        idx, valid, _ = guess._slots(s, a, b)
+        covered[idx[valid]] = True
+        off = covered.copy()
+        off[d] = False
+        assert np.max(np.abs(p[off] - o["p"][off])) < 1e-12
+        assert abs(p[d].sum() - 8 * len(waters)) < 1e-9
+        res.append(y[order])
+    assert np.max(np.abs(res[0] - res[1])) < 1e-6
+
+
+def test_mlp_reference_gradient_matches_finite_differences():
+    """The backward pass of the CPU MLP reference, with BF16 rounding switched
+    off and FP64 parameters, against central differences of its own loss."""
+    dims = [5, 7, 6, 3]
+    rng = np.random.default_rng(0)
+    nparam = sum(dims[i] * dims[i + 1] + dims[i + 1] for i in range(len(dims) - 1))
+    params = rng.standard_normal(nparam) * 0.5
+    x = rng.standard_normal((9, dims[0]))
+    t = rng.standard_normal((9, dims[-1]))
+    ident = lambda a: np.asarray(a, dtype=np.float64)  # noqa: E731
+    loss0, g = mlp_ref.loss_and_grad(params, dims, x, t, rnd=ident, pdtype=np.float64)
+    eps = 1e-6
+    fd = np.array([(mlp_ref.loss_and_grad(params + eps * e, dims, x, t, ident, np.float64)[0]
+                    - mlp_ref.loss_and_grad(params - eps * e, dims, x, t, ident, np.float64)[0]) / (2 * eps)
+                   for e in np.eye(nparam)])
+    assert np.max(np.abs(g - fd)) < 1e-7 * max(1.0, np.max(np.abs(fd)))
+    # and the default (BF16) forward differs from the FP64 one only by rounding
+    y16 = mlp_ref.forward(params.astype(np.float32), dims, x)[-1]
+    y64 = mlp_ref.forward(params, dims, x, ident, np.float64)[-1]
+    assert 0 < np.max(np.abs(y16 - y64)) < 5e-2 * np.max(np.abs(y64))
