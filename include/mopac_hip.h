@@ -95,7 +95,9 @@ typedef struct {
   double scfcrt;       /* SCFCRT= (default 1e-4) */
   int last;            /* 1 for a 1SCF job (phase_lock at exit) */
   int precision;       /* 0 = FP64; 1 = split-BF16 MFMA density with FP64 correction */
-  int pseudo_diag;     /* 1 = pseudo-diagonalize once newdg holds (iter.F90:843-861) */
+  int pseudo_diag;     /* once newdg holds (iter.F90:843-861): 0 = always full diagonalization
+                          (the reference), 1 = diag.F90's ordered rotation list, 2 = the same
+                          rotations applied at once as GEMMs + Bjorck re-orthonormalization */
   int adaptive_eigen;  /* 1 = loosen the eigensolver tolerance while PL is large */
   double enuclr;       /* eV */
   double atheat;       /* kcal/mol */

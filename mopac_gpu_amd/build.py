@@ -18,6 +18,7 @@ SOURCES = [
     "k_fock.hip",
     "k_eigen.hip",
     "k_pseudo.hip",
+    "k_occvirt.hip",
     "k_precision.hip",
     "k_mlp.hip",
     "k_batch.hip",

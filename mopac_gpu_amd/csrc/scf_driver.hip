@@ -15,6 +15,7 @@
 #include "common.h"
 #include "k_eigen.h"
 #include "k_fock.h"
+#include "k_occvirt.h"
 #include "k_pseudo.h"
 
 struct mhip_ctx {
@@ -35,6 +36,7 @@ struct mhip_ctx {
   double* hscal = nullptr;  // pinned host scalars
   // pseudo-diagonalization workspace
   mhip::PseudoWork pseudo;
+  mhip::OccVirtWork ov;
   // statistics of the last eigensolve
   int last_sweeps = 0;
   // Jacobi stopping tolerance (relative to max|diag|) inside the SCF loop;
@@ -429,7 +431,7 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
   const int ihomo = std::max(nclose, 1);
   double eig_homo = 0.0, eig_lumo = 0.0;
   PulayState pst;
-  int nfull = 0, npseudo = 0;
+  int nfull = 0, npseudo = 0, block_steps_since_diag = 0;
   for (;;) {
     niter += 1;
     int mode = kInitH;
@@ -496,10 +498,22 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
     if (newdg && okpuly && iredy > 1) pulay_step(c, pst, &pl);
     const bool use_pseudo = o->pseudo_diag && newdg && c->have_c && !okpuly;
     if (use_pseudo) {
-      pseudo_diag_device(s, c->pseudo, n, c->npad, c->f.as<double>(), c->C.as<double>(), nclose,
-                         c->eig.as<double>(), nullptr);
+      if (o->pseudo_diag == 2) {
+        occvirt_step(s, c->ov, n, c->npad, c->f.as<double>(), c->C.as<double>(), nclose,
+                     c->eig.as<double>());
+        block_steps_since_diag += 1;
+      } else {
+        pseudo_diag_device(s, c->pseudo, n, c->npad, c->f.as<double>(), c->C.as<double>(), nclose,
+                           c->eig.as<double>(), nullptr);
+      }
       ++npseudo;
     } else {
+      if (block_steps_since_diag > 0) {
+        // the warm start needs orthonormal columns to working precision
+        bjorck_step(s, c->ov, n, c->npad, c->C.as<double>(), 0, nclose);
+        bjorck_step(s, c->ov, n, c->npad, c->C.as<double>(), nclose, n - nclose);
+        block_steps_since_diag = 0;
+      }
       double tol = c->tol_iter;
       if (o->adaptive_eigen && pl > 10.0 * pltest) {
         // far from self-consistency the eigenvectors only need to be as good
@@ -532,6 +546,10 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
   }
   // ---- exit (iter.F90:1052-1106)
   ee = energy(c);
+  if (block_steps_since_diag > 0) {
+    bjorck_step(s, c->ov, n, c->npad, c->C.as<double>(), 0, nclose);
+    bjorck_step(s, c->ov, n, c->npad, c->C.as<double>(), nclose, n - nclose);
+  }
   copy_d2d(c->fcopy.ptr, c->f.ptr, sizeof(double) * mp, s);
   eigen_packed(c, c->fcopy.as<double>(), true);
   ++nfull;
@@ -596,6 +614,7 @@ int mhip_finalize(mhip_ctx* c) {
       b->release();
     c->plan.release();
     c->pseudo.release();
+    c->ov.release();
     if (c->hscal) (void)hipHostFree(c->hscal);
     (void)hipStreamDestroy(c->stream);
     delete c;

@@ -97,7 +97,7 @@ class Context:
         o.scfcrt = scfcrt
         o.last = last
         o.precision = precision
-        o.pseudo_diag = int(bool(pseudo_diag))
+        o.pseudo_diag = int(pseudo_diag)
         o.adaptive_eigen = int(bool(adaptive_eigen))
         o.enuclr = s.enuclr
         o.atheat = s.atheat
