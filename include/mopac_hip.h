@@ -94,7 +94,8 @@ typedef struct {
   int itrmax;          /* ITRY= (default 2000) */
   double scfcrt;       /* SCFCRT= (default 1e-4) */
   int last;            /* 1 for a 1SCF job (phase_lock at exit) */
-  int precision;       /* 0 = FP64; 1 = split-BF16 MFMA density with FP64 correction */
+  int precision;       /* 0 = FP64; 1 = split-BF16 MFMA contractions (density, block
+                          pseudo-diagonalization) while PL > 100 x PLTEST, FP64 after */
   int pseudo_diag;     /* once newdg holds (iter.F90:843-861): 0 = always full diagonalization
                           (the reference), 1 = diag.F90's ordered rotation list, 2 = the same
                           rotations applied at once as GEMMs + Bjorck re-orthonormalization */
@@ -103,6 +104,11 @@ typedef struct {
   double atheat;       /* kcal/mol */
   const double* pdiag; /* starting diagonal (moldat.F90:731-790), n */
   const double* p_init;/* optional starting total density (mpack), overrides pdiag */
+  int block_start;     /* with pseudo_diag == 2: 0 = block steps once newdg holds; 1 = whenever
+                          eigenvectors exist; 2 = also take the first eigenvectors from the
+                          fragment blocks of F (needs fragment and pdiag) */
+  const int* fragment; /* optional atom -> fragment id (0-based, e.g. the molecules of a
+                          cluster; <= 16 orbitals and an even electron count each) */
 } mhip_scf_opts;
 
 typedef struct {
@@ -114,6 +120,8 @@ typedef struct {
   int n_pseudo_diag;
   double seconds;      /* wall time of guess + iterations + final diag */
   int jacobi_sweeps;   /* block-Jacobi sweeps summed over all diagonalizations */
+  int n_fragment_diag; /* fragment-block starts (0 or 1) */
+  int n_split_iter;    /* iterations whose contractions ran split-BF16 */
 } mhip_scf_result;
 
 /* Run iter() (src/SCF/iter.F90:16-1107, RHF closed shell) with every matrix
@@ -141,6 +149,13 @@ void mhip_dsyevd(int n, double* a, int lda, double* w, int* info);
  * (n-nocc) x nocc occupied-virtual block, virtual index slow. */
 void mhip_call_rot(const double* fmo, const double* eig, double* v, const double* ci0,
                    const double* ca0, int nocc, int lumo, int n, double bigeps, double tiny);
+/* The precision form's contraction with call_gemm's signature: FP64 in and
+ * out, computed from a three-way BF16 split of each operand on the BF16
+ * matrix cores (FP32 accumulation in 32-term chunks, summed in FP64).  No
+ * counterpart in the reference; exported so the contraction's error against
+ * FP64 can be measured on its own. */
+void mhip_gemm_split_bf16(char tra, char trb, int m, int n, int k, double alpha, const double* a,
+                          int lda, const double* b, int ldb, double beta, double* c, int ldc);
 /* mopac_cuda_destroy_resources (src/gpu/cuda_wrappers.cu:830-856) */
 void mhip_destroy_resources(void);
 
@@ -178,6 +193,15 @@ int mhip_last_sweeps(mhip_ctx* ctx);
 int mhip_kernel_timing(mhip_ctx* ctx, int enable, double* solve_ms, long long* n_solve,
                        double* apply_ms, long long* n_apply, double* applyc_ms,
                        long long* n_applyc, long long* active_tiles, long long* active_pairs);
+
+/* Process-wide HIP-event timing of the other kernel classes (FP64 GEMM,
+ * split-BF16 GEMM, packed density, fock2), on the stream each launch uses.
+ * enable = 1 / 0 resets and switches it on / off; -1 fills, per class
+ * (index < max_classes, at most 4), the summed milliseconds, launches,
+ * algorithmic flops and algorithmic HBM bytes of the recorded launches. */
+int mhip_kernel_stats(mhip_ctx* ctx, int enable, int max_classes, double* ms, long long* launches,
+                      double* flops, double* bytes);
+const char* mhip_kernel_class_name(int cls);
 
 /* ---- PM7 Hamiltonian (host, once per geometry) ---------------------------
  * The producer of the loop's inputs (hcore, src/integrals/hcore.F90), for

@@ -19,6 +19,7 @@ SOURCES = [
     "k_eigen.hip",
     "k_pseudo.hip",
     "k_occvirt.hip",
+    "k_split.hip",
     "k_precision.hip",
     "k_mlp.hip",
     "k_batch.hip",

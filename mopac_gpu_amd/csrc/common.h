@@ -61,16 +61,65 @@ struct DevBuf {
 
 inline long long mpack_of(long long n) { return n * (n + 1) / 2; }
 
+// ---- per-class launch timing (HIP events on the launching stream) --------
+// Off unless mhip_kernel_stats switches it on; each timed launch records an
+// event pair and its algorithmic flops and HBM bytes.
+enum KClass {
+  K_GEMM_F64 = 0,    // gemm on the FP64 MFMA tiles: 2 m n k flops
+  K_GEMM_SPLIT,      // gemm_split_bf16: 2 m n k (FP64-equivalent) flops
+  K_SYRK_F64,        // packed density: n (n + 1) k flops
+  K_FOCK,            // fock_build (row + column pass): 8 (n2elec + 4 mpack) bytes
+  K_NCLASS
+};
+struct KTimer {
+  bool on = false;
+  struct Rec {
+    int cls;
+    hipEvent_t a, b;
+    double flops, bytes;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  hipEvent_t next() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      MHIP_CHECK(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+};
+extern KTimer g_ktimer;
+// Wrap one launch (or a short fixed sequence) of class cls.
+template <class F>
+inline void timed(hipStream_t s, int cls, double flops, double bytes, F&& launch) {
+  if (!g_ktimer.on) {
+    launch();
+    return;
+  }
+  hipEvent_t a = g_ktimer.next(), b = g_ktimer.next();
+  MHIP_CHECK(hipEventRecord(a, s));
+  launch();
+  MHIP_CHECK(hipEventRecord(b, s));
+  g_ktimer.recs.push_back({cls, a, b, flops, bytes});
+}
+
 // ---- dense kernels (k_dense.hip) ------------------------------------------
 // 1: FP64 MFMA tiles (default), 0: vector-FMA tiles (kept as a cross-check).
 extern int g_kernel_variant;
 // C = alpha * op(A) * op(B) + beta * C, column-major FP64.
 void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double* a,
           int lda, const double* b, int ldb, double beta, double* c, int ldc);
+// Same contract as gemm, computed on the BF16 matrix cores from a three-way
+// BF16 split of each FP64 operand (six cross products, FP32 accumulation in
+// 32-term chunks summed in FP64; k_split.hip).
+void gemm_split_bf16(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha,
+                     const double* a, int lda, const double* b, int ldb, double beta, double* c,
+                     int ldc);
 // Packed lower triangle p (mpack) = occ * A(:,0:k) A(:,0:k)^T over the first n rows.
 void syrk_packed(hipStream_t s, int n, int k, double occ, const double* a, int lda, double* p);
-// Split-BF16 variant: A = hi + lo (both bf16), products accumulated in FP32 on
-// MFMA, with the lo*lo term dropped and an FP64 correction of the diagonal.
+// Split-BF16 variant (gemm_split_bf16 into a dense n x n scratch, then packed).
 void syrk_packed_split_bf16(hipStream_t s, int n, int k, double occ, const double* a, int lda,
                             double* p, void* scratch);
 size_t syrk_split_scratch_bytes(int n, int k);

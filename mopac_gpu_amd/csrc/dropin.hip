@@ -51,7 +51,9 @@ using namespace mhip;
 
 extern "C" {
 
-void mhip_call_gemm(char tra, char trb, int m, int n, int k, double alpha, const double* a,
+}  // extern "C"
+
+static void call_gemm_impl(bool split, char tra, char trb, int m, int n, int k, double alpha, const double* a,
                     int lda, const double* b, int ldb, double beta, double* c, int ldc) {
   std::lock_guard<std::mutex> lk(g_mu);
   try {
@@ -62,14 +64,30 @@ void mhip_call_gemm(char tra, char trb, int m, int n, int k, double alpha, const
     h2d2(I.a, a, ar, ac, lda, ar, I.s);
     h2d2(I.b, b, br, bc, ldb, br, I.s);
     h2d2(I.c, c, m, n, ldc, m, I.s);
-    gemm(I.s, tra, trb, m, n, k, alpha, I.a.as<double>(), ar, I.b.as<double>(), br, beta,
-         I.c.as<double>(), m);
+    if (split)
+      gemm_split_bf16(I.s, tra, trb, m, n, k, alpha, I.a.as<double>(), ar, I.b.as<double>(), br,
+                      beta, I.c.as<double>(), m);
+    else
+      gemm(I.s, tra, trb, m, n, k, alpha, I.a.as<double>(), ar, I.b.as<double>(), br, beta,
+           I.c.as<double>(), m);
     MHIP_CHECK(hipMemcpy2DAsync(c, sizeof(double) * ldc, I.c.ptr, sizeof(double) * m,
                                 sizeof(double) * m, n, hipMemcpyDeviceToHost, I.s));
     MHIP_CHECK(hipStreamSynchronize(I.s));
   } catch (const HipError& e) {
     report(e);
   }
+}
+
+extern "C" {
+
+void mhip_call_gemm(char tra, char trb, int m, int n, int k, double alpha, const double* a,
+                    int lda, const double* b, int ldb, double beta, double* c, int ldc) {
+  call_gemm_impl(false, tra, trb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+void mhip_gemm_split_bf16(char tra, char trb, int m, int n, int k, double alpha, const double* a,
+                          int lda, const double* b, int ldb, double beta, double* c, int ldc) {
+  call_gemm_impl(true, tra, trb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
 }
 
 void mhip_call_syrk(char uplo, char tra, int n, int k, double alpha, const double* a, int lda,

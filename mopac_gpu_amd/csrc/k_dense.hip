@@ -5,6 +5,8 @@
 #include "mfma.h"
 
 namespace mhip {
+
+KTimer g_ktimer;
 namespace {
 
 constexpr int TB = 64;    // output tile edge
@@ -276,8 +278,9 @@ inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 int g_kernel_variant = 1;  // 1: FP64 MFMA tiles, 0: vector-FMA tiles
 
-void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double* a,
-          int lda, const double* b, int ldb, double beta, double* c, int ldc) {
+static void gemm_launch(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha,
+                        const double* a, int lda, const double* b, int ldb, double beta, double* c,
+                        int ldc) {
   if (m <= 0 || n <= 0) return;
   dim3 grid(cdiv(n, TB), cdiv(m, TB));
   const bool TA = (ta == 'T' || ta == 't'), TBT = (tb == 'T' || tb == 't');
@@ -296,14 +299,26 @@ void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
   MHIP_LAUNCH_CHECK();
 }
 
-void syrk_packed(hipStream_t s, int n, int k, double occ, const double* a, int lda, double* p) {
-  if (n <= 0) return;
+void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double* a,
+          int lda, const double* b, int ldb, double beta, double* c, int ldc) {
+  if (m <= 0 || n <= 0) return;
+  timed(s, K_GEMM_F64, 2.0 * m * n * (double)k, 8.0 * ((double)m * k + (double)k * n + 2.0 * m * n),
+        [&] { gemm_launch(s, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); });
+}
+
+static void syrk_launch(hipStream_t s, int n, int k, double occ, const double* a, int lda, double* p) {
   const int nt = cdiv(n, TB);
   if (g_kernel_variant == 1)
     hipLaunchKernelGGL(k_syrk_packed_mfma, dim3(nt * (nt + 1) / 2), dim3(NT), 0, s, n, k, occ, a, lda, p);
   else
     hipLaunchKernelGGL(k_syrk_packed, dim3(nt * (nt + 1) / 2), dim3(NT), 0, s, n, k, occ, a, lda, p);
   MHIP_LAUNCH_CHECK();
+}
+
+void syrk_packed(hipStream_t s, int n, int k, double occ, const double* a, int lda, double* p) {
+  if (n <= 0) return;
+  timed(s, K_SYRK_F64, (double)n * (n + 1) * (double)k, 8.0 * ((double)n * k + (double)n * (n + 1) / 2),
+        [&] { syrk_launch(s, n, k, occ, a, lda, p); });
 }
 
 void unpack_sym(hipStream_t s, int n, const double* p, double* a, int lda, int npad,
@@ -356,13 +371,15 @@ void dots(hipStream_t s, long long len, int nvec, const double* a, long long str
 
 namespace mhip {
 size_t syrk_split_scratch_bytes(int n, int k) {
-  (void)n;
   (void)k;
-  return 8;
+  return sizeof(double) * (size_t)n * n + 8;
 }
+// occ A A^T through the split-BF16 GEMM into a dense n x n scratch, then packed.
 void syrk_packed_split_bf16(hipStream_t s, int n, int k, double occ, const double* a, int lda,
                             double* p, void* scratch) {
-  (void)s; (void)n; (void)k; (void)occ; (void)a; (void)lda; (void)p; (void)scratch;
-  throw HipError{"split-BF16 density kernel is not available in this build"};
+  if (n <= 0) return;
+  double* d = static_cast<double*>(scratch);
+  gemm_split_bf16(s, 'N', 'T', n, n, k, occ, a, lda, a, lda, 0.0, d, n);
+  pack_lower(s, n, d, n, p);
 }
 }  // namespace mhip

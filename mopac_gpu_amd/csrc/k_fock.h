@@ -8,6 +8,7 @@ namespace mhip {
 struct SysDev {
   int numat = 0;
   int norbs = 0;
+  long long n2elec = 0;
   const int* nf0 = nullptr;             // first orbital of each atom (0-based)
   const int* norb = nullptr;            // 1 (H) or 4 (C, N, O)
   const long long* woff = nullptr;      // start of atom i's W row (pairs j < i)

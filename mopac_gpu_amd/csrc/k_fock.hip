@@ -401,10 +401,14 @@ __global__ void k_axpby_packed(long long len, double a, const double* __restrict
 }  // namespace
 
 void fock_build(hipStream_t s, const FockArgs& a) {
-  hipLaunchKernelGGL(k_fock_rows, dim3(a.sys.numat), dim3(256), 0, s, a);
-  MHIP_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_fock_cols, dim3(a.sys.numat), dim3(256), 0, s, a);
-  MHIP_LAUNCH_CHECK();
+  // algorithmic traffic: W once, plus ptot, p, f in and f out (packed)
+  const double mp = (double)a.sys.norbs * (a.sys.norbs + 1) / 2;
+  timed(s, K_FOCK, 0.0, 8.0 * ((double)a.sys.n2elec + 4.0 * mp), [&] {
+    hipLaunchKernelGGL(k_fock_rows, dim3(a.sys.numat), dim3(256), 0, s, a);
+    MHIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fock_cols, dim3(a.sys.numat), dim3(256), 0, s, a);
+    MHIP_LAUNCH_CHECK();
+  });
 }
 
 void helect(hipStream_t s, int n, const double* pa, const double* h, const double* f,

@@ -47,6 +47,53 @@ __global__ __launch_bounds__(256) void k_ov_sines(int nocc, int nvirt, const dou
   }
 }
 
+// colmax[blockIdx.x] = max over this block's occupied columns j of sum_v B(v,j)^2
+__global__ __launch_bounds__(256) void k_b_colnorm(int nocc, int nvirt, const double* __restrict__ b,
+                                                   double* __restrict__ part) {
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double mx = 0.0;
+  for (int j = blockIdx.x * 4 + w; j < nocc; j += gridDim.x * 4) {
+    double s = 0.0;
+    for (int v = lane; v < nvirt; v += 64) {
+      const double x = b[(size_t)v + (size_t)j * nvirt];
+      s += x * x;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    mx = fmax(mx, s);
+  }
+  if (lane == 0) red[w] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// B *= min(1, cap / max column norm): bounds the step so that the Bjorck
+// steps stay in their region of convergence far from self-consistency
+__global__ __launch_bounds__(256) void k_b_scale(long long len, int nparts, const double* __restrict__ part,
+                                                 double cap, double* __restrict__ b) {
+  double mx = 0.0;
+  for (int i = 0; i < nparts; ++i) mx = fmax(mx, part[i]);
+  const double nrm = sqrt(mx);
+  if (!(nrm > cap)) return;
+  const double sc = cap / nrm;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < len; i += (long long)gridDim.x * 256) b[i] *= sc;
+}
+
+// eig(nocc + v) = C_v(:, v) . (F C_v)(:, v): the virtual levels of the current
+// F (level shift and SCF changes included) in the current virtual orbitals
+__global__ __launch_bounds__(256) void k_virt_rayleigh(int n, int nocc, int nvirt, int ld,
+                                                       const double* __restrict__ cv,
+                                                       const double* __restrict__ wv,
+                                                       double* __restrict__ eig) {
+  const int lane = threadIdx.x & 63;
+  const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (v >= nvirt) return;
+  double s = 0.0;
+  for (int i = lane; i < n; i += 64) s += cv[(size_t)i + (size_t)v * ld] * wv[(size_t)i + (size_t)v * ld];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) eig[nocc + v] = s;
+}
+
 __global__ void k_sub_identity(int m, double* __restrict__ e, int lde) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m) e[(size_t)i + (size_t)i * lde] -= 1.0;
@@ -69,13 +116,14 @@ void OccVirtWork::ensure(int n, int nocc, int ld) {
   eo.ensure(sizeof(double) * (size_t)ld * ld);
   ev.ensure(sizeof(double) * (size_t)ld * ld);
   cw.ensure(sizeof(double) * (size_t)ld * ld);
+  part.ensure(sizeof(double) * 256);
 }
 
 void OccVirtWork::release() {
-  for (DevBuf* x : {&fd, &wv, &fmo, &b, &eo, &ev, &cw}) x->release();
+  for (DevBuf* x : {&fd, &wv, &fmo, &b, &eo, &ev, &cw, &part}) x->release();
 }
 
-void bjorck_step(hipStream_t s, OccVirtWork& w, int n, int ld, double* C, int c0, int m) {
+void bjorck_step(hipStream_t s, OccVirtWork& w, int n, int ld, double* C, int c0, int m, bool split) {
   if (m <= 0) return;
   MHIP_REQUIRE(c0 >= 0 && c0 + m <= ld && n <= ld, "bjorck_step: column block out of range");
   w.ensure(n, 0, ld);
@@ -88,12 +136,13 @@ void bjorck_step(hipStream_t s, OccVirtWork& w, int n, int ld, double* C, int c0
   gemm(s, 'T', 'N', m, m, n, 1.0, cw, ld, cw, ld, 0.0, e, m);
   hipLaunchKernelGGL(k_sub_identity, dim3((m + 255) / 256), dim3(256), 0, s, m, e, m);
   MHIP_LAUNCH_CHECK();
-  // C <- C - C E / 2
-  gemm(s, 'N', 'N', n, m, m, -0.5, cw, ld, e, m, 1.0, cb, ld);
+  // C <- C - C E / 2 (E is small: the split product's error is second order)
+  if (split) gemm_split_bf16(s, 'N', 'N', n, m, m, -0.5, cw, ld, e, m, 1.0, cb, ld);
+  else gemm(s, 'N', 'N', n, m, m, -0.5, cw, ld, e, m, 1.0, cb, ld);
 }
 
 void occvirt_step(hipStream_t s, OccVirtWork& w, int n, int ld, const double* fpacked, double* C,
-                  int nocc, const double* eig) {
+                  int nocc, double* eig, bool split, int bjorck_steps) {
   const int nvirt = n - nocc;
   if (nocc <= 0 || nvirt <= 0) return;
   MHIP_REQUIRE(nocc <= ld && n <= ld, "occvirt_step: bad sizes");
@@ -102,19 +151,143 @@ void occvirt_step(hipStream_t s, OccVirtWork& w, int n, int ld, const double* fp
   unpack_sym(s, n, fpacked, fd, ld, n, 0.0);
   double* cv = C + (size_t)nocc * ld;
   // Wv = F C_virt (n x nvirt), FMO = C_occ^T Wv (nocc x nvirt)
-  gemm(s, 'N', 'N', n, nvirt, n, 1.0, fd, ld, cv, ld, 0.0, w.wv.as<double>(), ld);
-  gemm(s, 'T', 'N', nocc, nvirt, n, 1.0, C, ld, w.wv.as<double>(), ld, 0.0, w.fmo.as<double>(), nocc);
+  auto mm = [&](char ta, char tb, int m, int nn, int k, double al, const double* a, int lda,
+                const double* b, int ldb, double be, double* c, int ldc) {
+    if (split) gemm_split_bf16(s, ta, tb, m, nn, k, al, a, lda, b, ldb, be, c, ldc);
+    else gemm(s, ta, tb, m, nn, k, al, a, lda, b, ldb, be, c, ldc);
+  };
+  mm('N', 'N', n, nvirt, n, 1.0, fd, ld, cv, ld, 0.0, w.wv.as<double>(), ld);
+  mm('T', 'N', nocc, nvirt, n, 1.0, C, ld, w.wv.as<double>(), ld, 0.0, w.fmo.as<double>(), nocc);
+  hipLaunchKernelGGL(k_virt_rayleigh, dim3((nvirt + 3) / 4), dim3(256), 0, s, n, nocc, nvirt, ld, cv,
+                     w.wv.as<double>(), eig);
+  MHIP_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_ov_sines, dim3((nocc + TT - 1) / TT, (nvirt + TT - 1) / TT), dim3(256), 0, s,
                      nocc, nvirt, w.fmo.as<double>(), eig, w.b.as<double>());
+  MHIP_LAUNCH_CHECK();
+  constexpr int kParts = 256;
+  hipLaunchKernelGGL(k_b_colnorm, dim3(kParts), dim3(256), 0, s, nocc, nvirt, w.b.as<double>(),
+                     w.part.as<double>());
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_b_scale, dim3(1024), dim3(256), 0, s, (long long)nocc * nvirt, kParts,
+                     w.part.as<double>(), kMaxStep, w.b.as<double>());
   MHIP_LAUNCH_CHECK();
   // simultaneous rotations, both from the old C
   double* cw = w.cw.as<double>();
   copy_cols(s, n, ld, C, cw, n);
-  gemm(s, 'N', 'N', n, nocc, nvirt, 1.0, cw + (size_t)nocc * ld, ld, w.b.as<double>(), nvirt, 1.0, C,
-       ld);
-  gemm(s, 'N', 'T', n, nvirt, nocc, -1.0, cw, ld, w.b.as<double>(), nvirt, 1.0, cv, ld);
-  bjorck_step(s, w, n, ld, C, 0, nocc);
-  bjorck_step(s, w, n, ld, C, nocc, nvirt);
+  mm('N', 'N', n, nocc, nvirt, 1.0, cw + (size_t)nocc * ld, ld, w.b.as<double>(), nvirt, 1.0, C, ld);
+  mm('N', 'T', n, nvirt, nocc, -1.0, cw, ld, w.b.as<double>(), nvirt, 1.0, cv, ld);
+  for (int it = 0; it < bjorck_steps; ++it) {
+    bjorck_step(s, w, n, ld, C, 0, nocc, split);
+    bjorck_step(s, w, n, ld, C, nocc, nvirt, split);
+  }
+}
+
+}  // namespace mhip
+
+namespace mhip {
+namespace {
+
+__global__ __launch_bounds__(64) void k_fragment_eigen(const double* __restrict__ fd, int ld,
+                                                       const int* __restrict__ fptr,
+                                                       const int* __restrict__ forb,
+                                                       const int* __restrict__ focc,
+                                                       const int* __restrict__ occ_off,
+                                                       const int* __restrict__ virt_off,
+                                                       double* __restrict__ C,
+                                                       double* __restrict__ eig) {
+  constexpr int M = kMaxFragOrbs;
+  __shared__ double a[M][M + 1], v[M][M + 1];
+  __shared__ double cs[2];
+  __shared__ int order[M];
+  __shared__ int done;
+  const int f = blockIdx.x, tid = threadIdx.x;
+  const int o0 = fptr[f], nf = fptr[f + 1] - o0;
+  for (int e = tid; e < M * M; e += 64) {
+    const int r = e / M, c = e % M;
+    a[r][c] = (r < nf && c < nf) ? fd[(size_t)forb[o0 + r] + (size_t)forb[o0 + c] * ld] : 0.0;
+    v[r][c] = (r == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < 50; ++sweep) {
+    if (tid == 0) {
+      double off = 0.0, dg = 0.0;
+      for (int r = 0; r < nf; ++r) {
+        dg = fmax(dg, fabs(a[r][r]));
+        for (int c = 0; c < r; ++c) off = fmax(off, fabs(a[r][c]));
+      }
+      done = off <= 1.0e-15 * fmax(dg, 1.0);
+    }
+    __syncthreads();
+    if (done) break;
+    for (int p = 0; p < nf - 1; ++p) {
+      for (int q = p + 1; q < nf; ++q) {
+        if (tid == 0) {
+          const double app = a[p][p], aqq = a[q][q], apq = a[p][q];
+          double c = 1.0, sn = 0.0;
+          if (fabs(apq) > 1.0e-300) {
+            const double theta = (aqq - app) / (2.0 * apq);
+            const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            sn = t * c;
+          }
+          cs[0] = c;
+          cs[1] = sn;
+        }
+        __syncthreads();
+        const double c = cs[0], sn = cs[1];
+        if (tid < nf) {  // rows p, q
+          const double x = a[p][tid], y = a[q][tid];
+          a[p][tid] = c * x - sn * y;
+          a[q][tid] = sn * x + c * y;
+        }
+        __syncthreads();
+        if (tid < nf) {  // columns p, q of a and v
+          const double x = a[tid][p], y = a[tid][q];
+          a[tid][p] = c * x - sn * y;
+          a[tid][q] = sn * x + c * y;
+          const double u = v[tid][p], w = v[tid][q];
+          v[tid][p] = c * u - sn * w;
+          v[tid][q] = sn * u + c * w;
+        }
+        __syncthreads();
+      }
+    }
+  }
+  if (tid == 0) {  // ascending order of the eigenvalues
+    for (int k = 0; k < nf; ++k) order[k] = k;
+    for (int k = 1; k < nf; ++k) {
+      const int key = order[k];
+      int m = k - 1;
+      while (m >= 0 && a[order[m]][order[m]] > a[key][key]) {
+        order[m + 1] = order[m];
+        --m;
+      }
+      order[m + 1] = key;
+    }
+  }
+  __syncthreads();
+  const int no = focc[f];
+  for (int e = tid; e < nf * nf; e += 64) {
+    const int k = e / nf, r = e % nf;  // k-th eigenvector, row r
+    const int col = (k < no) ? occ_off[f] + k : virt_off[f] + (k - no);
+    C[(size_t)forb[o0 + r] + (size_t)col * ld] = v[r][order[k]];
+  }
+  if (tid < nf) {
+    const int k = tid;
+    const int col = (k < no) ? occ_off[f] + k : virt_off[f] + (k - no);
+    eig[col] = a[order[k]][order[k]];
+  }
+}
+
+}  // namespace
+
+void fragment_eigen(hipStream_t s, int nfrag, const double* fd, int ld, const int* fptr,
+                    const int* forb, const int* focc, const int* occ_off, const int* virt_off,
+                    double* C, double* eig) {
+  if (nfrag <= 0) return;
+  hipLaunchKernelGGL(k_fragment_eigen, dim3(nfrag), dim3(64), 0, s, fd, ld, fptr, forb, focc, occ_off,
+                     virt_off, C, eig);
+  MHIP_LAUNCH_CHECK();
 }
 
 }  // namespace mhip

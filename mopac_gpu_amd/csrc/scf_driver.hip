@@ -11,6 +11,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <vector>
 
 #include "common.h"
 #include "k_eigen.h"
@@ -26,7 +27,10 @@ struct mhip_ctx {
   long long mpack = 0, n2elec = 0;
   mhip::DevBuf w, h, nf0, norb, woff, wcum, w1off, ccol, heavy, light, nhb, nlb, jrow, jcol;
   mhip::SysDev sys;
+  std::vector<int> nf0_h, no_h;  // first orbital (0-based) and orbital count per atom
   bool have_system = false;
+  // fragment start (CSR over fragments)
+  mhip::DevBuf frag_ptr, frag_orb, frag_occ, frag_ooff, frag_voff, frontier;
   // SCF state
   mhip::DevBuf p, pa, f, fcopy, pold, pold2, fd, A, C, C2, T, eig, scal, partial;
   mhip::DevBuf fhist, ehist, coef;
@@ -90,6 +94,42 @@ __global__ void k_get_diag_packed(int n, const double* p, double* d) {
   d[i] = p[(long long)i * (i + 1) / 2 + i];
 }
 
+// out[0] = max eig over the occupied columns, out[1] = min over the virtual
+// ones (the frontier levels whatever the column order)
+__global__ __launch_bounds__(256) void k_frontier(int n, int nocc, const double* __restrict__ eig,
+                                                  double* __restrict__ out) {
+  __shared__ double hi[256], lo[256];
+  double h = -1.0e300, l = 1.0e300;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    if (i < nocc) h = fmax(h, eig[i]);
+    else l = fmin(l, eig[i]);
+  }
+  hi[threadIdx.x] = h;
+  lo[threadIdx.x] = l;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + w]);
+      lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = hi[0];
+    out[1] = lo[0];
+  }
+}
+
+// identity on the padding columns/rows [n, npad), zero elsewhere
+__global__ void k_pad_identity(int n, int npad, double* __restrict__ c) {
+  const long long len = (long long)npad * npad;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(e % npad), col = (int)(e / npad);
+    c[e] = (r == col && r >= n) ? 1.0 : 0.0;
+  }
+}
+
 void copy_d2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
   MHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
 }
@@ -114,6 +154,8 @@ static void build_system(mhip_ctx* c, int numat, const int* nfirst, const int* n
     MHIP_REQUIRE(i == 0 || nf0[i] == nf0[i - 1] + no[i - 1], "orbitals must be contiguous by atom");
   }
   MHIP_REQUIRE(nf0[numat - 1] + no[numat - 1] == norbs, "nlast(numat) must equal norbs");
+  c->nf0_h = nf0;
+  c->no_h = no;
   for (int i = 0; i < numat; ++i) {
     const int lim = no[i] * (no[i] + 1) / 2;
     woff[i] = kr;
@@ -146,6 +188,7 @@ static void build_system(mhip_ctx* c, int numat, const int* nfirst, const int* n
   SysDev& d = c->sys;
   d.numat = numat;
   d.norbs = norbs;
+  d.n2elec = n2elec;
   d.nf0 = c->nf0.as<int>();
   d.norb = c->norb.as<int>();
   d.woff = c->woff.as<long long>();
@@ -379,6 +422,50 @@ static void density(mhip_ctx* c, int nocc, bool split) {
   }
 }
 
+// Host tables of the fragment start; returns the number of fragments, or 0
+// when the map is absent or unusable (then the first iteration diagonalizes F
+// in full, as the reference does).
+static int setup_fragments(mhip_ctx* c, const mhip_scf_opts* o) {
+  if (!o->fragment || o->pseudo_diag != 2 || o->block_start < 2 || !o->pdiag) return 0;
+  const int numat = c->numat;
+  int nfrag = 0;
+  for (int a = 0; a < numat; ++a) {
+    if (o->fragment[a] < 0) return 0;
+    nfrag = std::max(nfrag, o->fragment[a] + 1);
+  }
+  std::vector<std::vector<int>> orbs(nfrag);
+  for (int a = 0; a < numat; ++a)
+    for (int k = 0; k < c->no_h[a]; ++k) orbs[o->fragment[a]].push_back(c->nf0_h[a] + k);
+  std::vector<int> ptr(nfrag + 1, 0), orb, focc(nfrag), ooff(nfrag), voff(nfrag);
+  int nocc_sum = 0, nvirt_sum = 0;
+  for (int f = 0; f < nfrag; ++f) {
+    const int nf = (int)orbs[f].size();
+    if (nf == 0 || nf > kMaxFragOrbs) return 0;
+    double el = 0.0;
+    for (int i : orbs[f]) el += o->pdiag[i];
+    const long long ne = std::llround(el);
+    if (std::fabs(el - (double)ne) > 1e-6 || (ne & 1) || ne / 2 > nf) return 0;
+    focc[f] = (int)(ne / 2);
+    ptr[f + 1] = ptr[f] + nf;
+    orb.insert(orb.end(), orbs[f].begin(), orbs[f].end());
+    ooff[f] = nocc_sum;
+    nocc_sum += focc[f];
+  }
+  if (nocc_sum != o->nclose) return 0;
+  for (int f = 0; f < nfrag; ++f) {
+    voff[f] = o->nclose + nvirt_sum;
+    nvirt_sum += (int)orbs[f].size() - focc[f];
+  }
+  hipStream_t s = c->stream;
+  upload(c->frag_ptr, ptr.data(), sizeof(int) * ptr.size(), s);
+  upload(c->frag_orb, orb.data(), sizeof(int) * orb.size(), s);
+  upload(c->frag_occ, focc.data(), sizeof(int) * nfrag, s);
+  upload(c->frag_ooff, ooff.data(), sizeof(int) * nfrag, s);
+  upload(c->frag_voff, voff.data(), sizeof(int) * nfrag, s);
+  MHIP_CHECK(hipStreamSynchronize(s));  // the host vectors go out of scope
+  return nfrag;
+}
+
 void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_out,
              double* f_out, double* c_out, double* eig_out) {
   MHIP_REQUIRE(c->have_system, "mhip_upload_system must be called before mhip_scf");
@@ -431,7 +518,9 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
   const int ihomo = std::max(nclose, 1);
   double eig_homo = 0.0, eig_lumo = 0.0;
   PulayState pst;
-  int nfull = 0, npseudo = 0, block_steps_since_diag = 0;
+  int nfull = 0, npseudo = 0, block_steps_since_diag = 0, nfragdiag = 0, n_split = 0;
+  const int nfrag = setup_fragments(c, o);
+  c->frontier.ensure(sizeof(double) * 2);
   for (;;) {
     niter += 1;
     int mode = kInitH;
@@ -474,6 +563,7 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
       fock_build(s, fock_args(c, recompute ? kInitH : mode, shift));
       if (niter >= itrmax) break;
       ee = energy(c);
+      MHIP_REQUIRE(std::isfinite(ee), "SCF produced a non-finite electronic energy");
       escf = (ee + o->enuclr) * enrgy + o->atheat;
       diff = escf - eold;
       if (diff > 0) ten = ten - 1.0;
@@ -496,11 +586,30 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
     iredy += 1;
     eold = escf;
     if (newdg && okpuly && iredy > 1) pulay_step(c, pst, &pl);
-    const bool use_pseudo = o->pseudo_diag && newdg && c->have_c && !okpuly;
-    if (use_pseudo) {
+    // precision form: split-BF16 contractions while far from self-consistency,
+    // FP64 for the iterations that decide convergence
+    const bool split_now = o->precision == 1 && pl > 100.0 * pltest;
+    n_split += split_now ? 1 : 0;
+    const bool use_pseudo =
+        o->pseudo_diag && (newdg || (o->pseudo_diag == 2 && o->block_start >= 1)) && c->have_c && !okpuly;
+    if (!c->have_c && nfrag > 0) {
+      // fragment start: block-diagonal eigenvectors of F's fragment blocks
+      c->ov.ensure(n, nclose, c->npad);
+      unpack_sym(s, n, c->f.as<double>(), c->ov.fd.as<double>(), c->npad, n, 0.0);
+      hipLaunchKernelGGL(k_pad_identity, dim3(2048), dim3(256), 0, s, n, c->npad, c->C.as<double>());
+      MHIP_LAUNCH_CHECK();
+      fragment_eigen(s, nfrag, c->ov.fd.as<double>(), c->npad, c->frag_ptr.as<int>(),
+                     c->frag_orb.as<int>(), c->frag_occ.as<int>(), c->frag_ooff.as<int>(),
+                     c->frag_voff.as<int>(), c->C.as<double>(), c->eig.as<double>());
+      c->have_c = true;
+      block_steps_since_diag += 1;
+      ++nfragdiag;
+    } else if (use_pseudo) {
       if (o->pseudo_diag == 2) {
+        // Bjorck error after k steps ~ |B^T B|^(2^k): more steps while PL is large
+        const int nbj = pl > 0.05 ? 3 : (pl > 1.0e-3 ? 2 : 1);
         occvirt_step(s, c->ov, n, c->npad, c->f.as<double>(), c->C.as<double>(), nclose,
-                     c->eig.as<double>());
+                     c->eig.as<double>(), split_now, nbj);
         block_steps_since_diag += 1;
       } else {
         pseudo_diag_device(s, c->pseudo, n, c->npad, c->f.as<double>(), c->C.as<double>(), nclose,
@@ -523,14 +632,17 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
       eigen_packed(c, c->f.as<double>(), true, tol);
       ++nfull;
     }
-    density(c, nclose, o->precision == 1);
+    density(c, nclose, split_now);
     if (!(newdg && okpuly)) {
       cnvg(s, n, P, pold, c->pold2.as<double>(), niter, c->scal.as<double>());
       // pl, plus the two frontier eigenvalues for the next level shift
       MHIP_CHECK(hipMemcpyAsync(c->hscal, c->scal.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
     }
     if (ihomo < n) {
-      MHIP_CHECK(hipMemcpyAsync(c->hscal + 1, c->eig.as<double>() + ihomo - 1, 2 * sizeof(double),
+      hipLaunchKernelGGL(k_frontier, dim3(1), dim3(256), 0, s, n, nclose, c->eig.as<double>(),
+                         c->frontier.as<double>());
+      MHIP_LAUNCH_CHECK();
+      MHIP_CHECK(hipMemcpyAsync(c->hscal + 1, c->frontier.ptr, 2 * sizeof(double),
                                 hipMemcpyDeviceToHost, s));
     }
     MHIP_CHECK(hipStreamSynchronize(s));
@@ -564,6 +676,8 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
     r->converged = converged ? 1 : 0;
     r->n_full_diag = nfull;
     r->n_pseudo_diag = npseudo;
+    r->n_fragment_diag = nfragdiag;
+    r->n_split_iter = n_split;
     r->seconds = std::chrono::duration<double>(t1 - t0).count();
     r->jacobi_sweeps = c->total_sweeps;
   }
@@ -611,6 +725,9 @@ int mhip_finalize(mhip_ctx* c) {
                       &c->f, &c->fcopy, &c->pold, &c->pold2, &c->fd, &c->A, &c->C, &c->C2, &c->T,
                       &c->eig, &c->scal, &c->partial, &c->fhist, &c->ehist, &c->coef,
                       &c->split_scratch})
+      b->release();
+    for (DevBuf* b : {&c->frag_ptr, &c->frag_orb, &c->frag_occ, &c->frag_ooff, &c->frag_voff,
+                      &c->frontier})
       b->release();
     c->plan.release();
     c->pseudo.release();
@@ -741,6 +858,44 @@ int mhip_last_sweeps(mhip_ctx* c) { return c ? c->last_sweeps : 0; }
 
 // Event timing of the block-Jacobi kernels on the context stream: enable (1) /
 // disable (0) and reset; read back accumulated milliseconds and launch counts.
+const char* mhip_kernel_class_name(int cls) {
+  static const char* names[K_NCLASS] = {"k_gemm_mfma (FP64 GEMM)", "k_gemm_split_bf16 (split-BF16 GEMM)",
+                                        "k_syrk_packed_mfma (density)", "k_fock_rows+k_fock_cols (fock2)"};
+  return (cls >= 0 && cls < K_NCLASS) ? names[cls] : "";
+}
+
+int mhip_kernel_stats(mhip_ctx* c, int enable, int max_classes, double* ms, long long* launches,
+                      double* flops, double* bytes) {
+  return guarded([&] {
+    MHIP_REQUIRE(c != nullptr, "mhip_kernel_stats: null context");
+    MHIP_CHECK(hipSetDevice(c->device));
+    MHIP_CHECK(hipStreamSynchronize(c->stream));
+    if (enable == 0 || enable == 1) {
+      g_ktimer.recs.clear();
+      g_ktimer.used = 0;
+      g_ktimer.on = enable == 1;
+      return;
+    }
+    const int nc = std::min(max_classes, (int)K_NCLASS);
+    for (int i = 0; i < nc; ++i) {
+      if (ms) ms[i] = 0.0;
+      if (launches) launches[i] = 0;
+      if (flops) flops[i] = 0.0;
+      if (bytes) bytes[i] = 0.0;
+    }
+    for (const auto& r : g_ktimer.recs) {
+      if (r.cls >= nc) continue;
+      float e = 0.f;
+      MHIP_CHECK(hipEventSynchronize(r.b));
+      MHIP_CHECK(hipEventElapsedTime(&e, r.a, r.b));
+      if (ms) ms[r.cls] += e;
+      if (launches) launches[r.cls] += 1;
+      if (flops) flops[r.cls] += r.flops;
+      if (bytes) bytes[r.cls] += r.bytes;
+    }
+  });
+}
+
 int mhip_kernel_timing(mhip_ctx* c, int enable, double* solve_ms, long long* n_solve,
                        double* apply_ms, long long* n_apply, double* applyc_ms,
                        long long* n_applyc, long long* active_tiles, long long* active_pairs) {

@@ -176,3 +176,26 @@ def glycine_helix(nres, seed):
             nat.append(1)
             xyz.append(_place(CA, C, oh, 0.96, np.radians(107.0), np.radians(0.0)))
     return np.array(nat, dtype=np.int32), np.array(xyz, dtype=np.float64)
+
+
+COVALENT_RADIUS = {1: 0.31, 6: 0.76, 7: 0.71, 8: 0.66}
+
+
+def fragments(nat, xyz, scale=1.2):
+    """Atom -> fragment id: connected components of the covalent bond graph
+    (bonded when closer than scale x the sum of covalent radii)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    from scipy.spatial import cKDTree
+    nat = np.asarray(nat)
+    xyz = np.asarray(xyz, dtype=np.float64)
+    rad = np.array([COVALENT_RADIUS[int(z)] for z in nat])
+    pairs = cKDTree(xyz).query_pairs(2.0 * scale * rad.max(), output_type="ndarray")
+    if len(pairs):
+        d = np.linalg.norm(xyz[pairs[:, 0]] - xyz[pairs[:, 1]], axis=1)
+        pairs = pairs[d < scale * (rad[pairs[:, 0]] + rad[pairs[:, 1]])]
+    n = len(nat)
+    g = coo_matrix((np.ones(len(pairs)), (pairs[:, 0], pairs[:, 1])), shape=(n, n)) if len(pairs) else \
+        coo_matrix((n, n))
+    _, labels = connected_components(g, directed=False)
+    return labels.astype(np.int32)

@@ -26,13 +26,18 @@ from . import lib as L
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(HERE, "models", "water_guess.npz")
 
-NIN = 64
+NIN = 96
 NHID = 256
 NLAYERS = 3
 NOUT = 16
 RCUT = 4.0
 MUS = np.linspace(1.0, 4.0, 6)
 ETA = 3.0
+# environment field: point charges of the isolated PM7 water molecule
+# (the reference's API test values, tests/mopac_api_test.F90:92-94)
+Q_O, Q_H = -0.644520, 0.322260
+RFIELD = 9.0
+SHELLS = ((1.4, 2.6), (2.6, 4.0))
 
 # pair types
 SELF_O, SELF_H, INTRA_OH, INTRA_HH, INTER_OO, INTER_OH, INTER_HO, INTER_HH = range(8)
@@ -94,6 +99,41 @@ def _radial(nat, xyz, mol, tree):
     return feat
 
 
+def _environment(nat, xyz, mol, R):
+    """Per atom, in the frame of its own molecule: the electrostatic potential,
+    field and field gradient (5) of the other molecules' point charges within
+    RFIELD (smoothly switched off over the last 1 A), and the unit-vector sums
+    toward O and H neighbours of other molecules in two distance shells."""
+    nat = np.asarray(nat)
+    n = len(nat)
+    q = np.where(nat == 8, Q_O, Q_H)
+    feat = np.zeros((n, 9 + 3 * 2 * len(SHELLS)))
+    pairs = cKDTree(xyz).query_pairs(RFIELD, output_type="ndarray")
+    if not len(pairs):
+        return feat
+    i, j = pairs[:, 0], pairs[:, 1]
+    keep = mol[i] != mol[j]
+    i, j = i[keep], j[keep]
+    for a, b in ((i, j), (j, i)):
+        d = xyz[b] - xyz[a]
+        r = np.linalg.norm(d, axis=1)
+        u = np.einsum("mij,mj->mi", R[mol[a]], d / r[:, None])
+        sw = np.where(r < RFIELD - 1.0, 1.0, 0.5 * (1.0 + np.cos(np.pi * (r - (RFIELD - 1.0)))))
+        qs = q[b] * sw
+        np.add.at(feat[:, 0], a, qs / r)
+        np.add.at(feat[:, 1:4], a, (qs / r ** 2)[:, None] * u)
+        g = np.stack([3 * u[:, 0] ** 2 - 1, 3 * u[:, 1] ** 2 - 1, 3 * u[:, 0] * u[:, 1],
+                      3 * u[:, 0] * u[:, 2], 3 * u[:, 1] * u[:, 2]], axis=1)
+        np.add.at(feat[:, 4:9], a, (qs / r ** 3)[:, None] * g)
+        for si, (lo, hi) in enumerate(SHELLS):
+            in_shell = (r >= lo) & (r < hi)
+            for ei, el in enumerate((8, 1)):
+                m = in_shell & (nat[b] == el)
+                col = 9 + 3 * (2 * si + ei)
+                np.add.at(feat[:, col:col + 3], a[m], u[m])
+    return feat
+
+
 def pair_list(nat, xyz, mol):
     """All (A, B) with A >= B: self pairs, same-molecule pairs, pairs within RCUT."""
     tree = cKDTree(xyz)
@@ -147,6 +187,9 @@ def features(nat, xyz, mol, waters, a, b, tree):
     side[w[:, 2]] = -1.0
     X[:, 52] = side[a]
     X[:, 53] = side[b]
+    env = _environment(nat, xyz, mol, R)
+    X[:, 54:75] = env[a]
+    X[:, 75:96] = env[b]
     return X.astype(np.float32), typ
 
 
@@ -283,7 +326,8 @@ class MLP:
 
 
 class GuessModel:
-    """Trained network plus input / output normalization."""
+    """Trained network plus input normalization and per-pair-type output
+    normalization (y_mean, y_std: 8 x 16)."""
 
     def __init__(self, params, x_mean, x_std, y_mean, y_std):
         self.params = np.asarray(params, dtype=np.float32)
@@ -315,7 +359,10 @@ class GuessModel:
         a, b, tree = pair_list(system.nat, system.xyz, mol)
         X, _ = features(system.nat, system.xyz, mol, waters, a, b, tree)
         xn = ((X - self.x_mean) / self.x_std).astype(np.float32)
-        y = self.mlp().forward(xn) * self.y_std + self.y_mean
+        typ = np.argmax(X[:, :8], axis=1)
+        ym = self.y_mean[typ] if self.y_mean.ndim == 2 else self.y_mean
+        ys = self.y_std[typ] if self.y_std.ndim == 2 else self.y_std
+        y = self.mlp().forward(xn) * ys + ym
         return assemble(system, y, a, b, mol, waters)
 
 

@@ -31,6 +31,8 @@ class ScfOpts(ctypes.Structure):
         ("atheat", c_double),
         ("pdiag", c_void_p),
         ("p_init", c_void_p),
+        ("block_start", c_int),
+        ("fragment", c_void_p),
     ]
 
 
@@ -44,6 +46,8 @@ class ScfResult(ctypes.Structure):
         ("n_pseudo_diag", c_int),
         ("seconds", c_double),
         ("jacobi_sweeps", c_int),
+        ("n_fragment_diag", c_int),
+        ("n_split_iter", c_int),
     ]
 
 
@@ -61,12 +65,15 @@ SIGNATURES = {
     "mhip_pseudo_diag": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, ctypes.POINTER(c_ll)]),
     "mhip_scf": (c_int, [c_void_p, ctypes.POINTER(ScfOpts), ctypes.POINTER(ScfResult), c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhip_call_gemm": (None, [c_char, c_char, c_int, c_int, c_int, c_double, c_void_p, c_int, c_void_p, c_int, c_double, c_void_p, c_int]),
+    "mhip_gemm_split_bf16": (None, [c_char, c_char, c_int, c_int, c_int, c_double, c_void_p, c_int, c_void_p, c_int, c_double, c_void_p, c_int]),
     "mhip_call_syrk": (None, [c_char, c_char, c_int, c_int, c_double, c_void_p, c_int, c_double, c_void_p, c_int]),
     "mhip_dsyevd": (None, [c_int, c_void_p, c_int, c_void_p, ctypes.POINTER(c_int)]),
     "mhip_call_rot": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double]),
     "mhip_destroy_resources": (None, []),
     "mhip_set_tuning": (c_int, [c_void_p, c_int, c_int, c_double]),
     "mhip_last_sweeps": (c_int, [c_void_p]),
+    "mhip_kernel_stats": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mhip_kernel_class_name": (ctypes.c_char_p, [c_int]),
     "mhip_kernel_timing": (c_int, [c_void_p, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_ll),
                                    ctypes.POINTER(c_double), ctypes.POINTER(c_ll),
                                    ctypes.POINTER(c_double), ctypes.POINTER(c_ll),

@@ -87,7 +87,7 @@ class Context:
 
     # -- the whole loop ---------------------------------------------------
     def iter(self, pulay=False, scfcrt=1e-4, itrmax=2000, last=1, precision=0, pseudo_diag=False,
-             adaptive_eigen=False, p_init=None, want=("p", "eig")):
+             adaptive_eigen=False, p_init=None, want=("p", "eig"), block_start=0, fragment=None):
         s = self.system
         n = s.norbs
         o = L.ScfOpts()
@@ -107,6 +107,11 @@ class Context:
         if p_init is not None:
             pin = np.ascontiguousarray(p_init, dtype=np.float64)
             o.p_init = pin.ctypes.data
+        o.block_start = int(block_start)
+        frag = None
+        if fragment is not None:
+            frag = np.ascontiguousarray(fragment, dtype=np.int32)
+            o.fragment = frag.ctypes.data
         r = L.ScfResult()
         out = {}
         p = np.zeros(s.mpack) if "p" in want else None
@@ -117,6 +122,7 @@ class Context:
                                   L.ptr(c), L.ptr(e)))
         out.update(escf=r.escf, ee=r.ee, niter=r.niter, converged=bool(r.converged),
                    n_full_diag=r.n_full_diag, n_pseudo_diag=r.n_pseudo_diag, seconds=r.seconds,
-                   jacobi_sweeps=r.jacobi_sweeps,
+                   jacobi_sweeps=r.jacobi_sweeps, n_fragment_diag=r.n_fragment_diag,
+                   n_split_iter=r.n_split_iter,
                    p=p, f=f, c=c, eig=e)
         return out

@@ -9,6 +9,8 @@
 3. The MLP is trained on the GPU with the library's own forward/backward
    kernels and Adam; held-out clusters measure the error of the predicted
    density against the converged SCF and the iterations it saves.
+Outputs are standardized per pair type (self O, self H, intra O-H, intra H-H,
+inter O-O, O-H, H-O, H-H); the learning rate follows a cosine schedule.
 """
 import argparse
 import json
@@ -43,9 +45,9 @@ def samples_for(ctx, nat, xyz):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--clusters", type=int, default=240)
+    ap.add_argument("--clusters", type=int, default=600)
     ap.add_argument("--holdout", type=int, default=8)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=guess.DEFAULT_MODEL)
@@ -65,23 +67,30 @@ def main(argv=None):
         Ys.append(Y)
     X = np.concatenate(Xs).astype(np.float64)
     Y = np.concatenate(Ys).astype(np.float64)
+    T = np.argmax(X[:, :8], axis=1)
     t_data = time.time() - t0
     print(f"data: {nconv} converged clusters, {len(X)} pair samples, {t_data:.1f} s", flush=True)
     x_mean = X.mean(axis=0)
     x_std = np.maximum(X.std(axis=0), 1e-3)
-    y_mean = Y.mean(axis=0)
-    y_std = np.maximum(Y.std(axis=0), 1e-4)
+    # per pair type: the blocks of different types live on different scales
+    y_mean = np.zeros((8, guess.NOUT))
+    y_std = np.ones((8, guess.NOUT))
+    for t in range(8):
+        m = T == t
+        if m.any():
+            y_mean[t] = Y[m].mean(axis=0)
+            y_std[t] = np.maximum(Y[m].std(axis=0), 1e-6)
     Xn = ((X - x_mean) / x_std).astype(np.float32)
-    Yn = ((Y - y_mean) / y_std).astype(np.float32)
+    Yn = ((Y - y_mean[T]) / y_std[T]).astype(np.float32)
     mlp = guess.MLP(seed=1)
     rng = np.random.default_rng(2)
     t1 = time.time()
     losses = []
     for step in range(args.steps):
         idx = rng.integers(0, len(Xn), size=args.batch)
-        lr = args.lr * (0.1 ** (step / max(args.steps, 1)))
+        lr = args.lr * 0.5 * (1.0 + np.cos(np.pi * step / max(args.steps, 1)))
         loss = mlp.train_step(Xn[idx], Yn[idx], lr)
-        if step % 500 == 0 or step == args.steps - 1:
+        if step % 2000 == 0 or step == args.steps - 1:
             losses.append((step, float(loss)))
             print(f"step {step} loss {loss:.5f}", flush=True)
     t_train = time.time() - t1
