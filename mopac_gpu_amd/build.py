@@ -11,6 +11,10 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmopachip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("MHIP_FC", "/opt/rocm/bin/amdflang")
+FORTRAN_MODULE = os.path.join(HERE, "fortran", "mopac_hip.F90")
+FORTRAN_TEST = os.path.join(ROOT, "tests", "fortran", "test_mopac_hip.F90")
+FORTRAN_TEST_EXE = os.path.join(ROOT, "tests", "fortran", "_build", "test_mopac_hip")
 ARCH = os.environ.get("MHIP_ARCH", "gfx950")
 
 SOURCES = [
@@ -67,7 +71,34 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link of libmopachip.so failed")
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The Fortran host module (ISO_C_BINDING interfaces + reference-surface
+    wrappers) and the Fortran test program linked against libmopachip.so.
+    Skipped when no Fortran compiler is installed."""
+    if not os.path.exists(FC):
+        return ""
+    fdir = os.path.join(HERE, "_build", "fortran")
+    tdir = os.path.dirname(FORTRAN_TEST_EXE)
+    os.makedirs(fdir, exist_ok=True)
+    os.makedirs(tdir, exist_ok=True)
+    obj = os.path.join(fdir, "mopac_hip.o")
+    cmds = [
+        [FC, "-O2", "-fPIC", "-module-dir", fdir, "-c", FORTRAN_MODULE, "-o", obj],
+        [FC, "-O2", "-I", fdir, FORTRAN_TEST, obj, "-L", HERE, "-lmopachip",
+         "-Wl,-rpath,$ORIGIN/../../../mopac_gpu_amd", "-o", FORTRAN_TEST_EXE],
+    ]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("Fortran host build failed")
+    return FORTRAN_TEST_EXE
 
 
 if __name__ == "__main__":

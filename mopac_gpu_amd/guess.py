@@ -107,30 +107,40 @@ def _environment(nat, xyz, mol, R):
     nat = np.asarray(nat)
     n = len(nat)
     q = np.where(nat == 8, Q_O, Q_H)
-    feat = np.zeros((n, 9 + 3 * 2 * len(SHELLS)))
+    ncol = 9 + 3 * 2 * len(SHELLS)
+    feat = np.zeros((n, ncol))
     pairs = cKDTree(xyz).query_pairs(RFIELD, output_type="ndarray")
     if not len(pairs):
         return feat
     i, j = pairs[:, 0], pairs[:, 1]
     keep = mol[i] != mol[j]
     i, j = i[keep], j[keep]
-    for a, b in ((i, j), (j, i)):
-        d = xyz[b] - xyz[a]
-        r = np.linalg.norm(d, axis=1)
-        u = np.einsum("mij,mj->mi", R[mol[a]], d / r[:, None])
-        sw = np.where(r < RFIELD - 1.0, 1.0, 0.5 * (1.0 + np.cos(np.pi * (r - (RFIELD - 1.0)))))
-        qs = q[b] * sw
-        np.add.at(feat[:, 0], a, qs / r)
-        np.add.at(feat[:, 1:4], a, (qs / r ** 2)[:, None] * u)
-        g = np.stack([3 * u[:, 0] ** 2 - 1, 3 * u[:, 1] ** 2 - 1, 3 * u[:, 0] * u[:, 1],
-                      3 * u[:, 0] * u[:, 2], 3 * u[:, 1] * u[:, 2]], axis=1)
-        np.add.at(feat[:, 4:9], a, (qs / r ** 3)[:, None] * g)
-        for si, (lo, hi) in enumerate(SHELLS):
-            in_shell = (r >= lo) & (r < hi)
-            for ei, el in enumerate((8, 1)):
-                m = in_shell & (nat[b] == el)
-                col = 9 + 3 * (2 * si + ei)
-                np.add.at(feat[:, col:col + 3], a[m], u[m])
+    a = np.concatenate([i, j])  # both directions at once
+    b = np.concatenate([j, i])
+    d = xyz[b] - xyz[a]
+    r = np.sqrt(np.einsum("mi,mi->m", d, d))
+    u = np.einsum("mij,mj->mi", R[mol[a]], d / r[:, None])
+    sw = np.where(r < RFIELD - 1.0, 1.0, 0.5 * (1.0 + np.cos(np.pi * (r - (RFIELD - 1.0)))))
+    qs = q[b] * sw
+    cols = np.empty((len(a), ncol))
+    cols[:, 0] = qs / r
+    cols[:, 1:4] = (qs / r ** 2)[:, None] * u
+    g = qs / r ** 3
+    cols[:, 4] = g * (3 * u[:, 0] ** 2 - 1)
+    cols[:, 5] = g * (3 * u[:, 1] ** 2 - 1)
+    cols[:, 6] = g * 3 * u[:, 0] * u[:, 1]
+    cols[:, 7] = g * 3 * u[:, 0] * u[:, 2]
+    cols[:, 8] = g * 3 * u[:, 1] * u[:, 2]
+    cols[:, 9:] = 0.0
+    for si, (lo, hi) in enumerate(SHELLS):
+        in_shell = (r >= lo) & (r < hi)
+        for ei, el in enumerate((8, 1)):
+            m = in_shell & (nat[b] == el)
+            c0 = 9 + 3 * (2 * si + ei)
+            cols[m, c0:c0 + 3] = u[m]
+    # scatter-add by atom: one bincount per column (np.add.at is far slower)
+    for k in range(ncol):
+        feat[:, k] = np.bincount(a, weights=cols[:, k], minlength=n)
     return feat
 
 
@@ -294,6 +304,8 @@ class MLP:
 
     def set_params(self, params):
         params = np.ascontiguousarray(params, dtype=np.float32)
+        if params.size != n_params():
+            raise ValueError(f"MLP parameter vector has {params.size} values, the network {n_params()}")
         L.check(self.lib.mhip_mlp_set_params(self.h, L.ptr(params)))
 
     def get_params(self):
@@ -301,8 +313,13 @@ class MLP:
         L.check(self.lib.mhip_mlp_get_params(self.h, L.ptr(out)))
         return out
 
+    def _check_x(self, x):
+        if x.ndim != 2 or x.shape[1] != NIN or x.shape[0] == 0:
+            raise ValueError(f"MLP input must be (batch > 0, {NIN}), got {x.shape}")
+
     def forward(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)
+        self._check_x(x)
         y = np.zeros((x.shape[0], NOUT), dtype=np.float32)
         L.check(self.lib.mhip_mlp_forward(self.h, x.shape[0], L.ptr(x), L.ptr(y)))
         return y
@@ -310,6 +327,9 @@ class MLP:
     def gradients(self, x, t):
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
+        self._check_x(x)
+        if t.shape != (x.shape[0], NOUT):
+            raise ValueError(f"MLP targets must be ({x.shape[0]}, {NOUT}), got {t.shape}")
         g = np.zeros(n_params(), dtype=np.float32)
         loss = ctypes.c_float()
         L.check(self.lib.mhip_mlp_gradients(self.h, x.shape[0], L.ptr(x), L.ptr(t), L.ptr(g),
@@ -319,6 +339,9 @@ class MLP:
     def train_step(self, x, t, lr):
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
+        self._check_x(x)
+        if t.shape != (x.shape[0], NOUT):
+            raise ValueError(f"MLP targets must be ({x.shape[0]}, {NOUT}), got {t.shape}")
         loss = ctypes.c_float()
         L.check(self.lib.mhip_mlp_train_step(self.h, x.shape[0], L.ptr(x), L.ptr(t), ctypes.c_float(lr),
                                              ctypes.byref(loss)))

@@ -9,7 +9,7 @@ from oracle import mlp_ref
 
 pytestmark = pytest.mark.gpu
 
-DIMS = [64, 256, 256, 256, 16]
+DIMS = guess.layer_dims()  # [NIN, 256, 256, 256, 16]
 
 
 def _rand_params(seed, scale=1.0):

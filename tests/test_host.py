@@ -167,3 +167,20 @@ def test_mlp_reference_gradient_matches_finite_differences():
     y16 = mlp_ref.forward(params.astype(np.float32), dims, x)[-1]
     y64 = mlp_ref.forward(params, dims, x, ident, np.float64)[-1]
     assert 0 < np.max(np.abs(y16 - y64)) < 5e-2 * np.max(np.abs(y64))
+
+
+def test_fortran_interface_structs_match_c_layout(tmp_path):
+    """The bind(C) types of the Fortran host module have the C structs' sizes."""
+    import ctypes
+    from mopac_gpu_amd import build as B
+    if not os.path.exists(B.FC):
+        pytest.skip("no Fortran compiler")
+    src = tmp_path / "sz.F90"
+    src.write_text("program sz\n  use iso_c_binding\n  use mopac_hip\n  type(mhip_scf_opts) :: o\n"
+                   "  type(mhip_scf_result) :: r\n  print *, c_sizeof(o), c_sizeof(r)\nend program sz\n")
+    mod = os.path.join(ROOT, "mopac_gpu_amd", "_build", "fortran")
+    exe = tmp_path / "sz"
+    subprocess.run([B.FC, "-I", mod, str(src), os.path.join(mod, "mopac_hip.o"), "-L", B.HERE, "-lmopachip",
+                    "-Wl,-rpath," + B.HERE, "-o", str(exe)], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert [int(x) for x in out] == [ctypes.sizeof(L.ScfOpts), ctypes.sizeof(L.ScfResult)]
