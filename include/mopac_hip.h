@@ -130,6 +130,91 @@ typedef struct {
 int mhip_scf(mhip_ctx* ctx, const mhip_scf_opts* opts, mhip_scf_result* res, double* p,
              double* f, double* c, double* eig);
 
+/* ---- MOZYME localized-MO kernels -------------------------------------------
+ * The MOZYME_C module arrays, passed explicitly (C cannot see Fortran
+ * modules), with the reference's conventions: atom numbers 1-based, offsets
+ * 0-based (MOZYME_C.F90:116-162); nijbo numat x numat column-major, -1 where
+ * an atom pair is not stored; the sparse P and F of length mpack hold the
+ * atom-pair blocks at nijbo.  diagg2 grows LMOs inside the room each owns up
+ * to the next LMO's offset (or icocc_dim / cocc_dim for the last), and
+ * updates ncf/icocc/cocc and nce/icvir/cvir in place. */
+typedef struct {
+  int numat, norbs, nocc, nvir;
+  long long mpack;
+  const int* nfirst;   /* numat, 1-based */
+  const int* nlast;    /* numat, 1-based */
+  const int* nijbo;    /* numat * numat */
+  int* ncf;            /* nocc */
+  const int* nncf;     /* nocc */
+  const int* ncocc;    /* nocc */
+  int* icocc;          /* icocc_dim */
+  int icocc_dim;
+  double* cocc;        /* cocc_dim */
+  int cocc_dim;
+  int* nce;            /* nvir */
+  const int* nnce;     /* nvir */
+  const int* ncvir;    /* nvir */
+  int* icvir;          /* icvir_dim */
+  int icvir_dim;
+  double* cvir;        /* cvir_dim */
+  int cvir_dim;
+} mhip_lmo_set;
+
+/* density_for_MOZYME(p, mode, nclose, partp) over all nocc LMOs
+ * (src/MOZYME/density_for_MOZYME.F90:16-342): mode 0 full density, -1 and 1
+ * the partial forms with partp. */
+int mhip_mozyme_density(mhip_ctx* ctx, const mhip_lmo_set* lmo, int mode, const double* partp,
+                        double* p);
+
+/* The values diagg1 keeps between calls (SAVE, diagg1.F90:61-71) and the
+ * MOZYME_C scalars it reads and writes.  Set reset = 1 for a new SCF (oldens
+ * as the OLDENS keyword); nfmo (nvir ints, caller-owned) carries each
+ * virtual LMO's list length to the calls that reuse the list. */
+typedef struct {
+  int reset, oldens;
+  double fref, safety, oldlim;
+  double tiny, sumt, ovmax;
+  int* nfmo;
+} mhip_diagg1_state;
+
+/* diagg1(fao, nocc, nvir, eigv, ...) (src/MOZYME/diagg1.F90:16-649): the
+ * virtual LMO energies eigv, the occupied ones eigs (on the iterations the
+ * reference evaluates them) and the list of occupied-virtual elements
+ * fmo(1:nij), ifmo(2, 1:nij) (virtual index first, 1-based), at most fmo_dim. */
+int mhip_mozyme_diagg1(mhip_ctx* ctx, const mhip_lmo_set* lmo, const double* fao, const double* p,
+                       double* eigs, double* eigv, int idiagg, mhip_diagg1_state* st, int fmo_dim,
+                       double* fmo, int* ifmo, int* nij);
+
+/* diagg2's saved rejection counts (diagg2.F90:55,338-339) and the MOZYME_C
+ * scalars it uses: tiny (from diagg1, updated here), sumb (out), damp (the
+ * DAMP constant, 1 for main-group molecules), shift, thresh (1e-13 by
+ * default, iter_for_MOZYME.F90:86-97). */
+typedef struct {
+  int reset;
+  double tiny, sumb, damp, shift, thresh;
+  int nrejct[2];
+} mhip_diagg2_state;
+
+/* diagg2(nocc, nvir, eigv, ..., nij, idiagg, ...) (src/MOZYME/diagg2.F90:
+ * 16-359): the 2x2 annihilations of the listed elements, in list order, with
+ * LMO growth, rejection and the half-angle retry. */
+int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* lmo, const double* eigs, const double* eigv,
+                       const double* fmo, const int* ifmo, int nij, int idiagg,
+                       mhip_diagg2_state* st);
+
+/* helecz() (src/MOZYME/helecz.F90:16-52): E = sum over stored atom pairs of
+ * P (H + F), diagonal elements weighted 1/2, on the sparse nijbo storage. */
+int mhip_mozyme_helecz(mhip_ctx* ctx, int numat, const int* nfirst, const int* nlast, const int* nijbo,
+                       long long mpack, const double* p, const double* h, const double* f, double* e);
+
+/* cnvgz(pnew, p, p1, p2, p3, niter, idiag) (src/MOZYME/cnvgz.F90:16-85):
+ * three-point extrapolation every third cycle and diagonal damping from the
+ * fourth (three_point = use_three_point_extrap); idiag(i) is the 1-based
+ * position of P(i,i) in the sparse storage; pmax = max|pnew - p|.  All arrays
+ * are updated in place as the reference updates them. */
+int mhip_mozyme_cnvgz(mhip_ctx* ctx, int norbs, long long mpack, double* pnew, double* p, double* p1,
+                      double* p2, double* p3, int niter, const int* idiag, int three_point, double* pmax);
+
 /* ---- reference-boundary drop-ins -----------------------------------------
  * Same C signatures as the reference's exported symbols, so its Fortran
  * interfaces (src/gpu/cublas_interfaces.F90, src/matrix/eigenvectors_cuda.F90)

@@ -1,0 +1,1259 @@
+// MOZYME localized-MO kernels: density_for_MOZYME, diagg1, diagg2
+// (src/MOZYME/density_for_MOZYME.F90:16-342, diagg1.F90:16-649,
+// diagg2.F90:16-359), on the reference's own LMO arrays.
+//
+// Layout (MOZYME_C.F90:116-162): LMO i's atoms are icocc[nncf[i] + k]
+// (1-based atom numbers), its coefficients cocc[ncocc[i] + ...] atom after
+// atom; the same for the virtual set (nce, nnce, icvir, ncvir, cvir).  The
+// sparse P and F hold atom-pair blocks at nijbo(a, b) (0-based, -1 if absent;
+// a > b: iorbs(a) x iorbs(b), b fastest; a == b: packed lower triangle).
+//
+// density: one workgroup per occupied LMO adds 2 c c^T into every stored
+//   block of its atom pairs (FP64 atomics; the order of the additions is the
+//   only difference from the reference, <= 1e-15 relative).
+// diagg1: one workgroup per virtual LMO builds its F-weighted vector ws on
+//   its own atoms in LDS, then scans all occupied LMOs with the reference's
+//   fast test and cutoffs.  Two passes (count, then write at the prefix-sum
+//   offset) keep the reference's list order (virtual LMO, then occupied LMO)
+//   and its storage cap; a third kernel evaluates the occupied LMO energies.
+// diagg2: the annihilation list is processed in the reference's order by one
+//   workgroup (its rotations and the growth of the LMOs depend on each other
+//   through the LMOs they share); inside a rotation the atoms of both LMOs
+//   are handled in parallel, and growth keeps the reference's atom order by
+//   a prefix scan.
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+
+namespace mhip {
+namespace {
+
+struct LmoDev {
+  int numat, norbs, nocc, nvir;
+  const int* nfirst;  // 1-based
+  const int* iorbs;
+  const int* nijbo;   // column-major numat x numat
+  int* ncf;
+  const int* nncf;
+  const int* ncocc;
+  int* icocc;
+  int icocc_dim;
+  double* cocc;
+  int cocc_dim;
+  int* nce;
+  const int* nnce;
+  const int* ncvir;
+  int* icvir;
+  int icvir_dim;
+  double* cvir;
+  int cvir_dim;
+};
+
+__device__ __forceinline__ int nij_of(const LmoDev& L, int a1, int b1) {  // 1-based atoms
+  return L.nijbo[(size_t)(a1 - 1) + (size_t)(b1 - 1) * L.numat];
+}
+
+// ---------------------------------------------------------------- density
+__global__ __launch_bounds__(256) void k_mz_density(LmoDev L, double* __restrict__ p) {
+  const int i = blockIdx.x;
+  __shared__ int atoms[512];
+  __shared__ int coff[512];
+  const int nat_i = L.ncf[i], base = L.nncf[i], loop = L.ncocc[i];
+  if (threadIdx.x == 0) {
+    int o = 0;
+    for (int k = 0; k < nat_i && k < 512; ++k) {
+      const int a = L.icocc[base + k];
+      atoms[k] = a;
+      coff[k] = o;
+      o += L.iorbs[a - 1];
+    }
+  }
+  __syncthreads();
+  const int npair = nat_i * nat_i;
+  for (int e = threadIdx.x; e < npair; e += blockDim.x) {
+    const int jj = e / nat_i, kk = e % nat_i;
+    const int j = atoms[jj], k = atoms[kk];
+    if (j < k) continue;
+    const int l0 = nij_of(L, j, k);
+    if (l0 < 0) continue;
+    const int nj = L.iorbs[j - 1], nk = L.iorbs[k - 1];
+    const double* cj = L.cocc + loop + coff[jj];
+    const double* ck = L.cocc + loop + coff[kk];
+    if (j == k) {
+      int l = l0;
+      for (int j1 = 0; j1 < nj; ++j1)
+        for (int k1 = 0; k1 <= j1; ++k1) atomicAdd(p + l++, cj[j1] * ck[k1]);
+    } else {
+      int l = l0;
+      for (int j1 = 0; j1 < nj; ++j1)
+        for (int k1 = 0; k1 < nk; ++k1) atomicAdd(p + l++, cj[j1] * ck[k1]);
+    }
+  }
+}
+
+__global__ void k_mz_density_init(long long n, int mode, const double* __restrict__ partp,
+                                   double* __restrict__ p) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = (mode == 0) ? 0.0 : (mode == -1 ? -0.5 * partp[i] : 0.5 * partp[i]);
+}
+
+__global__ void k_scale(long long n, double s, double* __restrict__ p) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] *= s;
+}
+
+// ---------------------------------------------------------------- diagg1
+constexpr int MZ_MAXAT = 256;  // atoms per LMO handled in LDS
+
+struct D1Params {
+  double cutoff, flim, oldlim;
+  int full;        // idiagg <= 5 or even: fast test + new list; else reuse the old list
+  int fmo_dim;
+};
+
+// Per virtual LMO: ws on its atoms (position-indexed), aov, avir, eigv.
+// Shared by the count and the write passes.
+struct VirtLds {
+  int atoms[MZ_MAXAT];
+  int woff[MZ_MAXAT];      // offset of the atom's orbitals in ws
+  double avir[MZ_MAXAT];
+  double aov[MZ_MAXAT];
+  double ws[4 * MZ_MAXAT];
+  double red[256];
+  int nat;
+};
+
+// position of atom a (1-based) in the virtual LMO, or -1
+__device__ __forceinline__ int vpos(const VirtLds& V, int a) {
+  for (int k = 0; k < V.nat; ++k)
+    if (V.atoms[k] == a) return k;
+  return -1;
+}
+
+__device__ void virt_setup(const LmoDev& L, int i, const double* __restrict__ fao,
+                           const double* __restrict__ p, double cutoff, VirtLds& V, double* eigv) {
+  const int tid = threadIdx.x;
+  const int loopi = L.ncvir[i];
+  if (tid == 0) {
+    V.nat = min(L.nce[i], MZ_MAXAT);
+    int o = 0;
+    for (int k = 0; k < V.nat; ++k) {
+      const int a = L.icvir[L.nnce[i] + k];
+      V.atoms[k] = a;
+      V.woff[k] = o;
+      o += L.iorbs[a - 1];
+    }
+  }
+  __syncthreads();
+  // avir: squared weight of each atom in this virtual LMO
+  for (int k = tid; k < V.nat; k += blockDim.x) {
+    const int no = L.iorbs[V.atoms[k] - 1];
+    double s = 0.0;
+    for (int x = 0; x < no; ++x) {
+      const double c = L.cvir[loopi + V.woff[k] + x];
+      s += c * c;
+    }
+    V.avir[k] = s;
+  }
+  __syncthreads();
+  // ws(j1 orbitals) = sum over atoms k1 of the LMO of F(j1, k1) c_vir(k1), as
+  // diagg1.F90:142-195 accumulates it (k1 in LMO order)
+  for (int jj = tid; jj < V.nat; jj += blockDim.x) {
+    const int j1 = V.atoms[jj], nj = L.iorbs[j1 - 1];
+    double w[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int kk = 0; kk < V.nat; ++kk) {
+      const int k1 = V.atoms[kk], nk = L.iorbs[k1 - 1];
+      const int kj = nij_of(L, k1, j1);
+      if (kj < 0) continue;
+      if (!(V.avir[kk] * p[kj] > cutoff)) continue;
+      const double* cv = L.cvir + loopi + V.woff[kk];
+      if (nk == 1 && nj == 1) {
+        w[0] += fao[kj] * cv[0];
+      } else if (k1 > j1) {
+        int ii = kj;
+        for (int i4 = 0; i4 < nk; ++i4)
+          for (int jx = 0; jx < nj; ++jx) w[jx] += fao[ii++] * cv[i4];
+      } else if (k1 < j1) {
+        int ii = kj;
+        for (int jx = 0; jx < nj; ++jx)
+          for (int i4 = 0; i4 < nk; ++i4) w[jx] += fao[ii++] * cv[i4];
+      } else {
+        for (int jx = 1; jx <= nj; ++jx)
+          for (int i4 = 1; i4 <= nj; ++i4) {
+            const int ii = (i4 > jx) ? kj + (i4 * (i4 - 1)) / 2 + jx : kj + (jx * (jx - 1)) / 2 + i4;
+            w[jx - 1] += fao[ii - 1] * cv[i4 - 1];
+          }
+      }
+    }
+    for (int x = 0; x < nj; ++x) V.ws[V.woff[jj] + x] = w[x];
+  }
+  __syncthreads();
+  for (int k = tid; k < V.nat; k += blockDim.x) {
+    const int no = L.iorbs[V.atoms[k] - 1];
+    double s = 0.0;
+    for (int x = 0; x < no; ++x) s += V.ws[V.woff[k] + x] * V.ws[V.woff[k] + x];
+    V.aov[k] = s;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0;
+    for (int k = 0; k < V.nat; ++k) {
+      if (V.aov[k] * V.avir[k] > cutoff) {
+        const int no = L.iorbs[V.atoms[k] - 1];
+        for (int x = 0; x < no; ++x) s += V.ws[V.woff[k] + x] * L.cvir[loopi + V.woff[k] + x];
+      }
+    }
+    if (eigv) eigv[i] = s;
+  }
+  __syncthreads();
+}
+
+// Occupied LMO j against the virtual LMO in V: the fast test (full pass only)
+// and the sum.  Returns 0 if the pair is not evaluated, 1 if evaluated but
+// not stored, 2 if stored; value in *sum.
+__device__ int occ_virt_element(const LmoDev& L, const VirtLds& V, const double* __restrict__ fao,
+                                const double* __restrict__ aocc, int j, const D1Params& P,
+                                bool fast_test, double* sum_out) {
+  if (fast_test) {
+    const int i1 = V.atoms[0], i2 = (V.nat > 1) ? V.atoms[1] : i1;
+    const int j1 = L.icocc[L.nncf[j]], j2 = (L.ncf[j] > 1) ? L.icocc[L.nncf[j] + 1] : j1;
+    const int a = nij_of(L, i1, j1), b = nij_of(L, i1, j2), c = nij_of(L, i2, j1), d = nij_of(L, i2, j2);
+    if (a < 0 && b < 0 && c < 0 && d < 0) return 0;
+    double s = 0.0;
+    if (a >= 0) s = fabs(fao[a]);
+    if (b >= 0) s += fabs(fao[b]);
+    if (c >= 0) s += fabs(fao[c]);
+    if (d >= 0) s += fabs(fao[d]);
+    if (!(s >= P.flim)) return 0;
+  }
+  const int loopj = L.ncocc[j];
+  bool lij = false;
+  double sum = 0.0;
+  int kl = 0;
+  for (int kk = L.nncf[j]; kk < L.nncf[j] + L.ncf[j]; ++kk) {
+    const int k1 = L.icocc[kk], nk = L.iorbs[k1 - 1];
+    const int vp = vpos(V, k1);
+    if (vp < 0 || aocc[kk] * V.aov[vp] < P.cutoff) {
+      kl += nk;
+    } else {
+      lij = true;
+      for (int x = 0; x < nk; ++x) sum += V.ws[V.woff[vp] + x] * L.cocc[loopj + kl + x];
+      kl += nk;
+    }
+  }
+  *sum_out = sum;
+  if (!fast_test) return 2;  // the reuse path stores every listed element
+  return (lij && fabs(sum) > P.oldlim) ? 2 : 1;
+}
+
+// aocc[kk]: squared weight of the kk-th atom slot of the occupied LMOs
+__global__ void k_mz_aocc(LmoDev L, double* __restrict__ aocc) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= L.nocc) return;
+  const int loopj = L.ncocc[j];
+  int kl = 0;
+  for (int kk = L.nncf[j]; kk < L.nncf[j] + L.ncf[j]; ++kk) {
+    const int no = L.iorbs[L.icocc[kk] - 1];
+    double s = 0.0;
+    for (int x = 0; x < no; ++x) s += L.cocc[loopj + kl + x] * L.cocc[loopj + kl + x];
+    aocc[kk] = s;
+    kl += no;
+  }
+}
+
+// Pass 1 (full lists): per virtual LMO, the number of stored elements.
+__global__ __launch_bounds__(256) void k_mz_diagg1_count(LmoDev L, const double* __restrict__ fao,
+                                                         const double* __restrict__ p,
+                                                         const double* __restrict__ aocc,
+                                                         D1Params P, int* __restrict__ counts,
+                                                         double* __restrict__ eigv) {
+  __shared__ VirtLds V;
+  const int i = blockIdx.x;
+  virt_setup(L, i, fao, p, P.cutoff, V, eigv);
+  int cnt = 0;
+  for (int j = threadIdx.x; j < L.nocc; j += blockDim.x) {
+    double s;
+    if (occ_virt_element(L, V, fao, aocc, j, P, true, &s) == 2) ++cnt;
+  }
+  // block sum
+  __shared__ int red[256];
+  red[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[i] = red[0];
+}
+
+// Pass 2: write the list at the prefix offsets in the reference's order,
+// honouring the storage cap exactly (a candidate is evaluated only while the
+// list is not full); per-LMO maxima and sums of |element| for tiny and sumt.
+__global__ __launch_bounds__(256) void k_mz_diagg1_write(LmoDev L, const double* __restrict__ fao,
+                                                         const double* __restrict__ p,
+                                                         const double* __restrict__ aocc,
+                                                         D1Params P, const int* __restrict__ offs,
+                                                         int* __restrict__ nfmo,
+                                                         const int* __restrict__ old_ifmo,
+                                                         double* __restrict__ fmo, int* __restrict__ ifmo,
+                                                         double* __restrict__ vmax,
+                                                         double* __restrict__ vsum,
+                                                         double* __restrict__ eigv) {
+  __shared__ VirtLds V;
+  __shared__ int scan[256];
+  __shared__ int base_s;
+  const int i = blockIdx.x, tid = threadIdx.x;
+  virt_setup(L, i, fao, p, P.cutoff, V, eigv);
+  double mx = 0.0, sm = 0.0;
+  if (P.full) {
+    if (tid == 0) base_s = offs[i];
+    __syncthreads();
+    for (int j0 = 0; j0 < L.nocc; j0 += blockDim.x) {
+      const int base = base_s;
+      if (base >= P.fmo_dim) break;  // list full: nothing more is evaluated
+      const int j = j0 + tid;
+      double s = 0.0;
+      int kind = 0;
+      if (j < L.nocc) kind = occ_virt_element(L, V, fao, aocc, j, P, true, &s);
+      // inclusive scan of stored flags in j order
+      scan[tid] = (kind == 2);
+      __syncthreads();
+      for (int off = 1; off < 256; off <<= 1) {
+        const int t = (tid >= off) ? scan[tid - off] : 0;
+        __syncthreads();
+        scan[tid] += t;
+        __syncthreads();
+      }
+      const int before = base + scan[tid] - (kind == 2);  // entries stored before this j
+      if (kind != 0 && before < P.fmo_dim) {
+        mx = fmax(mx, fabs(s));
+        sm += fabs(s);
+        if (kind == 2) {
+          fmo[before] = s;
+          ifmo[2 * before] = i + 1;
+          ifmo[2 * before + 1] = j + 1;
+        }
+      }
+      __syncthreads();
+      if (tid == 255) base_s = base + scan[255];
+      __syncthreads();
+    }
+    if (tid == 0) nfmo[i] = max(0, min(base_s, P.fmo_dim) - offs[i]);
+  } else {
+    // reuse: the old list entries of this virtual LMO, recomputed in place
+    const int start = offs[i], cnt = nfmo[i];
+    for (int jj = tid; jj < cnt; jj += blockDim.x) {
+      const int pos = start + jj;
+      if (pos >= P.fmo_dim) break;
+      const int j = old_ifmo[2 * pos + 1] - 1;
+      double s;
+      occ_virt_element(L, V, fao, aocc, j, P, false, &s);
+      fmo[pos] = s;
+      ifmo[2 * pos] = i + 1;
+      ifmo[2 * pos + 1] = j + 1;
+      mx = fmax(mx, fabs(s));
+      sm += fabs(s);
+    }
+  }
+  V.red[tid] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) V.red[tid] = fmax(V.red[tid], V.red[tid + w]);
+    __syncthreads();
+  }
+  if (tid == 0) vmax[i] = V.red[0];
+  __syncthreads();
+  V.red[tid] = sm;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) V.red[tid] += V.red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) vsum[i] = V.red[0];
+}
+
+// Occupied LMO energies (diagg1.F90:484-640); one thread per occupied LMO.
+__global__ void k_mz_occ_energy(LmoDev L, const double* __restrict__ fao, const double* __restrict__ p,
+                                const double* __restrict__ aocc, double cutoff,
+                                double* __restrict__ eigs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.nocc) return;
+  const int loopi = L.ncocc[i], b0 = L.nncf[i], nat_i = L.ncf[i];
+  double sum = 0.0;
+  int jl = loopi;
+  for (int j = b0; j < b0 + nat_i; ++j) {
+    const int j1 = L.icocc[j], nj = L.iorbs[j1 - 1];
+    int kl = loopi;
+    for (int k = b0; k < b0 + nat_i; ++k) {
+      const int k1 = L.icocc[k], nk = L.iorbs[k1 - 1];
+      const int k1j1 = nij_of(L, k1, j1);
+      // avir(k1) of the reference here is the squared weight of atom k1 in LMO i
+      if (k1j1 >= 0) {
+        double wk = 0.0;
+        for (int x = 0; x < nk; ++x) wk += L.cocc[kl + x] * L.cocc[kl + x];
+        if (wk * p[k1j1] * aocc[k] >= cutoff) {
+          for (int jx = 1; jx <= nj; ++jx) {
+            double s1 = 0.0;
+            if (k1 > j1) {
+              for (int i4 = 1; i4 <= nk; ++i4) s1 += fao[k1j1 + (i4 - 1) * nj + jx - 1] * L.cocc[kl + i4 - 1];
+            } else if (k1 < j1) {
+              for (int i4 = 1; i4 <= nk; ++i4) s1 += fao[k1j1 + (jx - 1) * nk + i4 - 1] * L.cocc[kl + i4 - 1];
+            } else {
+              for (int j4 = 1; j4 <= jx; ++j4) s1 += fao[k1j1 + (jx * (jx - 1)) / 2 + j4 - 1] * L.cocc[kl + j4 - 1];
+              for (int i4 = jx + 1; i4 <= nk; ++i4)
+                s1 += fao[k1j1 + (i4 * (i4 - 1)) / 2 + jx - 1] * L.cocc[kl + i4 - 1];
+            }
+            sum += L.cocc[jl + jx - 1] * s1;
+          }
+        }
+      }
+      kl += nk;
+    }
+    jl += nj;
+  }
+  eigs[i] = sum;
+}
+
+// ---------------------------------------------------------------- diagg2
+struct D2Params {
+  double tiny, biglim, cnst, shift, thresh;
+  int retry, nij;
+};
+
+constexpr int D2_NT = 256;
+
+__global__ __launch_bounds__(D2_NT) void k_mz_diagg2(LmoDev L, const double* __restrict__ eigs,
+                                                     const double* __restrict__ eigv,
+                                                     const double* __restrict__ fmo,
+                                                     const int* __restrict__ ifmo, D2Params P,
+                                                     int* __restrict__ iused_g,  // numat, -1
+                                                     char* __restrict__ latoms_g,  // numat, 0
+                                                     double* __restrict__ storej,
+                                                     double* __restrict__ storei,
+                                                     double* __restrict__ out) {  // sumb, nrej
+  const int tid = threadIdx.x;
+  __shared__ int flag[D2_NT];
+  __shared__ int s_mlf, s_mle, s_fail, s_ncf, s_nce;
+  __shared__ double s_alpha;
+  double sumb = 0.0;
+  int nrej = 0;
+  for (int ij = 0; ij < P.nij; ++ij) {
+    const int i = ifmo[2 * ij] - 1, j = ifmo[2 * ij + 1] - 1;
+    if (!(fabs(fmo[ij]) >= P.tiny)) continue;
+    const double c = fmo[ij] * P.cnst;
+    const double d = eigs[j] - eigv[i] - P.shift;
+    if (!(fabs(c / d) >= P.biglim)) continue;
+    const int ncfj = L.ncf[j], ncei = L.nce[i];
+    const int jlr = L.ncocc[j];
+    int jur, jncf, iur, incv;
+    if (j != L.nocc - 1) { jur = L.ncocc[j + 1]; jncf = L.nncf[j + 1]; }
+    else { jur = L.cocc_dim; jncf = L.icocc_dim; }
+    jur = min(jlr + L.norbs, jur);  // exclusive end of LMO j's coefficients
+    const int ilr = L.ncvir[i];
+    if (i != L.nvir - 1) { iur = L.ncvir[i + 1]; incv = L.nnce[i + 1]; }
+    else { iur = L.cvir_dim; incv = L.icvir_dim; }
+    iur = min(ilr + L.norbs, iur);
+    for (int k = tid; k < jur - jlr; k += D2_NT) storej[k] = L.cocc[jlr + k];
+    for (int k = tid; k < iur - ilr; k += D2_NT) storei[k] = L.cvir[ilr + k];
+    const double e = copysign(sqrt(4.0 * c * c + d * d), d);
+    double alpha = sqrt(0.5 * (1.0 + d / e));
+    __syncthreads();
+    for (;;) {
+      const double beta = -copysign(sqrt(1.0 - alpha * alpha), c);
+      if (tid == 0) sumb += fabs(beta);
+      // iused: offset of each atom of occupied LMO j (its original atoms)
+      if (tid == 0) {
+        int mlf = 0;
+        for (int lf = L.nncf[j]; lf < L.nncf[j] + L.ncf[j]; ++lf) {
+          const int a = L.icocc[lf] - 1;
+          iused_g[a] = mlf;
+          mlf += L.iorbs[a];
+        }
+        s_mlf = mlf;
+        s_fail = 0;
+        s_ncf = L.ncf[j];
+        s_nce = L.nce[i];
+      }
+      __syncthreads();
+      const int loopi = L.ncvir[i], loopj = L.ncocc[j];
+      // ---- virtual LMO atoms, in order: rotate common atoms, grow LMO j
+      {
+        const int nv = L.nce[i];
+        int mle_base = 0;  // running orbital offset over virtual atoms
+        int mlf_run = s_mlf;
+        int ncf_run = s_ncf;
+        for (int c0 = 0; c0 < nv; c0 += D2_NT) {
+          const int le = c0 + tid;
+          int mie = -1, no = 0, grow = 0, mle = 0;
+          // offsets of this chunk's atoms: exclusive scan of iorbs
+          flag[tid] = (le < nv) ? L.iorbs[L.icvir[L.nnce[i] + le] - 1] : 0;
+          __syncthreads();
+          for (int off = 1; off < D2_NT; off <<= 1) {
+            const int t = (tid >= off) ? flag[tid - off] : 0;
+            __syncthreads();
+            flag[tid] += t;
+            __syncthreads();
+          }
+          if (le < nv) {
+            mie = L.icvir[L.nnce[i] + le] - 1;
+            no = L.iorbs[mie];
+            mle = mle_base + flag[tid] - no;
+          }
+          const int chunk_orbs = flag[D2_NT - 1];
+          __syncthreads();
+          if (le < nv) {
+            latoms_g[mie] = 1;
+            const int u = iused_g[mie];
+            if (u >= 0) {
+              for (int x = 0; x < no; ++x) {
+                const double a = L.cocc[loopj + u + x], b = L.cvir[loopi + mle + x];
+                L.cocc[loopj + u + x] = alpha * a + beta * b;
+                L.cvir[loopi + mle + x] = alpha * b - beta * a;
+              }
+            } else {
+              double s = 0.0;
+              for (int x = 0; x < no; ++x) {
+                const double t = beta * L.cvir[loopi + mle + x];
+                s += t * t;
+              }
+              grow = (s > P.thresh);
+            }
+          }
+          // growth in order: scan of grow flags
+          flag[tid] = grow;
+          __syncthreads();
+          for (int off = 1; off < D2_NT; off <<= 1) {
+            const int t = (tid >= off) ? flag[tid - off] : 0;
+            __syncthreads();
+            flag[tid] += t;
+            __syncthreads();
+          }
+          const int ngrow = flag[D2_NT - 1];
+          if (ngrow > 0) {
+            // orbital offsets of the grown atoms: second scan over their iorbs
+            const int rank = flag[tid] - grow;  // grown atoms before this one
+            __syncthreads();
+            flag[tid] = grow ? no : 0;
+            __syncthreads();
+            for (int off = 1; off < D2_NT; off <<= 1) {
+              const int t = (tid >= off) ? flag[tid - off] : 0;
+              __syncthreads();
+              flag[tid] += t;
+              __syncthreads();
+            }
+            if (grow) {
+              const int slot = L.nncf[j] + ncf_run + rank;
+              const int obase = mlf_run + flag[tid] - no;
+              // the reference stops at the first atom that would not fit
+              if (slot >= jncf || jlr + obase + no > jur) {
+                atomicOr(&s_fail, 1);
+              }
+            }
+            __syncthreads();
+            if (!s_fail && grow) {
+              const int slot = L.nncf[j] + ncf_run + rank;
+              const int obase = mlf_run + flag[tid] - no;
+              L.icocc[slot] = mie + 1;
+              iused_g[mie] = obase;
+              for (int x = 0; x < no; ++x) {
+                L.cocc[loopj + obase + x] = beta * L.cvir[loopi + mle + x];
+                L.cvir[loopi + mle + x] = alpha * L.cvir[loopi + mle + x];
+              }
+            }
+            const int grown_orbs = flag[D2_NT - 1];
+            __syncthreads();
+            if (s_fail) break;
+            ncf_run += ngrow;
+            mlf_run += grown_orbs;
+          }
+          mle_base += chunk_orbs;
+          __syncthreads();
+        }
+        if (tid == 0) {
+          s_ncf = ncf_run;
+          s_mlf = mlf_run;
+          s_mle = mle_base;
+        }
+        __syncthreads();
+      }
+      if (!s_fail) {
+        if (tid == 0) L.ncf[j] = s_ncf;
+        __syncthreads();
+        // ---- occupied LMO atoms not in the virtual LMO: grow LMO i, in order
+        const int nf = s_ncf;
+        int mle_run = s_mle;
+        int nce_run = s_nce;
+        for (int c0 = 0; c0 < nf; c0 += D2_NT) {
+          const int lf = c0 + tid;
+          int ii = -1, no = 0, grow = 0;
+          if (lf < nf) {
+            ii = L.icocc[L.nncf[j] + lf] - 1;
+            no = L.iorbs[ii];
+            if (!latoms_g[ii]) {
+              const int u = iused_g[ii];
+              double s = 0.0;
+              for (int x = 0; x < no; ++x) {
+                const double t = beta * L.cocc[loopj + u + x];
+                s += t * t;
+              }
+              grow = (s > P.thresh);
+            }
+          }
+          flag[tid] = grow;
+          __syncthreads();
+          for (int off = 1; off < D2_NT; off <<= 1) {
+            const int t = (tid >= off) ? flag[tid - off] : 0;
+            __syncthreads();
+            flag[tid] += t;
+            __syncthreads();
+          }
+          const int ngrow = flag[D2_NT - 1];
+          if (ngrow > 0) {
+            const int rank = flag[tid] - grow;
+            __syncthreads();
+            flag[tid] = grow ? no : 0;
+            __syncthreads();
+            for (int off = 1; off < D2_NT; off <<= 1) {
+              const int t = (tid >= off) ? flag[tid - off] : 0;
+              __syncthreads();
+              flag[tid] += t;
+              __syncthreads();
+            }
+            if (grow) {
+              const int slot = L.nnce[i] + nce_run + rank;
+              const int obase = mle_run + flag[tid] - no;
+              if (slot >= incv || ilr + obase + no > iur) atomicOr(&s_fail, 1);
+            }
+            __syncthreads();
+            if (!s_fail && grow) {
+              const int slot = L.nnce[i] + nce_run + rank;
+              const int obase = mle_run + flag[tid] - no;
+              const int u = iused_g[ii];
+              L.icvir[slot] = ii + 1;
+              latoms_g[ii] = 1;
+              for (int x = 0; x < no; ++x) {
+                L.cvir[loopi + obase + x] = -beta * L.cocc[loopj + u + x];
+                L.cocc[loopj + u + x] = alpha * L.cocc[loopj + u + x];
+              }
+            }
+            const int grown_orbs = flag[D2_NT - 1];
+            __syncthreads();
+            if (s_fail) break;
+            nce_run += ngrow;
+            mle_run += grown_orbs;
+          }
+          __syncthreads();
+        }
+        if (!s_fail) {
+          if (tid == 0) L.nce[i] = nce_run;
+          __syncthreads();
+          // reset the flags of the atoms now in the two LMOs
+          for (int le = tid; le < L.nce[i]; le += D2_NT) latoms_g[L.icvir[L.nnce[i] + le] - 1] = 0;
+          for (int lf = tid; lf < L.ncf[j]; lf += D2_NT) iused_g[L.icocc[L.nncf[j] + lf] - 1] = -1;
+          __syncthreads();
+          break;
+        }
+      }
+      // ---- rejection (label 1000): restore both LMOs and all flags
+      if (tid == 0) ++nrej;
+      for (int k = tid; k < jur - jlr; k += D2_NT) L.cocc[jlr + k] = storej[k];
+      for (int k = tid; k < iur - ilr; k += D2_NT) L.cvir[ilr + k] = storei[k];
+      for (int k = tid; k < L.numat; k += D2_NT) {
+        iused_g[k] = -1;
+        latoms_g[k] = 0;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        L.ncf[j] = ncfj;
+        L.nce[i] = ncei;
+      }
+      __syncthreads();
+      if (P.retry) {
+        alpha = 0.5 * (alpha + 1.0);
+        continue;
+      }
+      break;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out[0] = sumb;
+    out[1] = (double)nrej;
+  }
+}
+
+// ---------------------------------------------------------------- helecz
+// One workgroup per atom i: sum over its stored pairs (j < i) and its
+// diagonal block; partial sums combined in a fixed order by k_mz_sum2.
+__global__ __launch_bounds__(256) void k_mz_helecz(int numat, const int* __restrict__ iorbs,
+                                                   const int* __restrict__ nijbo,
+                                                   const double* __restrict__ p,
+                                                   const double* __restrict__ h,
+                                                   const double* __restrict__ f,
+                                                   double* __restrict__ part) {
+  const int i = blockIdx.x, tid = threadIdx.x;
+  const int ni = iorbs[i];
+  double ee = 0.0, ed = 0.0;
+  for (int j = tid; j < i; j += 256) {
+    const int k = nijbo[(size_t)i + (size_t)j * numat];
+    if (k < 0) continue;
+    const int len = ni * iorbs[j];
+    for (int m = k; m < k + len; ++m) ee += p[m] * (h[m] + f[m]);
+  }
+  if (tid == 0) {
+    int k = nijbo[(size_t)i + (size_t)i * numat];
+    for (int l1 = 0; l1 < ni; ++l1) {
+      for (int l2 = 0; l2 < l1; ++l2, ++k) ee += p[k] * (h[k] + f[k]);
+      ed += p[k] * (h[k] + f[k]);
+      ++k;
+    }
+  }
+  __shared__ double r1[256], r2[256];
+  r1[tid] = ee;
+  r2[tid] = ed;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) {
+      r1[tid] += r1[tid + w];
+      r2[tid] += r2[tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    part[2 * i] = r1[0];
+    part[2 * i + 1] = r2[0];
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_mz_sum2(int n, const double* __restrict__ part,
+                                                  double* __restrict__ out) {
+  __shared__ double r1[1024], r2[1024];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  r1[threadIdx.x] = a;
+  r2[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+      r2[threadIdx.x] += r2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = r1[0] + 0.5 * r2[0];
+}
+
+// ---------------------------------------------------------------- cnvgz
+// pmax partials over the packed vectors
+__global__ __launch_bounds__(256) void k_mz_absdiff_max(long long n, const double* __restrict__ a,
+                                                        const double* __restrict__ b,
+                                                        double* __restrict__ part) {
+  __shared__ double r[256];
+  double m = 0.0;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    m = fmax(m, fabs(a[i] - b[i]));
+  r[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) r[threadIdx.x] = fmax(r[threadIdx.x], r[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = r[0];
+}
+
+// One workgroup: diagonal bookkeeping, pmax, and the extrapolation factor
+// (cnvgz.F90:25-54); scal[0] = pmax, scal[1] = fac (0 when not applied).
+__global__ __launch_bounds__(1024) void k_mz_cnvgz_diag(int norbs, int nparts, const double* __restrict__ part,
+                                                        const int* __restrict__ idiag,
+                                                        const double* __restrict__ pnew,
+                                                        const double* __restrict__ p,
+                                                        const double* __restrict__ p1, double* __restrict__ p2,
+                                                        double* __restrict__ p3, int niter, int three,
+                                                        double* __restrict__ scal) {
+  __shared__ double ra[1024], rb[1024];
+  const int tid = threadIdx.x;
+  double fa = 0.0, fb = 0.0, m = 0.0;
+  for (int i = tid; i < norbs; i += 1024) {
+    const int j = idiag[i] - 1;
+    const double a3 = pnew[j], a2 = p[j];
+    p3[i] = a3;
+    p2[i] = a2;
+    const double sa = fabs(a3 - a2);
+    fa += sa * sa;
+    const double t = a3 - 2.0 * a2 + p1[i];
+    fb += t * t;
+  }
+  for (int k = tid; k < nparts; k += 1024) m = fmax(m, part[k]);
+  ra[tid] = fa;
+  rb[tid] = fb;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (tid < w) {
+      ra[tid] += ra[tid + w];
+      rb[tid] += rb[tid + w];
+    }
+    __syncthreads();
+  }
+  const double faca = ra[0], facb = rb[0];
+  __syncthreads();
+  ra[tid] = m;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (tid < w) ra[tid] = fmax(ra[tid], ra[tid + w]);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    scal[0] = ra[0];
+    double fac = 0.0;
+    if (three && niter % 3 == 0 && facb > 0.0 && faca < 100.0 * facb) fac = sqrt(faca / facb);
+    scal[1] = fac;
+  }
+}
+
+// pnew += fac (pnew - p); then p = pnew (after the damping pass)
+__global__ void k_mz_extrap(long long n, const double* __restrict__ scal, double* __restrict__ pnew,
+                            const double* __restrict__ p) {
+  const double fac = scal[1];
+  if (fac == 0.0) return;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    pnew[i] = pnew[i] + fac * (pnew[i] - p[i]);
+}
+
+// diagonal damping from iteration 4 on (cnvgz.F90:60-74), then p1 = p2
+__global__ void k_mz_damp(int norbs, const int* __restrict__ idiag, const double* __restrict__ scal,
+                          const double* __restrict__ p2, const double* __restrict__ p3, int niter, int three,
+                          double* __restrict__ pnew, double* __restrict__ p1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= norbs) return;
+  const double damp = 0.05;
+  if (three && niter > 3 && scal[0] > damp) {
+    const int j = idiag[i] - 1;
+    const double d = p3[i] - p2[i];
+    if (fabs(d) > damp) {
+      double v = p2[i] + copysign(damp, d);
+      pnew[j] = fmin(2.0, fmax(v, 0.0));
+    }
+  }
+  p1[i] = p2[i];
+}
+
+inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+}  // namespace mhip
+
+using namespace mhip;
+
+// ---- host side: device copies of an LMO set ---------------------------------
+namespace {
+
+struct LmoBufs {
+  DevBuf nfirst, iorbs, nijbo, ncf, nncf, ncocc, icocc, cocc, nce, nnce, ncvir, icvir, cvir;
+  LmoDev dev{};
+};
+
+void up(hipStream_t s, DevBuf& b, const void* src, size_t bytes) {
+  b.ensure(bytes ? bytes : 8);
+  if (bytes) MHIP_CHECK(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, s));
+}
+void down(hipStream_t s, void* dst, const DevBuf& b, size_t bytes) {
+  if (bytes) MHIP_CHECK(hipMemcpyAsync(dst, b.ptr, bytes, hipMemcpyDeviceToHost, s));
+}
+
+// Checks every index the kernels will follow, then uploads.
+void upload_lmos(hipStream_t s, const mhip_lmo_set* L, LmoBufs& B) {
+  MHIP_REQUIRE(L && L->numat > 0 && L->norbs > 0 && L->nocc >= 0 && L->nvir >= 0, "LMO set: bad sizes");
+  std::vector<int> iorbs(L->numat);
+  for (int a = 0; a < L->numat; ++a) {
+    iorbs[a] = L->nlast[a] - L->nfirst[a] + 1;
+    MHIP_REQUIRE(iorbs[a] == 1 || iorbs[a] == 4, "LMO set: only s and sp atoms");
+  }
+  auto check_set = [&](int n, const int* nc, const int* nn, const int* no, const int* idx, int idim,
+                       int cdim, const char* what) {
+    for (int i = 0; i < n; ++i) {
+      const int end_atoms = (i + 1 < n) ? nn[i + 1] : idim;
+      const int end_coef = (i + 1 < n) ? no[i + 1] : cdim;
+      MHIP_REQUIRE(nn[i] >= 0 && nc[i] >= 0 && nn[i] + nc[i] <= end_atoms && end_atoms <= idim,
+                   std::string("LMO set: atom list out of range in ") + what);
+      MHIP_REQUIRE(nc[i] <= MZ_MAXAT, std::string("LMO set: more than 256 atoms in one LMO in ") + what);
+      int orbs = 0;
+      for (int k = 0; k < nc[i]; ++k) {
+        const int a = idx[nn[i] + k];
+        MHIP_REQUIRE(a >= 1 && a <= L->numat, std::string("LMO set: bad atom number in ") + what);
+        orbs += iorbs[a - 1];
+      }
+      MHIP_REQUIRE(no[i] >= 0 && no[i] + orbs <= end_coef && end_coef <= cdim,
+                   std::string("LMO set: coefficients out of range in ") + what);
+    }
+  };
+  check_set(L->nocc, L->ncf, L->nncf, L->ncocc, L->icocc, L->icocc_dim, L->cocc_dim, "the occupied set");
+  check_set(L->nvir, L->nce, L->nnce, L->ncvir, L->icvir, L->icvir_dim, L->cvir_dim, "the virtual set");
+  const size_t nn = (size_t)L->numat * L->numat;
+  for (int b = 0; b < L->numat; ++b)
+    for (int a = 0; a < L->numat; ++a) {
+      const long long o = L->nijbo[(size_t)a + (size_t)b * L->numat];
+      if (o < 0) continue;
+      const long long blk = (a == b) ? iorbs[a] * (iorbs[a] + 1) / 2 : (long long)iorbs[a] * iorbs[b];
+      MHIP_REQUIRE(o + blk <= L->mpack, "LMO set: nijbo block beyond mpack");
+      MHIP_REQUIRE(L->nijbo[(size_t)b + (size_t)a * L->numat] == o, "LMO set: nijbo not symmetric");
+    }
+  up(s, B.nfirst, L->nfirst, sizeof(int) * L->numat);
+  up(s, B.iorbs, iorbs.data(), sizeof(int) * L->numat);
+  up(s, B.nijbo, L->nijbo, sizeof(int) * nn);
+  up(s, B.ncf, L->ncf, sizeof(int) * L->nocc);
+  up(s, B.nncf, L->nncf, sizeof(int) * L->nocc);
+  up(s, B.ncocc, L->ncocc, sizeof(int) * L->nocc);
+  up(s, B.icocc, L->icocc, sizeof(int) * L->icocc_dim);
+  up(s, B.cocc, L->cocc, sizeof(double) * L->cocc_dim);
+  up(s, B.nce, L->nce, sizeof(int) * L->nvir);
+  up(s, B.nnce, L->nnce, sizeof(int) * L->nvir);
+  up(s, B.ncvir, L->ncvir, sizeof(int) * L->nvir);
+  up(s, B.icvir, L->icvir, sizeof(int) * L->icvir_dim);
+  up(s, B.cvir, L->cvir, sizeof(double) * L->cvir_dim);
+  LmoDev& d = B.dev;
+  d.numat = L->numat;
+  d.norbs = L->norbs;
+  d.nocc = L->nocc;
+  d.nvir = L->nvir;
+  d.nfirst = B.nfirst.as<int>();
+  d.iorbs = B.iorbs.as<int>();
+  d.nijbo = B.nijbo.as<int>();
+  d.ncf = B.ncf.as<int>();
+  d.nncf = B.nncf.as<int>();
+  d.ncocc = B.ncocc.as<int>();
+  d.icocc = B.icocc.as<int>();
+  d.icocc_dim = L->icocc_dim;
+  d.cocc = B.cocc.as<double>();
+  d.cocc_dim = L->cocc_dim;
+  d.nce = B.nce.as<int>();
+  d.nnce = B.nnce.as<int>();
+  d.ncvir = B.ncvir.as<int>();
+  d.icvir = B.icvir.as<int>();
+  d.icvir_dim = L->icvir_dim;
+  d.cvir = B.cvir.as<double>();
+  d.cvir_dim = L->cvir_dim;
+}
+
+template <class F>
+int mz_guard(F&& fn) {
+  try {
+    fn();
+    return 0;
+  } catch (const HipError& e) {
+    set_error(e.msg);
+    return 1;
+  }
+}
+
+hipStream_t g_mz_stream = nullptr;
+hipStream_t mz_stream(mhip_ctx* ctx) {
+  (void)ctx;
+  if (!g_mz_stream) MHIP_CHECK(hipStreamCreateWithFlags(&g_mz_stream, hipStreamNonBlocking));
+  return g_mz_stream;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhip_mozyme_density(mhip_ctx* ctx, const mhip_lmo_set* L, int mode, const double* partp, double* p) {
+  return mz_guard([&] {
+    hipStream_t s = mz_stream(ctx);
+    LmoBufs B;
+    upload_lmos(s, L, B);
+    DevBuf dp, dpart;
+    dp.ensure(sizeof(double) * L->mpack);
+    if (mode != 0) {
+      MHIP_REQUIRE(partp != nullptr, "mhip_mozyme_density: partp required for mode != 0");
+      up(s, dpart, partp, sizeof(double) * L->mpack);
+    }
+    const int nb = std::min(4096, cdiv(L->mpack, 256));
+    hipLaunchKernelGGL(k_mz_density_init, dim3(nb), dim3(256), 0, s, L->mpack, mode,
+                       mode != 0 ? dpart.as<double>() : nullptr, dp.as<double>());
+    MHIP_LAUNCH_CHECK();
+    if (L->nocc > 0) {
+      hipLaunchKernelGGL(k_mz_density, dim3(L->nocc), dim3(256), 0, s, B.dev, dp.as<double>());
+      MHIP_LAUNCH_CHECK();
+    }
+    const double spinfa = (mode == 0 || mode == 1) ? 2.0 : (mode == -1 ? -2.0 : 1.0);
+    if (std::fabs(spinfa - 1.0) > 1e-10) {
+      hipLaunchKernelGGL(k_scale, dim3(nb), dim3(256), 0, s, L->mpack, spinfa, dp.as<double>());
+      MHIP_LAUNCH_CHECK();
+    }
+    down(s, p, dp, sizeof(double) * L->mpack);
+    MHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int mhip_mozyme_diagg1(mhip_ctx* ctx, const mhip_lmo_set* L, const double* fao, const double* p,
+                       double* eigs, double* eigv, int idiagg, mhip_diagg1_state* st, int fmo_dim,
+                       double* fmo, int* ifmo, int* nij) {
+  return mz_guard([&] {
+    MHIP_REQUIRE(st && fmo && ifmo && nij && eigs && eigv && fmo_dim > 0, "mhip_mozyme_diagg1: bad arguments");
+    hipStream_t s = mz_stream(ctx);
+    // saved state of diagg1.F90:61-71 (reset when st->reset is set)
+    if (st->reset) {
+      st->fref = st->oldens ? 0.0 : 10.0;
+      st->safety = 1.0;
+      st->oldlim = 0.0;
+      st->reset = 0;
+    }
+    const bool full = (idiagg <= 5 || idiagg % 2 == 0);
+    MHIP_REQUIRE(full || st->nfmo != nullptr, "mhip_mozyme_diagg1: nfmo needed to reuse the list");
+    LmoBufs B;
+    upload_lmos(s, L, B);
+    DevBuf dfao, dp, daocc, dcounts, doffs, dnfmo, dold, dfmo, difmo, dvmax, dvsum, deigv, deigs;
+    up(s, dfao, fao, sizeof(double) * L->mpack);
+    up(s, dp, p, sizeof(double) * L->mpack);
+    daocc.ensure(sizeof(double) * std::max(L->icocc_dim, 1));
+    MHIP_CHECK(hipMemsetAsync(daocc.ptr, 0, sizeof(double) * std::max(L->icocc_dim, 1), s));
+    up(s, deigv, eigv, sizeof(double) * std::max(L->nvir, 1));
+    up(s, deigs, eigs, sizeof(double) * std::max(L->nocc, 1));
+    D1Params P;
+    const double cutlim = 1.0e-8;
+    P.cutoff = std::max(cutlim, st->tiny * 10.0 * cutlim);
+    P.flim = std::min(3.0, st->fref * 0.5);
+    st->fref = 0.0;
+    if (idiagg <= 5) P.cutoff = cutlim;
+    P.oldlim = st->oldlim;
+    P.full = full ? 1 : 0;
+    P.fmo_dim = fmo_dim;
+    const int nv = std::max(L->nvir, 1);
+    dcounts.ensure(sizeof(int) * nv);
+    doffs.ensure(sizeof(int) * nv);
+    dnfmo.ensure(sizeof(int) * nv);
+    dvmax.ensure(sizeof(double) * nv);
+    dvsum.ensure(sizeof(double) * nv);
+    dfmo.ensure(sizeof(double) * fmo_dim);
+    difmo.ensure(sizeof(int) * 2 * fmo_dim);
+    if (L->nocc > 0) {
+      hipLaunchKernelGGL(k_mz_aocc, dim3(cdiv(L->nocc, 128)), dim3(128), 0, s, B.dev, daocc.as<double>());
+      MHIP_LAUNCH_CHECK();
+    }
+    std::vector<int> offs(nv, 0), counts(nv, 0);
+    if (full) {
+      if (L->nvir > 0) {
+        hipLaunchKernelGGL(k_mz_diagg1_count, dim3(L->nvir), dim3(256), 0, s, B.dev, dfao.as<double>(),
+                           dp.as<double>(), daocc.as<double>(), P, dcounts.as<int>(), deigv.as<double>());
+        MHIP_LAUNCH_CHECK();
+        down(s, counts.data(), dcounts, sizeof(int) * L->nvir);
+        MHIP_CHECK(hipStreamSynchronize(s));
+      }
+      int acc = 0;
+      for (int i = 0; i < L->nvir; ++i) {
+        offs[i] = acc;
+        acc += counts[i];
+      }
+    } else {
+      // reuse: the old list's per-LMO counts (nfmo) and entries
+      int acc = 0;
+      for (int i = 0; i < L->nvir; ++i) {
+        offs[i] = acc;
+        acc += st->nfmo[i];
+      }
+      for (int k = 0; k < std::min(acc, fmo_dim); ++k)
+        MHIP_REQUIRE(ifmo[2 * k] >= 1 && ifmo[2 * k] <= L->nvir && ifmo[2 * k + 1] >= 1 &&
+                         ifmo[2 * k + 1] <= L->nocc,
+                     "mhip_mozyme_diagg1: the list to reuse holds an entry out of range");
+      up(s, dnfmo, st->nfmo, sizeof(int) * L->nvir);
+      up(s, dold, ifmo, sizeof(int) * 2 * std::max(std::min(acc, fmo_dim), 1));
+    }
+    up(s, doffs, offs.data(), sizeof(int) * nv);
+    if (L->nvir > 0) {
+      hipLaunchKernelGGL(k_mz_diagg1_write, dim3(L->nvir), dim3(256), 0, s, B.dev, dfao.as<double>(),
+                         dp.as<double>(), daocc.as<double>(), P, doffs.as<int>(), dnfmo.as<int>(),
+                         full ? nullptr : dold.as<int>(), dfmo.as<double>(), difmo.as<int>(),
+                         dvmax.as<double>(), dvsum.as<double>(), deigv.as<double>());
+      MHIP_LAUNCH_CHECK();
+    }
+    std::vector<double> vmax(nv, 0.0), vsum(nv, 0.0);
+    std::vector<int> nfmo_h(nv, 0);
+    down(s, vmax.data(), dvmax, sizeof(double) * L->nvir);
+    down(s, vsum.data(), dvsum, sizeof(double) * L->nvir);
+    if (full) down(s, nfmo_h.data(), dnfmo, sizeof(int) * L->nvir);
+    MHIP_CHECK(hipStreamSynchronize(s));
+    double tiny = 0.0, sumt = 0.0;
+    int total = 0;
+    for (int i = 0; i < L->nvir; ++i) {
+      tiny = std::max(tiny, vmax[i]);
+      sumt += vsum[i];
+      total += full ? nfmo_h[i] : st->nfmo[i];
+    }
+    total = std::min(total, fmo_dim);
+    if (full && st->nfmo)
+      for (int i = 0; i < L->nvir; ++i) st->nfmo[i] = nfmo_h[i];
+    // storage full (diagg1.F90:458-470)
+    if (total == fmo_dim) st->safety *= 2.0;
+    else st->safety = std::max(st->safety * 0.5, 1.0);
+    *nij = total;
+    down(s, fmo, dfmo, sizeof(double) * total);
+    down(s, ifmo, difmo, sizeof(int) * 2 * total);
+    down(s, eigv, deigv, sizeof(double) * L->nvir);
+    if (idiagg > 2 && idiagg % 4 != 0) {
+      st->fref = (tiny * tiny) * (tiny * tiny);  // tiny**4 as the reference's powi
+    } else {
+      if (L->nocc > 0) {
+        hipLaunchKernelGGL(k_mz_occ_energy, dim3(cdiv(L->nocc, 64)), dim3(64), 0, s, B.dev, dfao.as<double>(),
+                           dp.as<double>(), daocc.as<double>(), P.cutoff, deigs.as<double>());
+        MHIP_LAUNCH_CHECK();
+        down(s, eigs, deigs, sizeof(double) * L->nocc);
+      }
+      st->oldlim = tiny * st->safety * 1.0e-3;
+      st->fref = (tiny * tiny) * (tiny * tiny);  // tiny**4 as the reference's powi
+    }
+    MHIP_CHECK(hipStreamSynchronize(s));
+    st->tiny = tiny;
+    st->sumt = sumt;
+    st->ovmax = tiny;
+  });
+}
+
+int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const double* eigv,
+                       const double* fmo, const int* ifmo, int nij, int idiagg, mhip_diagg2_state* st) {
+  return mz_guard([&] {
+    MHIP_REQUIRE(st && eigs && eigv && nij >= 0 && (nij == 0 || (fmo && ifmo)), "mhip_mozyme_diagg2: bad arguments");
+    hipStream_t s = mz_stream(ctx);
+    for (int k = 0; k < nij; ++k)
+      MHIP_REQUIRE(ifmo[2 * k] >= 1 && ifmo[2 * k] <= L->nvir && ifmo[2 * k + 1] >= 1 &&
+                       ifmo[2 * k + 1] <= L->nocc,
+                   "mhip_mozyme_diagg2: ifmo entry out of range");
+    if (st->reset) {
+      st->nrejct[0] = st->nrejct[1] = 0;
+      st->reset = 0;
+    }
+    D2Params P;
+    // epseta (src/MOZYME/epseta.F90): eps = max(tiny(1d0), 1d-39); bigeps = 50 sqrt(eps)
+    const double bigeps = 50.0 * std::sqrt(std::max(2.2250738585072014e-308, 1.0e-39));
+    const bool retry = (st->nrejct[0] == st->nrejct[1] && st->nrejct[0] != 0 && st->nrejct[0] < 20);
+    if (idiagg % 5 == 0 || idiagg <= 5) {
+      st->tiny = -1.0;
+      P.biglim = -1.0;
+    } else {
+      st->tiny = 0.01 * st->tiny;
+      P.biglim = bigeps;
+    }
+    P.tiny = st->tiny;
+    P.cnst = st->damp;
+    P.shift = st->shift;
+    P.thresh = st->thresh;
+    P.retry = retry ? 1 : 0;
+    P.nij = nij;
+    LmoBufs B;
+    upload_lmos(s, L, B);
+    DevBuf deigs, deigv, dfmo, difmo, dused, dlat, dsj, dsi, dout;
+    up(s, deigs, eigs, sizeof(double) * std::max(L->nocc, 1));
+    up(s, deigv, eigv, sizeof(double) * std::max(L->nvir, 1));
+    up(s, dfmo, fmo, sizeof(double) * std::max(nij, 1));
+    up(s, difmo, ifmo, sizeof(int) * 2 * std::max(nij, 1));
+    std::vector<int> minus1(L->numat, -1);
+    up(s, dused, minus1.data(), sizeof(int) * L->numat);
+    dlat.ensure(L->numat);
+    MHIP_CHECK(hipMemsetAsync(dlat.ptr, 0, L->numat, s));
+    dsj.ensure(sizeof(double) * (L->norbs + 8));
+    dsi.ensure(sizeof(double) * (L->norbs + 8));
+    dout.ensure(sizeof(double) * 2);
+    hipLaunchKernelGGL(k_mz_diagg2, dim3(1), dim3(D2_NT), 0, s, B.dev, deigs.as<double>(), deigv.as<double>(),
+                       dfmo.as<double>(), difmo.as<int>(), P, dused.as<int>(), (char*)dlat.ptr, dsj.as<double>(),
+                       dsi.as<double>(), dout.as<double>());
+    MHIP_LAUNCH_CHECK();
+    double out[2];
+    down(s, out, dout, sizeof(out));
+    down(s, L->ncf, B.ncf, sizeof(int) * L->nocc);
+    down(s, L->icocc, B.icocc, sizeof(int) * L->icocc_dim);
+    down(s, L->cocc, B.cocc, sizeof(double) * L->cocc_dim);
+    down(s, L->nce, B.nce, sizeof(int) * L->nvir);
+    down(s, L->icvir, B.icvir, sizeof(int) * L->icvir_dim);
+    down(s, L->cvir, B.cvir, sizeof(double) * L->cvir_dim);
+    MHIP_CHECK(hipStreamSynchronize(s));
+    st->sumb = out[0];
+    st->nrejct[1] = st->nrejct[0];
+    st->nrejct[0] = (int)out[1];
+  });
+}
+
+int mhip_mozyme_helecz(mhip_ctx* ctx, int numat, const int* nfirst, const int* nlast, const int* nijbo,
+                       long long mpack, const double* p, const double* h, const double* f, double* e) {
+  return mz_guard([&] {
+    MHIP_REQUIRE(numat > 0 && nfirst && nlast && nijbo && p && h && f && e, "mhip_mozyme_helecz: bad arguments");
+    hipStream_t s = mz_stream(ctx);
+    std::vector<int> iorbs(numat);
+    for (int a = 0; a < numat; ++a) iorbs[a] = nlast[a] - nfirst[a] + 1;
+    for (int b = 0; b < numat; ++b)
+      for (int a = 0; a < numat; ++a) {
+        const long long o = nijbo[(size_t)a + (size_t)b * numat];
+        if (o < 0) {
+          MHIP_REQUIRE(a != b, "mhip_mozyme_helecz: every diagonal block must be stored");
+          continue;
+        }
+        const long long blk = (a == b) ? iorbs[a] * (iorbs[a] + 1) / 2 : (long long)iorbs[a] * iorbs[b];
+        MHIP_REQUIRE(o + blk <= mpack, "mhip_mozyme_helecz: nijbo block beyond mpack");
+      }
+    DevBuf di, dn, dp, dh, df, dpart, dout;
+    up(s, di, iorbs.data(), sizeof(int) * numat);
+    up(s, dn, nijbo, sizeof(int) * (size_t)numat * numat);
+    up(s, dp, p, sizeof(double) * mpack);
+    up(s, dh, h, sizeof(double) * mpack);
+    up(s, df, f, sizeof(double) * mpack);
+    dpart.ensure(sizeof(double) * 2 * numat);
+    dout.ensure(sizeof(double));
+    hipLaunchKernelGGL(k_mz_helecz, dim3(numat), dim3(256), 0, s, numat, di.as<int>(), dn.as<int>(),
+                       dp.as<double>(), dh.as<double>(), df.as<double>(), dpart.as<double>());
+    MHIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mz_sum2, dim3(1), dim3(1024), 0, s, numat, dpart.as<double>(), dout.as<double>());
+    MHIP_LAUNCH_CHECK();
+    down(s, e, dout, sizeof(double));
+    MHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int mhip_mozyme_cnvgz(mhip_ctx* ctx, int norbs, long long mpack, double* pnew, double* p, double* p1,
+                      double* p2, double* p3, int niter, const int* idiag, int three_point, double* pmax) {
+  return mz_guard([&] {
+    MHIP_REQUIRE(norbs > 0 && mpack > 0 && pnew && p && p1 && p2 && p3 && idiag && pmax,
+                 "mhip_mozyme_cnvgz: bad arguments");
+    for (int i = 0; i < norbs; ++i)
+      MHIP_REQUIRE(idiag[i] >= 1 && idiag[i] <= mpack, "mhip_mozyme_cnvgz: idiag out of range");
+    hipStream_t s = mz_stream(ctx);
+    DevBuf dpn, dp, d1, d2, d3, di, dpart, dsc;
+    up(s, dpn, pnew, sizeof(double) * mpack);
+    up(s, dp, p, sizeof(double) * mpack);
+    up(s, d1, p1, sizeof(double) * norbs);
+    up(s, d2, p2, sizeof(double) * norbs);
+    up(s, d3, p3, sizeof(double) * norbs);
+    up(s, di, idiag, sizeof(int) * norbs);
+    const int nparts = std::min(1024, cdiv(mpack, 256));
+    dpart.ensure(sizeof(double) * nparts);
+    dsc.ensure(sizeof(double) * 2);
+    hipLaunchKernelGGL(k_mz_absdiff_max, dim3(nparts), dim3(256), 0, s, mpack, dpn.as<double>(), dp.as<double>(),
+                       dpart.as<double>());
+    MHIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mz_cnvgz_diag, dim3(1), dim3(1024), 0, s, norbs, nparts, dpart.as<double>(), di.as<int>(),
+                       dpn.as<double>(), dp.as<double>(), d1.as<double>(), d2.as<double>(), d3.as<double>(),
+                       niter, three_point, dsc.as<double>());
+    MHIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mz_extrap, dim3(std::min(4096, cdiv(mpack, 256))), dim3(256), 0, s, mpack,
+                       dsc.as<double>(), dpn.as<double>(), dp.as<double>());
+    MHIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mz_damp, dim3(cdiv(norbs, 256)), dim3(256), 0, s, norbs, di.as<int>(), dsc.as<double>(),
+                       d2.as<double>(), d3.as<double>(), niter, three_point, dpn.as<double>(), d1.as<double>());
+    MHIP_LAUNCH_CHECK();
+    double sc[2];
+    down(s, sc, dsc, sizeof(sc));
+    down(s, pnew, dpn, sizeof(double) * mpack);
+    down(s, p, dpn, sizeof(double) * mpack);  // p = pnew
+    down(s, p1, d1, sizeof(double) * norbs);
+    down(s, p2, d2, sizeof(double) * norbs);
+    down(s, p3, d3, sizeof(double) * norbs);
+    MHIP_CHECK(hipStreamSynchronize(s));
+    *pmax = sc[0];
+  });
+}
+
+}  // extern "C"

@@ -51,6 +51,26 @@ class ScfResult(ctypes.Structure):
     ]
 
 
+class LmoSetC(ctypes.Structure):
+    _fields_ = [("numat", c_int), ("norbs", c_int), ("nocc", c_int), ("nvir", c_int), ("mpack", c_ll),
+                ("nfirst", c_void_p), ("nlast", c_void_p), ("nijbo", c_void_p),
+                ("ncf", c_void_p), ("nncf", c_void_p), ("ncocc", c_void_p), ("icocc", c_void_p),
+                ("icocc_dim", c_int), ("cocc", c_void_p), ("cocc_dim", c_int),
+                ("nce", c_void_p), ("nnce", c_void_p), ("ncvir", c_void_p), ("icvir", c_void_p),
+                ("icvir_dim", c_int), ("cvir", c_void_p), ("cvir_dim", c_int)]
+
+
+class Diagg1State(ctypes.Structure):
+    _fields_ = [("reset", c_int), ("oldens", c_int), ("fref", c_double), ("safety", c_double),
+                ("oldlim", c_double), ("tiny", c_double), ("sumt", c_double), ("ovmax", c_double),
+                ("nfmo", c_void_p)]
+
+
+class Diagg2State(ctypes.Structure):
+    _fields_ = [("reset", c_int), ("tiny", c_double), ("sumb", c_double), ("damp", c_double),
+                ("shift", c_double), ("thresh", c_double), ("nrejct", c_int * 2)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "mhip_init": (c_int, [c_int, ctypes.POINTER(c_void_p)]),
@@ -72,6 +92,15 @@ SIGNATURES = {
     "mhip_destroy_resources": (None, []),
     "mhip_set_tuning": (c_int, [c_void_p, c_int, c_int, c_double]),
     "mhip_last_sweeps": (c_int, [c_void_p]),
+    "mhip_mozyme_density": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "mhip_mozyme_diagg1": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                   c_int, c_void_p, c_void_p, c_void_p]),
+    "mhip_mozyme_diagg2": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                   c_void_p]),
+    "mhip_mozyme_helecz": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p, c_void_p,
+                                   ctypes.POINTER(c_double)]),
+    "mhip_mozyme_cnvgz": (c_int, [c_void_p, c_int, c_ll, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                  c_void_p, c_int, ctypes.POINTER(c_double)]),
     "mhip_kernel_stats": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhip_kernel_class_name": (ctypes.c_char_p, [c_int]),
     "mhip_kernel_timing": (c_int, [c_void_p, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_ll),

@@ -1,0 +1,289 @@
+"""MOZYME localized-MO (LMO) data in the reference's layout, and a builder
+that makes it from a converged conventional SCF.
+
+Layout (src/MOZYME/MOZYME_C.F90:116-162), kept exactly because the GPU
+kernels and the reference routines read the same arrays:
+
+  occupied LMO i:  atoms icocc[nncf[i] : nncf[i] + ncf[i]] (1-based atom
+                   numbers), coefficients cocc[ncocc[i] : ...] atom after
+                   atom (iorbs of each); nncf/ncocc are 0-based offsets and
+                   each LMO owns the space up to the next LMO's offset (room
+                   to grow in diagg2);
+  virtual LMO i:   the same with nce, nnce, icvir, ncvir, cvir;
+  sparse P, F:     atom-pair blocks at nijbo[a, b] (0-based, -1 when the pair
+                   is not stored): a > b as an iorbs(a) x iorbs(b) block with
+                   b's orbital fastest, a == b as the packed lower triangle.
+
+The builder localizes the canonical occupied and virtual orbitals of a
+converged SCF separately (Pipek-Mezey), drops atoms whose weight in an LMO is
+below `keep`, and stores every atom pair within `rcut` (Angstrom).  It feeds
+the parity tests of the MOZYME kernels with realistic LMOs; the reference's
+own start (a Lewis structure, makvec) is not part of the SCF inner loop.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class LmoSet:
+    numat: int
+    norbs: int
+    nfirst: np.ndarray      # 1-based
+    nlast: np.ndarray       # 1-based
+    nat: np.ndarray
+    nijbo: np.ndarray       # (numat, numat) int32, symmetric, -1 = not stored
+    mpack: int              # size of the sparse storage
+    nocc: int
+    nvir: int
+    ncf: np.ndarray
+    nncf: np.ndarray
+    ncocc: np.ndarray
+    icocc: np.ndarray
+    cocc: np.ndarray
+    nce: np.ndarray
+    nnce: np.ndarray
+    ncvir: np.ndarray
+    icvir: np.ndarray
+    cvir: np.ndarray
+    eigs: np.ndarray        # occupied LMO energies
+    eigv: np.ndarray        # virtual LMO energies
+    fao: np.ndarray = field(default=None)    # sparse Fock matrix
+    p: np.ndarray = field(default=None)      # sparse density matrix
+
+    @property
+    def iorbs(self):
+        return (self.nlast - self.nfirst + 1).astype(np.int32)
+
+    def copy(self):
+        return LmoSet(**{k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in self.__dict__.items()})
+
+
+def build_nijbo(xyz, iorbs, rcut):
+    numat = len(iorbs)
+    nijbo = np.full((numat, numat), -1, dtype=np.int32)
+    off = 0
+    for a in range(numat):
+        d = np.linalg.norm(xyz[: a + 1] - xyz[a], axis=1)
+        for b in range(a + 1):
+            if b != a and d[b] > rcut:
+                continue
+            nijbo[a, b] = nijbo[b, a] = off
+            off += iorbs[a] * (iorbs[a] + 1) // 2 if a == b else iorbs[a] * iorbs[b]
+    return nijbo, off
+
+
+def sparse_from_dense(m, nfirst, iorbs, nijbo, size):
+    """Dense symmetric matrix (norbs x norbs) -> MOZYME sparse blocks."""
+    out = np.zeros(size)
+    numat = len(iorbs)
+    f0 = nfirst - 1
+    for a in range(numat):
+        for b in range(a + 1):
+            o = nijbo[a, b]
+            if o < 0:
+                continue
+            ra, rb = slice(f0[a], f0[a] + iorbs[a]), slice(f0[b], f0[b] + iorbs[b])
+            blk = m[ra, rb]
+            if a == b:
+                out[o:o + iorbs[a] * (iorbs[a] + 1) // 2] = blk[np.tril_indices(iorbs[a])]
+            else:
+                out[o:o + iorbs[a] * iorbs[b]] = blk.ravel()
+    return out
+
+
+def dense_from_sparse(v, nfirst, iorbs, nijbo, norbs):
+    m = np.zeros((norbs, norbs))
+    f0 = nfirst - 1
+    for a in range(len(iorbs)):
+        for b in range(a + 1):
+            o = nijbo[a, b]
+            if o < 0:
+                continue
+            ra, rb = slice(f0[a], f0[a] + iorbs[a]), slice(f0[b], f0[b] + iorbs[b])
+            if a == b:
+                t = np.zeros((iorbs[a], iorbs[a]))
+                t[np.tril_indices(iorbs[a])] = v[o:o + iorbs[a] * (iorbs[a] + 1) // 2]
+                m[ra, ra] = t + t.T - np.diag(np.diag(t))
+            else:
+                blk = v[o:o + iorbs[a] * iorbs[b]].reshape(iorbs[a], iorbs[b])
+                m[ra, rb] = blk
+                m[rb, ra] = blk.T
+    return m
+
+
+def pipek_mezey(c, nfirst, iorbs, sweeps=30, tol=1e-10):
+    """Pipek-Mezey localization of the columns of c (orthogonal AO basis):
+    maximize sum_i sum_A q_iA^2, q_iA = sum_{mu on A} c_mu,i^2, by 2x2
+    rotations (Pipek & Mezey, J. Chem. Phys. 90, 4916 (1989))."""
+    c = np.array(c, dtype=np.float64, copy=True)
+    n = c.shape[1]
+    f0 = nfirst - 1
+    atom_of = np.repeat(np.arange(len(iorbs)), iorbs)
+    nat = len(iorbs)
+
+    def pops(x):
+        return np.bincount(atom_of, weights=x, minlength=nat)
+
+    for _ in range(sweeps):
+        change = 0.0
+        for i in range(n - 1):
+            for j in range(i + 1, n):
+                ci, cj = c[:, i], c[:, j]
+                qii = pops(ci * ci)
+                qjj = pops(cj * cj)
+                qij = pops(ci * cj)
+                a = np.sum(qij ** 2) - 0.25 * np.sum((qii - qjj) ** 2)
+                b = np.sum(qij * (qii - qjj))
+                if abs(a) < 1e-300 and abs(b) < 1e-300:
+                    continue
+                g = np.hypot(a, b)
+                if a + g < 1e-14:
+                    continue
+                t = 0.25 * np.arctan2(b, -a)
+                if abs(t) < 1e-12:
+                    continue
+                cs, sn = np.cos(t), np.sin(t)
+                c[:, i], c[:, j] = cs * ci + sn * cj, -sn * ci + cs * cj
+                change = max(change, abs(t))
+        if change < tol:
+            break
+    return c
+
+
+def _pack(cols, nfirst, iorbs, keep, pad_atoms):
+    f0 = nfirst - 1
+    nat = len(iorbs)
+    ncf, nncf, ncoc, idx, coef = [], [], [], [], []
+    ia = ic = 0
+    for k in range(cols.shape[1]):
+        x = cols[:, k]
+        w = np.array([np.sum(x[f0[a]:f0[a] + iorbs[a]] ** 2) for a in range(nat)])
+        atoms = [a for a in np.argsort(-w) if w[a] > keep]
+        nncf.append(ia)
+        ncoc.append(ic)
+        ncf.append(len(atoms))
+        for a in atoms:
+            idx.append(a + 1)
+            coef.extend(x[f0[a]:f0[a] + iorbs[a]])
+        room = pad_atoms
+        idx.extend([0] * room)
+        coef.extend([0.0] * (4 * room))
+        ia += len(atoms) + room
+        ic += sum(iorbs[a] for a in atoms) + 4 * room
+    return (np.array(ncf, np.int32), np.array(nncf, np.int32), np.array(ncoc, np.int32),
+            np.array(idx, np.int32), np.array(coef))
+
+
+def from_scf(system, c, f_dense, p_dense, nocc, keep=1e-4, rcut=8.0, pad_atoms=6, sweeps=30):
+    """LMOs in MOZYME layout from canonical orbitals c (norbs x norbs)."""
+    nfirst = np.asarray(system.nfirst, dtype=np.int32)
+    nlast = np.asarray(system.nlast, dtype=np.int32)
+    iorbs = nlast - nfirst + 1
+    occ = pipek_mezey(c[:, :nocc], nfirst, iorbs, sweeps)
+    vir = pipek_mezey(c[:, nocc:], nfirst, iorbs, sweeps)
+    nijbo, size = build_nijbo(np.asarray(system.xyz), iorbs, rcut)
+    o = _pack(occ, nfirst, iorbs, keep, pad_atoms)
+    v = _pack(vir, nfirst, iorbs, keep, pad_atoms)
+    s = LmoSet(numat=system.numat, norbs=system.norbs, nfirst=nfirst, nlast=nlast,
+               nat=np.asarray(system.nat, dtype=np.int32), nijbo=nijbo, mpack=size, nocc=nocc,
+               nvir=c.shape[1] - nocc, ncf=o[0], nncf=o[1], ncocc=o[2], icocc=o[3], cocc=o[4],
+               nce=v[0], nnce=v[1], ncvir=v[2], icvir=v[3], cvir=v[4],
+               eigs=np.zeros(nocc), eigv=np.zeros(c.shape[1] - nocc))
+    s.fao = sparse_from_dense(f_dense, nfirst, iorbs, nijbo, size)
+    s.p = sparse_from_dense(p_dense, nfirst, iorbs, nijbo, size)
+    s.eigs = np.array([_energy(s, k, True, f_dense) for k in range(nocc)])
+    s.eigv = np.array([_energy(s, k, False, f_dense) for k in range(s.nvir)])
+    return s
+
+
+def lmo_dense(s, k, occupied):
+    """Column k of the occupied (or virtual) LMOs as a dense norbs vector."""
+    ncf, nncf, ncoc, idx, coef = ((s.ncf, s.nncf, s.ncocc, s.icocc, s.cocc) if occupied
+                                  else (s.nce, s.nnce, s.ncvir, s.icvir, s.cvir))
+    x = np.zeros(s.norbs)
+    pos = ncoc[k]
+    for a in idx[nncf[k]:nncf[k] + ncf[k]]:
+        a0 = a - 1
+        no = s.nlast[a0] - s.nfirst[a0] + 1
+        x[s.nfirst[a0] - 1:s.nfirst[a0] - 1 + no] = coef[pos:pos + no]
+        pos += no
+    return x
+
+
+def _energy(s, k, occupied, f_dense):
+    x = lmo_dense(s, k, occupied)
+    return float(x @ f_dense @ x)
+
+
+class GpuMozyme:
+    """The MOZYME kernels of libmopachip on an LmoSet, with the state the
+    reference keeps between calls (mhip_diagg1_state / mhip_diagg2_state)."""
+
+    def __init__(self, ctx, s, fmo_dim=None, thresh=1e-13, shift=0.0, damp=1.0, oldens=False):
+        import ctypes
+        from . import lib as L
+        self.ctypes, self.L = ctypes, L
+        self.ctx = ctx
+        self.s = s.copy()
+        self.fmo_dim = int(fmo_dim or max(1, s.nocc * s.nvir))
+        self.nfmo = np.zeros(max(s.nvir, 1), dtype=np.int32)
+        self.st1 = L.Diagg1State(reset=1, oldens=int(oldens))
+        self.st1.nfmo = self.nfmo.ctypes.data
+        self.st2 = L.Diagg2State(reset=1, damp=damp, shift=shift, thresh=thresh)
+        self.fmo = np.zeros(self.fmo_dim)
+        self.ifmo = np.zeros((self.fmo_dim, 2), dtype=np.int32)
+        self.nij = 0
+        self._arrays()
+
+    def _arrays(self):
+        s = self.s
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+        self.a = dict(nfirst=i32(s.nfirst), nlast=i32(s.nlast), nijbo=np.asfortranarray(i32(s.nijbo)),
+                      ncf=i32(s.ncf), nncf=i32(s.nncf), ncocc=i32(s.ncocc), icocc=i32(s.icocc),
+                      cocc=np.ascontiguousarray(s.cocc, dtype=np.float64), nce=i32(s.nce), nnce=i32(s.nnce),
+                      ncvir=i32(s.ncvir), icvir=i32(s.icvir),
+                      cvir=np.ascontiguousarray(s.cvir, dtype=np.float64))
+        a = self.a
+        self.c = self.L.LmoSetC(numat=s.numat, norbs=s.norbs, nocc=s.nocc, nvir=s.nvir, mpack=s.mpack,
+                                icocc_dim=len(a["icocc"]), cocc_dim=len(a["cocc"]),
+                                icvir_dim=len(a["icvir"]), cvir_dim=len(a["cvir"]))
+        for k in ("nfirst", "nlast", "nijbo", "ncf", "nncf", "ncocc", "icocc", "cocc", "nce", "nnce", "ncvir",
+                  "icvir", "cvir"):
+            setattr(self.c, k, a[k].ctypes.data)
+        self.eigs = np.array(s.eigs, dtype=np.float64)
+        self.eigv = np.array(s.eigv, dtype=np.float64)
+
+    def density(self, mode=0, partp=None):
+        p = np.zeros(self.s.mpack)
+        pp = None if partp is None else np.ascontiguousarray(partp, dtype=np.float64)
+        self.L.check(self.L.load().mhip_mozyme_density(self.ctx.h, self.ctypes.byref(self.c), mode,
+                                                       self.L.ptr(pp), self.L.ptr(p)))
+        return p
+
+    def diagg1(self, fao, p, idiagg):
+        nij = self.ctypes.c_int()
+        self.L.check(self.L.load().mhip_mozyme_diagg1(
+            self.ctx.h, self.ctypes.byref(self.c), self.L.ptr(np.ascontiguousarray(fao)),
+            self.L.ptr(np.ascontiguousarray(p)), self.L.ptr(self.eigs), self.L.ptr(self.eigv), idiagg,
+            self.ctypes.byref(self.st1), self.fmo_dim, self.L.ptr(self.fmo), self.L.ptr(self.ifmo),
+            self.ctypes.byref(nij)))
+        self.nij = nij.value
+        return dict(nij=self.nij, fmo=self.fmo[:self.nij].copy(), ifmo=self.ifmo[:self.nij].copy(),
+                    tiny=self.st1.tiny, sumt=self.st1.sumt, ovmax=self.st1.ovmax, eigv=self.eigv.copy())
+
+    def diagg2(self, idiagg):
+        self.st2.tiny = self.st1.tiny
+        self.L.check(self.L.load().mhip_mozyme_diagg2(
+            self.ctx.h, self.ctypes.byref(self.c), self.L.ptr(self.eigs), self.L.ptr(self.eigv),
+            self.L.ptr(self.fmo), self.L.ptr(self.ifmo), self.nij, idiagg, self.ctypes.byref(self.st2)))
+        self.st1.tiny = self.st2.tiny
+        return self.st2.sumb
+
+    def get(self):
+        s = self.s.copy()
+        a = self.a
+        s.ncf, s.icocc, s.cocc = a["ncf"].copy(), a["icocc"].copy(), a["cocc"].copy()
+        s.nce, s.icvir, s.cvir = a["nce"].copy(), a["icvir"].copy(), a["cvir"].copy()
+        s.eigs, s.eigv = self.eigs.copy(), self.eigv.copy()
+        return s

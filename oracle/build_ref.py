@@ -7,9 +7,11 @@ restatement, never part of the product).
 Only the routines on the SCF inner loop and the small modules they read are
 compiled, with amdflang, straight from where they lie.  The reference's own
 CMake build is not used.  Symbols those files reference but never reach on
-this path (the COSMO Fock term addfck, guarded by useps = .false.) stay
-unresolved, so the library is opened lazily (RTLD_LAZY); the BLAS/LAPACK
-routines pulay calls resolve from MKL, which the oracle loader makes global.
+this path stay unresolved (the COSMO Fock term addfck, guarded by
+useps = .false.; ijbo, used only when lijbo is false; mopend, reada and timer,
+reached only on error or by keywords the tests do not set), so the library is
+opened lazily (RTLD_LAZY); the BLAS/LAPACK routines pulay calls resolve from
+MKL, which the oracle loader makes global.
 """
 import argparse
 import os
@@ -36,7 +38,20 @@ SOURCES = [
     "SCF/pulay.F90",
     "matrix/diag.F90",
     "matrix/osinv.F90",
+    # MOZYME kernels and the modules they read
+    "Common_arrays_C.F90",
+    "models/parameters_C.F90",
+    "MOZYME/MOZYME_C.F90",
+    "MOZYME/epseta.F90",
+    "MOZYME/diagg1.F90",
+    "MOZYME/diagg2.F90",
+    "MOZYME/density_for_MOZYME.F90",
+    "overlaps_C.F90",
+    "utilities/ijbo.F90",
+    "MOZYME/helecz.F90",
+    "MOZYME/cnvgz.F90",
 ]
+HARNESSES = ["ref_harness.F90", "ref_harness_mozyme.F90"]
 
 
 def build(src="/root/reference/src"):
@@ -46,7 +61,7 @@ def build(src="/root/reference/src"):
     os.makedirs(mod, exist_ok=True)
     objs = []
     flags = ["-O2", "-fPIC", "-module-dir", mod, "-I", mod]
-    for rel in SOURCES + [os.path.join(HERE, "ref_harness.F90")]:
+    for rel in SOURCES + [os.path.join(HERE, h) for h in HARNESSES]:
         path = rel if os.path.isabs(rel) else os.path.join(src, rel)
         obj = os.path.join(OUT, os.path.basename(path).replace(".F90", ".o"))
         subprocess.run([FC] + flags + ["-c", path, "-o", obj], check=True, cwd=OUT)

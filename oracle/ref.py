@@ -36,6 +36,15 @@ def load():
         lib.ref_diag.argtypes = [V, V, I, V, I]
         lib.ref_osinv.argtypes = [V, I, V]
         lib.ref_pulay_seq.argtypes = [I, I, V, V, V]
+        lib.mz_setup.argtypes = [I, I, I, V, V, V, V, D, D]
+        lib.mz_set_lmos.argtypes = [I, I, V, V, V, I, V, I, V, V, V, V, I, V, I, V, I, V, V]
+        lib.mz_get_lmos.argtypes = [I, I, V, V, V, V, V, V, V]
+        lib.mz_density.argtypes = [V, I, I, V]
+        lib.mz_diagg1.argtypes = [V, I, I, V, I, V, V, V, V, V, V]
+        lib.mz_diagg2.argtypes = [I, I, V, I, I, V, V]
+        lib.mz_helecz.argtypes = [V, V, V]
+        lib.mz_helecz.restype = D
+        lib.mz_cnvgz.argtypes = [V, V, V, V, V, I, V, I, V]
         _lib = lib
     return _lib
 
@@ -87,3 +96,85 @@ def pulay_seq(n, f_seq, p_seq):
     pl = np.zeros(f.shape[0])
     load().ref_pulay_seq(n, f.shape[0], P(f), P(np.ascontiguousarray(p_seq)), P(pl))
     return f, pl
+
+
+# ---- MOZYME (src/MOZYME/diagg1.F90, diagg2.F90, density_for_MOZYME.F90) ----
+
+class Mozyme:
+    """Loads an LmoSet (mopac_gpu_amd.mozyme) into the reference's module
+    arrays and runs its MOZYME kernels; every call mutates the loaded set the
+    way the reference does, and get() returns it."""
+
+    def __init__(self, s, fmo_dim=None, thresh=1e-13, shift=0.0):
+        lib = load()
+        self.s = s.copy()
+        self.fmo_dim = int(fmo_dim or max(1, s.nocc * s.nvir))
+        self.i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        nij = np.asfortranarray(self.i32(s.nijbo))
+        lib.mz_setup(s.numat, s.norbs, s.mpack, P(self.i32(s.nfirst)), P(self.i32(s.nlast)),
+                     P(self.i32(s.nat)), P(nij), thresh, shift)
+        self._keep = [self.i32(x) for x in (s.ncf, s.nncf, s.ncocc, s.icocc, s.nce, s.nnce, s.ncvir, s.icvir)]
+        ncf, nncf, ncocc, icocc, nce, nnce, ncvir, icvir = self._keep
+        self.cocc = np.ascontiguousarray(s.cocc, dtype=np.float64)
+        self.cvir = np.ascontiguousarray(s.cvir, dtype=np.float64)
+        eigs = np.ascontiguousarray(s.eigs, dtype=np.float64)
+        p = np.ascontiguousarray(s.p, dtype=np.float64)
+        lib.mz_set_lmos(s.nocc, s.nvir, P(ncf), P(nncf), P(ncocc), len(icocc), P(icocc), len(self.cocc),
+                        P(self.cocc), P(nce), P(nnce), P(ncvir), len(icvir), P(icvir), len(self.cvir),
+                        P(self.cvir), self.fmo_dim, P(eigs), P(p))
+        self.eigv = np.array(s.eigv, dtype=np.float64)
+        self.tiny = 0.0
+
+    def density(self, mode=0, partp=None):
+        s = self.s
+        p = np.zeros(s.mpack)
+        pp = np.zeros(s.mpack) if partp is None else np.ascontiguousarray(partp, dtype=np.float64)
+        load().mz_density(P(p), mode, s.nocc, P(pp))
+        return p
+
+    def diagg1(self, fao, idiagg):
+        s = self.s
+        fmo = np.zeros(self.fmo_dim)
+        ifmo = np.zeros((self.fmo_dim, 2), dtype=np.int32)
+        nij = ctypes.c_int()
+        tiny = ctypes.c_double(self.tiny)
+        sumt, ovmax = ctypes.c_double(), ctypes.c_double()
+        load().mz_diagg1(P(np.ascontiguousarray(fao)), s.nocc, s.nvir, P(self.eigv), idiagg, ctypes.byref(nij),
+                         P(fmo), P(ifmo), ctypes.byref(tiny), ctypes.byref(sumt), ctypes.byref(ovmax))
+        self.tiny = tiny.value
+        k = nij.value
+        return dict(nij=k, fmo=fmo[:k].copy(), ifmo=ifmo[:k].copy(), tiny=tiny.value, sumt=sumt.value,
+                    ovmax=ovmax.value, eigv=self.eigv.copy())
+
+    def diagg2(self, nij, idiagg):
+        s = self.s
+        tiny = ctypes.c_double(self.tiny)
+        sumb = ctypes.c_double()
+        load().mz_diagg2(s.nocc, s.nvir, P(self.eigv), nij, idiagg, ctypes.byref(tiny), ctypes.byref(sumb))
+        self.tiny = tiny.value
+        return sumb.value
+
+    def helecz(self, p, h, f):
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+        return load().mz_helecz(P(c(p)), P(c(h)), P(c(f)))
+
+    def cnvgz(self, pnew, p, p1, p2, p3, niter, idiag, three_point=True):
+        """In place on pnew, p, p1, p2, p3 (float64 arrays); returns pmax."""
+        pm = ctypes.c_double()
+        load().mz_cnvgz(P(pnew), P(p), P(p1), P(p2), P(p3), niter, P(self.i32(idiag)), int(three_point),
+                        ctypes.byref(pm))
+        return pm.value
+
+    def get(self):
+        s = self.s.copy()
+        eigs = np.zeros(s.nocc)
+        ncf = np.zeros(s.nocc, np.int32)
+        nce = np.zeros(s.nvir, np.int32)
+        icocc = np.zeros(len(s.icocc), np.int32)
+        icvir = np.zeros(len(s.icvir), np.int32)
+        cocc = np.zeros(len(s.cocc))
+        cvir = np.zeros(len(s.cvir))
+        load().mz_get_lmos(s.nocc, s.nvir, P(ncf), P(icocc), P(cocc), P(nce), P(icvir), P(cvir), P(eigs))
+        s.ncf, s.nce, s.icocc, s.icvir, s.cocc, s.cvir, s.eigs = ncf, nce, icocc, icvir, cocc, cvir, eigs
+        s.eigv = self.eigv.copy()
+        return s

@@ -62,9 +62,14 @@ def test_fortran_host_matches_oracle(tmp_path):
     f_ref = oracle.fock2(s, s.h, ptest, 0.5 * ptest)
     assert np.max(np.abs(g["f"] - f_ref)) < 1e-11 * np.max(np.abs(f_ref))
     assert g["e"] == pytest.approx(oracle.helect(s.norbs, 0.5 * ptest, s.h, f_ref), rel=1e-12)
+    # eigenvectors_LAPACK's in-place +i*1e-10 on the diagonal of xmat, nothing else
+    d = np.array([i * (i + 1) // 2 + i for i in range(s.norbs)])
+    off = np.ones(s.mpack, bool)
+    off[d] = False
+    assert np.array_equal(g["fcopy"][off], g["f"][off])
+    assert np.max(np.abs(g["fcopy"][d] - g["f"][d] - 1e-10 * np.arange(1, s.norbs + 1))) < 1e-14
     fc = f_ref.copy()
     c_ref, e_ref = oracle.eigenvectors(fc)
-    assert np.array_equal(g["fcopy"], fc)  # the in-place +i*1e-10 diagonal shift
     assert np.max(np.abs(g["eig"] - e_ref)) < 1e-9
     p_ref = oracle.density(c_ref, s.nclose, 2.0)
     assert np.max(np.abs(g["p"] - p_ref)) < 1e-9

@@ -1,0 +1,121 @@
+"""GPU parity of the MOZYME kernels (density_for_MOZYME, diagg1, diagg2)
+against MOPAC's own Fortran routines compiled from the reference sources
+(oracle/_ref, oracle/ref.py), on localized MOs of a converged poly-glycine
+helix, over a sequence of calls that exercises every branch of the saved
+state (new lists, reused lists, occupied energies, tiny/biglim, growth)."""
+import numpy as np
+import pytest
+
+from mopac_gpu_amd import geometry, hamiltonian, mozyme
+from oracle import oracle
+from oracle import ref as REF
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not REF.available(), reason="oracle/_ref not built (needs the reference sources)")]
+
+
+def _unpack(p, n):
+    m = np.zeros((n, n))
+    m[np.tril_indices(n)] = p
+    return m + m.T - np.diag(np.diag(m))
+
+
+@pytest.fixture(scope="module")
+def helix_lmos():
+    nat, xyz = geometry.glycine_helix(3, 3)
+    s = hamiltonian.build(nat, xyz)
+    o = oracle.iterate(s, scfcrt=1e-6)
+    n = s.norbs
+    return mozyme.from_scf(s, o["c"], _unpack(o["f"], n), _unpack(o["p"], n), s.nclose, keep=1e-3, rcut=6.0)
+
+
+def _same_lmos(a, b):
+    assert np.array_equal(a.ncf, b.ncf) and np.array_equal(a.nce, b.nce)
+    for k in range(a.nocc):
+        sl = slice(a.nncf[k], a.nncf[k] + a.ncf[k])
+        assert np.array_equal(a.icocc[sl], b.icocc[sl])
+    for k in range(a.nvir):
+        sl = slice(a.nnce[k], a.nnce[k] + a.nce[k])
+        assert np.array_equal(a.icvir[sl], b.icvir[sl])
+    assert np.max(np.abs(a.cocc - b.cocc)) < 1e-12
+    assert np.max(np.abs(a.cvir - b.cvir)) < 1e-12
+
+
+def test_density_for_mozyme_matches_reference(gpu_ctx, helix_lmos):
+    L = helix_lmos
+    print(f"helix LMOs: {L.numat} atoms, {L.nocc} occupied, {L.nvir} virtual", flush=True)
+    R = REF.Mozyme(L)
+    G = mozyme.GpuMozyme(gpu_ctx, L)
+    for mode in (0, -1, 1):
+        part = None if mode == 0 else L.p
+        pr = R.density(mode, part)
+        pg = G.density(mode, part)
+        assert np.max(np.abs(pg - pr)) < 1e-13 * max(1.0, np.max(np.abs(pr)))
+
+
+@pytest.mark.parametrize("fmo_cap", [None, 60])
+def test_diagg1_diagg2_sequence_matches_reference(gpu_ctx, helix_lmos, fmo_cap):
+    L = helix_lmos
+    R = REF.Mozyme(L, fmo_dim=fmo_cap)
+    G = mozyme.GpuMozyme(gpu_ctx, L, fmo_dim=fmo_cap)
+    for idiagg in range(1, 11):
+        print(f"idiagg {idiagg} cap {fmo_cap}", flush=True)
+        r = R.diagg1(L.fao, idiagg)
+        g = G.diagg1(L.fao, L.p, idiagg)
+        assert g["nij"] == r["nij"]
+        assert np.array_equal(g["ifmo"], r["ifmo"])
+        assert np.max(np.abs(g["fmo"] - r["fmo"]), initial=0.0) < 1e-12
+        assert np.max(np.abs(g["eigv"] - r["eigv"])) < 1e-12
+        assert g["tiny"] == pytest.approx(r["tiny"], rel=1e-12, abs=1e-300)
+        assert g["sumt"] == pytest.approx(r["sumt"], rel=1e-12, abs=1e-300)
+        sr = R.diagg2(r["nij"], idiagg)
+        sg = G.diagg2(idiagg)
+        assert sg == pytest.approx(sr, rel=1e-12, abs=1e-300)
+        a, b = R.get(), G.get()
+        assert np.max(np.abs(a.eigs - b.eigs)) < 1e-12
+        _same_lmos(a, b)
+
+
+def _idiag(L):
+    out = []
+    for a in range(L.numat):
+        for l1 in range(L.iorbs[a]):
+            out.append(L.nijbo[a, a] + l1 * (l1 + 1) // 2 + l1 + 1)
+    return np.array(out, dtype=np.int32)
+
+
+def test_helecz_matches_reference(gpu_ctx, helix_lmos):
+    import ctypes
+    from mopac_gpu_amd import lib as L_
+    L = helix_lmos
+    R = REF.Mozyme(L)
+    rng = np.random.default_rng(3)
+    h = L.fao - rng.uniform(0.0, 1.0, L.mpack)
+    e_ref = R.helecz(L.p, h, L.fao)
+    e = ctypes.c_double()
+    nij = np.asfortranarray(L.nijbo.astype(np.int32))
+    L_.check(L_.load().mhip_mozyme_helecz(gpu_ctx.h, L.numat, L_.ptr(L.nfirst.astype(np.int32)),
+                                          L_.ptr(L.nlast.astype(np.int32)), L_.ptr(nij), L.mpack, L_.ptr(L.p),
+                                          L_.ptr(h), L_.ptr(L.fao), ctypes.byref(e)))
+    assert e.value == pytest.approx(e_ref, rel=1e-13)
+
+
+@pytest.mark.parametrize("niter,three", [(2, 1), (3, 1), (6, 1), (7, 0)])
+def test_cnvgz_matches_reference(gpu_ctx, helix_lmos, niter, three):
+    import ctypes
+    from mopac_gpu_amd import lib as L_
+    L = helix_lmos
+    R = REF.Mozyme(L)
+    rng = np.random.default_rng(niter)
+    idiag = _idiag(L)
+    base = [L.p + 0.2 * rng.standard_normal(L.mpack), L.p.copy()] + \
+           [L.p[idiag - 1] + 0.1 * rng.standard_normal(L.norbs) for _ in range(3)]
+    ref_arrays = [b.copy() for b in base]
+    gpu_arrays = [np.ascontiguousarray(b.copy()) for b in base]
+    pm_ref = R.cnvgz(*ref_arrays, niter, idiag, three_point=bool(three))
+    pm = ctypes.c_double()
+    L_.check(L_.load().mhip_mozyme_cnvgz(gpu_ctx.h, L.norbs, L.mpack, *[L_.ptr(a) for a in gpu_arrays], niter,
+                                         L_.ptr(idiag), three, ctypes.byref(pm)))
+    assert pm.value == pm_ref
+    for a, b in zip(gpu_arrays, ref_arrays):
+        assert np.max(np.abs(a - b)) < 1e-13

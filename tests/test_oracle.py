@@ -218,3 +218,45 @@ def test_scf_energy_invariant_under_rigid_motion():
     assert e0["converged"] and e1["converged"]
     assert abs(e0["escf"] - e1["escf"]) < 1e-6
     assert np.max(np.abs(e0["eig"] - e1["eig"])) < 1e-6
+
+
+# ---- MOZYME: the reference routines driven through the harness -------------
+
+def _helix_lmos(rcut):
+    from mopac_gpu_amd import mozyme
+    nat, xyz = geometry.glycine_helix(2, 3)
+    s = hamiltonian.build(nat, xyz)
+    o = oracle.iterate(s, scfcrt=1e-6)
+    n = s.norbs
+    return s, o, mozyme.from_scf(s, o["c"], _unpack(o["f"], n), _unpack(o["p"], n), s.nclose, keep=1e-3,
+                                 rcut=rcut)
+
+
+@needs_ref
+def test_reference_mozyme_density_is_the_lmo_projector():
+    from mopac_gpu_amd import mozyme
+    s, o, L = _helix_lmos(6.0)
+    p = REF.Mozyme(L).density()
+    d = np.zeros((s.norbs, s.norbs))
+    for k in range(L.nocc):
+        x = mozyme.lmo_dense(L, k, True)
+        d += 2.0 * np.outer(x, x)
+    assert np.max(np.abs(p - mozyme.sparse_from_dense(d, L.nfirst, L.iorbs, L.nijbo, L.mpack))) < 1e-14
+
+
+@needs_ref
+def test_reference_helecz_equals_dense_helect_when_all_pairs_are_stored():
+    from mopac_gpu_amd import mozyme
+    s, o, L = _helix_lmos(1e3)
+    h = mozyme.sparse_from_dense(_unpack(s.h, s.norbs), L.nfirst, L.iorbs, L.nijbo, L.mpack)
+    e = REF.Mozyme(L).helecz(L.p, h, L.fao)
+    assert e == pytest.approx(oracle.helect(s.norbs, o["p"], s.h, o["f"]), rel=1e-13)
+
+
+def test_localized_orbitals_span_the_occupied_space():
+    from mopac_gpu_amd import mozyme
+    s, o, L = _helix_lmos(6.0)
+    c = o["c"][:, :s.nclose]
+    q = mozyme.pipek_mezey(c, L.nfirst, L.iorbs)
+    assert np.max(np.abs(q.T @ q - np.eye(s.nclose))) < 1e-12
+    assert np.max(np.abs(q @ q.T - c @ c.T)) < 1e-12
