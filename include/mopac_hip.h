@@ -130,6 +130,40 @@ typedef struct {
 int mhip_scf(mhip_ctx* ctx, const mhip_scf_opts* opts, mhip_scf_result* res, double* p,
              double* f, double* c, double* eig);
 
+/* ---- batched SCF: many small molecules in one captured graph --------------
+ * No single counterpart in the reference (it runs one molecule per job); each
+ * molecule follows iter (src/SCF/iter.F90:16-1107, RHF, no PULAY) with its
+ * control logic on the device.  Molecules up to 256 basis functions; the
+ * arrays of each molecule are the ones mhip_upload_system / mhip_scf take. */
+typedef struct {
+  int numat, norbs, nclose;
+  const int* nfirst;       /* 1-based */
+  const int* nlast;
+  long long n2elec;
+  const double* h;         /* packed */
+  const double* w;         /* hcore order */
+  const double* pdiag;     /* starting diagonal */
+  double enuclr, atheat;   /* eV, kcal/mol */
+  double scfcrt;           /* SCFCRT (default 1e-4 when <= 0) */
+  int itrmax;              /* default 2000 when <= 0 */
+} mhip_batch_molecule;
+
+typedef struct mhip_batch mhip_batch;
+int mhip_batch_create(int nmol, const mhip_batch_molecule* mols, mhip_batch** out);
+/* Eigensolver workgroup size per size class (n <= 32, 64, 128, 256):
+ * 64, 128, 256 or 512 (chosen by the autotuner, mopac_gpu_amd/batch.py). */
+int mhip_batch_set_tiles(mhip_batch* b, const int* tiles);
+/* Capture iters_per_graph iterations of the whole batch into one hipGraph
+ * and replay it (at most max_launches times) until every molecule is done. */
+int mhip_batch_run(mhip_batch* b, int iters_per_graph, int max_launches, double* seconds,
+                   int* n_converged);
+/* Heats of formation (kcal/mol, without geometry-only terms), iterations and
+ * convergence flags of every molecule; any pointer may be NULL. */
+int mhip_batch_results(mhip_batch* b, double* escf, int* niter, int* converged);
+/* Packed total density of molecule m. */
+int mhip_batch_density(mhip_batch* b, int m, double* p);
+void mhip_batch_destroy(mhip_batch* b);
+
 /* ---- MOZYME localized-MO kernels -------------------------------------------
  * The MOZYME_C module arrays, passed explicitly (C cannot see Fortran
  * modules), with the reference's conventions: atom numbers 1-based, offsets

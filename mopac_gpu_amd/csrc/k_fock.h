@@ -38,6 +38,23 @@ struct FockArgs {
 };
 
 void fock_build(hipStream_t s, const FockArgs& a);
+
+// ---- batched forms (one launch for many molecules) --------------------------
+struct BatchCnvg {
+  int n;
+  double* p;     // packed total density (new, extrapolated in place)
+  double* pold;  // packed previous density
+  double* p1;    // diagonal history (n)
+};
+// args: one FockArgs per molecule (device array); blk_mol/blk_atom map each
+// workgroup to (molecule, atom); mode/shift per molecule override the args.
+void fock_build_batch(hipStream_t s, const FockArgs* args, int nblk, const int* blk_mol, const int* blk_atom,
+                      const int* active, const int* mode, const double* shift);
+void helect_batch(hipStream_t s, const FockArgs* args, int nmol, int nrow_blk, const int* blk_mol,
+                  const int* blk_row, const int* active, const long long* orb_off, double* partial,
+                  double scale, double* ee);
+void cnvg_batch(hipStream_t s, const BatchCnvg* b, int nmol, int nrow_blk, const int* blk_mol,
+                const int* blk_row, const int* active, const int* niter, double* scal);
 void helect(hipStream_t s, int n, const double* pa, const double* h, const double* f, double scale,
             double* partial, double* out);
 void cnvg(hipStream_t s, int n, double* pnew, double* pold, double* p1, int niter, double* scal);

@@ -43,8 +43,9 @@ __device__ __forceinline__ double finit(const FockArgs& a, long long q, bool dia
   }
 }
 
-__global__ __launch_bounds__(256) void k_fock_rows(FockArgs a) {
-  const int i = blockIdx.x;
+// Row pass for atom i (one 256-thread workgroup); shared by the single-system
+// and the batched launches.
+__device__ void fock_rows_body(const FockArgs& a, const int i) {
   const int ni = a.sys.norb[i];
   const long long fi = a.sys.nf0[i];
   const double* __restrict__ P = a.p;
@@ -199,8 +200,10 @@ __global__ __launch_bounds__(256) void k_fock_rows(FockArgs a) {
   if (threadIdx.x < limi) a.jrow[(long long)i * 10 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ __launch_bounds__(256) void k_fock_cols(FockArgs a) {
-  const int j = blockIdx.x;
+__global__ __launch_bounds__(256) void k_fock_rows(FockArgs a) { fock_rows_body(a, blockIdx.x); }
+
+// Column pass for atom j (one 256-thread workgroup).
+__device__ void fock_cols_body(const FockArgs& a, const int j) {
   const int nj = a.sys.norb[j];
   const int lim = (nj == 4) ? 10 : 1;
   const long long fj = a.sys.nf0[j];
@@ -242,6 +245,8 @@ __global__ __launch_bounds__(256) void k_fock_cols(FockArgs a) {
     a.f[q] = finit(a, q, x == y) + a.jrow[(long long)j * 10 + cd] + t + g;
   }
 }
+
+__global__ __launch_bounds__(256) void k_fock_cols(FockArgs a) { fock_cols_body(a, blockIdx.x); }
 
 // ---- electronic energy -------------------------------------------------------
 __global__ __launch_bounds__(256) void k_helect_rows(int n, const double* __restrict__ p,
@@ -308,11 +313,9 @@ __device__ double block_max(double v, double* red) {
   return r;
 }
 
-__global__ __launch_bounds__(1024) void k_cnvg_diag(int n, double* __restrict__ pnew,
-                                                    double* __restrict__ pold,
-                                                    double* __restrict__ p1, int niter,
-                                                    double* __restrict__ scal) {
-  constexpr int NTH = 1024;
+template <int NTH>
+__device__ void cnvg_diag_body(int n, double* __restrict__ pnew, double* __restrict__ pold,
+                               double* __restrict__ p1, int niter, double* __restrict__ scal) {
   __shared__ double red[NTH];
   const double rhfuhf = 2.0;
   const double damp = (niter > 3) ? 0.05 : 1.0e10;
@@ -378,6 +381,13 @@ __global__ __launch_bounds__(1024) void k_cnvg_diag(int n, double* __restrict__ 
   }
 }
 
+__global__ __launch_bounds__(1024) void k_cnvg_diag(int n, double* __restrict__ pnew,
+                                                    double* __restrict__ pold,
+                                                    double* __restrict__ p1, int niter,
+                                                    double* __restrict__ scal) {
+  cnvg_diag_body<1024>(n, pnew, pold, p1, niter, scal);
+}
+
 __global__ void k_cnvg_offdiag(int n, double* __restrict__ pnew, double* __restrict__ pold,
                                const double* __restrict__ scal) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -389,6 +399,115 @@ __global__ void k_cnvg_offdiag(int n, double* __restrict__ pnew, double* __restr
   const double v = a + fac * (a - pold[q]);
   pold[q] = v;
   pnew[q] = v;
+}
+
+// ---- batched forms: one launch for many molecules --------------------------
+// blk_mol / blk_idx map each workgroup to (molecule, atom or row); molecules
+// whose flag in `active` is 0 (converged) do nothing.
+__global__ __launch_bounds__(256) void k_bfock_rows(const FockArgs* __restrict__ args,
+                                                    const int* __restrict__ blk_mol,
+                                                    const int* __restrict__ blk_idx,
+                                                    const int* __restrict__ active,
+                                                    const int* __restrict__ mode,
+                                                    const double* __restrict__ shift) {
+  const int m = blk_mol[blockIdx.x];
+  if (!active[m]) return;
+  FockArgs a = args[m];
+  a.init_mode = mode[m];
+  a.shift = shift[m];
+  fock_rows_body(a, blk_idx[blockIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_bfock_cols(const FockArgs* __restrict__ args,
+                                                    const int* __restrict__ blk_mol,
+                                                    const int* __restrict__ blk_idx,
+                                                    const int* __restrict__ active,
+                                                    const int* __restrict__ mode,
+                                                    const double* __restrict__ shift) {
+  const int m = blk_mol[blockIdx.x];
+  if (!active[m]) return;
+  FockArgs a = args[m];
+  a.init_mode = mode[m];
+  a.shift = shift[m];
+  fock_cols_body(a, blk_idx[blockIdx.x]);
+}
+
+// electronic energy: rows (blk_mol, blk_idx = row), then one sum per molecule
+__global__ __launch_bounds__(256) void k_bhelect_rows(const FockArgs* __restrict__ args,
+                                                      const int* __restrict__ blk_mol,
+                                                      const int* __restrict__ blk_idx,
+                                                      const int* __restrict__ active,
+                                                      const long long* __restrict__ orb_off,
+                                                      double* __restrict__ partial) {
+  const int m = blk_mol[blockIdx.x];
+  if (!active[m]) return;
+  const FockArgs& a = args[m];
+  const long long r = blk_idx[blockIdx.x];
+  const long long base = r * (r + 1) / 2;
+  double s = 0.0;
+  for (long long c = threadIdx.x; c <= r; c += 256) {
+    const long long q = base + c;
+    const double v = a.pa[q] * (a.h[q] + a.f[q]);
+    s += (c == r) ? 0.5 * v : v;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[orb_off[m] + r] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_bhelect_sum(const FockArgs* __restrict__ args,
+                                                     const int* __restrict__ active,
+                                                     const long long* __restrict__ orb_off,
+                                                     const double* __restrict__ partial, double scale,
+                                                     double* __restrict__ ee) {
+  const int m = blockIdx.x;
+  if (!active[m]) return;
+  const int n = args[m].sys.norbs;
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += partial[orb_off[m] + i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ee[m] = scale * red[0];
+}
+
+// cnvg: the diagonal pass per molecule, then the off-diagonal rows
+__global__ __launch_bounds__(256) void k_bcnvg_diag(const BatchCnvg* __restrict__ b,
+                                                    const int* __restrict__ active,
+                                                    const int* __restrict__ niter,
+                                                    double* __restrict__ scal) {
+  const int m = blockIdx.x;
+  if (!active[m]) return;
+  const BatchCnvg& c = b[m];
+  cnvg_diag_body<256>(c.n, c.p, c.pold, c.p1, niter[m], scal + 2 * m);
+}
+
+__global__ __launch_bounds__(256) void k_bcnvg_offdiag(const BatchCnvg* __restrict__ b,
+                                                       const int* __restrict__ blk_mol,
+                                                       const int* __restrict__ blk_idx,
+                                                       const int* __restrict__ active,
+                                                       const double* __restrict__ scal) {
+  const int m = blk_mol[blockIdx.x];
+  if (!active[m]) return;
+  const BatchCnvg& c = b[m];
+  const int r = blk_idx[blockIdx.x];
+  const double fac = scal[2 * m + 1];
+  for (int col = threadIdx.x; col < r; col += 256) {
+    const long long q = (long long)r * (r + 1) / 2 + col;
+    const double a = c.p[q];
+    const double v = a + fac * (a - c.pold[q]);
+    c.pold[q] = v;
+    c.p[q] = v;
+  }
 }
 
 __global__ void k_axpby_packed(long long len, double a, const double* __restrict__ x, double b,
@@ -423,6 +542,35 @@ void cnvg(hipStream_t s, int n, double* pnew, double* pold, double* p1, int nite
   hipLaunchKernelGGL(k_cnvg_diag, dim3(1), dim3(1024), 0, s, n, pnew, pold, p1, niter, scal);
   MHIP_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_cnvg_offdiag, dim3((n + 255) / 256, n), dim3(256), 0, s, n, pnew, pold, scal);
+  MHIP_LAUNCH_CHECK();
+}
+
+void fock_build_batch(hipStream_t s, const FockArgs* args, int nblk, const int* blk_mol, const int* blk_atom,
+                      const int* active, const int* mode, const double* shift) {
+  if (nblk <= 0) return;
+  hipLaunchKernelGGL(k_bfock_rows, dim3(nblk), dim3(256), 0, s, args, blk_mol, blk_atom, active, mode, shift);
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bfock_cols, dim3(nblk), dim3(256), 0, s, args, blk_mol, blk_atom, active, mode, shift);
+  MHIP_LAUNCH_CHECK();
+}
+
+void helect_batch(hipStream_t s, const FockArgs* args, int nmol, int nrow_blk, const int* blk_mol,
+                  const int* blk_row, const int* active, const long long* orb_off, double* partial,
+                  double scale, double* ee) {
+  if (nrow_blk <= 0) return;
+  hipLaunchKernelGGL(k_bhelect_rows, dim3(nrow_blk), dim3(256), 0, s, args, blk_mol, blk_row, active, orb_off,
+                     partial);
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bhelect_sum, dim3(nmol), dim3(256), 0, s, args, active, orb_off, partial, scale, ee);
+  MHIP_LAUNCH_CHECK();
+}
+
+void cnvg_batch(hipStream_t s, const BatchCnvg* b, int nmol, int nrow_blk, const int* blk_mol,
+                const int* blk_row, const int* active, const int* niter, double* scal) {
+  if (nmol <= 0) return;
+  hipLaunchKernelGGL(k_bcnvg_diag, dim3(nmol), dim3(256), 0, s, b, active, niter, scal);
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bcnvg_offdiag, dim3(nrow_blk), dim3(256), 0, s, b, blk_mol, blk_row, active, scal);
   MHIP_LAUNCH_CHECK();
 }
 

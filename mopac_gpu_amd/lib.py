@@ -101,6 +101,12 @@ SIGNATURES = {
                                    ctypes.POINTER(c_double)]),
     "mhip_mozyme_cnvgz": (c_int, [c_void_p, c_int, c_ll, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                   c_void_p, c_int, ctypes.POINTER(c_double)]),
+    "mhip_batch_create": (c_int, [c_int, c_void_p, ctypes.POINTER(c_void_p)]),
+    "mhip_batch_set_tiles": (c_int, [c_void_p, c_void_p]),
+    "mhip_batch_run": (c_int, [c_void_p, c_int, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_int)]),
+    "mhip_batch_results": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mhip_batch_density": (c_int, [c_void_p, c_int, c_void_p]),
+    "mhip_batch_destroy": (None, [c_void_p]),
     "mhip_kernel_stats": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhip_kernel_class_name": (ctypes.c_char_p, [c_int]),
     "mhip_kernel_timing": (c_int, [c_void_p, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_ll),

@@ -1,0 +1,46 @@
+"""GPU tests of the batched SCF (one captured graph for many molecules):
+every molecule's heat of formation, iteration count and density against the
+CPU oracle running the reference's loop on the same molecule."""
+import numpy as np
+import pytest
+
+from mopac_gpu_amd import batch as B
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_batch():
+    mols = B.workload(48, seed=11)
+    systems = B.build_systems(mols)
+    b = B.Batch(systems, scfcrt=1e-6, tiles=[64, 128, 256, 256])
+    yield systems, b
+    b.close()
+
+
+def test_batch_matches_oracle_per_molecule(small_batch):
+    systems, b = small_batch
+    sec, nconv = b.run(iters_per_graph=20, max_launches=20)
+    print(f"batch of {len(systems)}: {sec:.3f} s, {nconv} converged", flush=True)
+    escf, niter, conv = b.results()
+    assert conv.all()
+    for m in range(0, len(systems), 3):
+        s = systems[m]
+        o = oracle.iterate(s, scfcrt=1e-6)
+        assert o["converged"]
+        assert abs(escf[m] - o["escf"]) < 1e-4, (m, s.norbs, escf[m], o["escf"])
+        assert abs(int(niter[m]) - o["niter"]) <= 1, (m, niter[m], o["niter"])
+        assert np.max(np.abs(b.density(m) - o["p"])) < 1e-4
+
+
+def test_batch_is_replayable_and_tiles_do_not_change_results(small_batch):
+    systems, b = small_batch
+    b.set_tiles([64, 128, 256, 256])
+    b.run(iters_per_graph=20, max_launches=20)
+    e1, n1, _ = b.results()
+    b.set_tiles([128, 64, 128, 512])
+    b.run(iters_per_graph=20, max_launches=20)
+    e2, n2, _ = b.results()
+    assert np.max(np.abs(e1 - e2)) < 1e-6
+    assert np.array_equal(n1, n2)

@@ -184,3 +184,20 @@ def test_fortran_interface_structs_match_c_layout(tmp_path):
                     "-Wl,-rpath," + B.HERE, "-o", str(exe)], check=True, capture_output=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
     assert [int(x) for x in out] == [ctypes.sizeof(L.ScfOpts), ctypes.sizeof(L.ScfResult)]
+
+
+def test_batch_workload_spans_the_configured_sizes():
+    from mopac_gpu_amd import batch as B
+    mols = B.workload(200, seed=4)
+    sizes = []
+    for nat, xyz in mols:
+        nat = np.asarray(nat)
+        heavy = int(np.sum(nat > 1))
+        sizes.append(4 * heavy + (len(nat) - heavy))
+    sizes = np.array(sizes)
+    assert sizes.min() >= 6 and sizes.max() <= 180
+    alk = sizes[0::2]
+    assert np.all((alk - 2) % 6 == 0)            # CnH2n+2: 6n + 2 basis functions
+    wat = sizes[1::2]
+    assert np.all(wat % 6 == 0)                  # (H2O)k: 6k
+    assert B.workload(10, seed=4)[3][1].tolist() == mols[3][1].tolist()  # seeded
