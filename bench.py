@@ -44,7 +44,27 @@ def parse():
     ap.add_argument("--waters", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--precision", type=int, default=FAST_PRECISION, choices=(0, 1),
+                    help="fast form: 1 = split-BF16 contractions while far from convergence")
+    ap.add_argument("--fast-only", action="store_true",
+                    help="time only the fast form (no plain runs, no error report, no CPU leg): "
+                         "the command the kernel profiles are taken from")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="instead of the SCF, copy a MIB-MiB buffer 10 times (the known-size "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.arange(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(10):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "bytes_read_per_launch": n * 8,
+                      "bytes_written_per_launch": n * 8, "launches": 10}), flush=True)
 
 
 def main():
@@ -53,6 +73,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU path is a design "
                                    "(see DESIGN.md), so --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     rank = int(os.environ.get("RANK", "0"))
     import ctypes
     from mopac_gpu_amd import geometry, hamiltonian
@@ -72,35 +95,34 @@ def main():
 
     model = learned.load_default()
     frag = geometry.fragments(nat, xyz)
-    p_guess = None
     t_guess = 0.0
 
     def run_plain():
         return ctx.iter(scfcrt=1e-6, want=("p", "eig"))
 
     def run_fast():
-        nonlocal p_guess, t_guess
+        nonlocal t_guess
         t0 = time.perf_counter()
-        pin = None
-        if model is not None:
-            pin = model.predict_density(s, ctx)
+        # learned starting density, built on the device (descriptors, network,
+        # assembly); the standard guess when the system is not all water
+        have = model is not None and model.to_device(ctx, s) is not None
         t_guess = time.perf_counter() - t0
-        p_guess = pin
-        r = ctx.iter(scfcrt=1e-6, adaptive_eigen=True, precision=FAST_PRECISION, p_init=pin,
+        r = ctx.iter(scfcrt=1e-6, adaptive_eigen=True, precision=args.precision, start_from_guess=have,
                      pseudo_diag=2, block_start=2, fragment=frag, want=("p", "eig"))
         r["seconds"] += t_guess
         return r
 
-    # reference FP64 answer for the error report
-    ref = run_plain()
-    # plain-form timing
-    for _ in range(args.warmup):
-        run_plain()
+    # reference FP64 answer for the error report, and the plain-form timing
+    ref = None
     plain_t = []
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        r = run_plain()
-        plain_t.append(time.perf_counter() - t0)
+    if not args.fast_only:
+        ref = run_plain()
+        for _ in range(args.warmup):
+            run_plain()
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            run_plain()
+            plain_t.append(time.perf_counter() - t0)
     # accelerated-form timing (the metric)
     for _ in range(args.warmup):
         run_fast()
@@ -111,9 +133,9 @@ def main():
         fast_last = run_fast()
         fast_t.append(time.perf_counter() - t0)
     t_fast = float(np.mean(fast_t))
-    t_plain = float(np.mean(plain_t))
-    dhf = abs(fast_last["escf"] - ref["escf"])
-    dp = float(np.max(np.abs(fast_last["p"] - ref["p"])))
+    t_plain = float(np.mean(plain_t)) if plain_t else None
+    dhf = abs(fast_last["escf"] - ref["escf"]) if ref else None
+    dp = float(np.max(np.abs(fast_last["p"] - ref["p"]))) if ref else None
 
     # ---- roofline: dominant kernel of the measured (fast) form, HIP events on
     # the stream every kernel is launched on
@@ -143,7 +165,13 @@ def main():
         name = lib.mhip_kernel_class_name(i).decode()
         bound = "hbm" if "fock" in name else "mfma"
         entries.append((name, bound, float(kms[i]), int(kl[i]), float(kf[i]), float(kb[i])))
-    breakdown = {e[0]: {"ms": round(e[2], 3), "launches": int(e[3])} for e in entries if e[3] > 0}
+    def rate(e):
+        if e[2] <= 0:
+            return None
+        return round((e[5] / 1e9 if e[1] == "hbm" else e[4] / 1e12) / (e[2] * 1e-3), 2)
+
+    breakdown = {e[0]: {"ms": round(e[2], 3), "launches": int(e[3]),
+                        ("GB/s" if e[1] == "hbm" else "TFLOP/s"): rate(e)} for e in entries if e[3] > 0}
     name, bound, tot_ms, nl, flops, nbytes = max(entries, key=lambda e: e[2])
     avg_s = (tot_ms / max(nl, 1)) * 1e-3
     if bound == "hbm":
@@ -161,7 +189,7 @@ def main():
 
     # ---- CPU baseline: the FP64 restatement (MKL dsyevd) on host cores
     cpu = None
-    if rank == 0 and not args.no_cpu:
+    if rank == 0 and not args.no_cpu and not args.fast_only:
         from oracle import oracle
         olib = oracle.load()
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -185,16 +213,21 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (+bf16 split MFMA density while far from convergence)",
+        "dtype": ("f64 + bf16 (split-BF16 MFMA contractions while far from convergence; "
+                  "BF16-weight network for the starting density)" if args.precision == 1 else
+                  "f64 (BF16-weight network for the starting density)"),
         "data": "synthetic seeded (H2O)512 lattice cluster; PM7 integrals generated in-process",
         "config": {"workload": f"(H2O){k} PM7 RHF 1SCF SCFCRT=1e-6", "norbs": n, "nocc": s.nclose,
-                   "form": "mixed+learned" if model is not None else "mixed (no trained model found)"},
-        "plain_fp64_s": round(t_plain, 4),
-        "speedup_vs_plain": round(t_plain / t_fast, 3),
-        "iterations": {"plain": ref["niter"], "fast": fast_last["niter"],
-                       "plain_full_diag": ref["n_full_diag"], "fast_full_diag": fast_last["n_full_diag"],
-                       "plain_jacobi_sweeps": ref["jacobi_sweeps"],
-                       "fast_jacobi_sweeps": fast_last["jacobi_sweeps"]},
+                   "form": ("mixed+learned" if model is not None else "mixed (no trained model found)"),
+                   "precision": args.precision},
+        "plain_fp64_s": round(t_plain, 4) if t_plain else None,
+        "speedup_vs_plain": round(t_plain / t_fast, 3) if t_plain else None,
+        "iterations": {"plain": ref["niter"] if ref else None, "fast": fast_last["niter"],
+                       "plain_full_diag": ref["n_full_diag"] if ref else None,
+                       "fast_full_diag": fast_last["n_full_diag"],
+                       "plain_jacobi_sweeps": ref["jacobi_sweeps"] if ref else None,
+                       "fast_jacobi_sweeps": fast_last["jacobi_sweeps"],
+                       "fast_split_iterations": fast_last["n_split_iter"]},
         "error_vs_fp64": {"dHf_kcal": dhf, "max_dP": dp},
         "guess_s": round(t_guess, 4),
         "device": dev_name,

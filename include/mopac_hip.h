@@ -109,6 +109,8 @@ typedef struct {
                           fragment blocks of F (needs fragment and pdiag) */
   const int* fragment; /* optional atom -> fragment id (0-based, e.g. the molecules of a
                           cluster; <= 16 orbitals and an even electron count each) */
+  int start_from_guess;/* 1 = start from the density mhip_guess_water left on the device
+                          (overrides p_init and pdiag) */
 } mhip_scf_opts;
 
 typedef struct {
@@ -295,6 +297,29 @@ int mhip_mlp_gradients(mhip_mlp* mlp, int batch, const float* x, const float* t,
                        float* loss);
 int mhip_mlp_train_step(mhip_mlp* mlp, int batch, const float* x, const float* t, float lr,
                         float* loss);
+
+/* Learned starting density of a water cluster, built on the device: pair
+ * descriptors, the network and the packed-density assembly all run in HBM and
+ * the density stays there for the next mhip_scf with start_from_guess = 1.
+ * The system must be uploaded and made of nw waters (O: 4 orbitals, H: 1).
+ * waters: nw x 3 atom indices (O, H1, H2; 0-based); pairs (pair_a[m] >=
+ * pair_b[m]): every atom with itself, the atoms of one molecule, and every
+ * pair closer than 4 A (other blocks start at zero).  mlp: 96 -> ... -> 16.
+ * x_out (npairs x 96, normalized descriptors) and y_out (npairs x 16, raw
+ * network outputs) are filled when not NULL.  No counterpart in the
+ * reference; the density plays the role of an OLDENS start
+ * (src/SCF/iter.F90:151-159). */
+typedef struct {
+  const float* x_mean; /* 96 */
+  const float* x_std;  /* 96 */
+  const float* y_mean; /* 8 pair types x 16 */
+  const float* y_std;  /* 8 x 16 */
+} mhip_guess_model;
+int mhip_guess_water(mhip_ctx* ctx, mhip_mlp* mlp, const mhip_guess_model* model, const int* nat,
+                     const double* xyz, int nw, const int* waters, int npairs, const int* pair_a,
+                     const int* pair_b, float* x_out, float* y_out);
+/* Copy the device guess (mpack) to the host. */
+int mhip_guess_density(mhip_ctx* ctx, double* p);
 
 /* ---- diagnostics / tuning --------------------------------------------------
  * No counterpart in the reference.  kernel_variant: 1 = FP64 MFMA tiles

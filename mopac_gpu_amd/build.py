@@ -27,6 +27,7 @@ SOURCES = [
     "k_precision.hip",
     "k_mlp.hip",
     "k_batch.hip",
+    "k_guess.hip",
     "k_mozyme.hip",
     "scf_driver.hip",
     "dropin.hip",

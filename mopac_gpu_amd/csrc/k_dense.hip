@@ -1,6 +1,8 @@
 // Dense FP64 building blocks: tiled GEMM, SYRK into MOPAC packed storage,
 // packed <-> dense conversion, the packed Pulay commutator, linear
 // combinations and deterministic dot products.
+#include <cstdint>
+
 #include "common.h"
 #include "mfma.h"
 
@@ -239,6 +241,122 @@ __global__ __launch_bounds__(NT) void k_gemm_mfma(int M, int N, int K, double al
   });
 }
 
+// Large-tile FP64 MFMA GEMM for shapes that divide the tile: BM x BN per
+// workgroup, four waves in 2 x 2 each owning (BM/2) x (BN/2) as 16 x 16 MFMA
+// blocks.  k advances in chunks of 16 that are double-buffered in LDS; the
+// next chunk is fetched from global memory into registers (16-byte loads)
+// while the current one is multiplied, so one barrier per chunk.  The product
+// is formed transposed (the MFMA A operand is taken from B, the B operand
+// from A), which puts each lane's outputs down a column of C: stores are
+// coalesced along the column-major rows.  LDS rows are padded to a stride of
+// 16 (mod 32) doubles so a half-wave's 16-lane groups hit disjoint banks.
+// Requires M % BM == 0, N % BN == 0, K % 16 == 0, K > 0, 16-byte aligned a, b
+// and even lda, ldb (checked by the launcher).
+constexpr int BK = 16;
+template <int BM, int BN, bool TA, bool TBT>
+__global__ __launch_bounds__(256) void k_gemm_big(int M, int N, int K, double alpha,
+                                                  const double* __restrict__ a, int lda,
+                                                  const double* __restrict__ b, int ldb, double beta,
+                                                  double* __restrict__ c, int ldc) {
+  constexpr int SA = BM + 16, SBn = BN + 16;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int NA2 = BM * BK / 2 / 256, NB2 = BN * BK / 2 / 256;
+  static_assert(NA2 >= 1 && NB2 >= 1 && FM >= 1 && FN >= 1, "tile too small");
+  __shared__ double As[2][BK][SA];
+  __shared__ double Bs[2][BK][SBn];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int wm = (w >> 1) * WM, wn = (w & 1) * WN;
+  double2 ra[NA2], rb[NB2];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < NA2; ++r) {
+      const int idx = tid + 256 * r;
+      const double* src;
+      if (!TA) src = a + (size_t)(m0 + 2 * (idx % (BM / 2))) + (size_t)(k0 + idx / (BM / 2)) * lda;
+      else src = a + (size_t)(k0 + 2 * (idx % (BK / 2))) + (size_t)(m0 + idx / (BK / 2)) * lda;
+      ra[r] = *reinterpret_cast<const double2*>(src);
+    }
+#pragma unroll
+    for (int r = 0; r < NB2; ++r) {
+      const int idx = tid + 256 * r;
+      const double* src;
+      if (!TBT) src = b + (size_t)(k0 + 2 * (idx % (BK / 2))) + (size_t)(n0 + idx / (BK / 2)) * ldb;
+      else src = b + (size_t)(n0 + 2 * (idx % (BN / 2))) + (size_t)(k0 + idx / (BN / 2)) * ldb;
+      rb[r] = *reinterpret_cast<const double2*>(src);
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < NA2; ++r) {
+      const int idx = tid + 256 * r;
+      if (!TA) {
+        const int mm = 2 * (idx % (BM / 2)), kk = idx / (BM / 2);
+        As[buf][kk][mm] = ra[r].x;
+        As[buf][kk][mm + 1] = ra[r].y;
+      } else {
+        const int kk = 2 * (idx % (BK / 2)), mm = idx / (BK / 2);
+        As[buf][kk][mm] = ra[r].x;
+        As[buf][kk + 1][mm] = ra[r].y;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NB2; ++r) {
+      const int idx = tid + 256 * r;
+      if (!TBT) {
+        const int kk = 2 * (idx % (BK / 2)), nn = idx / (BK / 2);
+        Bs[buf][kk][nn] = rb[r].x;
+        Bs[buf][kk + 1][nn] = rb[r].y;
+      } else {
+        const int nn = 2 * (idx % (BN / 2)), kk = idx / (BN / 2);
+        Bs[buf][kk][nn] = rb[r].x;
+        Bs[buf][kk][nn + 1] = rb[r].y;
+      }
+    }
+  };
+  d4_t acc[FN][FM];
+#pragma unroll
+  for (int j = 0; j < FN; ++j)
+#pragma unroll
+    for (int i = 0; i < FM; ++i) acc[j][i] = d4_t{0.0, 0.0, 0.0, 0.0};
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  const int nk = K / BK;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      double af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[i] = As[cur][ks + lk][wm + 16 * i + li];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = Bs[cur][ks + lk][wn + 16 * j + li];
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int i = 0; i < FM; ++i) acc[j][i] = mfma16x16x4(bf[j], af[i], acc[j][i]);
+    }
+    if (kt + 1 < nk) lstore(cur ^ 1);
+    __syncthreads();
+  }
+  // lane holds C(m0 + wm + 16 i + li, n0 + wn + 16 j + lk + 4 t) in acc[j][i][t]
+#pragma unroll
+  for (int j = 0; j < FN; ++j)
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int gm = m0 + wm + 16 * i + li, gn = n0 + wn + 16 * j + lk + 4 * t;
+        double* dst = c + (size_t)gm + (size_t)gn * ldc;
+        const double v = acc[j][i][t];
+        *dst = (beta == 0.0) ? alpha * v : fma(beta, *dst, alpha * v);
+      }
+}
+
 __global__ __launch_bounds__(NT) void k_syrk_packed_mfma(int n, int k, double occ, const double* a,
                                                          int lda, double* p) {
   __shared__ double As[TK][TB + 1];
@@ -278,12 +396,55 @@ inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 int g_kernel_variant = 1;  // 1: FP64 MFMA tiles, 0: vector-FMA tiles
 
+template <int BM, int BN>
+static void gemm_big_dispatch(hipStream_t s, bool TA, bool TBT, int m, int n, int k, double alpha,
+                              const double* a, int lda, const double* b, int ldb, double beta, double* c,
+                              int ldc) {
+  const dim3 grid(n / BN, m / BM);
+  if (!TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, false>), grid, dim3(256), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  else if (TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, true, false>), grid, dim3(256), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  else if (!TA && TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, true>), grid, dim3(256), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  else hipLaunchKernelGGL((k_gemm_big<BM, BN, true, true>), grid, dim3(256), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  MHIP_LAUNCH_CHECK();
+}
+
+// The large-tile kernel when the shape divides one of its tiles; the tile is
+// the one with the least estimated time: rounds of 256 workgroups (one per
+// CU) times the tile's work over its relative efficiency.
+static bool gemm_big_launch(hipStream_t s, bool TA, bool TBT, int m, int n, int k, double alpha,
+                            const double* a, int lda, const double* b, int ldb, double beta, double* c,
+                            int ldc) {
+  if (k < BK || k % BK != 0 || (lda & 1) || (ldb & 1)) return false;
+  if ((reinterpret_cast<uintptr_t>(a) & 15) || (reinterpret_cast<uintptr_t>(b) & 15)) return false;
+  struct Cand { int bm, bn; double eff; };
+  const Cand cands[3] = {{128, 128, 1.0}, {128, 64, 0.85}, {64, 64, 0.6}};
+  int best = -1;
+  double best_cost = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    const Cand& q = cands[i];
+    if (m % q.bm || n % q.bn) continue;
+    const long long tiles = (long long)(m / q.bm) * (n / q.bn);
+    const double cost = (double)((tiles + 255) / 256) * q.bm * q.bn / q.eff;
+    if (best < 0 || cost < best_cost) {
+      best = i;
+      best_cost = cost;
+    }
+  }
+  if (best < 0) return false;
+  if (best == 0) gemm_big_dispatch<128, 128>(s, TA, TBT, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  else if (best == 1) gemm_big_dispatch<128, 64>(s, TA, TBT, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  else gemm_big_dispatch<64, 64>(s, TA, TBT, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  return true;
+}
+
 static void gemm_launch(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha,
                         const double* a, int lda, const double* b, int ldb, double beta, double* c,
                         int ldc) {
   if (m <= 0 || n <= 0) return;
   dim3 grid(cdiv(n, TB), cdiv(m, TB));
   const bool TA = (ta == 'T' || ta == 't'), TBT = (tb == 'T' || tb == 't');
+  if (g_kernel_variant == 1 && gemm_big_launch(s, TA, TBT, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc))
+    return;
   if (g_kernel_variant == 1) {
     if (!TA && !TBT) hipLaunchKernelGGL((k_gemm_mfma<false, false>), grid, dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
     else if (TA && !TBT) hipLaunchKernelGGL((k_gemm_mfma<true, false>), grid, dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);

@@ -6,6 +6,10 @@ namespace mhip {
 
 struct JacobiPlan {
   int n = 0, npad = 0, nb = 0, npairs = 0;
+  int nsteps = 0;        // steps per sweep
+  bool grouped = false;  // pairs only within [0, split) and [split, n)
+  int split = 0;
+  int npo = 0;  // pairs of the first group in every step (npairs when not grouped)
   int max_inner = 2;  // inner Jacobi sweeps per 64x64 pair problem
   // Optional HIP-event timing of the solve and A-update launches.
   bool timing = false;
@@ -13,13 +17,16 @@ struct JacobiPlan {
   long long n_solve = 0, n_apply = 0, n_applyc = 0;
   long long active_tiles = 0, active_pairs = 0;  // summed over timed launches
   std::vector<hipEvent_t> ev;
-  DevBuf pairs;  // (nb-1) steps x npairs x 2 block indices
+  DevBuf pairs;  // nsteps x npairs x 2 block indices
   DevBuf qbuf;   // npairs x 64 x 64 rotations of the current step
   DevBuf part;   // reduction scratch
   DevBuf perm;   // sort permutation
   DevBuf active; // per pair: 1 when Q != I this step
   DevBuf counts; // active pairs per step (timing mode)
-  void build(int n);
+  // split > 0 (a multiple of 32 leaving an even number of blocks on both
+  // sides): rotations only within the leading split columns and within the
+  // rest, for a matrix already block-diagonal in that partition.
+  void build(int n, int split = 0);
   void release() {
     pairs.release(); qbuf.release(); part.release(); perm.release(); active.release();
     counts.release();

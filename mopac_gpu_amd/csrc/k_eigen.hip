@@ -40,14 +40,40 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// Rotation k of round `round` from the current submatrix (cyclic Jacobi
+// angle, the smaller of the two solutions).
+__device__ __forceinline__ void pair_rotation(const double (*S)[LDSW], int round, int k, int& p, int& q,
+                                              double& c, double& sn) {
+  rr_pair(round, k, p, q);
+  if (p > q) { const int t = p; p = q; q = t; }
+  const double app = S[p][p], aqq = S[q][q], apq = S[p][q];
+  c = 1.0;
+  sn = 0.0;
+  if (fabs(apq) > 1.0e-300 && fabs(apq) > 1.0e-18 * sqrt(fabs(app * aqq))) {
+    const double theta = (aqq - app) / (2.0 * apq);
+    double t;
+    if (fabs(theta) > 1.0e150) t = 0.5 / theta;
+    else t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+    c = 1.0 / sqrt(t * t + 1.0);
+    sn = t * c;
+  }
+}
+
 // One workgroup per block pair: load the 64x64 submatrix, run cyclic Jacobi
 // in parallel (round-robin) order until converged or max_inner sweeps, and
 // write the accumulated rotation Q.  A pair already diagonal to `skip_abs`
 // is marked inactive (Q = I) and skipped by the update kernels.
+//
+// A round: the first wave computes the round's 32 disjoint rotations into
+// LDS; then the 1024 threads each own one 2x2 block (row pair k1, column pair
+// k2) of the submatrix and apply rotation k1 to its rows and k2 to its
+// columns in one pass, in place (no other thread touches the block), and
+// rotate two rows of Q's columns p(k2), q(k2).  Two barriers per round.
 __global__ __launch_bounds__(SOLVE_THREADS) void k_jac_solve(
     const double* __restrict__ A, int lda, int nreal, const int* __restrict__ pairs,
     double* __restrict__ Qbuf, int* __restrict__ active, double tol, double skip_abs,
     int max_inner) {
+  static_assert(SOLVE_THREADS == (SB / 2) * (SB / 2), "one thread per 2x2 block");
   __shared__ double S[SB][LDSW];
   __shared__ double Q[SB][LDSW];
   __shared__ double rc[32], rs[32];
@@ -56,6 +82,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_jac_solve(
   const int pidx = blockIdx.x;
   const int I = pairs[2 * pidx], J = pairs[2 * pidx + 1];
   const int tid = threadIdx.x;
+  const int k1 = tid >> 5, k2 = tid & 31;
   for (int e = tid; e < SB * SB; e += SOLVE_THREADS) {
     const int r = e % SB, c = e / SB;
     S[r][c] = A[(size_t)gidx_dev(I, J, r) + (size_t)gidx_dev(I, J, c) * lda];
@@ -95,42 +122,28 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_jac_solve(
     for (int round = 0; round < 63; ++round) {
       if (tid < 32) {
         int p, q;
-        rr_pair(round, tid, p, q);
-        if (p > q) { const int t = p; p = q; q = t; }
-        const double app = S[p][p], aqq = S[q][q], apq = S[p][q];
-        double c = 1.0, sn = 0.0;
-        if (fabs(apq) > 1.0e-300 && fabs(apq) > 1.0e-18 * sqrt(fabs(app * aqq))) {
-          const double theta = (aqq - app) / (2.0 * apq);
-          double t;
-          if (fabs(theta) > 1.0e150) t = 0.5 / theta;
-          else t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-          c = 1.0 / sqrt(t * t + 1.0);
-          sn = t * c;
-        }
+        double c, sn;
+        pair_rotation(S, round, tid, p, q, c, sn);
         rc[tid] = c; rs[tid] = sn; rp[tid] = p; rq[tid] = q;
       }
       __syncthreads();
-      // rows of S: 32 rotations x 64 columns
-      for (int e = tid; e < 32 * SB; e += SOLVE_THREADS) {
-        const int k = e / SB, col = e % SB;
-        const int p = rp[k], q = rq[k];
-        const double c = rc[k], sn = rs[k];
-        const double x = S[p][col], y = S[q][col];
-        S[p][col] = c * x - sn * y;
-        S[q][col] = sn * x + c * y;
-      }
-      __syncthreads();
-      // columns of S and of Q
-      for (int e = tid; e < 32 * SB; e += SOLVE_THREADS) {
-        const int k = e / SB, row = e % SB;
-        const int p = rp[k], q = rq[k];
-        const double c = rc[k], sn = rs[k];
-        const double x = S[row][p], y = S[row][q];
-        S[row][p] = c * x - sn * y;
-        S[row][q] = sn * x + c * y;
-        const double u = Q[row][p], v = Q[row][q];
-        Q[row][p] = c * u - sn * v;
-        Q[row][q] = sn * u + c * v;
+      const int p1 = rp[k1], q1 = rq[k1], p2 = rp[k2], q2 = rq[k2];
+      const double c1 = rc[k1], s1 = rs[k1], c2 = rc[k2], s2 = rs[k2];
+      const double xpp = S[p1][p2], xpq = S[p1][q2], xqp = S[q1][p2], xqq = S[q1][q2];
+      // rows p1, q1 (rotation k1), then columns p2, q2 (rotation k2)
+      const double rpp = c1 * xpp - s1 * xqp, rpq = c1 * xpq - s1 * xqq;
+      const double rqp = s1 * xpp + c1 * xqp, rqq = s1 * xpq + c1 * xqq;
+      S[p1][p2] = c2 * rpp - s2 * rpq;
+      S[p1][q2] = s2 * rpp + c2 * rpq;
+      S[q1][p2] = c2 * rqp - s2 * rqq;
+      S[q1][q2] = s2 * rqp + c2 * rqq;
+      // Q <- Q J(k2) on rows 2 k1, 2 k1 + 1
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int row = 2 * k1 + h;
+        const double u = Q[row][p2], v = Q[row][q2];
+        Q[row][p2] = c2 * u - s2 * v;
+        Q[row][q2] = s2 * u + c2 * v;
       }
       __syncthreads();
     }
@@ -147,18 +160,36 @@ __device__ __forceinline__ int gidx(int pairI, int pairJ, int local) {
   return (local < BS) ? pairI * BS + local : pairJ * BS + (local - BS);
 }
 
+// Tile t -> pair indices P1 <= P2.  The pairs [0, npo) and [npo, npairs) form
+// two groups (npo == npairs: one group); tiles across the groups are not
+// enumerated, since the grouped schedule never couples them.
+__device__ __forceinline__ void tri_of(int t, int& P1, int& P2) {
+  P2 = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((P2 + 1) * (P2 + 2) / 2 <= t) ++P2;
+  while (P2 * (P2 + 1) / 2 > t) --P2;
+  P1 = t - P2 * (P2 + 1) / 2;
+}
+__device__ __forceinline__ void tile_of(int t, int npairs, int npo, int& P1, int& P2) {
+  (void)npairs;
+  const int t0 = npo * (npo + 1) / 2;
+  if (t < t0) {
+    tri_of(t, P1, P2);
+  } else {
+    tri_of(t - t0, P1, P2);
+    P1 += npo;
+    P2 += npo;
+  }
+}
+
 // A(rows of pair P1, cols of pair P2) <- Q1^T A Q2 for P1 <= P2 (the mirror
 // tile is written as the transpose, keeping A exactly symmetric).
 __global__ __launch_bounds__(256) void k_jac_apply_a(double* __restrict__ A, int lda, int npairs,
-                                                     const int* __restrict__ pairs,
+                                                     int npo, const int* __restrict__ pairs,
                                                      const double* __restrict__ Qbuf,
                                                      const int* __restrict__ active) {
   // blockIdx.x enumerates (P1 <= P2)
-  const int t = blockIdx.x;
-  int P2 = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while ((P2 + 1) * (P2 + 2) / 2 <= t) ++P2;
-  while (P2 * (P2 + 1) / 2 > t) --P2;
-  const int P1 = t - P2 * (P2 + 1) / 2;
+  int P1, P2;
+  tile_of(blockIdx.x, npairs, npo, P1, P2);
   const bool a1 = active[P1] != 0, a2 = active[P2] != 0;
   if (!a1 && !a2) return;
   __shared__ double At[SB][LDSW];
@@ -283,14 +314,12 @@ __global__ __launch_bounds__(256) void k_jac_apply_c(double* __restrict__ C, int
 
 // MFMA versions of the two update kernels (same data flow as above).
 __global__ __launch_bounds__(256) void k_jac_apply_a_mfma(double* __restrict__ A, int lda,
-                                                          int npairs, const int* __restrict__ pairs,
+                                                          int npairs, int npo,
+                                                          const int* __restrict__ pairs,
                                                           const double* __restrict__ Qbuf,
                                                           const int* __restrict__ active) {
-  const int t = blockIdx.x;
-  int P2 = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while ((P2 + 1) * (P2 + 2) / 2 <= t) ++P2;
-  while (P2 * (P2 + 1) / 2 > t) --P2;
-  const int P1 = t - P2 * (P2 + 1) / 2;
+  int P1, P2;
+  tile_of(blockIdx.x, npairs, npo, P1, P2);
   const bool a1 = active[P1] != 0, a2 = active[P2] != 0;
   if (!a1 && !a2) return;
   __shared__ double At[SB][LDSW];
@@ -363,14 +392,16 @@ __global__ __launch_bounds__(256) void k_jac_apply_c_mfma(double* __restrict__ C
 }
 
 // max |A_ij| (i != j) and max |A_ii| over the leading n x n block.
+// split > 0: only entries inside [0, split) x [0, split) and its complement
+// count (the grouped schedule never rotates across the two groups)
 __global__ __launch_bounds__(256) void k_offmax_partial(int n, const double* __restrict__ A,
-                                                        int lda, double* __restrict__ part) {
+                                                        int lda, int split, double* __restrict__ part) {
   const int j = blockIdx.x;
   double off = 0.0, dg = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) {
     const double v = fabs(A[(size_t)i + (size_t)j * lda]);
     if (i == j) dg = v;
-    else off = fmax(off, v);
+    else if (split <= 0 || ((i < split) == (j < split))) off = fmax(off, v);
   }
   __shared__ double r1[256], r2[256];
   r1[threadIdx.x] = off;
@@ -413,11 +444,16 @@ __global__ __launch_bounds__(1024) void k_offmax_final(int n, const double* __re
   }
 }
 
-__global__ void k_count_active(int npairs, const int* __restrict__ active, int* __restrict__ out) {
+// active pairs of the step in each group: out[0] over [0, npo), out[1] over [npo, npairs)
+__global__ void k_count_active(int npairs, int npo, const int* __restrict__ active, int* __restrict__ out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    int a = 0;
-    for (int i = 0; i < npairs; ++i) a += active[i] != 0;
-    *out = a;
+    int a = 0, b = 0;
+    for (int i = 0; i < npairs; ++i) {
+      if (i < npo) a += active[i] != 0;
+      else b += active[i] != 0;
+    }
+    out[0] = a;
+    out[1] = b;
   }
 }
 
@@ -483,28 +519,52 @@ __global__ __launch_bounds__(256) void k_phase_lock(int n, double* __restrict__ 
 
 }  // namespace
 
-void JacobiPlan::build(int n_in) {
+void JacobiPlan::build(int n_in, int split) {
   n = n_in;
   npad = ((n + SB - 1) / SB) * SB;
   nb = npad / BS;
   npairs = nb / 2;
-  std::vector<int> h((size_t)(nb - 1) * npairs * 2);
-  // circle method on nb blocks
-  for (int step = 0; step < nb - 1; ++step)
-    for (int k = 0; k < npairs; ++k) {
-      auto pos = [&](int slot) { return slot == 0 ? 0 : 1 + ((slot - 1 + step) % (nb - 1)); };
-      int a = pos(k), b = pos(nb - 1 - k);
-      if (a > b) std::swap(a, b);
-      h[((size_t)step * npairs + k) * 2] = a;
-      h[((size_t)step * npairs + k) * 2 + 1] = b;
+  // circle method over the blocks [b0, b0 + m): round r, pair k
+  auto circle = [](int b0, int m, int r, int k, int& a, int& b) {
+    auto pos = [&](int slot) { return slot == 0 ? 0 : 1 + ((slot - 1 + r) % (m - 1)); };
+    a = b0 + pos(k);
+    b = b0 + pos(m - 1 - k);
+    if (a > b) std::swap(a, b);
+  };
+  const int nbo = (split > 0 && split % BS == 0) ? split / BS : 0;
+  const int nbv = nb - nbo;
+  grouped = nbo >= 2 && nbv >= 2 && nbo % 2 == 0 && nbv % 2 == 0;
+  this->split = grouped ? split : 0;
+  npo = grouped ? nbo / 2 : npairs;
+  std::vector<int> h;
+  if (grouped) {
+    // two independent problems (blocks [0, nbo) and [nbo, nb)) in every
+    // step; the smaller one repeats its rounds while the larger finishes
+    nsteps = std::max(nbo, nbv) - 1;
+    h.resize((size_t)nsteps * npairs * 2);
+    for (int step = 0; step < nsteps; ++step) {
+      int k = 0;
+      for (int q = 0; q < nbo / 2; ++q, ++k)
+        circle(0, nbo, step % (nbo - 1), q, h[((size_t)step * npairs + k) * 2],
+               h[((size_t)step * npairs + k) * 2 + 1]);
+      for (int q = 0; q < nbv / 2; ++q, ++k)
+        circle(nbo, nbv, step % (nbv - 1), q, h[((size_t)step * npairs + k) * 2],
+               h[((size_t)step * npairs + k) * 2 + 1]);
     }
+  } else {
+    nsteps = nb - 1;
+    h.resize((size_t)nsteps * npairs * 2);
+    for (int step = 0; step < nsteps; ++step)
+      for (int k = 0; k < npairs; ++k)
+        circle(0, nb, step, k, h[((size_t)step * npairs + k) * 2], h[((size_t)step * npairs + k) * 2 + 1]);
+  }
   pairs.ensure(h.size() * sizeof(int));
   MHIP_CHECK(hipMemcpy(pairs.ptr, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
   qbuf.ensure((size_t)npairs * SB * SB * sizeof(double));
   part.ensure((size_t)2 * npad * sizeof(double) + 64);
   perm.ensure((size_t)npad * sizeof(int));
   active.ensure((size_t)npairs * sizeof(int));
-  counts.ensure((size_t)nb * sizeof(int));
+  counts.ensure((size_t)2 * nb * sizeof(int));
 }
 
 void jacobi_set_identity(hipStream_t s, int npad, double* C, int ldc) {
@@ -518,7 +578,7 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
   int done = 0;
   for (int sw = 0; sw <= max_sweeps; ++sw) {
     hipLaunchKernelGGL(k_offmax_partial, dim3(plan.n), dim3(256), 0, s, plan.n, A, lda,
-                       plan.part.as<double>());
+                       plan.grouped ? plan.split : 0, plan.part.as<double>());
     MHIP_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_offmax_final, dim3(1), dim3(1024), 0, s, plan.n, plan.part.as<double>(),
                        dev_scal);
@@ -533,12 +593,14 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
       if (sweeps_done) *sweeps_done = done;
       return 1;
     }
-    const int nsteps = plan.nb - 1;
+    const int nsteps = plan.nsteps;
     if (plan.timing && plan.ev.size() < (size_t)(6 * nsteps)) {
       for (auto e : plan.ev) (void)hipEventDestroy(e);
       plan.ev.assign(6 * nsteps, nullptr);
       for (auto& e : plan.ev) MHIP_CHECK(hipEventCreate(&e));
     }
+    const int npv = plan.npairs - plan.npo;
+    const int ntiles = plan.npo * (plan.npo + 1) / 2 + npv * (npv + 1) / 2;
     for (int step = 0; step < nsteps; ++step) {
       const int* pr = plan.pairs.as<int>() + (size_t)step * plan.npairs * 2;
       if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 0], s));
@@ -548,14 +610,13 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
       MHIP_LAUNCH_CHECK();
       if (plan.timing) {
         MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 1], s));
-        hipLaunchKernelGGL(k_count_active, dim3(1), dim3(64), 0, s, plan.npairs,
-                           plan.active.as<int>(), plan.counts.as<int>() + step);
+        hipLaunchKernelGGL(k_count_active, dim3(1), dim3(64), 0, s, plan.npairs, plan.npo,
+                           plan.active.as<int>(), plan.counts.as<int>() + 2 * step);
       }
       if (g_kernel_variant == 1) {
         if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 2], s));
-        hipLaunchKernelGGL(k_jac_apply_a_mfma, dim3(plan.npairs * (plan.npairs + 1) / 2), dim3(256),
-                           0, s, A, lda, plan.npairs, pr, plan.qbuf.as<double>(),
-                           plan.active.as<int>());
+        hipLaunchKernelGGL(k_jac_apply_a_mfma, dim3(ntiles), dim3(256), 0, s, A, lda, plan.npairs,
+                           plan.npo, pr, plan.qbuf.as<double>(), plan.active.as<int>());
         MHIP_LAUNCH_CHECK();
         if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 3], s));
         if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 4], s));
@@ -563,8 +624,8 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
                            C, lda, plan.npad, pr, plan.qbuf.as<double>(), plan.active.as<int>());
         if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 5], s));
       } else {
-        hipLaunchKernelGGL(k_jac_apply_a, dim3(plan.npairs * (plan.npairs + 1) / 2), dim3(256), 0,
-                           s, A, lda, plan.npairs, pr, plan.qbuf.as<double>(), plan.active.as<int>());
+        hipLaunchKernelGGL(k_jac_apply_a, dim3(ntiles), dim3(256), 0, s, A, lda, plan.npairs, plan.npo,
+                           pr, plan.qbuf.as<double>(), plan.active.as<int>());
         MHIP_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_jac_apply_c, dim3(plan.npad / SB, plan.npairs), dim3(256), 0, s, C,
                            lda, plan.npad, pr, plan.qbuf.as<double>(), plan.active.as<int>());
@@ -573,12 +634,13 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
     }
     if (plan.timing && g_kernel_variant == 1) {
       MHIP_CHECK(hipStreamSynchronize(s));
-      std::vector<int> act(nsteps);
-      MHIP_CHECK(hipMemcpy(act.data(), plan.counts.ptr, sizeof(int) * nsteps, hipMemcpyDeviceToHost));
+      std::vector<int> act(2 * nsteps);
+      MHIP_CHECK(hipMemcpy(act.data(), plan.counts.ptr, sizeof(int) * 2 * nsteps, hipMemcpyDeviceToHost));
       for (int step = 0; step < nsteps; ++step) {
-        const long long u = plan.npairs - act[step];  // inactive pairs
-        plan.active_pairs += act[step];
-        plan.active_tiles += (long long)plan.npairs * (plan.npairs + 1) / 2 - u * (u + 1) / 2;
+        // tiles processed: those of each group with at least one active pair
+        const long long u0 = plan.npo - act[2 * step], u1 = npv - act[2 * step + 1];
+        plan.active_pairs += act[2 * step] + act[2 * step + 1];
+        plan.active_tiles += (long long)ntiles - u0 * (u0 + 1) / 2 - u1 * (u1 + 1) / 2;
         float a = 0.f, b = 0.f, c2 = 0.f;
         MHIP_CHECK(hipEventElapsedTime(&a, plan.ev[6 * step + 0], plan.ev[6 * step + 1]));
         MHIP_CHECK(hipEventElapsedTime(&b, plan.ev[6 * step + 2], plan.ev[6 * step + 3]));

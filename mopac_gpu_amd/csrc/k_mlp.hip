@@ -266,6 +266,24 @@ int guard(F&& fn) {
 
 }  // namespace
 
+// Device-resident forward pass for callers in other files (the learned
+// guess): they write the batch into mlp_input, then mlp_forward_resident
+// runs the network on the network's own stream, waits for it and returns the
+// device pointer of the outputs (batch x dims[nl], row-major).
+namespace mhip {
+float* mlp_input(mhip_mlp* q, int batch) {
+  mlp_ensure_batch(q, batch);
+  return act_ptr(q, batch, 0);
+}
+const float* mlp_forward_resident(mhip_mlp* q, int batch) {
+  forward(q, batch);
+  MHIP_CHECK(hipStreamSynchronize(q->s));
+  return act_ptr(q, batch, q->nl);
+}
+int mlp_layers(const mhip_mlp* q) { return q->nl; }
+int mlp_dim(const mhip_mlp* q, int l) { return q->dims[l]; }
+}  // namespace mhip
+
 extern "C" {
 
 int mhip_mlp_create(int nlayers, const int* dims, mhip_mlp** out) {

@@ -16,6 +16,7 @@
 #include "common.h"
 #include "k_eigen.h"
 #include "k_fock.h"
+#include "k_guess.h"
 #include "k_occvirt.h"
 #include "k_pseudo.h"
 
@@ -35,7 +36,12 @@ struct mhip_ctx {
   mhip::DevBuf p, pa, f, fcopy, pold, pold2, fd, A, C, C2, T, eig, scal, partial;
   mhip::DevBuf fhist, ehist, coef;
   mhip::DevBuf split_scratch;
+  // learned starting density (mhip_guess_water), valid for the uploaded system
+  mhip::GuessWork gw;
+  mhip::DevBuf pguess;
+  bool have_guess = false;
   mhip::JacobiPlan plan;
+  mhip::JacobiPlan plan_split;  // occupied / virtual groups, for the exit after block steps
   bool have_c = false;
   double* hscal = nullptr;  // pinned host scalars
   // pseudo-diagonalization workspace
@@ -213,6 +219,7 @@ static void build_system(mhip_ctx* c, int numat, const int* nfirst, const int* n
   c->partial.ensure(sizeof(double) * std::max<size_t>(norbs, dots_partial_doubles(mp, 11)) + 64);
   c->have_c = false;
   c->have_system = true;
+  c->have_guess = false;
   MHIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -233,7 +240,9 @@ FockArgs fock_args(mhip_ctx* c, int mode, double shift) {
 
 // eigenvectors_LAPACK on the device copy of a packed matrix: adds i*1e-10 to
 // the diagonal in place, then solves; ctx C/eig hold the result (sorted).
-static void eigen_packed(mhip_ctx* c, double* fpacked, bool perturb, double tol = 1.0e-12) {
+static void eigen_packed(mhip_ctx* c, double* fpacked, bool perturb, double tol = 1.0e-12,
+                         JacobiPlan* plan_in = nullptr) {
+  JacobiPlan& plan = plan_in ? *plan_in : c->plan;
   hipStream_t s = c->stream;
   const int n = c->norbs, np = c->npad;
   const double pad_base = 1.0e4;
@@ -252,12 +261,12 @@ static void eigen_packed(mhip_ctx* c, double* fpacked, bool perturb, double tol 
     jacobi_set_identity(s, np, C, np);
   }
   int sweeps = 0;
-  const int rc = jacobi_sweeps(s, c->plan, A, C, tol, 40, c->hscal + 32,
+  const int rc = jacobi_sweeps(s, plan, A, C, tol, 40, c->hscal + 32,
                                c->scal.as<double>() + 32, &sweeps);
   MHIP_REQUIRE(rc == 0, "block-Jacobi eigensolver did not converge in 40 sweeps");
   c->last_sweeps = sweeps;
   c->total_sweeps += sweeps;
-  jacobi_sort(s, c->plan, A, C, c->C2.as<double>(), c->eig.as<double>());
+  jacobi_sort(s, plan, A, C, c->C2.as<double>(), c->eig.as<double>());
   std::swap(c->C, c->C2);
   c->have_c = true;
 }
@@ -489,7 +498,12 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
   if (std::fabs(bshift) > 1e-20) ten = bshift;
   const int nclose = o->nclose;
   const double w1 = nclose / (nclose + 1.0e-6 + nclose);
-  if (o->p_init) {
+  if (o->start_from_guess) {
+    MHIP_REQUIRE(c->have_guess, "start_from_guess needs mhip_guess_water on the uploaded system");
+    copy_d2d(P, c->pguess.ptr, sizeof(double) * mp, s);
+    axpby(s, mp, 0.5, P, 0.0, Pa);
+    copy_d2d(pold, P, sizeof(double) * mp, s);
+  } else if (o->p_init) {
     MHIP_CHECK(hipMemcpyAsync(P, o->p_init, sizeof(double) * mp, hipMemcpyHostToDevice, s));
     axpby(s, mp, 0.5, P, 0.0, Pa);
     copy_d2d(pold, P, sizeof(double) * mp, s);
@@ -658,12 +672,20 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
   }
   // ---- exit (iter.F90:1052-1106)
   ee = energy(c);
+  JacobiPlan* exit_plan = nullptr;
   if (block_steps_since_diag > 0) {
     bjorck_step(s, c->ov, n, c->npad, c->C.as<double>(), 0, nclose);
     bjorck_step(s, c->ov, n, c->npad, c->C.as<double>(), nclose, n - nclose);
+    // C^T F C is block diagonal between the occupied and virtual columns up
+    // to the converged FMO: diagonalize the two blocks as independent
+    // problems in the same steps
+    if (c->plan_split.n != n || c->plan_split.split != nclose) c->plan_split.build(n, nclose);
+    c->plan_split.max_inner = c->plan.max_inner;
+    c->plan_split.timing = c->plan.timing;
+    if (c->plan_split.grouped && o->pseudo_diag == 2) exit_plan = &c->plan_split;
   }
   copy_d2d(c->fcopy.ptr, c->f.ptr, sizeof(double) * mp, s);
-  eigen_packed(c, c->fcopy.as<double>(), true);
+  eigen_packed(c, c->fcopy.as<double>(), true, 1.0e-12, exit_plan);
   ++nfull;
   if (o->last == 1) phase_lock(s, n, c->C.as<double>(), c->npad);
   MHIP_CHECK(hipStreamSynchronize(s));
@@ -724,12 +746,17 @@ int mhip_finalize(mhip_ctx* c) {
                       &c->heavy, &c->light, &c->nhb, &c->nlb, &c->jrow, &c->jcol, &c->p, &c->pa,
                       &c->f, &c->fcopy, &c->pold, &c->pold2, &c->fd, &c->A, &c->C, &c->C2, &c->T,
                       &c->eig, &c->scal, &c->partial, &c->fhist, &c->ehist, &c->coef,
-                      &c->split_scratch})
+                      &c->split_scratch, &c->pguess})
+      b->release();
+    for (DevBuf* b : {&c->gw.xyz, &c->gw.nat, &c->gw.mol, &c->gw.side, &c->gw.waters, &c->gw.R,
+                      &c->gw.geo, &c->gw.atomf, &c->gw.pa, &c->gw.pb, &c->gw.typ, &c->gw.nf0,
+                      &c->gw.no, &c->gw.norm})
       b->release();
     for (DevBuf* b : {&c->frag_ptr, &c->frag_orb, &c->frag_occ, &c->frag_ooff, &c->frag_voff,
                       &c->frontier})
       b->release();
     c->plan.release();
+    c->plan_split.release();
     c->pseudo.release();
     c->ov.release();
     if (c->hscal) (void)hipHostFree(c->hscal);
@@ -854,6 +881,53 @@ int mhip_scf(mhip_ctx* c, const mhip_scf_opts* opts, mhip_scf_result* res, doubl
   });
 }
 
+int mhip_guess_water(mhip_ctx* c, mhip_mlp* mlp, const mhip_guess_model* model, const int* nat,
+                     const double* xyz, int nw, const int* waters, int npairs, const int* pair_a,
+                     const int* pair_b, float* x_out, float* y_out) {
+  return guarded([&] {
+    MHIP_REQUIRE(c && mlp && model && nat && xyz && waters && pair_a && pair_b,
+                 "mhip_guess_water: null argument");
+    MHIP_REQUIRE(model->x_mean && model->x_std && model->y_mean && model->y_std,
+                 "mhip_guess_water: model normalization arrays missing");
+    MHIP_REQUIRE(c->have_system, "mhip_guess_water: no system uploaded");
+    MHIP_CHECK(hipSetDevice(c->device));
+    const int n = c->numat;
+    MHIP_REQUIRE(nw > 0 && 3 * nw == n, "mhip_guess_water: the system is not nw water molecules");
+    MHIP_REQUIRE(npairs > 0, "mhip_guess_water: empty pair list");
+    const int nl = mhip::mlp_layers(mlp);
+    MHIP_REQUIRE(mhip::mlp_dim(mlp, 0) == mhip::kGuessNin && mhip::mlp_dim(mlp, nl) == mhip::kGuessNout,
+                 "mhip_guess_water: network is not 96 -> ... -> 16");
+    std::vector<int> mol(n, -1);
+    for (int k = 0; k < nw; ++k)
+      for (int q = 0; q < 3; ++q) {
+        const int at = waters[3 * k + q];
+        MHIP_REQUIRE(at >= 0 && at < n && mol[at] < 0, "mhip_guess_water: waters is not a partition of the atoms");
+        mol[at] = k;
+        const bool oxy = q == 0;
+        MHIP_REQUIRE(nat[at] == (oxy ? 8 : 1) && c->no_h[at] == (oxy ? 4 : 1),
+                     "mhip_guess_water: each water must be O (4 orbitals), H, H (1 orbital each)");
+      }
+    for (int m = 0; m < npairs; ++m)
+      MHIP_REQUIRE(pair_b[m] >= 0 && pair_b[m] <= pair_a[m] && pair_a[m] < n,
+                   "mhip_guess_water: pairs must satisfy 0 <= b <= a < numat");
+    mhip::GuessHost in{n,           nw,          npairs,        nat,           xyz,
+                       mol.data(),  waters,      pair_a,        pair_b,        c->nf0_h.data(),
+                       c->no_h.data(), model->x_mean, model->x_std, model->y_mean, model->y_std};
+    c->pguess.ensure(sizeof(double) * c->mpack);
+    mhip::guess_water(c->stream, c->gw, mlp, in, c->pguess.as<double>(), c->mpack, x_out, y_out);
+    c->have_guess = true;
+  });
+}
+
+int mhip_guess_density(mhip_ctx* c, double* p) {
+  return guarded([&] {
+    MHIP_REQUIRE(c && p && c->have_guess, "mhip_guess_density: no guess on the device");
+    MHIP_CHECK(hipSetDevice(c->device));
+    MHIP_CHECK(hipMemcpyAsync(p, c->pguess.ptr, sizeof(double) * c->mpack, hipMemcpyDeviceToHost, c->stream));
+    MHIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
 int mhip_last_sweeps(mhip_ctx* c) { return c ? c->last_sweeps : 0; }
 
 // Event timing of the block-Jacobi kernels on the context stream: enable (1) /
@@ -900,19 +974,24 @@ int mhip_kernel_timing(mhip_ctx* c, int enable, double* solve_ms, long long* n_s
                        double* apply_ms, long long* n_apply, double* applyc_ms,
                        long long* n_applyc, long long* active_tiles, long long* active_pairs) {
   if (!c) return 1;
-  if (active_tiles) *active_tiles = c->plan.active_tiles;
-  if (active_pairs) *active_pairs = c->plan.active_pairs;
-  if (solve_ms) *solve_ms = c->plan.t_solve_ms;
-  if (n_solve) *n_solve = c->plan.n_solve;
-  if (apply_ms) *apply_ms = c->plan.t_apply_ms;
-  if (n_apply) *n_apply = c->plan.n_apply;
-  if (applyc_ms) *applyc_ms = c->plan.t_applyc_ms;
-  if (n_applyc) *n_applyc = c->plan.n_applyc;
+  // both plans: the in-loop one and the grouped one of the exit diagonalization
+  const mhip::JacobiPlan& a = c->plan;
+  const mhip::JacobiPlan& b = c->plan_split;
+  if (active_tiles) *active_tiles = a.active_tiles + b.active_tiles;
+  if (active_pairs) *active_pairs = a.active_pairs + b.active_pairs;
+  if (solve_ms) *solve_ms = a.t_solve_ms + b.t_solve_ms;
+  if (n_solve) *n_solve = a.n_solve + b.n_solve;
+  if (apply_ms) *apply_ms = a.t_apply_ms + b.t_apply_ms;
+  if (n_apply) *n_apply = a.n_apply + b.n_apply;
+  if (applyc_ms) *applyc_ms = a.t_applyc_ms + b.t_applyc_ms;
+  if (n_applyc) *n_applyc = a.n_applyc + b.n_applyc;
   if (enable >= 0) {
-    c->plan.timing = enable != 0;
-    c->plan.t_solve_ms = c->plan.t_apply_ms = c->plan.t_applyc_ms = 0.0;
-    c->plan.n_solve = c->plan.n_apply = c->plan.n_applyc = 0;
-    c->plan.active_tiles = c->plan.active_pairs = 0;
+    for (mhip::JacobiPlan* p : {&c->plan, &c->plan_split}) {
+      p->timing = enable != 0;
+      p->t_solve_ms = p->t_apply_ms = p->t_applyc_ms = 0.0;
+      p->n_solve = p->n_apply = p->n_applyc = 0;
+      p->active_tiles = p->active_pairs = 0;
+    }
   }
   return 0;
 }

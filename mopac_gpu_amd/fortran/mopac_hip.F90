@@ -33,6 +33,7 @@ module mopac_hip
     type(c_ptr) :: pdiag = c_null_ptr, p_init = c_null_ptr
     integer(c_int) :: block_start = 0
     type(c_ptr) :: fragment = c_null_ptr
+    integer(c_int) :: start_from_guess = 0
   end type
 
   type, bind(C) :: mhip_scf_result

@@ -45,26 +45,21 @@ SELF_O, SELF_H, INTRA_OH, INTRA_HH, INTER_OO, INTER_OH, INTER_HO, INTER_HH = ran
 
 def find_waters(nat, xyz):
     """(mol_of_atom, waters) when every atom belongs to a water molecule, else None.
-    waters[k] = (iO, iH1, iH2)."""
+    waters[k] = (iO, iH1, iH2), the two H in atom order."""
     nat = np.asarray(nat)
     xyz = np.asarray(xyz, dtype=np.float64)
     o_idx = np.where(nat == 8)[0]
     h_idx = np.where(nat == 1)[0]
     if len(o_idx) == 0 or len(o_idx) * 3 != len(nat) or len(h_idx) != 2 * len(o_idx):
         return None
-    tree = cKDTree(xyz[o_idx])
-    d, owner = tree.query(xyz[h_idx])
-    if np.any(d > 1.25):
+    d, owner = cKDTree(xyz[o_idx]).query(xyz[h_idx])
+    if np.any(d > 1.25) or np.any(np.bincount(owner, minlength=len(o_idx)) != 2):
         return None
-    waters = []
-    mol = np.full(len(nat), -1, dtype=np.int64)
-    for k, io in enumerate(o_idx):
-        hs = h_idx[owner == k]
-        if len(hs) != 2:
-            return None
-        waters.append((int(io), int(hs[0]), int(hs[1])))
-        mol[[io, hs[0], hs[1]]] = k
-    return mol, waters
+    hs = h_idx[np.argsort(owner, kind="stable")].reshape(-1, 2)
+    w = np.column_stack([o_idx, hs])
+    mol = np.empty(len(nat), dtype=np.int64)
+    mol[w.ravel()] = np.repeat(np.arange(len(o_idx)), 3)
+    return mol, [tuple(int(v) for v in row) for row in w]
 
 
 def frames(xyz, waters):
@@ -373,8 +368,37 @@ class GuessModel:
             self._mlp = MLP(self.params)
         return self._mlp
 
+    def to_device(self, ctx, system, want_xy=False):
+        """Build the starting density on the device (mhip_guess_water); the
+        next ctx.iter(start_from_guess=True) starts from it.  Returns None
+        (standard guess) when the system is not all water, else a dict with
+        the pair list and, with want_xy, the normalized descriptors and raw
+        network outputs."""
+        fw = find_waters(system.nat, system.xyz)
+        if fw is None:
+            return None
+        mol, waters = fw
+        a, b, _ = pair_list(system.nat, system.xyz, mol)
+        nat = np.ascontiguousarray(system.nat, dtype=np.int32)
+        xyz = np.ascontiguousarray(system.xyz, dtype=np.float64)
+        w = np.ascontiguousarray(np.array(waters), dtype=np.int32)
+        pa = np.ascontiguousarray(a, dtype=np.int32)
+        pb = np.ascontiguousarray(b, dtype=np.int32)
+        norm = [np.ascontiguousarray(v, dtype=np.float32) for v in (self.x_mean, self.x_std)]
+        norm += [np.ascontiguousarray(np.broadcast_to(v, (8, NOUT)), dtype=np.float32)
+                 for v in (self.y_mean, self.y_std)]
+        model = L.GuessModelC(*[v.ctypes.data for v in norm])
+        x = np.zeros((len(a), NIN), dtype=np.float32) if want_xy else None
+        y = np.zeros((len(a), NOUT), dtype=np.float32) if want_xy else None
+        L.check(ctx.lib.mhip_guess_water(ctx.h, self.mlp().h, ctypes.byref(model), L.ptr(nat), L.ptr(xyz),
+                                         len(waters), L.ptr(w), len(a), L.ptr(pa), L.ptr(pb), L.ptr(x),
+                                         L.ptr(y)))
+        return {"a": a, "b": b, "mol": mol, "waters": waters, "x": x, "y": y}
+
     def predict_density(self, system, ctx=None):
-        """Packed starting density, or None (standard guess) when not all water."""
+        """Packed starting density computed on the host (descriptors and
+        assembly in numpy, network on the device), or None (standard guess)
+        when not all water.  The device path is to_device."""
         fw = find_waters(system.nat, system.xyz)
         if fw is None:
             return None

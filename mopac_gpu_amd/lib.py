@@ -33,7 +33,12 @@ class ScfOpts(ctypes.Structure):
         ("p_init", c_void_p),
         ("block_start", c_int),
         ("fragment", c_void_p),
+        ("start_from_guess", c_int),
     ]
+
+
+class GuessModelC(ctypes.Structure):
+    _fields_ = [("x_mean", c_void_p), ("x_std", c_void_p), ("y_mean", c_void_p), ("y_std", c_void_p)]
 
 
 class ScfResult(ctypes.Structure):
@@ -120,6 +125,8 @@ SIGNATURES = {
     "mhip_mlp_forward": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "mhip_mlp_gradients": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "mhip_mlp_train_step": (c_int, [c_void_p, c_int, c_void_p, c_void_p, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
+    "mhip_guess_water": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mhip_guess_density": (c_int, [c_void_p, c_void_p]),
     "mhip_pm7_sizes": (c_int, [c_int, c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_ll)]),
     "mhip_pm7_hamiltonian": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double), ctypes.POINTER(c_double), c_void_p]),
     "mhip_ev_to_kcal": (c_double, []),

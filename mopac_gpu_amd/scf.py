@@ -87,7 +87,8 @@ class Context:
 
     # -- the whole loop ---------------------------------------------------
     def iter(self, pulay=False, scfcrt=1e-4, itrmax=2000, last=1, precision=0, pseudo_diag=False,
-             adaptive_eigen=False, p_init=None, want=("p", "eig"), block_start=0, fragment=None):
+             adaptive_eigen=False, p_init=None, want=("p", "eig"), block_start=0, fragment=None,
+             start_from_guess=False):
         s = self.system
         n = s.norbs
         o = L.ScfOpts()
@@ -108,6 +109,7 @@ class Context:
             pin = np.ascontiguousarray(p_init, dtype=np.float64)
             o.p_init = pin.ctypes.data
         o.block_start = int(block_start)
+        o.start_from_guess = int(bool(start_from_guess))
         frag = None
         if fragment is not None:
             frag = np.ascontiguousarray(fragment, dtype=np.int32)

@@ -188,3 +188,21 @@ def test_mfma_tiles_match_vector_tiles(gpu_ctx):
     lib.mhip_set_tuning(gpu_ctx.h, 1, -1, -1.0)
     assert np.max(np.abs(outs[0] - ref)) < 1e-12
     assert np.max(np.abs(outs[1] - ref)) < 1e-12
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+@pytest.mark.parametrize("m,n,k", [(256, 256, 64), (384, 192, 272), (64, 64, 16), (256, 128, 3072)])
+def test_large_tile_gemm_matches_numpy(gpu_ctx, ta, tb, m, n, k):
+    """Shapes that divide the 128x128, 128x64 or 64x64 tiles take the
+    double-buffered large-tile kernel; alpha and beta on a random C."""
+    lib = L.load()
+    rng = np.random.default_rng(m + 3 * n + 7 * k)
+    a = np.asfortranarray(rng.standard_normal((k, m) if ta == "T" else (m, k)))
+    b = np.asfortranarray(rng.standard_normal((n, k) if tb == "T" else (k, n)))
+    c = np.asfortranarray(rng.standard_normal((m, n)))
+    opa = a.T if ta == "T" else a
+    opb = b.T if tb == "T" else b
+    ref = 0.75 * opa @ opb - 0.5 * c
+    lib.mhip_call_gemm(ta.encode(), tb.encode(), m, n, k, 0.75, L.ptr(a), a.shape[0], L.ptr(b), b.shape[0],
+                       -0.5, L.ptr(c), m)
+    assert np.max(np.abs(c - ref)) < 1e-12 * max(1.0, np.sqrt(k))
