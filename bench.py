@@ -181,8 +181,19 @@ def main():
         achieved = (flops / max(nl, 1)) / avg_s / 1e12 if avg_s > 0 else 0.0
         peak = PEAK_BF16_TFLOPS if "split" in name else PEAK_FP64_TFLOPS
         unit = "TFLOP/s"
+    # HBM bytes per launch of that kernel class, measured by rocprofv3 PMC runs
+    # (FETCH_SIZE and WRITE_SIZE, scaled by a copy of known size) of the same
+    # workload; the numbers and the commands are in profiles/
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_h2o512_fast.json")
+    if k == 512 and os.path.exists(tfile):
+        with open(tfile) as fh:
+            tc = json.load(fh).get("classes", {}).get(name)
+        if tc:
+            traffic = round(tc["traffic_bytes"])
     roof = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-            "frac": round(achieved / peak, 4), "traffic": None, "kernel": name,
+            "frac": round(achieved / peak, 4), "traffic": traffic,
+            "algorithmic_bytes_per_launch": round(nbytes / max(nl, 1)) if nbytes else None, "kernel": name,
             "avg_launch_us": round(avg_s * 1e6, 2), "launches": int(nl),
             "algorithmic_per_launch": round((nbytes if bound == "hbm" else flops) / max(nl, 1)),
             "kernel_ms_in_one_fast_scf": breakdown}
