@@ -33,7 +33,7 @@ METRIC = "SCF time-to-converge (s) at |ΔHf|<1e-3 kcal/mol vs FP64: mixed+learne
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
-FAST_PRECISION = 0
+FAST_PRECISION = 1
 
 
 def parse():

@@ -117,6 +117,9 @@ void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
 void gemm_split_bf16(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha,
                      const double* a, int lda, const double* b, int ldb, double beta, double* c,
                      int ldc);
+// Frees the BF16 plane buffers gemm_split_bf16 keeps per (device, stream):
+// those of stream s, or all of them when s is null.
+void release_split_scratch(hipStream_t s);
 // Packed lower triangle p (mpack) = occ * A(:,0:k) A(:,0:k)^T over the first n rows.
 void syrk_packed(hipStream_t s, int n, int k, double occ, const double* a, int lda, double* p);
 // Split-BF16 variant (gemm_split_bf16 into a dense n x n scratch, then packed).

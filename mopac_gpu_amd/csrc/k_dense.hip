@@ -252,6 +252,23 @@ __global__ __launch_bounds__(NT) void k_gemm_mfma(int M, int N, int K, double al
 // 16 (mod 32) doubles so a half-wave's 16-lane groups hit disjoint banks.
 // Requires M % BM == 0, N % BN == 0, K % 16 == 0, K > 0, 16-byte aligned a, b
 // and even lda, ldb (checked by the launcher).
+// Workgroup -> output tile.  Workgroups are dealt round-robin to the eight
+// XCDs, each with its own L2: the first step gives every XCD a contiguous
+// chunk of the tile sequence; the second orders that sequence in groups of
+// eight tile rows, column by column, so one XCD's chunk is a compact block
+// that shares A row panels and B column panels in its L2.
+__device__ __forceinline__ void xcd_tile(int L, int gm, int gn, int& tm, int& tn) {
+  const int T = gm * gn;
+  const int per = T / 8;
+  const int t = (L < per * 8) ? (L % 8) * per + L / 8 : L;
+  constexpr int G = 8;
+  const int first = (t / (G * gn)) * G;
+  const int rows = min(gm - first, G);
+  const int r = t - first * gn;
+  tm = first + r % rows;
+  tn = r / rows;
+}
+
 constexpr int BK = 16;
 template <int BM, int BN, bool TA, bool TBT>
 __global__ __launch_bounds__(256) void k_gemm_big(int M, int N, int K, double alpha,
@@ -267,7 +284,9 @@ __global__ __launch_bounds__(256) void k_gemm_big(int M, int N, int K, double al
   __shared__ double Bs[2][BK][SBn];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  int tm, tn;
+  xcd_tile(blockIdx.x, M / BM, N / BN, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
   const int wm = (w >> 1) * WM, wn = (w & 1) * WN;
   double2 ra[NA2], rb[NB2];
   auto gload = [&](int k0) {
@@ -357,6 +376,87 @@ __global__ __launch_bounds__(256) void k_gemm_big(int M, int N, int K, double al
       }
 }
 
+// Packed SYRK on the large tiles: p = occ * A A^T over the lower triangle
+// of 128 x 128 tiles (XCD-chunked), the k loop of k_gemm_big with both
+// operands taken from rows of A (column-major, n x k).  The MFMA output keeps
+// lanes along columns, which are contiguous in the packed rows.  Requires
+// n % 128 == 0, k % 16 == 0, k > 0, 16-byte aligned a and even lda.
+__global__ __launch_bounds__(256) void k_syrk_big(int n, int k, double occ, const double* __restrict__ a,
+                                                  int lda, double* __restrict__ p) {
+  constexpr int BM = 128, S = BM + 16, WM = 64, F = 4, NL = BM * BK / 2 / 256;
+  __shared__ double As[2][BK][S];
+  __shared__ double Bs[2][BK][S];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nt = n / BM, T = nt * (nt + 1) / 2;
+  const int per = T / 8, L = blockIdx.x;
+  const int t = (L < per * 8) ? (L % 8) * per + L / 8 : L;
+  int tm = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((tm + 1) * (tm + 2) / 2 <= t) ++tm;
+  while (tm * (tm + 1) / 2 > t) --tm;
+  const int tn = t - tm * (tm + 1) / 2;
+  const int m0 = tm * BM, n0 = tn * BM;
+  const int wm = (w >> 1) * WM, wn = (w & 1) * WM;
+  double2 ra[NL], rb[NL];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      const int idx = tid + 256 * r;
+      const int mm = 2 * (idx % (BM / 2)), kk = idx / (BM / 2);
+      ra[r] = *reinterpret_cast<const double2*>(a + (size_t)(m0 + mm) + (size_t)(k0 + kk) * lda);
+      rb[r] = *reinterpret_cast<const double2*>(a + (size_t)(n0 + mm) + (size_t)(k0 + kk) * lda);
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      const int idx = tid + 256 * r;
+      const int mm = 2 * (idx % (BM / 2)), kk = idx / (BM / 2);
+      As[buf][kk][mm] = ra[r].x;
+      As[buf][kk][mm + 1] = ra[r].y;
+      Bs[buf][kk][mm] = rb[r].x;
+      Bs[buf][kk][mm + 1] = rb[r].y;
+    }
+  };
+  d4_t acc[F][F];
+#pragma unroll
+  for (int i = 0; i < F; ++i)
+#pragma unroll
+    for (int j = 0; j < F; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  const int nk = k / BK;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      double af[F], bf[F];
+#pragma unroll
+      for (int i = 0; i < F; ++i) af[i] = As[cur][ks + lk][wm + 16 * i + li];
+#pragma unroll
+      for (int j = 0; j < F; ++j) bf[j] = Bs[cur][ks + lk][wn + 16 * j + li];
+#pragma unroll
+      for (int i = 0; i < F; ++i)
+#pragma unroll
+        for (int j = 0; j < F; ++j) acc[i][j] = mfma16x16x4(af[i], bf[j], acc[i][j]);
+    }
+    if (kt + 1 < nk) lstore(cur ^ 1);
+    __syncthreads();
+  }
+  // lane holds (m0 + wm + 16 i + lk + 4 t, n0 + wn + 16 j + li) in acc[i][j][t]
+#pragma unroll
+  for (int i = 0; i < F; ++i)
+#pragma unroll
+    for (int j = 0; j < F; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const long long gm = m0 + wm + 16 * i + lk + 4 * q, gn = n0 + wn + 16 * j + li;
+        if (gn <= gm) p[gm * (gm + 1) / 2 + gn] = occ * acc[i][j][q];
+      }
+}
+
 __global__ __launch_bounds__(NT) void k_syrk_packed_mfma(int n, int k, double occ, const double* a,
                                                          int lda, double* p) {
   __shared__ double As[TK][TB + 1];
@@ -400,7 +500,7 @@ template <int BM, int BN>
 static void gemm_big_dispatch(hipStream_t s, bool TA, bool TBT, int m, int n, int k, double alpha,
                               const double* a, int lda, const double* b, int ldb, double beta, double* c,
                               int ldc) {
-  const dim3 grid(n / BN, m / BM);
+  const dim3 grid((m / BM) * (n / BN));
   if (!TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, false>), grid, dim3(256), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
   else if (TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, true, false>), grid, dim3(256), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
   else if (!TA && TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, true>), grid, dim3(256), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
@@ -468,6 +568,13 @@ void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
 }
 
 static void syrk_launch(hipStream_t s, int n, int k, double occ, const double* a, int lda, double* p) {
+  if (g_kernel_variant == 1 && n % 128 == 0 && k >= BK && k % BK == 0 && !(lda & 1) &&
+      !(reinterpret_cast<uintptr_t>(a) & 15)) {
+    const int nb = n / 128;
+    hipLaunchKernelGGL(k_syrk_big, dim3(nb * (nb + 1) / 2), dim3(256), 0, s, n, k, occ, a, lda, p);
+    MHIP_LAUNCH_CHECK();
+    return;
+  }
   const int nt = cdiv(n, TB);
   if (g_kernel_variant == 1)
     hipLaunchKernelGGL(k_syrk_packed_mfma, dim3(nt * (nt + 1) / 2), dim3(NT), 0, s, n, k, occ, a, lda, p);

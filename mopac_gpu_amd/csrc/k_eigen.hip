@@ -50,11 +50,13 @@ __device__ __forceinline__ void pair_rotation(const double (*S)[LDSW], int round
   c = 1.0;
   sn = 0.0;
   if (fabs(apq) > 1.0e-300 && fabs(apq) > 1.0e-18 * sqrt(fabs(app * aqq))) {
-    const double theta = (aqq - app) / (2.0 * apq);
-    double t;
-    if (fabs(theta) > 1.0e150) t = 0.5 / theta;
-    else t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-    c = 1.0 / sqrt(t * t + 1.0);
+    // t = tan of the rotation angle, the root of t^2 + 2 theta t - 1 = 0
+    // (theta = d / h) of smaller magnitude, written without theta:
+    // t = sign(d) h / (|d| + sqrt(d^2 + h^2)); one division instead of three
+    const double d = aqq - app, h = 2.0 * apq;
+    double t = h / (fabs(d) + sqrt(d * d + h * h));
+    if (d < 0.0) t = -t;
+    c = rsqrt(t * t + 1.0);
     sn = t * c;
   }
 }
@@ -345,10 +347,12 @@ __global__ __launch_bounds__(256) void k_jac_apply_a_mfma(double* __restrict__ A
   __syncthreads();
   for (int i = 0; i < 4; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
   mfma_tile64([&](int r, int k) { return At[r][k]; }, [&](int k, int c) { return Q2[k][c]; }, SB, acc);
+  // stage the result in LDS and write both the tile and its mirror with
+  // consecutive lanes on consecutive rows of A
+  __syncthreads();
+  mfma_tile64_store(acc, [&](int r, int c, double v) { Q1[r][c] = v; });
+  __syncthreads();
   if (P1 == P2) {
-    __syncthreads();
-    mfma_tile64_store(acc, [&](int r, int c, double v) { Q1[r][c] = v; });
-    __syncthreads();
     for (int e = tid; e < SB * SB; e += 256) {
       const int r = e % SB, c = e / SB;
       const double v = (r == c) ? Q1[r][c] : 0.5 * (Q1[r][c] + Q1[c][r]);
@@ -356,11 +360,14 @@ __global__ __launch_bounds__(256) void k_jac_apply_a_mfma(double* __restrict__ A
     }
     return;
   }
-  mfma_tile64_store(acc, [&](int r, int c, double v) {
-    const size_t gr = gidx(I1, J1, r), gc = gidx(I2, J2, c);
-    A[gr + gc * lda] = v;
-    A[gc + gr * lda] = v;
-  });
+  for (int e = tid; e < SB * SB; e += 256) {
+    const int r = e % SB, c = e / SB;
+    A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda] = Q1[r][c];
+  }
+  for (int e = tid; e < SB * SB; e += 256) {
+    const int c = e % SB, r = e / SB;
+    A[(size_t)gidx(I2, J2, c) + (size_t)gidx(I1, J1, r) * lda] = Q1[r][c];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_jac_apply_c_mfma(double* __restrict__ C, int ldc, int nrows,
@@ -385,10 +392,15 @@ __global__ __launch_bounds__(256) void k_jac_apply_c_mfma(double* __restrict__ C
   d4_t acc[4];
   for (int i = 0; i < 4; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
   mfma_tile64([&](int r, int k) { return Ct[r][k]; }, [&](int k, int c) { return Qs[k][c]; }, SB, acc);
-  mfma_tile64_store(acc, [&](int r, int c, double v) {
+  // stage in LDS, then store with consecutive lanes on consecutive rows of C
+  __syncthreads();
+  mfma_tile64_store(acc, [&](int r, int c, double v) { Ct[r][c] = v; });
+  __syncthreads();
+  for (int e = tid; e < SB * SB; e += 256) {
+    const int r = e % SB, c = e / SB;
     const int gr = row0 + r;
-    if (gr < nrows) C[(size_t)gr + (size_t)gidx(I, J, c) * ldc] = v;
-  });
+    if (gr < nrows) C[(size_t)gr + (size_t)gidx(I, J, c) * ldc] = Ct[r][c];
+  }
 }
 
 // max |A_ij| (i != j) and max |A_ii| over the leading n x n block.

@@ -760,6 +760,7 @@ int mhip_finalize(mhip_ctx* c) {
     c->pseudo.release();
     c->ov.release();
     if (c->hscal) (void)hipHostFree(c->hscal);
+    mhip::release_split_scratch(c->stream);
     (void)hipStreamDestroy(c->stream);
     delete c;
   });

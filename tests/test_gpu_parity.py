@@ -50,15 +50,20 @@ def test_fock2_matches_oracle(gpu_ctx, which):
     assert np.max(np.abs(f_gpu - f_ref)) < 1e-11 * max(1.0, np.max(np.abs(f_ref)))
 
 
-def test_density_and_energy_match_oracle(gpu_ctx):
-    s = _water(8, 5)
+@pytest.mark.parametrize("nw", [8, 64])
+def test_density_and_energy_match_oracle(gpu_ctx, nw):
+    """(H2O)64 (n = 384, 256 occupied) takes the large-tile packed SYRK."""
+    s = _water(nw, 5)
     gpu_ctx.upload(s)
     rng = np.random.default_rng(1)
     q, _ = np.linalg.qr(rng.standard_normal((s.norbs, s.norbs)))
     c = np.asfortranarray(q)
     p_gpu = gpu_ctx.density(c, s.nclose, 2.0)
     p_ref = oracle.density(c, s.nclose, 2.0)
-    assert np.max(np.abs(p_gpu - p_ref)) < 1e-13
+    assert np.max(np.abs(p_gpu - p_ref)) < 1e-13 * np.sqrt(s.nclose)
+    # an occupied count that is not a multiple of the k step takes the 64 x 64 tiles
+    p_odd = gpu_ctx.density(c, s.nclose - 3, 2.0)
+    assert np.max(np.abs(p_odd - oracle.density(c, s.nclose - 3, 2.0))) < 1e-13 * np.sqrt(s.nclose)
     f = oracle.fock2(s, s.h, p_ref, 0.5 * p_ref)
     e_gpu = gpu_ctx.helect(0.5 * p_ref, s.h, f)
     e_ref = oracle.helect(s.norbs, 0.5 * p_ref, s.h, f)
