@@ -251,6 +251,62 @@ int mhip_mozyme_helecz(mhip_ctx* ctx, int numat, const int* nfirst, const int* n
 int mhip_mozyme_cnvgz(mhip_ctx* ctx, int norbs, long long mpack, double* pnew, double* p, double* p1,
                       double* p2, double* p3, int niter, const int* idiag, int three_point, double* pmax);
 
+/* The MOZYME system for the sparse Fock build: the one-electron matrix on the
+ * sparse storage and the two-electron integrals as hcore_for_MOZYME stores
+ * them for SEMIDIRECT (src/MOZYME/hcore_for_MOZYME.F90:100-405): for each atom
+ * i, the blocks of the stored pairs (i, j < i) in order (100 / 10 / 10 / 1
+ * values), then i's one-centre block; pairs that are not stored (nijbo -1,
+ * or -2 for the dipole shell) take point-charge terms from the per-atom
+ * parameters am, dd, ad (parameters_C), evaluated on the fly (semidr). */
+typedef struct {
+  int numat, norbs;
+  long long mpack;      /* sparse storage length */
+  const int* nfirst;    /* numat, 1-based */
+  const int* nlast;
+  const int* nijbo;     /* numat * numat, column-major, symmetric */
+  const double* xyz;    /* 3 * numat, Angstrom */
+  const double* am;     /* per atom */
+  const double* dd;
+  const double* ad;
+  const double* h;      /* mpack */
+  const double* w;      /* nw */
+  long long nw;
+} mhip_mozyme_system;
+/* Upload a MOZYME system into the context (replaces the conventional one). */
+int mhip_mozyme_setup(mhip_ctx* ctx, const mhip_mozyme_system* sys);
+/* buildf(f, partf, 0) (src/MOZYME/buildf.F90:16-52): f = h + G(p) on the
+ * sparse storage, G as fock2z/fz2n (src/MOZYME/fock2z.F90:16-1362) forms it for
+ * a discrete system (ione = 1, direct = false, semidr = true). */
+int mhip_mozyme_fock2z(mhip_ctx* ctx, const double* p, double* f);
+
+/* The MOZYME SCF (iter_for_MOZYME, src/MOZYME/iter_for_MOZYME.F90:16-691)
+ * for a single point of a discrete system, after mhip_mozyme_setup: starting
+ * from the LMOs in lmo (updated in place; their capacities must leave room to
+ * grow), each iteration runs tidy and check on the host, eimp, diagg
+ * (diagg1, diagg2, density_for_MOZYME), cnvgz while the three-point
+ * extrapolation is on, buildf/fock2z and helecz, with isitsc's test and the
+ * reference's level-shift schedule.  The hydrogen-bond search addhb is not
+ * run. */
+typedef struct {
+  double scfcrt;    /* SELCON, kcal/mol (scfcri: 1e-2 when <= 0) */
+  int itrmax;       /* 2000 when <= 0 */
+  double thresh;    /* LMO coefficient threshold (1e-13 when <= 0) */
+  double damp;      /* DAMP (1 when <= 0) */
+  int fmo_dim;      /* occupied-virtual list capacity (0: automatic) */
+  double enuclr;    /* eV */
+  double atheat;    /* kcal/mol */
+} mhip_mozyme_opts;
+typedef struct {
+  double escf;      /* heat of formation, kcal/mol (no geometry-only terms) */
+  double ee;        /* electronic energy, eV */
+  int niter;
+  int converged;
+  double ovmax;     /* largest occupied-virtual element of the last diagg1 */
+  double seconds;
+} mhip_mozyme_result;
+int mhip_mozyme_scf(mhip_ctx* ctx, mhip_lmo_set* lmo, const mhip_mozyme_opts* opts, mhip_mozyme_result* res,
+                    double* p, double* f);
+
 /* ---- reference-boundary drop-ins -----------------------------------------
  * Same C signatures as the reference's exported symbols, so its Fortran
  * interfaces (src/gpu/cublas_interfaces.F90, src/matrix/eigenvectors_cuda.F90)
@@ -355,6 +411,10 @@ int mhip_pm7_hamiltonian(int numat, const int* nat, const double* coord, int cha
                          int* nfirst, int* nlast, double* h, double* w, double* enuclr,
                          double* atheat, double* pdiag);
 double mhip_ev_to_kcal(void);
+/* Per-element PM7 values the point-charge terms need (parameters_C after
+ * calpar): out[0..4] = natorb, tore, am, ad, dd (atomic units).  Elements 1,
+ * 6, 7, 8; -1 for any other. */
+int mhip_pm7_element(int z, double* out);
 
 #ifdef __cplusplus
 }

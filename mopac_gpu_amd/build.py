@@ -28,6 +28,7 @@ SOURCES = [
     "k_mlp.hip",
     "k_batch.hip",
     "k_guess.hip",
+    "mozyme_scf.hip",
     "k_mozyme.hip",
     "scf_driver.hip",
     "dropin.hip",

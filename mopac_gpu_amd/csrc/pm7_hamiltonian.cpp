@@ -950,4 +950,16 @@ int mhip_pm7_hamiltonian(int numat, const int* nat, const double* coord, int cha
 
 double mhip_ev_to_kcal(void) { return kFPC9; }
 
+int mhip_pm7_element(int z, double* out) {
+  init_params();
+  if (z < 0 || z >= 9 || !g_elem[z].ok || !out) return -1;
+  const Elem& E = g_elem[z];
+  out[0] = E.natorb;
+  out[1] = E.tore;
+  out[2] = E.am;
+  out[3] = E.ad;
+  out[4] = E.dd;
+  return 0;
+}
+
 }  // extern "C"

@@ -20,40 +20,8 @@
 #include "k_occvirt.h"
 #include "k_pseudo.h"
 
-struct mhip_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  // system
-  int numat = 0, norbs = 0, npad = 0;
-  long long mpack = 0, n2elec = 0;
-  mhip::DevBuf w, h, nf0, norb, woff, wcum, w1off, ccol, heavy, light, nhb, nlb, jrow, jcol;
-  mhip::SysDev sys;
-  std::vector<int> nf0_h, no_h;  // first orbital (0-based) and orbital count per atom
-  bool have_system = false;
-  // fragment start (CSR over fragments)
-  mhip::DevBuf frag_ptr, frag_orb, frag_occ, frag_ooff, frag_voff, frontier;
-  // SCF state
-  mhip::DevBuf p, pa, f, fcopy, pold, pold2, fd, A, C, C2, T, eig, scal, partial;
-  mhip::DevBuf fhist, ehist, coef;
-  mhip::DevBuf split_scratch;
-  // learned starting density (mhip_guess_water), valid for the uploaded system
-  mhip::GuessWork gw;
-  mhip::DevBuf pguess;
-  bool have_guess = false;
-  mhip::JacobiPlan plan;
-  mhip::JacobiPlan plan_split;  // occupied / virtual groups, for the exit after block steps
-  bool have_c = false;
-  double* hscal = nullptr;  // pinned host scalars
-  // pseudo-diagonalization workspace
-  mhip::PseudoWork pseudo;
-  mhip::OccVirtWork ov;
-  // statistics of the last eigensolve
-  int last_sweeps = 0;
-  // Jacobi stopping tolerance (relative to max|diag|) inside the SCF loop;
-  // the exit diagonalization always uses 1e-12.
-  double tol_iter = 1.0e-10;
-  int total_sweeps = 0;
-};
+#include "ctx.h"
+
 
 namespace mhip {
 
@@ -77,10 +45,6 @@ int guarded(F&& fn) {
   }
 }
 
-void upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s) {
-  b.ensure(bytes ? bytes : 8);
-  if (bytes) MHIP_CHECK(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, s));
-}
 
 __global__ void k_diag_add_index(int n, double* p, double scale) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -136,14 +100,19 @@ __global__ void k_pad_identity(int n, int npad, double* __restrict__ c) {
   }
 }
 
+}  // namespace
+
+void upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s) {
+  b.ensure(bytes ? bytes : 8);
+  if (bytes) MHIP_CHECK(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, s));
+}
+
 void copy_d2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
   MHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
 }
 
-}  // namespace
-
 // Build the pair tables for the Fock kernels from nfirst/nlast (1-based).
-static void build_system(mhip_ctx* c, int numat, const int* nfirst, const int* nlast, int norbs,
+void build_system(mhip_ctx* c, int numat, const int* nfirst, const int* nlast, int norbs,
                          long long n2elec, const double* h, const double* w) {
   hipStream_t s = c->stream;
   c->numat = numat;
@@ -761,6 +730,8 @@ int mhip_finalize(mhip_ctx* c) {
     c->ov.release();
     if (c->hscal) (void)hipHostFree(c->hscal);
     mhip::release_split_scratch(c->stream);
+    mhip::mz_release(c->mz);
+    c->mz = nullptr;
     (void)hipStreamDestroy(c->stream);
     delete c;
   });

@@ -37,6 +37,12 @@ class ScfOpts(ctypes.Structure):
     ]
 
 
+class MozymeSystemC(ctypes.Structure):
+    _fields_ = [("numat", c_int), ("norbs", c_int), ("mpack", c_ll), ("nfirst", c_void_p), ("nlast", c_void_p),
+                ("nijbo", c_void_p), ("xyz", c_void_p), ("am", c_void_p), ("dd", c_void_p), ("ad", c_void_p),
+                ("h", c_void_p), ("w", c_void_p), ("nw", c_ll)]
+
+
 class GuessModelC(ctypes.Structure):
     _fields_ = [("x_mean", c_void_p), ("x_std", c_void_p), ("y_mean", c_void_p), ("y_std", c_void_p)]
 
@@ -63,6 +69,16 @@ class LmoSetC(ctypes.Structure):
                 ("icocc_dim", c_int), ("cocc", c_void_p), ("cocc_dim", c_int),
                 ("nce", c_void_p), ("nnce", c_void_p), ("ncvir", c_void_p), ("icvir", c_void_p),
                 ("icvir_dim", c_int), ("cvir", c_void_p), ("cvir_dim", c_int)]
+
+
+class MozymeOptsC(ctypes.Structure):
+    _fields_ = [("scfcrt", c_double), ("itrmax", c_int), ("thresh", c_double), ("damp", c_double),
+                ("fmo_dim", c_int), ("enuclr", c_double), ("atheat", c_double)]
+
+
+class MozymeResultC(ctypes.Structure):
+    _fields_ = [("escf", c_double), ("ee", c_double), ("niter", c_int), ("converged", c_int),
+                ("ovmax", c_double), ("seconds", c_double)]
 
 
 class Diagg1State(ctypes.Structure):
@@ -127,6 +143,10 @@ SIGNATURES = {
     "mhip_mlp_train_step": (c_int, [c_void_p, c_int, c_void_p, c_void_p, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
     "mhip_guess_water": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhip_guess_density": (c_int, [c_void_p, c_void_p]),
+    "mhip_mozyme_setup": (c_int, [c_void_p, c_void_p]),
+    "mhip_pm7_element": (c_int, [c_int, c_void_p]),
+    "mhip_mozyme_fock2z": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "mhip_mozyme_scf": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhip_pm7_sizes": (c_int, [c_int, c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_ll)]),
     "mhip_pm7_hamiltonian": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double), ctypes.POINTER(c_double), c_void_p]),
     "mhip_ev_to_kcal": (c_double, []),

@@ -20,6 +20,7 @@ below `keep`, and stores every atom pair within `rcut` (Angstrom).  It feeds
 the parity tests of the MOZYME kernels with realistic LMOs; the reference's
 own start (a Lewis structure, makvec) is not part of the SCF inner loop.
 """
+import ctypes
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -71,6 +72,144 @@ def build_nijbo(xyz, iorbs, rcut):
             nijbo[a, b] = nijbo[b, a] = off
             off += iorbs[a] * (iorbs[a] + 1) // 2 if a == b else iorbs[a] * iorbs[b]
     return nijbo, off
+
+
+def build_nijbo_shells(xyz, iorbs, r_near, r_dipole):
+    """nijbo with the reference's three classes of pairs (fillij.F90): blocks
+    stored for pairs within r_near, -2 (point charge + dipole) up to r_dipole,
+    -1 (point charge) beyond."""
+    nijbo, size = build_nijbo(xyz, iorbs, r_near)
+    d = np.linalg.norm(xyz[:, None, :] - xyz[None, :, :], axis=2)
+    nijbo[(nijbo < 0) & (d <= r_dipole)] = -2
+    return nijbo, size
+
+
+def mozyme_integrals(system, nijbo):
+    """The two-electron integrals as hcore_for_MOZYME stores them for
+    SEMIDIRECT: the conventional hcore-order array with the blocks of pairs
+    that are not stored dropped (per atom i: stored pairs j < i, then i's
+    one-centre block)."""
+    no = (np.asarray(system.nlast) - np.asarray(system.nfirst) + 1).astype(np.int64)
+    lim = no * (no + 1) // 2
+    w = np.asarray(system.w)
+    keep = np.zeros(len(w), dtype=bool)
+    kr = 0
+    for i in range(len(no)):
+        for j in range(i):
+            n = lim[i] * lim[j]
+            if nijbo[i, j] >= 0:
+                keep[kr:kr + n] = True
+            kr += n
+        keep[kr:kr + lim[i] * lim[i]] = True
+        kr += lim[i] * lim[i]
+    assert kr == len(w)
+    return w[keep]
+
+
+def element_parameters(nat):
+    """Per-atom (am, ad, dd) of PM7 (parameters_C after calpar)."""
+    from . import lib as L
+    lib = L.load()
+    out = np.zeros(5)
+    am, ad, dd = [], [], []
+    for z in nat:
+        L.check(0 if lib.mhip_pm7_element(int(z), L.ptr(out)) == 0 else 1)
+        am.append(out[2])
+        ad.append(out[3])
+        dd.append(out[4])
+    return np.array(am), np.array(ad), np.array(dd)
+
+
+class MozymeSystem:
+    """A system in MOZYME storage: sparse H, integrals for SEMIDIRECT, nijbo
+    with near / dipole / point-charge pairs, per-atom parameters."""
+
+    def __init__(self, system, r_near=6.0, r_dipole=7.0, nijbo=None, mpack=None):
+        self.s = system
+        self.numat, self.norbs = system.numat, system.norbs
+        self.nat = np.asarray(system.nat, dtype=np.int32)
+        self.xyz = np.ascontiguousarray(system.xyz, dtype=np.float64)
+        self.nfirst = np.asarray(system.nfirst, dtype=np.int32)
+        self.nlast = np.asarray(system.nlast, dtype=np.int32)
+        iorbs = self.nlast - self.nfirst + 1
+        if nijbo is None:
+            self.nijbo, self.mpack = build_nijbo_shells(self.xyz, iorbs, r_near, r_dipole)
+        else:
+            self.nijbo, self.mpack = np.asarray(nijbo, dtype=np.int32), int(mpack)
+        hd = np.zeros((self.norbs, self.norbs))
+        hd[np.tril_indices(self.norbs)] = system.h
+        hd = hd + hd.T - np.diag(np.diag(hd))
+        self.h = sparse_from_dense(hd, self.nfirst, iorbs, self.nijbo, self.mpack)
+        self.w = mozyme_integrals(system, self.nijbo)
+        self.am, self.ad, self.dd = element_parameters(self.nat)
+
+    def upload(self, ctx):
+        from . import lib as L
+        c = lambda a, t=np.float64: np.ascontiguousarray(a, dtype=t)  # noqa: E731
+        self._keep = [c(self.nfirst, np.int32), c(self.nlast, np.int32),
+                      np.asfortranarray(self.nijbo.astype(np.int32)), c(self.xyz), c(self.am), c(self.dd),
+                      c(self.ad), c(self.h), c(self.w)]
+        nf, nl, nij, xyz, am, dd, ad, h, w = self._keep
+        sysc = L.MozymeSystemC(self.numat, self.norbs, self.mpack, nf.ctypes.data, nl.ctypes.data,
+                               nij.ctypes.data, xyz.ctypes.data, am.ctypes.data, dd.ctypes.data, ad.ctypes.data,
+                               h.ctypes.data, w.ctypes.data, len(w))
+        L.check(ctx.lib.mhip_mozyme_setup(ctx.h, ctypes.byref(sysc)))
+
+    def scf(self, ctx, lmos, scfcrt=1e-2, itrmax=2000, thresh=1e-13, fmo_dim=0):
+        """iter_for_MOZYME on the device from the LMOs of `lmos` (an LmoSet on
+        this system's nijbo); returns the result and the updated LMOs."""
+        from . import lib as L
+        s = reference_capacity(lmos)
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+        a = dict(nfirst=i32(s.nfirst), nlast=i32(s.nlast), nijbo=np.asfortranarray(i32(s.nijbo)),
+                 ncf=i32(s.ncf), nncf=i32(s.nncf), ncocc=i32(s.ncocc), icocc=i32(s.icocc),
+                 cocc=np.ascontiguousarray(s.cocc, dtype=np.float64), nce=i32(s.nce), nnce=i32(s.nnce),
+                 ncvir=i32(s.ncvir), icvir=i32(s.icvir), cvir=np.ascontiguousarray(s.cvir, dtype=np.float64))
+        c = L.LmoSetC(numat=s.numat, norbs=s.norbs, nocc=s.nocc, nvir=s.nvir, mpack=s.mpack,
+                      icocc_dim=len(a["icocc"]), cocc_dim=len(a["cocc"]), icvir_dim=len(a["icvir"]),
+                      cvir_dim=len(a["cvir"]))
+        for k, v in a.items():
+            setattr(c, k, v.ctypes.data)
+        o = L.MozymeOptsC(scfcrt=scfcrt, itrmax=itrmax, thresh=thresh, damp=1.0, fmo_dim=fmo_dim,
+                          enuclr=self.s.enuclr, atheat=self.s.atheat)
+        r = L.MozymeResultC()
+        p = np.zeros(self.mpack)
+        f = np.zeros(self.mpack)
+        L.check(ctx.lib.mhip_mozyme_scf(ctx.h, ctypes.byref(c), ctypes.byref(o), ctypes.byref(r), L.ptr(p),
+                                        L.ptr(f)))
+        out = s.copy()
+        for k in ("ncf", "nncf", "ncocc", "icocc", "cocc", "nce", "nnce", "ncvir", "icvir", "cvir"):
+            setattr(out, k, a[k].copy())
+        out.p, out.fao = p, f
+        return dict(escf=r.escf, ee=r.ee, niter=r.niter, converged=bool(r.converged), ovmax=r.ovmax,
+                    seconds=r.seconds, p=p, f=f, lmos=out)
+
+    def fock2z(self, ctx, p):
+        from . import lib as L
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        f = np.zeros(self.mpack)
+        L.check(ctx.lib.mhip_mozyme_fock2z(ctx.h, L.ptr(p), L.ptr(f)))
+        return f
+
+
+def reference_capacity(s):
+    """A copy of an LmoSet whose LMO arrays have the capacities the reference
+    allocates (set_up_MOZYME_arrays.F90:53-58), the LMOs kept where they are
+    and the extra room at the end (tidy spreads it over the LMOs)."""
+    numat, norbs = s.numat, s.norbs
+    ipad2 = min(numat + 10, int(round(numat ** 0.25 * 50)))
+    ipad4 = int(round(ipad2 * norbs / numat))
+    out = s.copy()
+
+    def grow(a, n, dtype):
+        b = np.zeros(max(n, len(a)), dtype=dtype)
+        b[:len(a)] = a
+        return b
+    out.icocc = grow(s.icocc, ipad2 * s.nocc + 1000, np.int32)
+    out.icvir = grow(s.icvir, ipad2 * s.nvir + 1000, np.int32)
+    out.cocc = grow(s.cocc, ipad4 * s.nocc + 1000, np.float64)
+    out.cvir = grow(s.cvir, ipad4 * s.nvir + 1000, np.float64)
+    return out
 
 
 def sparse_from_dense(m, nfirst, iorbs, nijbo, size):
@@ -175,14 +314,18 @@ def _pack(cols, nfirst, iorbs, keep, pad_atoms):
             np.array(idx, np.int32), np.array(coef))
 
 
-def from_scf(system, c, f_dense, p_dense, nocc, keep=1e-4, rcut=8.0, pad_atoms=6, sweeps=30):
-    """LMOs in MOZYME layout from canonical orbitals c (norbs x norbs)."""
+def from_scf(system, c, f_dense, p_dense, nocc, keep=1e-4, rcut=8.0, pad_atoms=6, sweeps=30, r_dipole=None):
+    """LMOs in MOZYME layout from canonical orbitals c (norbs x norbs); pairs
+    within rcut are stored, with r_dipole the dipole shell is marked -2."""
     nfirst = np.asarray(system.nfirst, dtype=np.int32)
     nlast = np.asarray(system.nlast, dtype=np.int32)
     iorbs = nlast - nfirst + 1
     occ = pipek_mezey(c[:, :nocc], nfirst, iorbs, sweeps)
     vir = pipek_mezey(c[:, nocc:], nfirst, iorbs, sweeps)
-    nijbo, size = build_nijbo(np.asarray(system.xyz), iorbs, rcut)
+    if r_dipole is None:
+        nijbo, size = build_nijbo(np.asarray(system.xyz), iorbs, rcut)
+    else:
+        nijbo, size = build_nijbo_shells(np.asarray(system.xyz, dtype=np.float64), iorbs, rcut, r_dipole)
     o = _pack(occ, nfirst, iorbs, keep, pad_atoms)
     v = _pack(vir, nfirst, iorbs, keep, pad_atoms)
     s = LmoSet(numat=system.numat, norbs=system.norbs, nfirst=nfirst, nlast=nlast,

@@ -50,6 +50,16 @@ SOURCES = [
     "utilities/ijbo.F90",
     "MOZYME/helecz.F90",
     "MOZYME/cnvgz.F90",
+    # the sparse Fock build and what it reads (linear_cosmo only for the
+    # module interface of its COSMO term, never reached with useps false)
+    "funcon_C.F90",
+    "solvation/afmm_C.F90",
+    "solvation/linear_cosmo.F90",
+    "MOZYME/chrge_for_MOZYME.F90",
+    "MOZYME/jab_for_MOZYME.F90",
+    "MOZYME/kab_for_MOZYME.F90",
+    "MOZYME/fock1_for_MOZYME.F90",
+    "MOZYME/fock2z.F90",
 ]
 HARNESSES = ["ref_harness.F90", "ref_harness_mozyme.F90"]
 

@@ -133,3 +133,119 @@ def iterate(system, pulay=False, scfcrt=1e-4, itrmax=2000, last=1, p_init=None, 
     k = nit.value
     return dict(rc=rc, converged=(rc == 0), escf=escf.value, ee=ee.value, niter=k, p=p, f=f, c=c, eig=e,
                 trace=tr[:min(k, trace_cap)], trace_pl=trp[:min(k, trace_cap)])
+
+
+# ---- MOZYME sparse Fock build (numpy over the C fock2) -----------------------
+
+A0, EV = 0.529177210903, 27.211386245988
+
+
+def _unpack_sparse(v, nfirst, iorbs, nijbo, n):
+    m = np.zeros((n, n))
+    f0 = np.asarray(nfirst) - 1
+    for a in range(len(iorbs)):
+        for b in range(a + 1):
+            o = nijbo[a, b]
+            if o < 0:
+                continue
+            ra, rb = slice(f0[a], f0[a] + iorbs[a]), slice(f0[b], f0[b] + iorbs[b])
+            if a == b:
+                t = np.zeros((iorbs[a], iorbs[a]))
+                t[np.tril_indices(iorbs[a])] = v[o:o + iorbs[a] * (iorbs[a] + 1) // 2]
+                m[ra, ra] = t + t.T - np.diag(np.diag(t))
+            else:
+                blk = v[o:o + iorbs[a] * iorbs[b]].reshape(iorbs[a], iorbs[b])
+                m[ra, rb] = blk
+                m[rb, ra] = blk.T
+    return m
+
+
+def _pack_sparse(m, nfirst, iorbs, nijbo, size):
+    out = np.zeros(size)
+    f0 = np.asarray(nfirst) - 1
+    for a in range(len(iorbs)):
+        for b in range(a + 1):
+            o = nijbo[a, b]
+            if o < 0:
+                continue
+            ra, rb = slice(f0[a], f0[a] + iorbs[a]), slice(f0[b], f0[b] + iorbs[b])
+            blk = m[ra, rb]
+            if a == b:
+                out[o:o + iorbs[a] * (iorbs[a] + 1) // 2] = blk[np.tril_indices(iorbs[a])]
+            else:
+                out[o:o + iorbs[a] * iorbs[b]] = blk.ravel()
+    return out
+
+
+def mozyme_fock2z(system, nijbo, mpack, h_sparse, am, ad, dd, p):
+    """buildf(f, partf, 0) with fock2z/fz2n for a discrete system, SEMIDIRECT
+    (src/MOZYME/buildf.F90:16-52, src/MOZYME/fock2z.F90:753-1362).
+
+    Stored pairs (nijbo >= 0): the same jab/kab/fock1 terms as the
+    conventional fock2 (fock2z.F90:813-1073), so the C fock2 runs on the dense
+    form of p with the integral blocks of the other pairs zeroed.  Other
+    pairs: the point-charge monopole with the electron populations of
+    chrge_for_MOZYME, and for nijbo = -2 the dipole terms, from am, ad, dd
+    (fock2z.F90:1074-1200).  `system` supplies nfirst, nlast, xyz and the
+    hcore-order integrals of every pair."""
+    n = system.norbs
+    nf = np.asarray(system.nfirst)
+    iorbs = (np.asarray(system.nlast) - nf + 1).astype(np.int64)
+    lim = iorbs * (iorbs + 1) // 2
+    w = np.array(system.w, dtype=np.float64)
+    kr = 0
+    for i in range(len(iorbs)):
+        for j in range(i):
+            m = lim[i] * lim[j]
+            if nijbo[i, j] < 0:
+                w[kr:kr + m] = 0.0
+            kr += m
+        kr += lim[i] * lim[i]
+
+    class _S:
+        pass
+    s2 = _S()
+    s2.norbs, s2.numat, s2.nfirst, s2.nlast, s2.w = n, system.numat, system.nfirst, system.nlast, w
+    pk = _unpack_sparse(p, nf, iorbs, nijbo, n)[np.tril_indices(n)]
+    g = fock2(s2, np.zeros(len(pk)), pk, 0.5 * pk)
+    gd = np.zeros((n, n))
+    gd[np.tril_indices(n)] = g
+    gd = gd + gd.T - np.diag(np.diag(gd))
+    f = np.asarray(h_sparse, dtype=np.float64) + _pack_sparse(gd, nf, iorbs, nijbo, mpack)
+    xyz = np.asarray(system.xyz, dtype=np.float64)
+    qe = np.array([sum(p[nijbo[a, a] + (k * (k + 1)) // 2 + k] for k in range(iorbs[a]))
+                   for a in range(len(iorbs))])
+    for A in range(len(iorbs)):
+        mono = dg = 0.0
+        sp = np.zeros(3)
+        for B in range(len(iorbs)):
+            if B == A or nijbo[A, B] >= 0:
+                continue
+            d = xyz[A] - xyz[B]
+            r2 = d @ d
+            aee = 0.5 / am[A] + 0.5 / am[B]
+            mono += qe[B] * EV / np.sqrt(r2 / A0 ** 2 + aee ** 2)
+            if nijbo[A, B] != -2:
+                continue
+            r = np.sqrt(r2)
+            u = d / r
+            if abs(u[2]) > 0.99999999:
+                u[2] = np.sign(u[2])
+            if iorbs[A] > 1:
+                ade = 0.5 / ad[A] + 0.5 / am[B]
+                rp, rm = np.hypot(r / A0 + dd[A], ade), np.hypot(r / A0 - dd[A], ade)
+                sp += qe[B] * (EV * (0.5 / rp - 0.5 / rm)) * u
+            if iorbs[B] > 1:
+                ade = 0.5 / ad[B] + 0.5 / am[A]
+                rp, rm = np.hypot(r / A0 + dd[B], ade), np.hypot(r / A0 - dd[B], ade)
+                sb = EV * (0.5 / rp - 0.5 / rm)
+                j1 = nijbo[B, B]
+                dg -= 2.0 * sb * (p[j1 + 1] * u[0] + p[j1 + 3] * u[1] + p[j1 + 6] * u[2])
+        i1 = nijbo[A, A]
+        for k in range(iorbs[A]):
+            f[i1 + (k * (k + 1)) // 2 + k] += mono + dg
+        if iorbs[A] > 1:
+            f[i1 + 1] += sp[0]
+            f[i1 + 3] += sp[1]
+            f[i1 + 6] += sp[2]
+    return f

@@ -45,6 +45,7 @@ def load():
         lib.mz_helecz.argtypes = [V, V, V]
         lib.mz_helecz.restype = D
         lib.mz_cnvgz.argtypes = [V, V, V, V, V, I, V, I, V]
+        lib.mz_fock2z.argtypes = [V, V, V, V, ctypes.c_longlong, V, V, V, D, D, V]
         _lib = lib
     return _lib
 
@@ -99,6 +100,25 @@ def pulay_seq(n, f_seq, p_seq):
 
 
 # ---- MOZYME (src/MOZYME/diagg1.F90, diagg2.F90, density_for_MOZYME.F90) ----
+
+A0, EV = 0.529177210903, 27.211386245988
+
+
+def mozyme_fock2z(ms, p):
+    """The reference's buildf(f, partf, 0) with fock2z (SEMIDIRECT storage) on
+    a mopac_gpu_amd.mozyme.MozymeSystem and a sparse density p."""
+    lib = load()
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    nij = np.asfortranarray(i32(ms.nijbo))
+    lib.mz_setup(ms.numat, ms.norbs, ms.mpack, P(i32(ms.nfirst)), P(i32(ms.nlast)), P(i32(ms.nat)), P(nij),
+                 1e-13, 0.0)
+    f = np.zeros(ms.mpack)
+    xyz = f64(ms.xyz)
+    w = f64(ms.w)
+    lib.mz_fock2z(P(xyz), P(f64(ms.am)), P(f64(ms.dd)), P(f64(ms.ad)), len(w), P(w), P(f64(p)), P(f64(ms.h)),
+                  A0, EV, P(f))
+    return f
 
 class Mozyme:
     """Loads an LmoSet (mopac_gpu_amd.mozyme) into the reference's module

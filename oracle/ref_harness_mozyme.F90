@@ -181,4 +181,49 @@ contains
     pmax_out = pmax
   end subroutine mz_cnvgz
 
+  ! buildf(f, partf, 0) with fock2z/fz2n (src/MOZYME/buildf.F90:16-52,
+  ! src/MOZYME/fock2z.F90): f = h + G(p) on the sparse storage of mz_setup,
+  ! discrete system (ione = 1), integrals as stored for SEMIDIRECT (direct
+  ! false, semidr true).  Per-atom parameters go into parameters_C at each
+  ! atom's element; a0 and ev into funcon_C.
+  subroutine mz_fock2z(xyz, am_in, dd_in, ad_in, nw, w_in, p_in, h_in, a0_in, ev_in, f_out) &
+      bind(C, name="mz_fock2z")
+    use molkst_C, only: numat, norbs, mpack, l_feather
+    use common_arrays_C, only: coord, p, nat, ifact
+    use MOZYME_C, only: direct, semidr, kopt
+    use parameters_C, only: am, dd, ad
+    use funcon_C, only: a0, ev
+    use cosmo_C, only: useps
+    integer(c_long_long), value :: nw
+    real(c_double) :: xyz(3, numat), am_in(numat), dd_in(numat), ad_in(numat)
+    real(c_double) :: w_in(nw), p_in(mpack), h_in(mpack), f_out(mpack)
+    real(c_double), value :: a0_in, ev_in
+    double precision, allocatable :: q(:), qe(:), ptot2(:, :)
+    integer :: i
+    if (allocated(coord)) deallocate (coord)
+    if (allocated(ifact)) deallocate (ifact)
+    if (allocated(kopt)) deallocate (kopt)
+    allocate (coord(3, numat), ifact(norbs + 1), kopt(numat), q(numat), qe(numat), ptot2(numat, 81))
+    coord = xyz
+    do i = 1, norbs + 1
+      ifact(i) = (i * (i - 1)) / 2
+    end do
+    kopt = 0
+    do i = 1, numat
+      am(nat(i)) = am_in(i)
+      dd(nat(i)) = dd_in(i)
+      ad(nat(i)) = ad_in(i)
+    end do
+    a0 = a0_in
+    ev = ev_in
+    direct = .false.
+    semidr = .true.
+    l_feather = .false.
+    useps = .false.
+    p = p_in
+    f_out = h_in
+    call fock2z(f_out, q, qe, w_in, w_in, ptot2, 0, 1)
+    deallocate (q, qe, ptot2)
+  end subroutine mz_fock2z
+
 end module ref_harness_mozyme

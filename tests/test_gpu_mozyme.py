@@ -119,3 +119,51 @@ def test_cnvgz_matches_reference(gpu_ctx, helix_lmos, niter, three):
     assert pm.value == pm_ref
     for a, b in zip(gpu_arrays, ref_arrays):
         assert np.max(np.abs(a - b)) < 1e-13
+
+
+@pytest.mark.parametrize("r_near,r_dipole", [(6.0, 7.0), (4.0, 6.0), (30.0, 31.0)])
+def test_fock2z_matches_reference(gpu_ctx, r_near, r_dipole):
+    """buildf/fock2z on the device (stored pairs, dipole shell, point charges)
+    against the reference's own fz2n, SEMIDIRECT, and against the oracle."""
+    nat, xyz = geometry.glycine_helix(6, 3)
+    s = hamiltonian.build(nat, xyz)
+    ms = mozyme.MozymeSystem(s, r_near=r_near, r_dipole=r_dipole)
+    rng = np.random.default_rng(1)
+    pd = rng.standard_normal((s.norbs, s.norbs)) * 0.1
+    pd = pd + pd.T + np.eye(s.norbs)
+    p = mozyme.sparse_from_dense(pd, ms.nfirst, ms.nlast - ms.nfirst + 1, ms.nijbo, ms.mpack)
+    print(f"fock2z helix {ms.numat} atoms, near {r_near}, dipole {r_dipole}", flush=True)
+    ms.upload(gpu_ctx)
+    f = ms.fock2z(gpu_ctx, p)
+    f_ref = REF.mozyme_fock2z(ms, p)
+    scale = np.max(np.abs(f_ref))
+    assert np.max(np.abs(f - f_ref)) < 1e-11 * scale
+    f_o = oracle.mozyme_fock2z(s, ms.nijbo, ms.mpack, ms.h, ms.am, ms.ad, ms.dd, p)
+    assert np.max(np.abs(f - f_o)) < 1e-11 * scale
+
+
+def test_mozyme_scf_reaches_the_conventional_energy(gpu_ctx):
+    """iter_for_MOZYME on the device from LMOs of an unconverged start (two
+    conventional iterations), every atom pair stored: at MOZYME's default
+    criterion (SELCON 0.01 kcal/mol) the heat of formation is the
+    conventional SCF's within 0.05 kcal/mol."""
+    nat, xyz = geometry.glycine_helix(6, 3)
+    s = hamiltonian.build(nat, xyz)
+    n = s.norbs
+    gpu_ctx.upload(s)
+    conv = gpu_ctx.iter(scfcrt=1e-6)
+    r2 = gpu_ctx.iter(scfcrt=1e-6, itrmax=2, want=("p", "f", "c", "eig"))
+    fd, pd = _unpack(r2["f"], n), _unpack(r2["p"], n)
+    lmos = mozyme.from_scf(s, r2["c"], fd, pd, s.nclose, keep=1e-8, rcut=30.0)
+    assert (lmos.nijbo >= 0).all()
+    ms = mozyme.MozymeSystem(s, nijbo=lmos.nijbo, mpack=lmos.mpack)
+    ms.upload(gpu_ctx)
+    print(f"MOZYME SCF helix {s.numat} atoms from a two-iteration start", flush=True)
+    r = ms.scf(gpu_ctx, lmos)
+    print(f"escf {r['escf']:.6f} conventional {conv['escf']:.6f} niter {r['niter']}", flush=True)
+    assert r["converged"] and 3 < r["niter"] < 100
+    assert abs(r["escf"] - conv["escf"]) < 0.05
+    # the density left by the loop carries 2 electrons per occupied LMO
+    d = np.array([ms.nijbo[a, a] + (k * (k + 1)) // 2 + k for a in range(s.numat)
+                  for k in range(int(s.nlast[a] - s.nfirst[a] + 1))])
+    assert abs(np.sum(r["p"][d]) - 2 * s.nclose) < 1e-3 * s.nclose

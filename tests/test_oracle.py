@@ -260,3 +260,24 @@ def test_localized_orbitals_span_the_occupied_space():
     q = mozyme.pipek_mezey(c, L.nfirst, L.iorbs)
     assert np.max(np.abs(q.T @ q - np.eye(s.nclose))) < 1e-12
     assert np.max(np.abs(q @ q.T - c @ c.T)) < 1e-12
+
+
+@pytest.mark.skipif(not REF.available(), reason="oracle/_ref not built (needs the reference sources)")
+@pytest.mark.parametrize("r_near,r_dipole", [(6.0, 7.0), (4.0, 6.0), (30.0, 31.0)])
+def test_mozyme_fock2z_matches_reference_fortran(r_near, r_dipole):
+    """The restatement of buildf/fock2z (stored pairs through the C fock2,
+    point-charge and dipole terms for the others) against the reference's
+    own fz2n, SEMIDIRECT, on a poly-glycine helix with all three pair classes."""
+    from mopac_gpu_amd import mozyme
+    nat, xyz = geometry.glycine_helix(6, 3)
+    s = hamiltonian.build(nat, xyz)
+    ms = mozyme.MozymeSystem(s, r_near=r_near, r_dipole=r_dipole)
+    if r_near < 10:
+        assert (ms.nijbo == -2).any() and (ms.nijbo == -1).any()
+    rng = np.random.default_rng(1)
+    pd = rng.standard_normal((s.norbs, s.norbs)) * 0.1
+    pd = pd + pd.T + np.eye(s.norbs)
+    p = mozyme.sparse_from_dense(pd, ms.nfirst, ms.nlast - ms.nfirst + 1, ms.nijbo, ms.mpack)
+    f_ref = REF.mozyme_fock2z(ms, p)
+    f = oracle.mozyme_fock2z(s, ms.nijbo, ms.mpack, ms.h, ms.am, ms.ad, ms.dd, p)
+    assert np.max(np.abs(f - f_ref)) < 1e-11 * np.max(np.abs(f_ref))
