@@ -271,6 +271,8 @@ typedef struct {
   const double* h;      /* mpack */
   const double* w;      /* nw */
   long long nw;
+  int feather;          /* l_feather (1 for PM7): smooth switch to point charges */
+  double trunc1, trunc2;/* its radius (Angstrom) and Gaussian exponent (molkst_C trunc_1, trunc_2) */
 } mhip_mozyme_system;
 /* Upload a MOZYME system into the context (replaces the conventional one). */
 int mhip_mozyme_setup(mhip_ctx* ctx, const mhip_mozyme_system* sys);
@@ -415,6 +417,14 @@ double mhip_ev_to_kcal(void);
  * calpar): out[0..4] = natorb, tore, am, ad, dd (atomic units).  Elements 1,
  * 6, 7, 8; -1 for any other. */
 int mhip_pm7_element(int z, double* out);
+/* The one-electron matrix on the MOZYME sparse storage (nijbo, mpack) and
+ * the core-core energy as hcore_for_MOZYME forms them for a discrete system
+ * (src/MOZYME/hcore_for_MOZYME.F90:100-260): NDDO terms for stored pairs,
+ * outer2's point charge + dipole for the nijbo = -2 shell, outer1's point
+ * charge for the rest.  The near-pair integrals are the conventional ones
+ * (mhip_pm7_hamiltonian's w without the other pairs' blocks). */
+int mhip_pm7_hamiltonian_mozyme(int numat, const int* nat, const double* coord, const int* nijbo,
+                                long long mpack, double* h, double* enuclr);
 
 #ifdef __cplusplus
 }

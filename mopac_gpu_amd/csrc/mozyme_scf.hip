@@ -37,6 +37,8 @@ struct MzFock {
   DevBuf xyz, am, dd, ad;
   DevBuf qe;            // electron population per atom
   DevBuf ps, fs;        // sparse P and F (device copies for the host entry point)
+  int feather = 0;      // PM7's smooth switch to point charges
+  double trunc1 = 0.0, trunc2 = 0.0;
 };
 
 namespace {
@@ -80,7 +82,9 @@ __global__ void k_mzf_qe(int numat, const int* __restrict__ iorbs, const int* __
   qe[a] = sum;
 }
 
-// The far-pair terms of fz2n (fock2z.F90:1074-1200) for atom A: over every
+// The far-pair terms of fz2n (fock2z.F90:1074-1200) for atom A (with
+// l_feather, PM7's default, the monopole blends into the bare Coulomb
+// ev a0 / r and the dipole integrals fade out below trunc1): over every
 // atom B whose pair is not stored, the monopole qe(B) w1 on A's diagonal;
 // for the dipole shell (nijbo = -2) also qe(B) times A's dipole integrals on
 // A's s-p elements (A with p orbitals) and 2 P_B(s,p) times B's dipole
@@ -93,6 +97,7 @@ __global__ __launch_bounds__(256) void k_mzf_far(int numat, const int* __restric
                                                  const double* __restrict__ xyz, const double* __restrict__ am,
                                                  const double* __restrict__ dd, const double* __restrict__ ad,
                                                  const double* __restrict__ qe, const double* __restrict__ ps,
+                                                 int feather, double trunc1, double trunc2,
                                                  double* __restrict__ fs) {
   const int A = blockIdx.x;
   const int na = iorbs[A];
@@ -108,17 +113,24 @@ __global__ __launch_bounds__(256) void k_mzf_far(int numat, const int* __restric
       const double dx = xa - xyz[3 * B], dy = ya - xyz[3 * B + 1], dz = za - xyz[3 * B + 2];
       const double r2 = dx * dx + dy * dy + dz * dz;
       const double aee = 0.5 / am[A] + 0.5 / am[B];
-      const double w1 = kEV / sqrt(r2 / (kA0 * kA0) + aee * aee);
+      double w1 = kEV / sqrt(r2 / (kA0 * kA0) + aee * aee);
+      const double r = sqrt(r2);
+      // PM7's smooth switch to point charges (to_point, src/integrals/mndod.F90:3462-3486)
+      double cf = 1.0;
+      if (feather) {
+        cf = r < trunc1 ? 1.0 - exp(-(r - trunc1) * (r - trunc1) * trunc2) : 0.0;
+        w1 = w1 * cf + (1.0 - cf) * (kEV * kA0 / r);
+      }
       mono += qe[B] * w1;
       if (o != -2) continue;
-      const double r = sqrt(r2);
       double ux = dx / r, uy = dy / r, uz = dz / r;  // from B to A
       if (fabs(uz) > 0.99999999) uz = copysign(1.0, uz);
       if (na > 1) {
         const double da = dd[A], ade = 0.5 / ad[A] + 0.5 / am[B];
         const double rp = sqrt((r / kA0 + da) * (r / kA0 + da) + ade * ade);
         const double rm = sqrt((r / kA0 - da) * (r / kA0 - da) + ade * ade);
-        const double s = kEV * (0.5 / rp - 0.5 / rm);
+        double s = kEV * (0.5 / rp - 0.5 / rm);
+        if (feather) s = s * cf;
         sp0 += qe[B] * (s * ux);
         sp1 += qe[B] * (s * uy);
         sp2 += qe[B] * (s * uz);
@@ -127,7 +139,8 @@ __global__ __launch_bounds__(256) void k_mzf_far(int numat, const int* __restric
         const double da = dd[B], ade = 0.5 / ad[B] + 0.5 / am[A];
         const double rp = sqrt((r / kA0 + da) * (r / kA0 + da) + ade * ade);
         const double rm = sqrt((r / kA0 - da) * (r / kA0 - da) + ade * ade);
-        const double s = kEV * (0.5 / rp - 0.5 / rm);  // along -u
+        double s = kEV * (0.5 / rp - 0.5 / rm);  // along -u
+        if (feather) s = s * cf;
         const long long j1 = diag[B];
         dg += ps[j1 + 1] * (s * -ux) * 2.0 + ps[j1 + 3] * (s * -uy) * 2.0 + ps[j1 + 6] * (s * -uz) * 2.0;
       }
@@ -193,7 +206,7 @@ void mz_fock2z(mhip_ctx* c, const double* ps, double* fs) {
   MHIP_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_mzf_far, dim3(z->numat), dim3(256), 0, s, z->numat, z->nijbo.as<int>(), z->iorbs.as<int>(),
                      z->diag.as<int>(), z->xyz.as<double>(), z->am.as<double>(), z->dd.as<double>(),
-                     z->ad.as<double>(), z->qe.as<double>(), ps, fs);
+                     z->ad.as<double>(), z->qe.as<double>(), ps, z->feather, z->trunc1, z->trunc2, fs);
   MHIP_LAUNCH_CHECK();
 }
 
@@ -270,6 +283,9 @@ int mhip_mozyme_setup(mhip_ctx* c, const mhip_mozyme_system* sys) {
     MzFock* z = c->mz;
     z->numat = n;
     z->norbs = norbs;
+    z->feather = sys->feather;
+    z->trunc1 = sys->trunc1;
+    z->trunc2 = sys->trunc2;
     z->mpack = sys->mpack;
     hipStream_t s = c->stream;
     upload(z->map, map.data(), sizeof(int) * map.size(), s);

@@ -950,6 +950,126 @@ int mhip_pm7_hamiltonian(int numat, const int* nat, const double* coord, int cha
 
 double mhip_ev_to_kcal(void) { return kFPC9; }
 
+// The one-electron matrix on the MOZYME sparse storage and the core-core
+// energy, as hcore_for_MOZYME forms them for a discrete system
+// (src/MOZYME/hcore_for_MOZYME.F90:100-260): stored pairs (nijbo >= 0) take
+// the NDDO terms (resonance block, electron-core attraction, core-core);
+// pairs of the dipole shell (-2) outer2's point charge plus dipole
+// (src/MOZYME/outer2.F90:16-150, from the NDDO (ss|ss), (s sigma|ss) and
+// (ss|s sigma) integrals); the others outer1's point charge
+// (src/MOZYME/outer1.F90:16-72) with PM7's switch to the bare Coulomb.
+int mhip_pm7_hamiltonian_mozyme(int numat, const int* nat, const double* coord, const int* nijbo,
+                                long long mpack, double* h, double* enuclr) {
+  init_params();
+  int norbs = 0;
+  long long nw = 0;
+  if (mhip_pm7_sizes(numat, nat, &norbs, &nw) != 0) return -1;
+  auto nij = [&](int a, int b) { return nijbo[a + (size_t)b * numat]; };
+  for (int i = 0; i < numat; ++i) {
+    const int no = g_elem[nat[i]].natorb;
+    if (nij(i, i) < 0 || nij(i, i) + no * (no + 1) / 2 > mpack) return -1;
+    for (int j = 0; j < i; ++j)
+      if (nij(i, j) >= 0 && nij(i, j) + no * g_elem[nat[j]].natorb > mpack) return -1;
+  }
+  for (long long q = 0; q < mpack; ++q) h[q] = 0.0;
+  for (int i = 0; i < numat; ++i) {
+    const Elem& E = g_elem[nat[i]];
+    for (int a = 0; a < E.natorb; ++a) h[nij(i, i) + a * (a + 1) / 2 + a] = (a == 0) ? E.uss : E.upp;
+  }
+  const long long npairs = (long long)numat * (numat - 1) / 2;
+  std::vector<double> e1b_acc((size_t)numat * 10, 0.0), enuc_row(numat, 0.0);
+  std::vector<double> e2a_pair((size_t)npairs * 10, 0.0);
+  std::atomic<int> next_row{0};
+  auto worker = [&]() {
+    for (int i = next_row++; i < numat; i = next_row++) {
+      const Elem& A = g_elem[nat[i]];
+      const int li = A.natorb;
+      double rowsum = 0.0;
+      for (int j = 0; j < i; ++j) {
+        const Elem& B = g_elem[nat[j]];
+        const int lj = B.natorb;
+        double e1b[10] = {0}, e2a[10] = {0}, enuc = 0.0;
+        const int o = nij(i, j);
+        if (o >= 0) {
+          PairOut po;
+          pair_integrals(A, B, coord + 3 * i, coord + 3 * j, po);
+          for (int a = 0; a < li; ++a)
+            for (int b = 0; b < lj; ++b) h[o + a * lj + b] += po.di[a][b];
+          std::memcpy(e1b, po.e1b, sizeof(e1b));
+          std::memcpy(e2a, po.e2a, sizeof(e2a));
+          enuc = po.enuc;
+        } else {
+          const double* xi = coord + 3 * i;
+          const double* xj = coord + 3 * j;
+          double x[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+          const double rij = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+          double w1;
+          if (o == -2) {
+            double ri[22], gab;
+            local_repulsion(A, B, rij, ri, gab);
+            w1 = ri[0];
+            for (int k = 0; k < 3; ++k) x[k] /= rij;
+            if (std::fabs(x[2]) > 0.99999999) x[2] = std::copysign(1.0, x[2]);
+            // the (p, s) elements: outer2's -w tore with w = -ri(2) x (atom i) and
+            // -ri(5) x (atom j); the sigma-odd integrals of local_repulsion carry
+            // the opposite sign to the reference's (its sigma axis runs from B
+            // to A), hence -ri here, which also makes these terms continue the
+            // NDDO attraction across the edge of the stored pairs
+            if (li > 1) {
+              e1b[1] = -ri[1] * x[0] * B.tore;
+              e1b[3] = -ri[1] * x[1] * B.tore;
+              e1b[6] = -ri[1] * x[2] * B.tore;
+            }
+            if (lj > 1) {
+              e2a[1] = -ri[4] * x[0] * A.tore;
+              e2a[3] = -ri[4] * x[1] * A.tore;
+              e2a[6] = -ri[4] * x[2] * A.tore;
+            }
+          } else {
+            const double r = rij / kA0;
+            double aee = 0.5 / A.am + 0.5 / B.am;
+            aee = aee * aee;
+            w1 = kEV / std::sqrt(r * r + aee);
+            double point;
+            const double cf = feather(rij, point);
+            w1 = w1 * cf + (1.0 - cf) * point;
+          }
+          for (int a = 0; a < li; ++a) e1b[pidx(a, a)] = -w1 * B.tore;
+          for (int a = 0; a < lj; ++a) e2a[pidx(a, a)] = -w1 * A.tore;
+          enuc = A.tore * B.tore * w1;
+        }
+        rowsum += enuc;
+        for (int q = 0; q < 10; ++q) e1b_acc[(size_t)i * 10 + q] += e1b[q];
+        const long long pid = (long long)i * (i - 1) / 2 + j;
+        for (int q = 0; q < 10; ++q) e2a_pair[(size_t)pid * 10 + q] = e2a[q];
+      }
+      enuc_row[i] = rowsum;
+    }
+  };
+  unsigned nthr = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (numat < 64) nthr = 1;
+  std::vector<std::thread> pool;
+  for (unsigned t2 = 1; t2 < nthr; ++t2) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
+  double en = 0.0;
+  for (int i = 0; i < numat; ++i) {
+    en += enuc_row[i];
+    const int li = g_elem[nat[i]].natorb;
+    for (int a = 0; a < li; ++a)
+      for (int b = 0; b <= a; ++b) h[nij(i, i) + pidx(a, b)] += e1b_acc[(size_t)i * 10 + pidx(a, b)];
+  }
+  for (int i = 0; i < numat; ++i)
+    for (int j = 0; j < i; ++j) {
+      const long long pid = (long long)i * (i - 1) / 2 + j;
+      const int lj = g_elem[nat[j]].natorb;
+      for (int a = 0; a < lj; ++a)
+        for (int b = 0; b <= a; ++b) h[nij(j, j) + pidx(a, b)] += e2a_pair[(size_t)pid * 10 + pidx(a, b)];
+    }
+  *enuclr = en;
+  return 0;
+}
+
 int mhip_pm7_element(int z, double* out) {
   init_params();
   if (z < 0 || z >= 9 || !g_elem[z].ok || !out) return -1;

@@ -40,7 +40,8 @@ class ScfOpts(ctypes.Structure):
 class MozymeSystemC(ctypes.Structure):
     _fields_ = [("numat", c_int), ("norbs", c_int), ("mpack", c_ll), ("nfirst", c_void_p), ("nlast", c_void_p),
                 ("nijbo", c_void_p), ("xyz", c_void_p), ("am", c_void_p), ("dd", c_void_p), ("ad", c_void_p),
-                ("h", c_void_p), ("w", c_void_p), ("nw", c_ll)]
+                ("h", c_void_p), ("w", c_void_p), ("nw", c_ll), ("feather", c_int), ("trunc1", c_double),
+                ("trunc2", c_double)]
 
 
 class GuessModelC(ctypes.Structure):
@@ -145,6 +146,7 @@ SIGNATURES = {
     "mhip_guess_density": (c_int, [c_void_p, c_void_p]),
     "mhip_mozyme_setup": (c_int, [c_void_p, c_void_p]),
     "mhip_pm7_element": (c_int, [c_int, c_void_p]),
+    "mhip_pm7_hamiltonian_mozyme": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_ll, c_void_p, ctypes.POINTER(c_double)]),
     "mhip_mozyme_fock2z": (c_int, [c_void_p, c_void_p, c_void_p]),
     "mhip_mozyme_scf": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhip_pm7_sizes": (c_int, [c_int, c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_ll)]),

@@ -124,7 +124,9 @@ class MozymeSystem:
     """A system in MOZYME storage: sparse H, integrals for SEMIDIRECT, nijbo
     with near / dipole / point-charge pairs, per-atom parameters."""
 
-    def __init__(self, system, r_near=6.0, r_dipole=7.0, nijbo=None, mpack=None):
+    TRUNC1, TRUNC2 = 7.0, 0.22  # PM7's switch to point charges (molkst_C trunc_1, trunc_2)
+
+    def __init__(self, system, r_near=6.0, r_dipole=7.0, nijbo=None, mpack=None, feather=True):
         self.s = system
         self.numat, self.norbs = system.numat, system.norbs
         self.nat = np.asarray(system.nat, dtype=np.int32)
@@ -136,12 +138,19 @@ class MozymeSystem:
             self.nijbo, self.mpack = build_nijbo_shells(self.xyz, iorbs, r_near, r_dipole)
         else:
             self.nijbo, self.mpack = np.asarray(nijbo, dtype=np.int32), int(mpack)
-        hd = np.zeros((self.norbs, self.norbs))
-        hd[np.tril_indices(self.norbs)] = system.h
-        hd = hd + hd.T - np.diag(np.diag(hd))
-        self.h = sparse_from_dense(hd, self.nfirst, iorbs, self.nijbo, self.mpack)
+        # one-electron matrix and core-core energy as hcore_for_MOZYME forms them
+        from . import lib as L
+        self.h = np.zeros(self.mpack)
+        en = ctypes.c_double()
+        rc = L.load().mhip_pm7_hamiltonian_mozyme(
+            self.numat, L.ptr(np.ascontiguousarray(self.nat)), L.ptr(self.xyz),
+            L.ptr(np.asfortranarray(self.nijbo.astype(np.int32))), self.mpack, L.ptr(self.h), ctypes.byref(en))
+        if rc != 0:
+            raise ValueError("mhip_pm7_hamiltonian_mozyme: unsupported element or nijbo beyond mpack")
+        self.enuclr = en.value
         self.w = mozyme_integrals(system, self.nijbo)
         self.am, self.ad, self.dd = element_parameters(self.nat)
+        self.feather = bool(feather)
 
     def upload(self, ctx):
         from . import lib as L
@@ -152,7 +161,7 @@ class MozymeSystem:
         nf, nl, nij, xyz, am, dd, ad, h, w = self._keep
         sysc = L.MozymeSystemC(self.numat, self.norbs, self.mpack, nf.ctypes.data, nl.ctypes.data,
                                nij.ctypes.data, xyz.ctypes.data, am.ctypes.data, dd.ctypes.data, ad.ctypes.data,
-                               h.ctypes.data, w.ctypes.data, len(w))
+                               h.ctypes.data, w.ctypes.data, len(w), int(self.feather), self.TRUNC1, self.TRUNC2)
         L.check(ctx.lib.mhip_mozyme_setup(ctx.h, ctypes.byref(sysc)))
 
     def scf(self, ctx, lmos, scfcrt=1e-2, itrmax=2000, thresh=1e-13, fmo_dim=0):
@@ -171,7 +180,7 @@ class MozymeSystem:
         for k, v in a.items():
             setattr(c, k, v.ctypes.data)
         o = L.MozymeOptsC(scfcrt=scfcrt, itrmax=itrmax, thresh=thresh, damp=1.0, fmo_dim=fmo_dim,
-                          enuclr=self.s.enuclr, atheat=self.s.atheat)
+                          enuclr=self.enuclr, atheat=self.s.atheat)
         r = L.MozymeResultC()
         p = np.zeros(self.mpack)
         f = np.zeros(self.mpack)

@@ -59,6 +59,8 @@ SOURCES = [
     "MOZYME/jab_for_MOZYME.F90",
     "MOZYME/kab_for_MOZYME.F90",
     "MOZYME/fock1_for_MOZYME.F90",
+    "integrals/mndod_C.F90",
+    "integrals/mndod.F90",
     "MOZYME/fock2z.F90",
 ]
 HARNESSES = ["ref_harness.F90", "ref_harness_mozyme.F90"]

@@ -177,7 +177,7 @@ def _pack_sparse(m, nfirst, iorbs, nijbo, size):
     return out
 
 
-def mozyme_fock2z(system, nijbo, mpack, h_sparse, am, ad, dd, p):
+def mozyme_fock2z(system, nijbo, mpack, h_sparse, am, ad, dd, p, feather=False, trunc1=7.0, trunc2=0.22):
     """buildf(f, partf, 0) with fock2z/fz2n for a discrete system, SEMIDIRECT
     (src/MOZYME/buildf.F90:16-52, src/MOZYME/fock2z.F90:753-1362).
 
@@ -186,8 +186,10 @@ def mozyme_fock2z(system, nijbo, mpack, h_sparse, am, ad, dd, p):
     form of p with the integral blocks of the other pairs zeroed.  Other
     pairs: the point-charge monopole with the electron populations of
     chrge_for_MOZYME, and for nijbo = -2 the dipole terms, from am, ad, dd
-    (fock2z.F90:1074-1200).  `system` supplies nfirst, nlast, xyz and the
-    hcore-order integrals of every pair."""
+    (fock2z.F90:1074-1200); with feather (l_feather, PM7) the monopole
+    blends into ev a0 / r and the dipole terms fade by to_point's factor
+    (src/integrals/mndod.F90:3462-3486).  `system` supplies nfirst, nlast,
+    xyz and the hcore-order integrals of every pair."""
     n = system.norbs
     nf = np.asarray(system.nfirst)
     iorbs = (np.asarray(system.nlast) - nf + 1).astype(np.int64)
@@ -224,21 +226,26 @@ def mozyme_fock2z(system, nijbo, mpack, h_sparse, am, ad, dd, p):
             d = xyz[A] - xyz[B]
             r2 = d @ d
             aee = 0.5 / am[A] + 0.5 / am[B]
-            mono += qe[B] * EV / np.sqrt(r2 / A0 ** 2 + aee ** 2)
+            w1 = EV / np.sqrt(r2 / A0 ** 2 + aee ** 2)
+            r = np.sqrt(r2)
+            cf = 1.0
+            if feather:
+                cf = 1.0 - np.exp(-(r - trunc1) ** 2 * trunc2) if r < trunc1 else 0.0
+                w1 = w1 * cf + (1.0 - cf) * EV * A0 / r
+            mono += qe[B] * w1
             if nijbo[A, B] != -2:
                 continue
-            r = np.sqrt(r2)
             u = d / r
             if abs(u[2]) > 0.99999999:
                 u[2] = np.sign(u[2])
             if iorbs[A] > 1:
                 ade = 0.5 / ad[A] + 0.5 / am[B]
                 rp, rm = np.hypot(r / A0 + dd[A], ade), np.hypot(r / A0 - dd[A], ade)
-                sp += qe[B] * (EV * (0.5 / rp - 0.5 / rm)) * u
+                sp += qe[B] * (EV * (0.5 / rp - 0.5 / rm) * cf) * u
             if iorbs[B] > 1:
                 ade = 0.5 / ad[B] + 0.5 / am[A]
                 rp, rm = np.hypot(r / A0 + dd[B], ade), np.hypot(r / A0 - dd[B], ade)
-                sb = EV * (0.5 / rp - 0.5 / rm)
+                sb = EV * (0.5 / rp - 0.5 / rm) * cf
                 j1 = nijbo[B, B]
                 dg -= 2.0 * sb * (p[j1 + 1] * u[0] + p[j1 + 3] * u[1] + p[j1 + 6] * u[2])
         i1 = nijbo[A, A]

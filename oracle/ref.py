@@ -45,7 +45,7 @@ def load():
         lib.mz_helecz.argtypes = [V, V, V]
         lib.mz_helecz.restype = D
         lib.mz_cnvgz.argtypes = [V, V, V, V, V, I, V, I, V]
-        lib.mz_fock2z.argtypes = [V, V, V, V, ctypes.c_longlong, V, V, V, D, D, V]
+        lib.mz_fock2z.argtypes = [V, V, V, V, ctypes.c_longlong, V, V, V, D, D, I, D, D, V]
         _lib = lib
     return _lib
 
@@ -102,9 +102,10 @@ def pulay_seq(n, f_seq, p_seq):
 # ---- MOZYME (src/MOZYME/diagg1.F90, diagg2.F90, density_for_MOZYME.F90) ----
 
 A0, EV = 0.529177210903, 27.211386245988
+TRUNC1, TRUNC2 = 7.0, 0.22  # PM7's point-charge switch (the values the PM7 producer uses)
 
 
-def mozyme_fock2z(ms, p):
+def mozyme_fock2z(ms, p, feather=False):
     """The reference's buildf(f, partf, 0) with fock2z (SEMIDIRECT storage) on
     a mopac_gpu_amd.mozyme.MozymeSystem and a sparse density p."""
     lib = load()
@@ -117,7 +118,7 @@ def mozyme_fock2z(ms, p):
     xyz = f64(ms.xyz)
     w = f64(ms.w)
     lib.mz_fock2z(P(xyz), P(f64(ms.am)), P(f64(ms.dd)), P(f64(ms.ad)), len(w), P(w), P(f64(p)), P(f64(ms.h)),
-                  A0, EV, P(f))
+                  A0, EV, int(feather), TRUNC1, TRUNC2, P(f))
     return f
 
 class Mozyme:

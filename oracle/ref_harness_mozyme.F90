@@ -184,11 +184,12 @@ contains
   ! buildf(f, partf, 0) with fock2z/fz2n (src/MOZYME/buildf.F90:16-52,
   ! src/MOZYME/fock2z.F90): f = h + G(p) on the sparse storage of mz_setup,
   ! discrete system (ione = 1), integrals as stored for SEMIDIRECT (direct
-  ! false, semidr true).  Per-atom parameters go into parameters_C at each
-  ! atom's element; a0 and ev into funcon_C.
-  subroutine mz_fock2z(xyz, am_in, dd_in, ad_in, nw, w_in, p_in, h_in, a0_in, ev_in, f_out) &
-      bind(C, name="mz_fock2z")
-    use molkst_C, only: numat, norbs, mpack, l_feather
+  ! false, semidr true), with or without PM7's smooth switch to point charges
+  ! (l_feather, trunc_1, trunc_2).  Per-atom parameters go into parameters_C
+  ! at each atom's element; a0 and ev into funcon_C.
+  subroutine mz_fock2z(xyz, am_in, dd_in, ad_in, nw, w_in, p_in, h_in, a0_in, ev_in, feather, trunc1, trunc2, &
+      f_out) bind(C, name="mz_fock2z")
+    use molkst_C, only: numat, norbs, mpack, l_feather, trunc_1, trunc_2
     use common_arrays_C, only: coord, p, nat, ifact
     use MOZYME_C, only: direct, semidr, kopt
     use parameters_C, only: am, dd, ad
@@ -197,7 +198,8 @@ contains
     integer(c_long_long), value :: nw
     real(c_double) :: xyz(3, numat), am_in(numat), dd_in(numat), ad_in(numat)
     real(c_double) :: w_in(nw), p_in(mpack), h_in(mpack), f_out(mpack)
-    real(c_double), value :: a0_in, ev_in
+    real(c_double), value :: a0_in, ev_in, trunc1, trunc2
+    integer(c_int), value :: feather
     double precision, allocatable :: q(:), qe(:), ptot2(:, :)
     integer :: i
     if (allocated(coord)) deallocate (coord)
@@ -218,7 +220,9 @@ contains
     ev = ev_in
     direct = .false.
     semidr = .true.
-    l_feather = .false.
+    l_feather = feather /= 0
+    trunc_1 = trunc1
+    trunc_2 = trunc2
     useps = .false.
     p = p_in
     f_out = h_in

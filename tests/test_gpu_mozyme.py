@@ -121,13 +121,14 @@ def test_cnvgz_matches_reference(gpu_ctx, helix_lmos, niter, three):
         assert np.max(np.abs(a - b)) < 1e-13
 
 
-@pytest.mark.parametrize("r_near,r_dipole", [(6.0, 7.0), (4.0, 6.0), (30.0, 31.0)])
-def test_fock2z_matches_reference(gpu_ctx, r_near, r_dipole):
+@pytest.mark.parametrize("r_near,r_dipole,feather", [(6.0, 7.0, False), (4.0, 6.0, False), (30.0, 31.0, False),
+                                                     (4.0, 6.0, True), (6.0, 8.0, True)])
+def test_fock2z_matches_reference(gpu_ctx, r_near, r_dipole, feather):
     """buildf/fock2z on the device (stored pairs, dipole shell, point charges)
     against the reference's own fz2n, SEMIDIRECT, and against the oracle."""
     nat, xyz = geometry.glycine_helix(6, 3)
     s = hamiltonian.build(nat, xyz)
-    ms = mozyme.MozymeSystem(s, r_near=r_near, r_dipole=r_dipole)
+    ms = mozyme.MozymeSystem(s, r_near=r_near, r_dipole=r_dipole, feather=feather)
     rng = np.random.default_rng(1)
     pd = rng.standard_normal((s.norbs, s.norbs)) * 0.1
     pd = pd + pd.T + np.eye(s.norbs)
@@ -135,18 +136,20 @@ def test_fock2z_matches_reference(gpu_ctx, r_near, r_dipole):
     print(f"fock2z helix {ms.numat} atoms, near {r_near}, dipole {r_dipole}", flush=True)
     ms.upload(gpu_ctx)
     f = ms.fock2z(gpu_ctx, p)
-    f_ref = REF.mozyme_fock2z(ms, p)
+    f_ref = REF.mozyme_fock2z(ms, p, feather=feather)
     scale = np.max(np.abs(f_ref))
     assert np.max(np.abs(f - f_ref)) < 1e-11 * scale
-    f_o = oracle.mozyme_fock2z(s, ms.nijbo, ms.mpack, ms.h, ms.am, ms.ad, ms.dd, p)
+    f_o = oracle.mozyme_fock2z(s, ms.nijbo, ms.mpack, ms.h, ms.am, ms.ad, ms.dd, p, feather=feather)
     assert np.max(np.abs(f - f_o)) < 1e-11 * scale
 
 
-def test_mozyme_scf_reaches_the_conventional_energy(gpu_ctx):
+@pytest.mark.parametrize("r_near,r_dipole", [(30.0, 31.0), (8.0, 9.0)])
+def test_mozyme_scf_reaches_the_conventional_energy(gpu_ctx, r_near, r_dipole):
     """iter_for_MOZYME on the device from LMOs of an unconverged start (two
-    conventional iterations), every atom pair stored: at MOZYME's default
-    criterion (SELCON 0.01 kcal/mol) the heat of formation is the
-    conventional SCF's within 0.05 kcal/mol."""
+    conventional iterations), with every atom pair stored or with point
+    charges and dipoles beyond 8 A: at MOZYME's default criterion (SELCON
+    0.01 kcal/mol) the heat of formation is the conventional SCF's within
+    0.05 kcal/mol."""
     nat, xyz = geometry.glycine_helix(6, 3)
     s = hamiltonian.build(nat, xyz)
     n = s.norbs
@@ -154,8 +157,8 @@ def test_mozyme_scf_reaches_the_conventional_energy(gpu_ctx):
     conv = gpu_ctx.iter(scfcrt=1e-6)
     r2 = gpu_ctx.iter(scfcrt=1e-6, itrmax=2, want=("p", "f", "c", "eig"))
     fd, pd = _unpack(r2["f"], n), _unpack(r2["p"], n)
-    lmos = mozyme.from_scf(s, r2["c"], fd, pd, s.nclose, keep=1e-8, rcut=30.0)
-    assert (lmos.nijbo >= 0).all()
+    lmos = mozyme.from_scf(s, r2["c"], fd, pd, s.nclose, keep=1e-8, rcut=r_near, r_dipole=r_dipole)
+    assert (lmos.nijbo >= 0).all() == (r_near > 20)
     ms = mozyme.MozymeSystem(s, nijbo=lmos.nijbo, mpack=lmos.mpack)
     ms.upload(gpu_ctx)
     print(f"MOZYME SCF helix {s.numat} atoms from a two-iteration start", flush=True)

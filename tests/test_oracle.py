@@ -263,21 +263,64 @@ def test_localized_orbitals_span_the_occupied_space():
 
 
 @pytest.mark.skipif(not REF.available(), reason="oracle/_ref not built (needs the reference sources)")
-@pytest.mark.parametrize("r_near,r_dipole", [(6.0, 7.0), (4.0, 6.0), (30.0, 31.0)])
-def test_mozyme_fock2z_matches_reference_fortran(r_near, r_dipole):
+@pytest.mark.parametrize("r_near,r_dipole,feather", [(6.0, 7.0, False), (4.0, 6.0, False), (30.0, 31.0, False),
+                                                     (4.0, 6.0, True), (6.0, 8.0, True)])
+def test_mozyme_fock2z_matches_reference_fortran(r_near, r_dipole, feather):
     """The restatement of buildf/fock2z (stored pairs through the C fock2,
     point-charge and dipole terms for the others) against the reference's
     own fz2n, SEMIDIRECT, on a poly-glycine helix with all three pair classes."""
     from mopac_gpu_amd import mozyme
     nat, xyz = geometry.glycine_helix(6, 3)
     s = hamiltonian.build(nat, xyz)
-    ms = mozyme.MozymeSystem(s, r_near=r_near, r_dipole=r_dipole)
+    ms = mozyme.MozymeSystem(s, r_near=r_near, r_dipole=r_dipole, feather=feather)
     if r_near < 10:
         assert (ms.nijbo == -2).any() and (ms.nijbo == -1).any()
     rng = np.random.default_rng(1)
     pd = rng.standard_normal((s.norbs, s.norbs)) * 0.1
     pd = pd + pd.T + np.eye(s.norbs)
     p = mozyme.sparse_from_dense(pd, ms.nfirst, ms.nlast - ms.nfirst + 1, ms.nijbo, ms.mpack)
-    f_ref = REF.mozyme_fock2z(ms, p)
-    f = oracle.mozyme_fock2z(s, ms.nijbo, ms.mpack, ms.h, ms.am, ms.ad, ms.dd, p)
+    f_ref = REF.mozyme_fock2z(ms, p, feather=feather)
+    f = oracle.mozyme_fock2z(s, ms.nijbo, ms.mpack, ms.h, ms.am, ms.ad, ms.dd, p, feather=feather)
     assert np.max(np.abs(f - f_ref)) < 1e-11 * np.max(np.abs(f_ref))
+
+
+@pytest.mark.parametrize("r_near,r_dipole,tol", [(30.0, 31.0, 1e-6), (8.0, 9.0, 1e-4), (6.0, 7.0, 0.15)])
+def test_mozyme_energy_at_the_conventional_density(r_near, r_dipole, tol):
+    """The MOZYME Hamiltonian (mhip_pm7_hamiltonian_mozyme) and sparse Fock
+    build (the restated fz2n with PM7's point-charge switch) evaluated at the
+    converged conventional density give the conventional heat of formation:
+    exactly when every pair is stored; to 1e-4 kcal/mol when the approximated
+    pairs lie beyond the 7 A switch (PM7's NDDO terms are point charges there
+    already; what remains is the dropped resonance and core-core exponentials);
+    within 0.15 kcal/mol when the dipole shell starts at 6 A."""
+    import ctypes
+    from mopac_gpu_amd import lib as L
+    from mopac_gpu_amd import mozyme
+    nat, xyz = geometry.glycine_helix(6, 3)
+    s = hamiltonian.build(nat, xyz)
+    n = s.norbs
+    o = oracle.iterate(s, scfcrt=1e-7)
+    pd = np.zeros((n, n))
+    pd[np.tril_indices(n)] = o["p"]
+    pd = pd + pd.T - np.diag(np.diag(pd))
+    ms = mozyme.MozymeSystem(s, r_near=r_near, r_dipole=r_dipole, feather=True)
+    iorbs = ms.nlast - ms.nfirst + 1
+    p = mozyme.sparse_from_dense(pd, ms.nfirst, iorbs, ms.nijbo, ms.mpack)
+    f = oracle.mozyme_fock2z(s, ms.nijbo, ms.mpack, ms.h, ms.am, ms.ad, ms.dd, p, feather=True)
+    e = 0.0  # helecz: stored blocks, diagonal elements of diagonal blocks at one half
+    for a in range(ms.numat):
+        for b in range(a + 1):
+            k = ms.nijbo[a, b]
+            if k < 0:
+                continue
+            if a == b:
+                q = 0
+                for i in range(iorbs[a]):
+                    for j in range(i + 1):
+                        e += (0.5 if i == j else 1.0) * p[k + q] * (ms.h[k + q] + f[k + q])
+                        q += 1
+            else:
+                m = iorbs[a] * iorbs[b]
+                e += np.sum(p[k:k + m] * (ms.h[k:k + m] + f[k:k + m]))
+    escf = (e + ms.enuclr) * L.load().mhip_ev_to_kcal() + s.atheat
+    assert abs(escf - o["escf"]) < tol
