@@ -23,15 +23,32 @@ struct JacobiPlan {
   DevBuf perm;   // sort permutation
   DevBuf active; // per pair: 1 when Q != I this step
   DevBuf counts; // active pairs per step (timing mode)
+  // C <- C Q runs on its own stream, behind by up to a sweep: Q and the
+  // active flags have a slot per step, and events order each slot's C update
+  // after its step's A update and before the slot's next solve
+  hipStream_t cstream = nullptr;
+  std::vector<hipEvent_t> ev_solved, ev_cdone;
+  std::vector<char> cdone_pending;
+  hipEvent_t ev_join = nullptr;
   // split > 0 (a multiple of 32 leaving an even number of blocks on both
   // sides): rotations only within the leading split columns and within the
   // rest, for a matrix already block-diagonal in that partition.
   void build(int n, int split = 0);
   void release() {
+    if (cstream) (void)hipStreamSynchronize(cstream);
     pairs.release(); qbuf.release(); part.release(); perm.release(); active.release();
     counts.release();
     for (auto e : ev) (void)hipEventDestroy(e);
     ev.clear();
+    for (auto e : ev_solved) (void)hipEventDestroy(e);
+    for (auto e : ev_cdone) (void)hipEventDestroy(e);
+    ev_solved.clear();
+    ev_cdone.clear();
+    cdone_pending.clear();
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    ev_join = nullptr;
+    if (cstream) (void)hipStreamDestroy(cstream);
+    cstream = nullptr;
   }
 };
 

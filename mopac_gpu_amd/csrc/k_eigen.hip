@@ -324,9 +324,10 @@ __global__ __launch_bounds__(256) void k_jac_apply_a_mfma(double* __restrict__ A
   tile_of(blockIdx.x, npairs, npo, P1, P2);
   const bool a1 = active[P1] != 0, a2 = active[P2] != 0;
   if (!a1 && !a2) return;
+  // two 64 x 65 LDS tiles (66.5 KB) so two workgroups fit on a CU: the tile of
+  // A, then Q1^T A, then the result; Q1, then Q2
   __shared__ double At[SB][LDSW];
-  __shared__ double Q1[SB][LDSW];
-  __shared__ double Q2[SB][LDSW];
+  __shared__ double Qs[SB][LDSW];
   const int I1 = pairs[2 * P1], J1 = pairs[2 * P1 + 1];
   const int I2 = pairs[2 * P2], J2 = pairs[2 * P2 + 1];
   const int tid = threadIdx.x;
@@ -335,38 +336,43 @@ __global__ __launch_bounds__(256) void k_jac_apply_a_mfma(double* __restrict__ A
   for (int e = tid; e < SB * SB; e += 256) {
     const int r = e % SB, c = e / SB;
     At[r][c] = A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda];
-    Q1[r][c] = a1 ? q1[e] : (r == c ? 1.0 : 0.0);
-    Q2[r][c] = a2 ? q2[e] : (r == c ? 1.0 : 0.0);
+    Qs[r][c] = a1 ? q1[e] : (r == c ? 1.0 : 0.0);
   }
   __syncthreads();
   d4_t acc[4];
   for (int i = 0; i < 4; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
-  mfma_tile64([&](int r, int k) { return Q1[k][r]; }, [&](int k, int c) { return At[k][c]; }, SB, acc);
+  mfma_tile64([&](int r, int k) { return Qs[k][r]; }, [&](int k, int c) { return At[k][c]; }, SB, acc);
   __syncthreads();
   mfma_tile64_store(acc, [&](int r, int c, double v) { At[r][c] = v; });
+  if (P1 != P2) {
+    for (int e = tid; e < SB * SB; e += 256) {
+      const int r = e % SB, c = e / SB;
+      Qs[r][c] = a2 ? q2[e] : (r == c ? 1.0 : 0.0);
+    }
+  }
   __syncthreads();
   for (int i = 0; i < 4; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
-  mfma_tile64([&](int r, int k) { return At[r][k]; }, [&](int k, int c) { return Q2[k][c]; }, SB, acc);
+  mfma_tile64([&](int r, int k) { return At[r][k]; }, [&](int k, int c) { return Qs[k][c]; }, SB, acc);
   // stage the result in LDS and write both the tile and its mirror with
   // consecutive lanes on consecutive rows of A
   __syncthreads();
-  mfma_tile64_store(acc, [&](int r, int c, double v) { Q1[r][c] = v; });
+  mfma_tile64_store(acc, [&](int r, int c, double v) { At[r][c] = v; });
   __syncthreads();
   if (P1 == P2) {
     for (int e = tid; e < SB * SB; e += 256) {
       const int r = e % SB, c = e / SB;
-      const double v = (r == c) ? Q1[r][c] : 0.5 * (Q1[r][c] + Q1[c][r]);
+      const double v = (r == c) ? At[r][c] : 0.5 * (At[r][c] + At[c][r]);
       A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda] = v;
     }
     return;
   }
   for (int e = tid; e < SB * SB; e += 256) {
     const int r = e % SB, c = e / SB;
-    A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda] = Q1[r][c];
+    A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda] = At[r][c];
   }
   for (int e = tid; e < SB * SB; e += 256) {
     const int c = e % SB, r = e / SB;
-    A[(size_t)gidx(I2, J2, c) + (size_t)gidx(I1, J1, r) * lda] = Q1[r][c];
+    A[(size_t)gidx(I2, J2, c) + (size_t)gidx(I1, J1, r) * lda] = At[r][c];
   }
 }
 
@@ -532,6 +538,9 @@ __global__ __launch_bounds__(256) void k_phase_lock(int n, double* __restrict__ 
 }  // namespace
 
 void JacobiPlan::build(int n_in, int split) {
+  // buffers below may be re-allocated: no C update of an earlier solve may
+  // still be reading them
+  if (cstream) MHIP_CHECK(hipStreamSynchronize(cstream));
   n = n_in;
   npad = ((n + SB - 1) / SB) * SB;
   nb = npad / BS;
@@ -572,10 +581,20 @@ void JacobiPlan::build(int n_in, int split) {
   }
   pairs.ensure(h.size() * sizeof(int));
   MHIP_CHECK(hipMemcpy(pairs.ptr, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
-  qbuf.ensure((size_t)npairs * SB * SB * sizeof(double));
+  qbuf.ensure((size_t)nsteps * npairs * SB * SB * sizeof(double));
   part.ensure((size_t)2 * npad * sizeof(double) + 64);
   perm.ensure((size_t)npad * sizeof(int));
-  active.ensure((size_t)npairs * sizeof(int));
+  active.ensure((size_t)nsteps * npairs * sizeof(int));
+  if (!cstream) MHIP_CHECK(hipStreamCreateWithFlags(&cstream, hipStreamNonBlocking));
+  if (!ev_join) MHIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+  while ((int)ev_solved.size() < nsteps) {
+    hipEvent_t a, b;
+    MHIP_CHECK(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+    MHIP_CHECK(hipEventCreateWithFlags(&b, hipEventDisableTiming));
+    ev_solved.push_back(a);
+    ev_cdone.push_back(b);
+  }
+  cdone_pending.assign(ev_cdone.size(), 0);
   counts.ensure((size_t)2 * nb * sizeof(int));
 }
 
@@ -588,6 +607,15 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
                   double* host_scal, double* dev_scal, int* sweeps_done) {
   const int lda = plan.npad;
   int done = 0;
+  hipStream_t cs = plan.cstream;
+  // the C updates start after whatever the caller queued on C
+  MHIP_CHECK(hipEventRecord(plan.ev_join, s));
+  MHIP_CHECK(hipStreamWaitEvent(cs, plan.ev_join, 0));
+  // and the caller's later work on C waits for the last of them
+  auto join = [&] {
+    MHIP_CHECK(hipEventRecord(plan.ev_join, cs));
+    MHIP_CHECK(hipStreamWaitEvent(s, plan.ev_join, 0));
+  };
   for (int sw = 0; sw <= max_sweeps; ++sw) {
     hipLaunchKernelGGL(k_offmax_partial, dim3(plan.n), dim3(256), 0, s, plan.n, A, lda,
                        plan.grouped ? plan.split : 0, plan.part.as<double>());
@@ -602,6 +630,7 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
     // pairs whose whole 64x64 block is already this diagonal are left alone
     const double skip_abs = 0.1 * thr;
     if (sw == max_sweeps) {
+      join();
       if (sweeps_done) *sweeps_done = done;
       return 1;
     }
@@ -615,36 +644,49 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
     const int ntiles = plan.npo * (plan.npo + 1) / 2 + npv * (npv + 1) / 2;
     for (int step = 0; step < nsteps; ++step) {
       const int* pr = plan.pairs.as<int>() + (size_t)step * plan.npairs * 2;
+      double* q = plan.qbuf.as<double>() + (size_t)step * plan.npairs * SB * SB;
+      int* act = plan.active.as<int>() + (size_t)step * plan.npairs;
+      // the slot's previous C update (one sweep earlier) must have read it
+      if (plan.cdone_pending[step]) MHIP_CHECK(hipStreamWaitEvent(s, plan.ev_cdone[step], 0));
       if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 0], s));
-      hipLaunchKernelGGL(k_jac_solve, dim3(plan.npairs), dim3(SOLVE_THREADS), 0, s, A, lda, plan.n, pr,
-                         plan.qbuf.as<double>(), plan.active.as<int>(), 1.0e-14, skip_abs,
-                         plan.max_inner);
+      hipLaunchKernelGGL(k_jac_solve, dim3(plan.npairs), dim3(SOLVE_THREADS), 0, s, A, lda, plan.n, pr, q, act,
+                         1.0e-14, skip_abs, plan.max_inner);
       MHIP_LAUNCH_CHECK();
       if (plan.timing) {
         MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 1], s));
-        hipLaunchKernelGGL(k_count_active, dim3(1), dim3(64), 0, s, plan.npairs, plan.npo,
-                           plan.active.as<int>(), plan.counts.as<int>() + 2 * step);
+        hipLaunchKernelGGL(k_count_active, dim3(1), dim3(64), 0, s, plan.npairs, plan.npo, act,
+                           plan.counts.as<int>() + 2 * step);
       }
       if (g_kernel_variant == 1) {
         if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 2], s));
-        hipLaunchKernelGGL(k_jac_apply_a_mfma, dim3(ntiles), dim3(256), 0, s, A, lda, plan.npairs,
-                           plan.npo, pr, plan.qbuf.as<double>(), plan.active.as<int>());
+        hipLaunchKernelGGL(k_jac_apply_a_mfma, dim3(ntiles), dim3(256), 0, s, A, lda, plan.npairs, plan.npo, pr, q,
+                           act);
         MHIP_LAUNCH_CHECK();
         if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 3], s));
-        if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 4], s));
-        hipLaunchKernelGGL(k_jac_apply_c_mfma, dim3(plan.npad / SB, plan.npairs), dim3(256), 0, s,
-                           C, lda, plan.npad, pr, plan.qbuf.as<double>(), plan.active.as<int>());
-        if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 5], s));
-      } else {
-        hipLaunchKernelGGL(k_jac_apply_a, dim3(ntiles), dim3(256), 0, s, A, lda, plan.npairs, plan.npo,
-                           pr, plan.qbuf.as<double>(), plan.active.as<int>());
+        // C <- C Q on the side stream once this step's A update is done, so
+        // it overlaps the next step's pair solves, which leave most CUs idle
+        MHIP_CHECK(hipEventRecord(plan.ev_solved[step], s));
+        MHIP_CHECK(hipStreamWaitEvent(cs, plan.ev_solved[step], 0));
+        if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 4], cs));
+        hipLaunchKernelGGL(k_jac_apply_c_mfma, dim3(plan.npad / SB, plan.npairs), dim3(256), 0, cs, C, lda,
+                           plan.npad, pr, q, act);
         MHIP_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_jac_apply_c, dim3(plan.npad / SB, plan.npairs), dim3(256), 0, s, C,
-                           lda, plan.npad, pr, plan.qbuf.as<double>(), plan.active.as<int>());
+        if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 5], cs));
+      } else {
+        hipLaunchKernelGGL(k_jac_apply_a, dim3(ntiles), dim3(256), 0, s, A, lda, plan.npairs, plan.npo, pr, q,
+                           act);
+        MHIP_LAUNCH_CHECK();
+        MHIP_CHECK(hipEventRecord(plan.ev_solved[step], s));
+        MHIP_CHECK(hipStreamWaitEvent(cs, plan.ev_solved[step], 0));
+        hipLaunchKernelGGL(k_jac_apply_c, dim3(plan.npad / SB, plan.npairs), dim3(256), 0, cs, C, lda,
+                           plan.npad, pr, q, act);
+        MHIP_LAUNCH_CHECK();
       }
-      MHIP_LAUNCH_CHECK();
+      MHIP_CHECK(hipEventRecord(plan.ev_cdone[step], cs));
+      plan.cdone_pending[step] = 1;
     }
     if (plan.timing && g_kernel_variant == 1) {
+      MHIP_CHECK(hipStreamSynchronize(cs));
       MHIP_CHECK(hipStreamSynchronize(s));
       std::vector<int> act(2 * nsteps);
       MHIP_CHECK(hipMemcpy(act.data(), plan.counts.ptr, sizeof(int) * 2 * nsteps, hipMemcpyDeviceToHost));
@@ -667,6 +709,7 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
     }
     ++done;
   }
+  join();
   if (sweeps_done) *sweeps_done = done;
   return 0;
 }
