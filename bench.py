@@ -141,7 +141,8 @@ def main():
     # the stream every kernel is launched on
     n = s.norbs
     jv = [ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double(), ctypes.c_longlong(),
-          ctypes.c_double(), ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()]
+          ctypes.c_double(), ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong(),
+          ctypes.c_longlong()]
     jrefs = [ctypes.byref(v) for v in jv]
     nk = 4
     kms, kl, kf, kb = (np.zeros(nk), np.zeros(nk, dtype=np.int64), np.zeros(nk), np.zeros(nk))
@@ -151,13 +152,14 @@ def main():
     lib.mhip_kernel_stats(ctx.h, -1, nk, L.ptr(kms), L.ptr(kl), L.ptr(kf), L.ptr(kb))
     lib.mhip_kernel_stats(ctx.h, 0, nk, None, None, None, None)
     lib.mhip_kernel_timing(ctx.h, 0, *jrefs)
-    solve_ms, n_solve, apply_ms, n_apply, applyc_ms, n_applyc, tiles, pairs = [v.value for v in jv]
+    solve_ms, n_solve, apply_ms, n_apply, applyc_ms, n_applyc, tiles, pairs, rotations = [v.value for v in jv]
     npad = ((n + 63) // 64) * 64
     # block-Jacobi algorithmic work: A-update = two 64^3 products per processed
-    # tile; C-update = one n x 64 x 64 product per processed pair; solve = one
-    # inner sweep = 2016 rotations x (64 + 64 + 64 element pairs) x 6 flops
+    # tile; C-update = one n x 64 x 64 product per processed pair; solve = the
+    # rotations the library counted (32 x 32 cross rounds per pair, plus 2 x 496
+    # inside the blocks once per sweep) x (64 + 64 + 64 element pairs) x 6 flops
     entries = [
-        ("k_jac_solve", "mfma", solve_ms, n_solve, pairs * 2016.0 * 3 * 64 * 6, 0.0),
+        ("k_jac_solve", "mfma", solve_ms, n_solve, rotations * 3.0 * 64 * 6, 0.0),
         ("k_jac_apply_a_mfma", "mfma", apply_ms, n_apply, tiles * 2.0 * 2.0 * 64 ** 3, 0.0),
         ("k_jac_apply_c_mfma", "mfma", applyc_ms, n_applyc, pairs * 2.0 * npad * 64 * 64, 0.0),
     ]

@@ -385,16 +385,23 @@ int mhip_guess_density(mhip_ctx* ctx, double* p);
  * sweeps per 64x64 pair problem (<=0 unchanged); tol_iter: eigensolver
  * stopping tolerance inside the SCF loop (<=0 unchanged). */
 int mhip_set_tuning(mhip_ctx* ctx, int kernel_variant, int max_inner, double tol_iter);
+/* No counterpart in the reference.  Pair-solve schedule of the block-Jacobi
+ * eigensolver for the whole process: 1 (default) = per step, 32 rounds over
+ * the index pairs across the two blocks, with the pairs inside each block
+ * rotated once per sweep; 0 = a full cyclic sweep over all 64 indices in
+ * every step.  Other values leave it unchanged.  Returns the previous one. */
+int mhip_set_jacobi_schedule(int schedule);
 /* Block-Jacobi sweeps used by the last eigensolve of this context. */
 int mhip_last_sweeps(mhip_ctx* ctx);
 /* HIP-event timing of the block-Jacobi solve / A-update / C-update launches
  * on the context's stream.  Returns the accumulated milliseconds and launch
- * counts, and the 64x64 tiles / block pairs the timed launches actually
- * processed; enable = 1 or 0 resets and switches timing on or off, -1 only
- * reads. */
+ * counts, the 64x64 tiles / block pairs the timed launches actually
+ * processed and the 2x2 rotations of the timed pair solves; enable = 1 or 0
+ * resets and switches timing on or off, -1 only reads. */
 int mhip_kernel_timing(mhip_ctx* ctx, int enable, double* solve_ms, long long* n_solve,
                        double* apply_ms, long long* n_apply, double* applyc_ms,
-                       long long* n_applyc, long long* active_tiles, long long* active_pairs);
+                       long long* n_applyc, long long* active_tiles, long long* active_pairs,
+                       long long* rotations);
 
 /* Process-wide HIP-event timing of the other kernel classes (FP64 GEMM,
  * split-BF16 GEMM, packed density, fock2), on the stream each launch uses.

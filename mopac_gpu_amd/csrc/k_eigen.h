@@ -16,12 +16,16 @@ struct JacobiPlan {
   double t_solve_ms = 0.0, t_apply_ms = 0.0, t_applyc_ms = 0.0;
   long long n_solve = 0, n_apply = 0, n_applyc = 0;
   long long active_tiles = 0, active_pairs = 0;  // summed over timed launches
+  long long active_rotations = 0;                 // 2x2 rotations of the timed solves
+  std::vector<int> intra_host;                    // host copy of intra
   std::vector<hipEvent_t> ev;
   DevBuf pairs;  // nsteps x npairs x 2 block indices
   DevBuf qbuf;   // npairs x 64 x 64 rotations of the current step
   DevBuf part;   // reduction scratch
   DevBuf perm;   // sort permutation
   DevBuf active; // per pair: 1 when Q != I this step
+  DevBuf intra;  // nsteps x npairs: 1 where the cross-round solve also
+                 // rotates inside the two blocks (once per sweep of a group)
   DevBuf counts; // active pairs per step (timing mode)
   // C <- C Q runs on its own stream, behind by up to a sweep: Q and the
   // active flags have a slot per step, and events order each slot's C update
@@ -37,7 +41,7 @@ struct JacobiPlan {
   void release() {
     if (cstream) (void)hipStreamSynchronize(cstream);
     pairs.release(); qbuf.release(); part.release(); perm.release(); active.release();
-    counts.release();
+    intra.release(); counts.release();
     for (auto e : ev) (void)hipEventDestroy(e);
     ev.clear();
     for (auto e : ev_solved) (void)hipEventDestroy(e);
@@ -51,6 +55,11 @@ struct JacobiPlan {
     cstream = nullptr;
   }
 };
+
+// Pair-solve schedule of every plan in the process: 1 (default) = cross
+// rounds with one intra-block pass per sweep, 0 = a full 64-index cyclic
+// sweep in every step.
+extern int g_jacobi_schedule;
 
 void jacobi_set_identity(hipStream_t s, int npad, double* C, int ldc);
 // Sweeps until max|offdiag| <= tol * max(1, max|diag|).  Returns 0 when

@@ -158,6 +158,167 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_jac_solve(
   if (tid == 0) active[pidx] = 1;
 }
 
+// Angle of the rotation that zeroes apq in [[app, apq], [apq, aqq]] (as in
+// pair_rotation), with t = tan: after it the diagonal is app - t apq,
+// aqq + t apq.
+__device__ __forceinline__ void rotation_of(double app, double aqq, double apq, double& c, double& sn,
+                                            double& t) {
+  c = 1.0;
+  sn = 0.0;
+  t = 0.0;
+  if (fabs(apq) > 1.0e-300 && fabs(apq) > 1.0e-18 * sqrt(fabs(app * aqq))) {
+    const double d = aqq - app, h = 2.0 * apq;
+    t = h / (fabs(d) + sqrt(d * d + h * h));
+    if (d < 0.0) t = -t;
+    c = rsqrt(t * t + 1.0);
+    sn = t * c;
+  }
+}
+
+// Thread (k1, k2) of a round: rotation k1 on rows p1, q1 and k2 on columns
+// p2, q2 of the 2x2 block it owns, and on two rows of Q's columns p2, q2.
+// Returns the new S[p1][q2].
+__device__ __forceinline__ double round_update(double (*S)[LDSW], double (*Q)[LDSW], int k1, int p1,
+                                               int q1, double c1, double s1, int p2, int q2, double c2,
+                                               double s2) {
+  const double xpp = S[p1][p2], xpq = S[p1][q2], xqp = S[q1][p2], xqq = S[q1][q2];
+  const double rpp = c1 * xpp - s1 * xqp, rpq = c1 * xpq - s1 * xqq;
+  const double rqp = s1 * xpp + c1 * xqp, rqq = s1 * xpq + c1 * xqq;
+  const double npq = s2 * rpp + c2 * rpq;
+  S[p1][p2] = c2 * rpp - s2 * rpq;
+  S[p1][q2] = npq;
+  S[q1][p2] = c2 * rqp - s2 * rqq;
+  S[q1][q2] = s2 * rqp + c2 * rqq;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int row = 2 * k1 + h;
+    const double u = Q[row][p2], v = Q[row][q2];
+    Q[row][p2] = c2 * u - s2 * v;
+    Q[row][q2] = s2 * u + c2 * v;
+  }
+  return npq;
+}
+
+// The pair solve with cross rounds.  An inner sweep is, when intra[pair] is
+// set, 31 rounds inside each 32-wide half (circle order on 32 indices, both
+// halves at once), then 32 cross rounds: round r pairs local index k with
+// 32 + (k + r) % 32, so the 1024 index pairs across the two blocks are each
+// rotated once.  The sweep schedule sets intra once per sweep of the block's
+// group, so every index pair of the matrix is rotated once per sweep, in
+// about half the sequential rounds of a full 64-index inner sweep.
+//
+// Cross rounds need one barrier each: the element rotation k of round r + 1
+// acts on, S[k][32 + (k + r + 1) % 32], lies in the 2x2 block of thread
+// (k, k + 1) in round r, so that thread computes the next angle right after
+// its own update, from the diagonals tracked in dgl (a - t apq, d + t apq
+// per rotation), and publishes it in the other buffer.
+__global__ __launch_bounds__(SOLVE_THREADS) void k_jac_solve_x(
+    const double* __restrict__ A, int lda, int nreal, const int* __restrict__ pairs,
+    const int* __restrict__ intra, double* __restrict__ Qbuf, int* __restrict__ active, double tol,
+    double skip_abs, int max_inner) {
+  __shared__ double S[SB][LDSW];
+  __shared__ double Q[SB][LDSW];
+  __shared__ double2 rot[2][32];
+  __shared__ double dgl[2][SB];
+  __shared__ int rp[32], rq[32];
+  __shared__ double red_off[SOLVE_THREADS / 64], red_dg[SOLVE_THREADS / 64];
+  const int pidx = blockIdx.x;
+  const int I = pairs[2 * pidx], J = pairs[2 * pidx + 1];
+  const bool do_intra = intra[pidx] != 0;
+  const int tid = threadIdx.x;
+  const int k1 = tid >> 5, k2 = tid & 31;
+  for (int e = tid; e < SB * SB; e += SOLVE_THREADS) {
+    const int r = e % SB, c = e / SB;
+    S[r][c] = A[(size_t)gidx_dev(I, J, r) + (size_t)gidx_dev(I, J, c) * lda];
+    Q[r][c] = (r == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int sw = 0; sw < max_inner; ++sw) {
+    double off = 0.0, dg = 0.0;
+    for (int e = tid; e < SB * SB; e += SOLVE_THREADS) {
+      const int r = e % SB, c = e / SB;
+      if (r == c) {
+        if (gidx_dev(I, J, r) < nreal) dg = fmax(dg, fabs(S[r][c]));
+      } else {
+        off = fmax(off, fabs(S[r][c]));
+      }
+    }
+    off = wave_max(off);
+    dg = wave_max(dg);
+    if ((tid & 63) == 0) {
+      red_off[tid >> 6] = off;
+      red_dg[tid >> 6] = dg;
+    }
+    __syncthreads();
+    off = 0.0;
+    dg = 0.0;
+#pragma unroll
+    for (int w = 0; w < SOLVE_THREADS / 64; ++w) {
+      off = fmax(off, red_off[w]);
+      dg = fmax(dg, red_dg[w]);
+    }
+    __syncthreads();
+    if (sw == 0 && off <= skip_abs) {
+      if (tid == 0) active[pidx] = 0;
+      return;
+    }
+    if (off <= tol * fmax(dg, 1.0)) break;
+    if (do_intra) {
+      for (int round = 0; round < 31; ++round) {
+        if (tid < 32) {
+          // circle order on the 32 indices of half tid / 16
+          const int h = tid >> 4, kk = tid & 15;
+          auto pos = [&](int slot) { return slot == 0 ? 0 : 1 + ((slot - 1 + round) % 31); };
+          int p = 32 * h + pos(kk), q = 32 * h + pos(31 - kk);
+          if (p > q) { const int x = p; p = q; q = x; }
+          double c, sn, t;
+          rotation_of(S[p][p], S[q][q], S[p][q], c, sn, t);
+          rot[0][tid] = make_double2(c, sn);
+          rp[tid] = p;
+          rq[tid] = q;
+        }
+        __syncthreads();
+        const double2 r1 = rot[0][k1], r2 = rot[0][k2];
+        (void)round_update(S, Q, k1, rp[k1], rq[k1], r1.x, r1.y, rp[k2], rq[k2], r2.x, r2.y);
+        __syncthreads();
+      }
+    }
+    // cross rounds: angles of round 0 and the diagonals after it
+    if (tid < 32) {
+      const int p = tid, q = 32 + tid;
+      const double app = S[p][p], aqq = S[q][q], apq = S[p][q];
+      double c, sn, t;
+      rotation_of(app, aqq, apq, c, sn, t);
+      rot[0][tid] = make_double2(c, sn);
+      dgl[0][p] = app - t * apq;
+      dgl[0][q] = aqq + t * apq;
+    }
+    __syncthreads();
+    for (int r = 0; r < 32; ++r) {
+      const int b = r & 1;
+      const double2 r1 = rot[b][k1], r2 = rot[b][k2];
+      const int q1 = 32 + ((k1 + r) & 31), q2 = 32 + ((k2 + r) & 31);
+      const double npq = round_update(S, Q, k1, k1, q1, r1.x, r1.y, k2, q2, r2.x, r2.y);
+      if (r < 31 && k2 == ((k1 + 1) & 31)) {
+        // rotation k1 of round r + 1 pairs k1 with q2
+        const double dp = dgl[b][k1], dq = dgl[b][q2];
+        double c, sn, t;
+        rotation_of(dp, dq, npq, c, sn, t);
+        rot[b ^ 1][k1] = make_double2(c, sn);
+        dgl[b ^ 1][k1] = dp - t * npq;
+        dgl[b ^ 1][q2] = dq + t * npq;
+      }
+      __syncthreads();
+    }
+  }
+  double* Qo = Qbuf + (size_t)pidx * SB * SB;
+  for (int e = tid; e < SB * SB; e += SOLVE_THREADS) {
+    const int r = e % SB, c = e / SB;
+    Qo[e] = Q[r][c];
+  }
+  if (tid == 0) active[pidx] = 1;
+}
+
 __device__ __forceinline__ int gidx(int pairI, int pairJ, int local) {
   return (local < BS) ? pairI * BS + local : pairJ * BS + (local - BS);
 }
@@ -537,6 +698,8 @@ __global__ __launch_bounds__(256) void k_phase_lock(int n, double* __restrict__ 
 
 }  // namespace
 
+int g_jacobi_schedule = 1;
+
 void JacobiPlan::build(int n_in, int split) {
   // buffers below may be re-allocated: no C update of an earlier solve may
   // still be reading them
@@ -581,6 +744,16 @@ void JacobiPlan::build(int n_in, int split) {
   }
   pairs.ensure(h.size() * sizeof(int));
   MHIP_CHECK(hipMemcpy(pairs.ptr, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+  // intra-block rounds at the first step of each sweep of a pair's group
+  std::vector<int> fl((size_t)nsteps * npairs, 0);
+  for (int step = 0; step < nsteps; ++step)
+    for (int k = 0; k < npairs; ++k) {
+      const bool first = grouped ? (k < npo ? step % (nbo - 1) == 0 : step % (nbv - 1) == 0) : step == 0;
+      fl[(size_t)step * npairs + k] = first ? 1 : 0;
+    }
+  intra_host = fl;
+  intra.ensure(fl.size() * sizeof(int));
+  MHIP_CHECK(hipMemcpy(intra.ptr, fl.data(), fl.size() * sizeof(int), hipMemcpyHostToDevice));
   qbuf.ensure((size_t)nsteps * npairs * SB * SB * sizeof(double));
   part.ensure((size_t)2 * npad * sizeof(double) + 64);
   perm.ensure((size_t)npad * sizeof(int));
@@ -649,8 +822,14 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
       // the slot's previous C update (one sweep earlier) must have read it
       if (plan.cdone_pending[step]) MHIP_CHECK(hipStreamWaitEvent(s, plan.ev_cdone[step], 0));
       if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 0], s));
-      hipLaunchKernelGGL(k_jac_solve, dim3(plan.npairs), dim3(SOLVE_THREADS), 0, s, A, lda, plan.n, pr, q, act,
-                         1.0e-14, skip_abs, plan.max_inner);
+      if (g_jacobi_schedule == 1) {
+        const int* fl = plan.intra.as<int>() + (size_t)step * plan.npairs;
+        hipLaunchKernelGGL(k_jac_solve_x, dim3(plan.npairs), dim3(SOLVE_THREADS), 0, s, A, lda, plan.n, pr, fl,
+                           q, act, 1.0e-14, skip_abs, plan.max_inner);
+      } else {
+        hipLaunchKernelGGL(k_jac_solve, dim3(plan.npairs), dim3(SOLVE_THREADS), 0, s, A, lda, plan.n, pr, q,
+                           act, 1.0e-14, skip_abs, plan.max_inner);
+      }
       MHIP_LAUNCH_CHECK();
       if (plan.timing) {
         MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 1], s));
@@ -694,6 +873,17 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
         // tiles processed: those of each group with at least one active pair
         const long long u0 = plan.npo - act[2 * step], u1 = npv - act[2 * step + 1];
         plan.active_pairs += act[2 * step] + act[2 * step + 1];
+        // a full inner sweep is 2016 rotations; cross rounds are 32 x 32,
+        // plus 2 x 496 inside the blocks where the step's intra flag is set
+        for (int g = 0; g < 2; ++g) {
+          const int first = g == 0 ? 0 : plan.npo;
+          if (first >= plan.npairs) continue;
+          const long long per =
+              g_jacobi_schedule == 1
+                  ? 1024 + (plan.intra_host[(size_t)step * plan.npairs + first] ? 992 : 0)
+                  : 2016LL * plan.max_inner;
+          plan.active_rotations += per * act[2 * step + g];
+        }
         plan.active_tiles += (long long)ntiles - u0 * (u0 + 1) / 2 - u1 * (u1 + 1) / 2;
         float a = 0.f, b = 0.f, c2 = 0.f;
         MHIP_CHECK(hipEventElapsedTime(&a, plan.ev[6 * step + 0], plan.ev[6 * step + 1]));

@@ -944,13 +944,15 @@ int mhip_kernel_stats(mhip_ctx* c, int enable, int max_classes, double* ms, long
 
 int mhip_kernel_timing(mhip_ctx* c, int enable, double* solve_ms, long long* n_solve,
                        double* apply_ms, long long* n_apply, double* applyc_ms,
-                       long long* n_applyc, long long* active_tiles, long long* active_pairs) {
+                       long long* n_applyc, long long* active_tiles, long long* active_pairs,
+                       long long* rotations) {
   if (!c) return 1;
   // both plans: the in-loop one and the grouped one of the exit diagonalization
   const mhip::JacobiPlan& a = c->plan;
   const mhip::JacobiPlan& b = c->plan_split;
   if (active_tiles) *active_tiles = a.active_tiles + b.active_tiles;
   if (active_pairs) *active_pairs = a.active_pairs + b.active_pairs;
+  if (rotations) *rotations = a.active_rotations + b.active_rotations;
   if (solve_ms) *solve_ms = a.t_solve_ms + b.t_solve_ms;
   if (n_solve) *n_solve = a.n_solve + b.n_solve;
   if (apply_ms) *apply_ms = a.t_apply_ms + b.t_apply_ms;
@@ -962,7 +964,7 @@ int mhip_kernel_timing(mhip_ctx* c, int enable, double* solve_ms, long long* n_s
       p->timing = enable != 0;
       p->t_solve_ms = p->t_apply_ms = p->t_applyc_ms = 0.0;
       p->n_solve = p->n_apply = p->n_applyc = 0;
-      p->active_tiles = p->active_pairs = 0;
+      p->active_tiles = p->active_pairs = p->active_rotations = 0;
     }
   }
   return 0;
@@ -975,6 +977,14 @@ int mhip_set_tuning(mhip_ctx* c, int kernel_variant, int max_inner, double tol_i
   if (c && max_inner > 0) c->plan.max_inner = max_inner;
   if (c && tol_iter > 0) c->tol_iter = tol_iter;
   return 0;
+}
+
+// Pair-solve schedule of the block-Jacobi eigensolver, process-wide:
+// 1 = cross rounds + one intra-block pass per sweep, 0 = full inner sweeps.
+int mhip_set_jacobi_schedule(int schedule) {
+  const int prev = mhip::g_jacobi_schedule;
+  if (schedule == 0 || schedule == 1) mhip::g_jacobi_schedule = schedule;
+  return prev;
 }
 
 }  // extern "C"

@@ -113,6 +113,7 @@ SIGNATURES = {
     "mhip_call_rot": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double]),
     "mhip_destroy_resources": (None, []),
     "mhip_set_tuning": (c_int, [c_void_p, c_int, c_int, c_double]),
+    "mhip_set_jacobi_schedule": (c_int, [c_int]),
     "mhip_last_sweeps": (c_int, [c_void_p]),
     "mhip_mozyme_density": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "mhip_mozyme_diagg1": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
@@ -134,7 +135,7 @@ SIGNATURES = {
     "mhip_kernel_timing": (c_int, [c_void_p, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_ll),
                                    ctypes.POINTER(c_double), ctypes.POINTER(c_ll),
                                    ctypes.POINTER(c_double), ctypes.POINTER(c_ll),
-                                   ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)]),
+                                   ctypes.POINTER(c_ll), ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)]),
     "mhip_mlp_create": (c_int, [c_int, c_void_p, ctypes.POINTER(c_void_p)]),
     "mhip_mlp_destroy": (None, [c_void_p]),
     "mhip_mlp_set_params": (c_int, [c_void_p, c_void_p]),

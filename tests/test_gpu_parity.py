@@ -70,14 +70,21 @@ def test_density_and_energy_match_oracle(gpu_ctx, nw):
     assert abs(e_gpu - e_ref) < 1e-10 * max(1.0, abs(e_ref))
 
 
-@pytest.mark.parametrize("k", [8, 32])
-def test_eigen_matches_lapack(gpu_ctx, k):
+@pytest.mark.parametrize("schedule", [1, 0])
+@pytest.mark.parametrize("k", [8, 32, 64])
+def test_eigen_matches_lapack(gpu_ctx, k, schedule):
     s = _water(k, 7)
     gpu_ctx.upload(s)
     p = _rand_density(s, 9)
     f = oracle.fock2(s, s.h, p, 0.5 * p)
     f1, f2 = f.copy(), f.copy()
-    c_gpu, e_gpu = gpu_ctx.eigenvectors(f1)
+    print(f"eigen (H2O){k} schedule {schedule}", flush=True)
+    lib = L.load()
+    prev = lib.mhip_set_jacobi_schedule(schedule)
+    try:
+        c_gpu, e_gpu = gpu_ctx.eigenvectors(f1)
+    finally:
+        lib.mhip_set_jacobi_schedule(prev)
     c_ref, e_ref = oracle.eigenvectors(f2)
     # the reference perturbs its argument in place; both must agree
     assert np.array_equal(f1, f2)
