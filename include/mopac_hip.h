@@ -432,6 +432,19 @@ int mhip_pm7_element(int z, double* out);
  * (mhip_pm7_hamiltonian's w without the other pairs' blocks). */
 int mhip_pm7_hamiltonian_mozyme(int numat, const int* nat, const double* coord, const int* nijbo,
                                 long long mpack, double* h, double* enuclr);
+/* mhip_pm7_hamiltonian's atom-level outputs alone (nfirst, nlast, atheat,
+ * pdiag), for systems whose pair terms are built sparsely. */
+int mhip_pm7_atoms(int numat, const int* nat, int charge, int* nfirst, int* nlast, double* atheat,
+                   double* pdiag);
+/* The two-electron integrals of the pairs stored in nijbo (>= 0), in hcore
+ * order (per atom i: its stored pairs j < i, then its one-centre block):
+ * the rotate blocks hcore_for_MOZYME stores for near pairs
+ * (hcore_for_MOZYME.F90:205-214); the far pairs' point-charge terms are
+ * formed on the fly by the MOZYME Fock build, so none are stored here.
+ * With w == NULL only *nw is set (the length); otherwise *nw is the capacity
+ * on entry and the length on return. */
+int mhip_pm7_integrals_mozyme(int numat, const int* nat, const double* coord, const int* nijbo,
+                              long long* nw, double* w);
 
 #ifdef __cplusplus
 }

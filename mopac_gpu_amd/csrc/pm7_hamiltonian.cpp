@@ -1070,6 +1070,88 @@ int mhip_pm7_hamiltonian_mozyme(int numat, const int* nat, const double* coord, 
   return 0;
 }
 
+// The atom-level part of mhip_pm7_hamiltonian: orbital ranges, the
+// isolated-atom reference heat and the diagonal starting density, with no
+// pair work (for systems whose pairs are built sparsely).
+int mhip_pm7_atoms(int numat, const int* nat, int charge, int* nfirst, int* nlast, double* atheat,
+                   double* pdiag) {
+  init_params();
+  int norbs = 0;
+  long long nw = 0;
+  if (mhip_pm7_sizes(numat, nat, &norbs, &nw) != 0) return -1;
+  int k = 1;
+  double eheat = 0.0, eisol = 0.0;
+  for (int i = 0; i < numat; ++i) {
+    const Elem& E = g_elem[nat[i]];
+    nfirst[i] = k;
+    k += E.natorb;
+    nlast[i] = k - 1;
+    eheat += E.eheat;
+    eisol += E.eisol;
+  }
+  *atheat = eheat - eisol * kFPC9;
+  const double yy = (double)charge / (norbs + 1.0e-10);
+  for (int i = 0; i < numat; ++i) {
+    const Elem& E = g_elem[nat[i]];
+    for (int a = 0; a < E.natorb; ++a)
+      pdiag[nfirst[i] - 1 + a] = (E.natorb == 1) ? E.tore - yy : E.tore * 0.25 - yy;
+  }
+  return 0;
+}
+
+// Two-electron integrals of the stored pairs only, in hcore order (per atom
+// i: the blocks of its stored pairs j < i, then its one-centre block), as
+// hcore_for_MOZYME keeps them for SEMIDIRECT.  w == nullptr: size only.
+int mhip_pm7_integrals_mozyme(int numat, const int* nat, const double* coord, const int* nijbo,
+                              long long* nw, double* w) {
+  init_params();
+  int norbs = 0;
+  long long nfull = 0;
+  if (!nw || mhip_pm7_sizes(numat, nat, &norbs, &nfull) != 0) return -1;
+  auto nij = [&](int a, int b) { return nijbo[a + (size_t)b * numat]; };
+  std::vector<long long> off(numat + 1);
+  long long kr = 0;
+  for (int i = 0; i < numat; ++i) {
+    off[i] = kr;
+    const int li = g_elem[nat[i]].natorb * (g_elem[nat[i]].natorb + 1) / 2;
+    for (int j = 0; j < i; ++j)
+      if (nij(i, j) >= 0) kr += (long long)li * (g_elem[nat[j]].natorb * (g_elem[nat[j]].natorb + 1) / 2);
+    kr += (long long)li * li;
+  }
+  off[numat] = kr;
+  if (!w) {
+    *nw = kr;
+    return 0;
+  }
+  if (*nw < kr) return -1;
+  std::atomic<int> next_row{0};
+  auto worker = [&]() {
+    for (int i = next_row++; i < numat; i = next_row++) {
+      const Elem& A = g_elem[nat[i]];
+      const int limij = A.natorb * (A.natorb + 1) / 2;
+      long long kk = off[i];
+      for (int j = 0; j < i; ++j) {
+        if (nij(i, j) < 0) continue;
+        const Elem& B = g_elem[nat[j]];
+        const int limkl = B.natorb * (B.natorb + 1) / 2;
+        PairOut po;
+        pair_integrals(A, B, coord + 3 * i, coord + 3 * j, po);
+        for (int q = 0; q < limij * limkl; ++q) w[kk + q] = po.w[q];
+        kk += (long long)limij * limkl;
+      }
+      one_center_block(A, w + kk);
+    }
+  };
+  unsigned nthr = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (numat < 64) nthr = 1;
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < nthr; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
+  *nw = kr;
+  return 0;
+}
+
 int mhip_pm7_element(int z, double* out) {
   init_params();
   if (z < 0 || z >= 9 || !g_elem[z].ok || !out) return -1;

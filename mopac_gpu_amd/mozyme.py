@@ -61,17 +61,24 @@ class LmoSet:
 
 
 def build_nijbo(xyz, iorbs, rcut):
+    """Offsets of the stored atom-pair blocks: pairs within rcut (Angstrom)
+    and every atom with itself, laid out row by row (a, b <= a), a > b as
+    iorbs[a] x iorbs[b], a == b as a packed triangle (ijbo.F90:16-83)."""
+    xyz = np.asarray(xyz, dtype=np.float64)
+    iorbs = np.asarray(iorbs, dtype=np.int64)
     numat = len(iorbs)
+    d = np.sqrt(((xyz[:, None, :] - xyz[None, :, :]) ** 2).sum(axis=2))
+    lower = np.tril(np.ones((numat, numat), dtype=bool))
+    keep = lower & ((d <= rcut) | np.eye(numat, dtype=bool))
+    size = np.outer(iorbs, iorbs)
+    np.fill_diagonal(size, iorbs * (iorbs + 1) // 2)
+    a_idx, b_idx = np.nonzero(keep)          # row-major: a ascending, then b
+    blk = size[a_idx, b_idx]
+    off = np.concatenate(([0], np.cumsum(blk)[:-1]))
     nijbo = np.full((numat, numat), -1, dtype=np.int32)
-    off = 0
-    for a in range(numat):
-        d = np.linalg.norm(xyz[: a + 1] - xyz[a], axis=1)
-        for b in range(a + 1):
-            if b != a and d[b] > rcut:
-                continue
-            nijbo[a, b] = nijbo[b, a] = off
-            off += iorbs[a] * (iorbs[a] + 1) // 2 if a == b else iorbs[a] * iorbs[b]
-    return nijbo, off
+    nijbo[a_idx, b_idx] = off
+    nijbo[b_idx, a_idx] = off
+    return nijbo, int(blk.sum())
 
 
 def build_nijbo_shells(xyz, iorbs, r_near, r_dipole):
@@ -148,9 +155,53 @@ class MozymeSystem:
         if rc != 0:
             raise ValueError("mhip_pm7_hamiltonian_mozyme: unsupported element or nijbo beyond mpack")
         self.enuclr = en.value
-        self.w = mozyme_integrals(system, self.nijbo)
+        # near-pair integrals in hcore order; the far pairs are point charges
+        nijf = np.asfortranarray(self.nijbo.astype(np.int32))
+        nw = ctypes.c_longlong()
+        lib = L.load()
+        L.check(lib.mhip_pm7_integrals_mozyme(self.numat, L.ptr(np.ascontiguousarray(self.nat)), L.ptr(self.xyz),
+                                              L.ptr(nijf), ctypes.byref(nw), None))
+        self.w = np.zeros(nw.value)
+        L.check(lib.mhip_pm7_integrals_mozyme(self.numat, L.ptr(np.ascontiguousarray(self.nat)), L.ptr(self.xyz),
+                                              L.ptr(nijf), ctypes.byref(nw), L.ptr(self.w)))
         self.am, self.ad, self.dd = element_parameters(self.nat)
         self.feather = bool(feather)
+
+    @classmethod
+    def from_geometry(cls, nat, xyz, r_near=6.0, r_dipole=7.0, charge=0, feather=True):
+        """A MOZYME system straight from the geometry: the atom data and
+        the sparse one- and two-electron terms only, without the
+        conventional system's full W and packed H (which do not fit a
+        protein-sized problem comfortably)."""
+        from types import SimpleNamespace
+        from . import lib as L
+        from .hamiltonian import TORE
+        nat = np.ascontiguousarray(nat, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        numat = len(nat)
+        norbs = ctypes.c_int()
+        n2 = ctypes.c_longlong()
+        lib = L.load()
+        L.check(lib.mhip_pm7_sizes(numat, L.ptr(nat), ctypes.byref(norbs), ctypes.byref(n2)))
+        nf = np.zeros(numat, np.int32)
+        nl = np.zeros(numat, np.int32)
+        pd = np.zeros(norbs.value)
+        at = ctypes.c_double()
+        L.check(lib.mhip_pm7_atoms(numat, L.ptr(nat), charge, L.ptr(nf), L.ptr(nl), ctypes.byref(at), L.ptr(pd)))
+        nel = int(sum(TORE[int(z)] for z in nat)) - charge
+        system = SimpleNamespace(nat=nat, xyz=xyz, numat=numat, norbs=norbs.value, nfirst=nf, nlast=nl,
+                                 atheat=at.value, pdiag=pd, nclose=nel // 2, charge=charge)
+        return cls(system, r_near=r_near, r_dipole=r_dipole, feather=feather)
+
+    def diagonal_density(self):
+        """The standard starting density (pdiag on the diagonal) in sparse storage."""
+        p = np.zeros(self.mpack)
+        iorbs = self.nlast - self.nfirst + 1
+        for a in range(self.numat):
+            o = self.nijbo[a, a]
+            for k in range(iorbs[a]):
+                p[o + k * (k + 1) // 2 + k] = self.s.pdiag[self.nfirst[a] - 1 + k]
+        return p
 
     def upload(self, ctx):
         from . import lib as L
@@ -347,6 +398,157 @@ def from_scf(system, c, f_dense, p_dense, nocc, keep=1e-4, rcut=8.0, pad_atoms=6
     s.eigs = np.array([_energy(s, k, True, f_dense) for k in range(nocc)])
     s.eigv = np.array([_energy(s, k, False, f_dense) for k in range(s.nvir)])
     return s
+
+
+VALENCE = {1: 1, 6: 4, 7: 5, 8: 6}
+
+
+def _hybrids(u):
+    """Orthonormal directed hybrids (s, px, py, pz) for the bond directions
+    u (k x 3 unit vectors), closest in the least-squares sense to the
+    sp3-like targets (1/2, sqrt(3)/2 u) (symmetric orthonormalization), and
+    an orthonormal basis of the rest of the valence space."""
+    k = len(u)
+    t = np.zeros((4, k))
+    t[0] = 0.5
+    t[1:] = 0.8660254037844386 * np.asarray(u).T
+    g = t.T @ t
+    ev, vec = np.linalg.eigh(g)
+    if ev.min() < 1e-6:
+        raise ValueError("Lewis structure: linearly dependent bond directions")
+    h = t @ (vec @ np.diag(ev ** -0.5) @ vec.T)
+    proj = np.eye(4) - h @ h.T
+    ev2, vec2 = np.linalg.eigh(proj)
+    rest = vec2[:, ev2 > 0.5]
+    return h, rest
+
+
+def lewis_lmos(ms, fock=None, pad_atoms=6, scale=1.2):
+    """Starting LMOs from a Lewis structure, the role makvec plays in the
+    reference (src/MOZYME/makvec.F90:16-143): two-centre sigma bonds from
+    directed hybrids, pi bonds between atoms left with an odd electron, and
+    lone pairs, each bond with its antibond as the virtual partner.  In the
+    orthonormal NDDO basis these are orthonormal by construction (every
+    hybrid is used once).  Bonds are polarized by the 2 x 2 Fock matrix of
+    their two hybrids when `fock` (sparse, on ms.nijbo; e.g. the Fock matrix
+    of the diagonal starting density, as makvec builds it) is given, else
+    equal weights.  Supports closed-shell H, C, N, O molecules whose
+    bonding follows from the covalent radii."""
+    from .geometry import COVALENT_RADIUS
+    nat = np.asarray(ms.nat)
+    xyz = np.asarray(ms.xyz, dtype=np.float64)
+    numat = len(nat)
+    iorbs = (ms.nlast - ms.nfirst + 1).astype(np.int64)
+    rad = np.array([COVALENT_RADIUS[int(z)] for z in nat])
+    from scipy.spatial import cKDTree
+    pairs = cKDTree(xyz).query_pairs(2.0 * scale * rad.max(), output_type="ndarray")
+    d = np.linalg.norm(xyz[pairs[:, 0]] - xyz[pairs[:, 1]], axis=1)
+    pairs = pairs[d < scale * (rad[pairs[:, 0]] + rad[pairs[:, 1]])]
+    nbr = [[] for _ in range(numat)]
+    for a, b in pairs:
+        nbr[a].append(int(b))
+        nbr[b].append(int(a))
+    # hybrids: hyb[a][b] = the orbital of atom a pointing at b; rest[a] = the others
+    hyb, rest, elec = [dict() for _ in range(numat)], [None] * numat, np.zeros(numat, dtype=np.int64)
+    for a in range(numat):
+        k = len(nbr[a])
+        if iorbs[a] == 1:
+            if k != 1:
+                raise ValueError(f"Lewis structure: hydrogen {a + 1} has {k} bonds")
+            hyb[a][nbr[a][0]] = np.array([1.0])
+            rest[a] = np.zeros((1, 0))
+        else:
+            if not 1 <= k <= 4:
+                raise ValueError(f"Lewis structure: atom {a + 1} has {k} bonds")
+            u = np.array([(xyz[b] - xyz[a]) / np.linalg.norm(xyz[b] - xyz[a]) for b in nbr[a]])
+            h, r = _hybrids(u)
+            for i, b in enumerate(nbr[a]):
+                hyb[a][b] = h[:, i]
+            rest[a] = r
+        elec[a] = VALENCE[int(nat[a])] - k
+
+    def block(a, b):
+        """F block rows on a, columns on b (dense, from the sparse storage)."""
+        o = ms.nijbo[a, b]
+        if a == b:
+            n = iorbs[a]
+            t = np.zeros((n, n))
+            t[np.tril_indices(n)] = fock[o:o + n * (n + 1) // 2]
+            return t + t.T - np.diag(np.diag(t))
+        if a > b:
+            return fock[o:o + iorbs[a] * iorbs[b]].reshape(iorbs[a], iorbs[b])
+        return block(b, a).T
+
+    def bond(a, ha, b, hb):
+        """(bonding, antibonding) as [(atom, coefficient vector), ...]."""
+        if fock is None:
+            c1 = c2 = np.sqrt(0.5)
+        else:
+            m = np.array([[ha @ block(a, a) @ ha, ha @ block(a, b) @ hb],
+                          [hb @ block(b, a) @ ha, hb @ block(b, b) @ hb]])
+            ev, v = np.linalg.eigh(m)
+            c1, c2 = v[0, 0], v[1, 0]
+        return [(a, c1 * ha), (b, c2 * hb)], [(a, -c2 * ha), (b, c1 * hb)]
+
+    occ, vir = [], []
+    for a, b in pairs:
+        o, v = bond(int(a), hyb[a][int(b)], int(b), hyb[b][int(a)])
+        occ.append(o)
+        vir.append(v)
+    # pi bonds between bonded atoms that each keep an odd electron
+    for a in range(numat):
+        if elec[a] % 2 == 0:
+            continue
+        partners = [b for b in nbr[a] if elec[b] % 2 == 1 and rest[b].shape[1] > 0]
+        if rest[a].shape[1] == 0 or not partners:
+            raise ValueError(f"Lewis structure: no pi partner for atom {a + 1}")
+        b = partners[0]
+        ua, sv, vt = np.linalg.svd(rest[a][1:].T @ rest[b][1:])
+        pa = rest[a] @ ua[:, 0]
+        pb = rest[b] @ vt[0]
+        rest[a] = rest[a] @ ua[:, 1:]
+        rest[b] = rest[b] @ vt[1:].T
+        elec[a] -= 1
+        elec[b] -= 1
+        o, v = bond(a, pa, b, pb)
+        occ.append(o)
+        vir.append(v)
+    for a in range(numat):
+        n = rest[a].shape[1]
+        if elec[a] != 2 * n:
+            raise ValueError(f"Lewis structure: atom {a + 1} has {elec[a]} electrons for {n} lone pairs")
+        for i in range(n):
+            occ.append([(a, rest[a][:, i])])
+    nocc = int(ms.s.nclose)
+    if len(occ) != nocc or len(occ) + len(vir) != ms.norbs:
+        raise ValueError(f"Lewis structure: {len(occ)} occupied / {len(vir)} virtual LMOs for "
+                         f"{nocc} / {ms.norbs - nocc}")
+
+    def pack(lmos):
+        ncf, nncf, ncoc, idx, coef = [], [], [], [], []
+        ia = ic = 0
+        for parts in lmos:
+            parts = sorted(parts, key=lambda t: -float(t[1] @ t[1]))
+            nncf.append(ia)
+            ncoc.append(ic)
+            ncf.append(len(parts))
+            for a, c in parts:
+                idx.append(a + 1)
+                coef.extend(c)
+            idx.extend([0] * pad_atoms)
+            coef.extend([0.0] * (4 * pad_atoms))
+            ia += len(parts) + pad_atoms
+            ic += sum(len(c) for _, c in parts) + 4 * pad_atoms
+        return (np.array(ncf, np.int32), np.array(nncf, np.int32), np.array(ncoc, np.int32),
+                np.array(idx, np.int32), np.array(coef))
+
+    o = pack(occ)
+    v = pack(vir)
+    return LmoSet(numat=numat, norbs=ms.norbs, nfirst=np.asarray(ms.nfirst, np.int32),
+                  nlast=np.asarray(ms.nlast, np.int32), nat=nat.astype(np.int32), nijbo=ms.nijbo,
+                  mpack=ms.mpack, nocc=nocc, nvir=len(vir), ncf=o[0], nncf=o[1], ncocc=o[2], icocc=o[3],
+                  cocc=o[4], nce=v[0], nnce=v[1], ncvir=v[2], icvir=v[3], cvir=v[4], eigs=np.zeros(nocc),
+                  eigv=np.zeros(len(vir)))
 
 
 def lmo_dense(s, k, occupied):

@@ -201,3 +201,71 @@ def test_batch_workload_spans_the_configured_sizes():
     wat = sizes[1::2]
     assert np.all(wat % 6 == 0)                  # (H2O)k: 6k
     assert B.workload(10, seed=4)[3][1].tolist() == mols[3][1].tolist()  # seeded
+
+
+@pytest.mark.parametrize("r_near", [6.0, 30.0])
+def test_mozyme_system_from_geometry_matches_the_conventional_build(r_near):
+    """The sparse build (atom data, near-pair integrals, sparse H) equals the
+    one cut out of the conventional system's full W and H."""
+    from mopac_gpu_amd import mozyme
+    nat, xyz = geometry.glycine_helix(6, 3)
+    s = hamiltonian.build(nat, xyz)
+    a = mozyme.MozymeSystem.from_geometry(nat, xyz, r_near=r_near, r_dipole=r_near + 1.0)
+    b = mozyme.MozymeSystem(s, r_near=r_near, r_dipole=r_near + 1.0)
+    assert np.array_equal(a.nijbo, b.nijbo) and a.mpack == b.mpack
+    assert np.array_equal(a.w, mozyme.mozyme_integrals(s, a.nijbo))
+    assert np.array_equal(a.h, b.h) and a.enuclr == b.enuclr
+    assert a.s.atheat == s.atheat and a.s.nclose == s.nclose
+    assert np.array_equal(a.s.pdiag, s.pdiag)
+    # the nijbo layout: blocks row by row, each the size of its pair
+    io = (s.nlast - s.nfirst + 1).astype(np.int64)
+    off = 0
+    for i in range(s.numat):
+        for j in range(i + 1):
+            if a.nijbo[i, j] >= 0:
+                assert a.nijbo[i, j] == off and a.nijbo[j, i] == off
+                off += io[i] * (io[i] + 1) // 2 if i == j else io[i] * io[j]
+    assert off == a.mpack
+
+
+@pytest.mark.parametrize("kind", ["helix", "water", "alkane"])
+def test_lewis_lmos_are_orthonormal_and_fill_the_shells(kind):
+    """The Lewis-structure start: every valence orbital used once (an
+    orthogonal transformation of the AO basis), one occupied LMO per electron
+    pair, and a determinant whose energy lies above the converged SCF's."""
+    from mopac_gpu_amd import mozyme
+    nat, xyz = {"helix": lambda: geometry.glycine_helix(3, 3),
+                "water": lambda: geometry.water_cluster(4, 1),
+                "alkane": lambda: geometry.alkane(5, 2)}[kind]()
+    s = hamiltonian.build(nat, xyz)
+    ms = mozyme.MozymeSystem.from_geometry(nat, xyz, r_near=30.0, r_dipole=31.0)
+    lm = mozyme.lewis_lmos(ms)
+    assert lm.nocc == s.nclose and lm.nocc + lm.nvir == s.norbs
+    c = np.column_stack([mozyme.lmo_dense(lm, k, True) for k in range(lm.nocc)] +
+                        [mozyme.lmo_dense(lm, k, False) for k in range(lm.nvir)])
+    assert np.max(np.abs(c.T @ c - np.eye(s.norbs))) < 1e-13
+    # every LMO lives on one or two atoms
+    assert set(lm.ncf) <= {1, 2} and set(lm.nce) <= {2}
+    p = 2.0 * c[:, :lm.nocc] @ c[:, :lm.nocc].T
+    pk = np.concatenate([p[i, :i + 1] for i in range(s.norbs)])
+    f = oracle.fock2(s, s.h, pk, 0.5 * pk)
+    e_lewis = 2.0 * oracle.helect(s.norbs, 0.5 * pk, s.h, f)
+    r = oracle.iterate(s, scfcrt=1e-6)
+    assert r["converged"] and e_lewis > r["ee"]
+    # with polarization from a Fock matrix the start is lower still
+    fs = mozyme.sparse_from_dense(_dense(f, s.norbs), ms.nfirst, ms.nlast - ms.nfirst + 1, ms.nijbo, ms.mpack)
+    lp = mozyme.lewis_lmos(ms, fs)
+    c2 = np.column_stack([mozyme.lmo_dense(lp, k, True) for k in range(lp.nocc)])
+    p2 = 2.0 * c2 @ c2.T
+    pk2 = np.concatenate([p2[i, :i + 1] for i in range(s.norbs)])
+    e2 = 2.0 * oracle.helect(s.norbs, 0.5 * pk2, s.h, oracle.fock2(s, s.h, pk2, 0.5 * pk2))
+    assert r["ee"] < e2 < e_lewis
+
+
+def _dense(packed, n):
+    m = np.zeros((n, n))
+    k = 0
+    for i in range(n):
+        m[i, :i + 1] = packed[k:k + i + 1]
+        k += i + 1
+    return m + np.tril(m, -1).T
