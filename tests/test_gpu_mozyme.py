@@ -170,3 +170,24 @@ def test_mozyme_scf_reaches_the_conventional_energy(gpu_ctx, r_near, r_dipole):
     d = np.array([ms.nijbo[a, a] + (k * (k + 1)) // 2 + k for a in range(s.numat)
                   for k in range(int(s.nlast[a] - s.nfirst[a] + 1))])
     assert abs(np.sum(r["p"][d]) - 2 * s.nclose) < 1e-3 * s.nclose
+
+
+def test_lewis_start_reaches_the_conventional_energy(gpu_ctx):
+    """The makvec-style start: Lewis-structure LMOs polarized by the Fock
+    matrix of the diagonal density (built by the device fock2z), on a system
+    built straight from the geometry.  With every pair stored the MOZYME SCF
+    ends within 0.05 kcal/mol of the conventional SCF at MOZYME's default
+    criterion, as the start from localized conventional orbitals does."""
+    nat, xyz = geometry.glycine_helix(6, 3)
+    s = hamiltonian.build(nat, xyz)
+    gpu_ctx.upload(s)
+    conv = gpu_ctx.iter(scfcrt=1e-6)
+    ms = mozyme.MozymeSystem.from_geometry(nat, xyz, r_near=30.0, r_dipole=31.0)
+    ms.upload(gpu_ctx)
+    f0 = ms.fock2z(gpu_ctx, ms.diagonal_density())
+    lmos = mozyme.lewis_lmos(ms, f0)
+    print(f"MOZYME SCF helix {ms.numat} atoms from Lewis LMOs", flush=True)
+    r = ms.scf(gpu_ctx, lmos)
+    print(f"escf {r['escf']:.6f} conventional {conv['escf']:.6f} niter {r['niter']}", flush=True)
+    assert r["converged"] and 3 < r["niter"] < 60
+    assert abs(r["escf"] - conv["escf"]) < 0.05
