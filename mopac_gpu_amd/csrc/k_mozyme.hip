@@ -425,27 +425,39 @@ struct D2Params {
 
 constexpr int D2_NT = 256;
 
+// One rotation of the (virtual i, occupied j) list per workgroup.  A rotation
+// reads and writes only LMOs i and j (coefficients, atom lists, counts), so
+// rotations that share neither commute exactly: the host runs the list as
+// levels (a rotation one level after the last earlier one on its i or its
+// j), one launch per level, which reproduces the sequential order of
+// diagg2.F90 for every LMO.  Each workgroup has its own slot of the atom
+// flags (iused = -1, latoms = 0 between rotations) and of the LMO copies.
 __global__ __launch_bounds__(D2_NT) void k_mz_diagg2(LmoDev L, const double* __restrict__ eigs,
                                                      const double* __restrict__ eigv,
                                                      const double* __restrict__ fmo,
                                                      const int* __restrict__ ifmo, D2Params P,
-                                                     int* __restrict__ iused_g,  // numat, -1
-                                                     char* __restrict__ latoms_g,  // numat, 0
-                                                     double* __restrict__ storej,
-                                                     double* __restrict__ storei,
-                                                     double* __restrict__ out) {  // sumb, nrej
+                                                     const int* __restrict__ order, int base,
+                                                     int* __restrict__ iused_all,   // slots x numat, -1
+                                                     char* __restrict__ latoms_all,  // slots x numat, 0
+                                                     double* __restrict__ storej_all, int span_j,
+                                                     double* __restrict__ storei_all, int span_i,
+                                                     double* __restrict__ sumb_out,
+                                                     int* __restrict__ nrej_out) {
   const int tid = threadIdx.x;
+  const int slot = blockIdx.x;
+  int* __restrict__ iused_g = iused_all + (size_t)slot * L.numat;
+  char* __restrict__ latoms_g = latoms_all + (size_t)slot * L.numat;
+  double* __restrict__ storej = storej_all + (size_t)slot * span_j;
+  double* __restrict__ storei = storei_all + (size_t)slot * span_i;
   __shared__ int flag[D2_NT];
   __shared__ int s_mlf, s_mle, s_fail, s_ncf, s_nce;
-  __shared__ double s_alpha;
   double sumb = 0.0;
   int nrej = 0;
-  for (int ij = 0; ij < P.nij; ++ij) {
+  const int ij = order[base + slot];
+  {
     const int i = ifmo[2 * ij] - 1, j = ifmo[2 * ij + 1] - 1;
-    if (!(fabs(fmo[ij]) >= P.tiny)) continue;
     const double c = fmo[ij] * P.cnst;
     const double d = eigs[j] - eigv[i] - P.shift;
-    if (!(fabs(c / d) >= P.biglim)) continue;
     const int ncfj = L.ncf[j], ncei = L.nce[i];
     const int jlr = L.ncocc[j];
     int jur, jncf, iur, incv;
@@ -680,8 +692,8 @@ __global__ __launch_bounds__(D2_NT) void k_mz_diagg2(LmoDev L, const double* __r
     __syncthreads();
   }
   if (tid == 0) {
-    out[0] = sumb;
-    out[1] = (double)nrej;
+    sumb_out[ij] = sumb;
+    nrej_out[ij] = nrej;
   }
 }
 
@@ -1145,26 +1157,79 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
     P.thresh = st->thresh;
     P.retry = retry ? 1 : 0;
     P.nij = nij;
+    // the rotations diagg2.F90 takes (same tests, same arithmetic as the
+    // reference's loop), each at one level past the last earlier rotation
+    // on its virtual or its occupied LMO
+    std::vector<int> lev_v(std::max(L->nvir, 1), 0), lev_o(std::max(L->nocc, 1), 0), level;
+    std::vector<int> taken;
+    taken.reserve(nij);
+    level.reserve(nij);
+    int nlev = 0;
+    for (int ij = 0; ij < nij; ++ij) {
+      if (!(std::fabs(fmo[ij]) >= P.tiny)) continue;
+      const int i = ifmo[2 * ij] - 1, j = ifmo[2 * ij + 1] - 1;
+      const double c = fmo[ij] * P.cnst;
+      const double d = eigs[j] - eigv[i] - P.shift;
+      if (!(std::fabs(c / d) >= P.biglim)) continue;
+      const int l = std::max(lev_v[i], lev_o[j]) + 1;
+      lev_v[i] = lev_o[j] = l;
+      taken.push_back(ij);
+      level.push_back(l);
+      nlev = std::max(nlev, l);
+    }
+    // entries grouped by level (counting sort, list order inside a level)
+    std::vector<int> start(nlev + 2, 0), order(taken.size());
+    for (int l : level) ++start[l + 1];
+    for (int l = 1; l <= nlev + 1; ++l) start[l] += start[l - 1];
+    {
+      std::vector<int> pos(start.begin(), start.end());
+      for (size_t k = 0; k < taken.size(); ++k) order[pos[level[k]]++] = taken[k];
+    }
+    int widest = 0;
+    for (int l = 1; l <= nlev; ++l) widest = std::max(widest, start[l + 1] - start[l]);
+    // each LMO's coefficient span, as the kernel bounds it
+    auto span_of = [&](int n, const int* no, int cdim) {
+      int m = 1;
+      for (int k = 0; k < n; ++k) {
+        const int lo = no[k], hi = (k + 1 < n) ? no[k + 1] : cdim;
+        m = std::max(m, std::min(lo + L->norbs, hi) - lo);
+      }
+      return m;
+    };
+    const int span_j = span_of(L->nocc, L->ncocc, L->cocc_dim);
+    const int span_i = span_of(L->nvir, L->ncvir, L->cvir_dim);
+    const int nslots = std::max(1, std::min(widest, 2048));
     LmoBufs B;
     upload_lmos(s, L, B);
-    DevBuf deigs, deigv, dfmo, difmo, dused, dlat, dsj, dsi, dout;
+    DevBuf deigs, deigv, dfmo, difmo, dorder, dused, dlat, dsj, dsi, dsumb, dnrej;
     up(s, deigs, eigs, sizeof(double) * std::max(L->nocc, 1));
     up(s, deigv, eigv, sizeof(double) * std::max(L->nvir, 1));
     up(s, dfmo, fmo, sizeof(double) * std::max(nij, 1));
     up(s, difmo, ifmo, sizeof(int) * 2 * std::max(nij, 1));
-    std::vector<int> minus1(L->numat, -1);
-    up(s, dused, minus1.data(), sizeof(int) * L->numat);
-    dlat.ensure(L->numat);
-    MHIP_CHECK(hipMemsetAsync(dlat.ptr, 0, L->numat, s));
-    dsj.ensure(sizeof(double) * (L->norbs + 8));
-    dsi.ensure(sizeof(double) * (L->norbs + 8));
-    dout.ensure(sizeof(double) * 2);
-    hipLaunchKernelGGL(k_mz_diagg2, dim3(1), dim3(D2_NT), 0, s, B.dev, deigs.as<double>(), deigv.as<double>(),
-                       dfmo.as<double>(), difmo.as<int>(), P, dused.as<int>(), (char*)dlat.ptr, dsj.as<double>(),
-                       dsi.as<double>(), dout.as<double>());
-    MHIP_LAUNCH_CHECK();
-    double out[2];
-    down(s, out, dout, sizeof(out));
+    up(s, dorder, order.data(), sizeof(int) * order.size());
+    dused.ensure(sizeof(int) * (size_t)nslots * L->numat);
+    MHIP_CHECK(hipMemsetAsync(dused.ptr, 0xff, sizeof(int) * (size_t)nslots * L->numat, s));  // -1
+    dlat.ensure((size_t)nslots * L->numat);
+    MHIP_CHECK(hipMemsetAsync(dlat.ptr, 0, (size_t)nslots * L->numat, s));
+    dsj.ensure(sizeof(double) * (size_t)nslots * span_j);
+    dsi.ensure(sizeof(double) * (size_t)nslots * span_i);
+    dsumb.ensure(sizeof(double) * std::max(nij, 1));
+    dnrej.ensure(sizeof(int) * std::max(nij, 1));
+    for (int l = 1; l <= nlev; ++l)
+      for (int b = start[l]; b < start[l + 1]; b += nslots) {
+        const int cnt = std::min(nslots, start[l + 1] - b);
+        hipLaunchKernelGGL(k_mz_diagg2, dim3(cnt), dim3(D2_NT), 0, s, B.dev, deigs.as<double>(),
+                           deigv.as<double>(), dfmo.as<double>(), difmo.as<int>(), P, dorder.as<int>(), b,
+                           dused.as<int>(), (char*)dlat.ptr, dsj.as<double>(), span_j, dsi.as<double>(), span_i,
+                           dsumb.as<double>(), dnrej.as<int>());
+        MHIP_LAUNCH_CHECK();
+      }
+    std::vector<double> sb(std::max(nij, 1));
+    std::vector<int> nr(std::max(nij, 1));
+    if (!taken.empty()) {
+      down(s, sb.data(), dsumb, sizeof(double) * nij);
+      down(s, nr.data(), dnrej, sizeof(int) * nij);
+    }
     down(s, L->ncf, B.ncf, sizeof(int) * L->nocc);
     down(s, L->icocc, B.icocc, sizeof(int) * L->icocc_dim);
     down(s, L->cocc, B.cocc, sizeof(double) * L->cocc_dim);
@@ -1172,9 +1237,15 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
     down(s, L->icvir, B.icvir, sizeof(int) * L->icvir_dim);
     down(s, L->cvir, B.cvir, sizeof(double) * L->cvir_dim);
     MHIP_CHECK(hipStreamSynchronize(s));
-    st->sumb = out[0];
+    double sumb = 0.0;
+    int nrej = 0;
+    for (int ij : taken) {  // list order
+      sumb += sb[ij];
+      nrej += nr[ij];
+    }
+    st->sumb = sumb;
     st->nrejct[1] = st->nrejct[0];
-    st->nrejct[0] = (int)out[1];
+    st->nrejct[0] = nrej;
   });
 }
 
