@@ -25,6 +25,8 @@
 #include <vector>
 
 #include "common.h"
+#include <cstring>
+#include <map>
 
 namespace mhip {
 namespace {
@@ -423,7 +425,7 @@ struct D2Params {
   int retry, nij;
 };
 
-constexpr int D2_NT = 256;
+constexpr int D2_NT = 64;  // one wave: the per-rotation scans run at wave latency
 
 // One rotation of the (virtual i, occupied j) list per workgroup.  A rotation
 // reads and writes only LMOs i and j (coefficients, atom lists, counts), so
@@ -450,7 +452,7 @@ __global__ __launch_bounds__(D2_NT) void k_mz_diagg2(LmoDev L, const double* __r
   double* __restrict__ storej = storej_all + (size_t)slot * span_j;
   double* __restrict__ storei = storei_all + (size_t)slot * span_i;
   __shared__ int flag[D2_NT];
-  __shared__ int s_mlf, s_mle, s_fail, s_ncf, s_nce;
+  __shared__ int s_mlf, s_mle, s_fail, s_ncf, s_nce, s_jext, s_iext;
   double sumb = 0.0;
   int nrej = 0;
   const int ij = order[base + slot];
@@ -468,26 +470,48 @@ __global__ __launch_bounds__(D2_NT) void k_mz_diagg2(LmoDev L, const double* __r
     if (i != L.nvir - 1) { iur = L.ncvir[i + 1]; incv = L.nnce[i + 1]; }
     else { iur = L.cvir_dim; incv = L.icvir_dim; }
     iur = min(ilr + L.norbs, iur);
-    for (int k = tid; k < jur - jlr; k += D2_NT) storej[k] = L.cocc[jlr + k];
-    for (int k = tid; k < iur - ilr; k += D2_NT) storei[k] = L.cvir[ilr + k];
+    // undo log for a rejection: here the coefficients of the atoms the two
+    // LMOs hold now (at most 4 per atom), later each grown position just
+    // before it is written; a rejection puts back exactly what was written,
+    // which leaves both stores as a full copy-and-restore would
+    const int bj = min(4 * L.ncf[j], jur - jlr), bi = min(4 * L.nce[i], iur - ilr);
+    for (int k = tid; k < bj; k += D2_NT) storej[k] = L.cocc[jlr + k];
+    for (int k = tid; k < bi; k += D2_NT) storei[k] = L.cvir[ilr + k];
     const double e = copysign(sqrt(4.0 * c * c + d * d), d);
     double alpha = sqrt(0.5 * (1.0 + d / e));
     __syncthreads();
     for (;;) {
       const double beta = -copysign(sqrt(1.0 - alpha * alpha), c);
       if (tid == 0) sumb += fabs(beta);
-      // iused: offset of each atom of occupied LMO j (its original atoms)
-      if (tid == 0) {
-        int mlf = 0;
-        for (int lf = L.nncf[j]; lf < L.nncf[j] + L.ncf[j]; ++lf) {
-          const int a = L.icocc[lf] - 1;
-          iused_g[a] = mlf;
-          mlf += L.iorbs[a];
+      // iused: offset of each atom of occupied LMO j (its original atoms),
+      // an exclusive scan of their orbital counts
+      {
+        const int n0 = L.nncf[j], nc = L.ncf[j];
+        int base = 0;
+        for (int c0 = 0; c0 < nc; c0 += D2_NT) {
+          const int lf = c0 + tid;
+          const int a = (lf < nc) ? L.icocc[n0 + lf] - 1 : -1;
+          const int no = (a >= 0) ? L.iorbs[a] : 0;
+          flag[tid] = no;
+          __syncthreads();
+          for (int off = 1; off < D2_NT; off <<= 1) {
+            const int t = (tid >= off) ? flag[tid - off] : 0;
+            __syncthreads();
+            flag[tid] += t;
+            __syncthreads();
+          }
+          if (a >= 0) iused_g[a] = base + flag[tid] - no;
+          base += flag[D2_NT - 1];
+          __syncthreads();
         }
-        s_mlf = mlf;
-        s_fail = 0;
-        s_ncf = L.ncf[j];
-        s_nce = L.nce[i];
+        if (tid == 0) {
+          s_mlf = base;
+          s_fail = 0;
+          s_ncf = L.ncf[j];
+          s_nce = L.nce[i];
+          s_jext = bj;
+          s_iext = bi;
+        }
       }
       __syncthreads();
       const int loopi = L.ncvir[i], loopj = L.ncocc[j];
@@ -571,6 +595,7 @@ __global__ __launch_bounds__(D2_NT) void k_mz_diagg2(LmoDev L, const double* __r
               L.icocc[slot] = mie + 1;
               iused_g[mie] = obase;
               for (int x = 0; x < no; ++x) {
+                storej[obase + x] = L.cocc[loopj + obase + x];
                 L.cocc[loopj + obase + x] = beta * L.cvir[loopi + mle + x];
                 L.cvir[loopi + mle + x] = alpha * L.cvir[loopi + mle + x];
               }
@@ -580,6 +605,7 @@ __global__ __launch_bounds__(D2_NT) void k_mz_diagg2(LmoDev L, const double* __r
             if (s_fail) break;
             ncf_run += ngrow;
             mlf_run += grown_orbs;
+            if (tid == 0) s_jext = max(s_jext, mlf_run);
           }
           mle_base += chunk_orbs;
           __syncthreads();
@@ -647,6 +673,7 @@ __global__ __launch_bounds__(D2_NT) void k_mz_diagg2(LmoDev L, const double* __r
               L.icvir[slot] = ii + 1;
               latoms_g[ii] = 1;
               for (int x = 0; x < no; ++x) {
+                storei[obase + x] = L.cvir[loopi + obase + x];
                 L.cvir[loopi + obase + x] = -beta * L.cocc[loopj + u + x];
                 L.cocc[loopj + u + x] = alpha * L.cocc[loopj + u + x];
               }
@@ -656,6 +683,7 @@ __global__ __launch_bounds__(D2_NT) void k_mz_diagg2(LmoDev L, const double* __r
             if (s_fail) break;
             nce_run += ngrow;
             mle_run += grown_orbs;
+            if (tid == 0) s_iext = max(s_iext, mle_run);
           }
           __syncthreads();
         }
@@ -671,8 +699,10 @@ __global__ __launch_bounds__(D2_NT) void k_mz_diagg2(LmoDev L, const double* __r
       }
       // ---- rejection (label 1000): restore both LMOs and all flags
       if (tid == 0) ++nrej;
-      for (int k = tid; k < jur - jlr; k += D2_NT) L.cocc[jlr + k] = storej[k];
-      for (int k = tid; k < iur - ilr; k += D2_NT) L.cvir[ilr + k] = storei[k];
+      __syncthreads();
+      const int rj = max(bj, s_jext), ri = max(bi, s_iext);
+      for (int k = tid; k < rj; k += D2_NT) L.cocc[jlr + k] = storej[k];
+      for (int k = tid; k < ri; k += D2_NT) L.cvir[ilr + k] = storei[k];
       for (int k = tid; k < L.numat; k += D2_NT) {
         iused_g[k] = -1;
         latoms_g[k] = 0;
@@ -871,6 +901,22 @@ struct LmoBufs {
   LmoDev dev{};
 };
 
+// Device copies and scratch of the MOZYME entry points, one set per device,
+// grown as needed and kept until the process ends: no allocation per call.
+struct MzWork {
+  LmoBufs B;
+  std::vector<int> nijbo_seen, iorbs_seen;  // the nijbo (and sizes) last checked and uploaded
+  long long nijbo_mpack = -1;
+  DevBuf dp, dpart, dfao, daocc, dcounts, doffs, dnfmo, dold, dfmo, difmo, dvmax, dvsum, deigv, deigs, dorder,
+      dused, dlat, dsj, dsi, dsumb, dnrej, dh, df, dout, dpn, d1, d2, d3, di, dsc;
+};
+MzWork& mz_work() {
+  static std::map<int, MzWork> work;
+  int d = 0;
+  MHIP_CHECK(hipGetDevice(&d));
+  return work[d];
+}
+
 void up(hipStream_t s, DevBuf& b, const void* src, size_t bytes) {
   b.ensure(bytes ? bytes : 8);
   if (bytes) MHIP_CHECK(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, s));
@@ -879,8 +925,38 @@ void down(hipStream_t s, void* dst, const DevBuf& b, size_t bytes) {
   if (bytes) MHIP_CHECK(hipMemcpyAsync(dst, b.ptr, bytes, hipMemcpyDeviceToHost, s));
 }
 
+// nijbo into W.B.nijbo (and iorbs into W.B.iorbs), checked first: every
+// block inside mpack, the matrix symmetric, every diagonal block stored.
+// A nijbo identical to the last one checked is neither checked nor sent again.
+void upload_nijbo(hipStream_t s, MzWork& W, int numat, const int* nijbo, long long mpack,
+                  const std::vector<int>& iorbs) {
+  const size_t nn = (size_t)numat * numat;
+  if (W.nijbo_seen.size() == nn && W.nijbo_mpack == mpack && W.iorbs_seen == iorbs &&
+      std::memcmp(W.nijbo_seen.data(), nijbo, sizeof(int) * nn) == 0)
+    return;
+  for (int b = 0; b < numat; ++b)
+    for (int a = 0; a < numat; ++a) {
+      const long long o = nijbo[(size_t)a + (size_t)b * numat];
+      if (o < 0) {
+        MHIP_REQUIRE(a != b, "LMO set: every diagonal block of nijbo must be stored");
+        continue;
+      }
+      const long long blk = (a == b) ? iorbs[a] * (iorbs[a] + 1) / 2 : (long long)iorbs[a] * iorbs[b];
+      MHIP_REQUIRE(o + blk <= mpack, "LMO set: nijbo block beyond mpack");
+      MHIP_REQUIRE(nijbo[(size_t)b + (size_t)a * numat] == o, "LMO set: nijbo not symmetric");
+    }
+  W.nijbo_seen.clear();  // stays empty if an upload below fails
+  up(s, W.B.nijbo, nijbo, sizeof(int) * nn);
+  up(s, W.B.iorbs, iorbs.data(), sizeof(int) * numat);
+  MHIP_CHECK(hipStreamSynchronize(s));
+  W.nijbo_seen.assign(nijbo, nijbo + nn);
+  W.iorbs_seen = iorbs;
+  W.nijbo_mpack = mpack;
+}
+
 // Checks every index the kernels will follow, then uploads.
-void upload_lmos(hipStream_t s, const mhip_lmo_set* L, LmoBufs& B) {
+void upload_lmos(hipStream_t s, const mhip_lmo_set* L, MzWork& W) {
+  LmoBufs& B = W.B;
   MHIP_REQUIRE(L && L->numat > 0 && L->norbs > 0 && L->nocc >= 0 && L->nvir >= 0, "LMO set: bad sizes");
   std::vector<int> iorbs(L->numat);
   for (int a = 0; a < L->numat; ++a) {
@@ -907,18 +983,8 @@ void upload_lmos(hipStream_t s, const mhip_lmo_set* L, LmoBufs& B) {
   };
   check_set(L->nocc, L->ncf, L->nncf, L->ncocc, L->icocc, L->icocc_dim, L->cocc_dim, "the occupied set");
   check_set(L->nvir, L->nce, L->nnce, L->ncvir, L->icvir, L->icvir_dim, L->cvir_dim, "the virtual set");
-  const size_t nn = (size_t)L->numat * L->numat;
-  for (int b = 0; b < L->numat; ++b)
-    for (int a = 0; a < L->numat; ++a) {
-      const long long o = L->nijbo[(size_t)a + (size_t)b * L->numat];
-      if (o < 0) continue;
-      const long long blk = (a == b) ? iorbs[a] * (iorbs[a] + 1) / 2 : (long long)iorbs[a] * iorbs[b];
-      MHIP_REQUIRE(o + blk <= L->mpack, "LMO set: nijbo block beyond mpack");
-      MHIP_REQUIRE(L->nijbo[(size_t)b + (size_t)a * L->numat] == o, "LMO set: nijbo not symmetric");
-    }
+  upload_nijbo(s, W, L->numat, L->nijbo, L->mpack, iorbs);
   up(s, B.nfirst, L->nfirst, sizeof(int) * L->numat);
-  up(s, B.iorbs, iorbs.data(), sizeof(int) * L->numat);
-  up(s, B.nijbo, L->nijbo, sizeof(int) * nn);
   up(s, B.ncf, L->ncf, sizeof(int) * L->nocc);
   up(s, B.nncf, L->nncf, sizeof(int) * L->nocc);
   up(s, B.ncocc, L->ncocc, sizeof(int) * L->nocc);
@@ -978,9 +1044,10 @@ extern "C" {
 int mhip_mozyme_density(mhip_ctx* ctx, const mhip_lmo_set* L, int mode, const double* partp, double* p) {
   return mz_guard([&] {
     hipStream_t s = mz_stream(ctx);
-    LmoBufs B;
-    upload_lmos(s, L, B);
-    DevBuf dp, dpart;
+    MzWork& W = mz_work();
+    upload_lmos(s, L, W);
+    const LmoBufs& B = W.B;
+    DevBuf &dp = W.dp, &dpart = W.dpart;
     dp.ensure(sizeof(double) * L->mpack);
     if (mode != 0) {
       MHIP_REQUIRE(partp != nullptr, "mhip_mozyme_density: partp required for mode != 0");
@@ -1019,9 +1086,12 @@ int mhip_mozyme_diagg1(mhip_ctx* ctx, const mhip_lmo_set* L, const double* fao, 
     }
     const bool full = (idiagg <= 5 || idiagg % 2 == 0);
     MHIP_REQUIRE(full || st->nfmo != nullptr, "mhip_mozyme_diagg1: nfmo needed to reuse the list");
-    LmoBufs B;
-    upload_lmos(s, L, B);
-    DevBuf dfao, dp, daocc, dcounts, doffs, dnfmo, dold, dfmo, difmo, dvmax, dvsum, deigv, deigs;
+    MzWork& W = mz_work();
+    upload_lmos(s, L, W);
+    const LmoBufs& B = W.B;
+    DevBuf &dfao = W.dfao, &dp = W.dp, &daocc = W.daocc, &dcounts = W.dcounts, &doffs = W.doffs,
+           &dnfmo = W.dnfmo, &dold = W.dold, &dfmo = W.dfmo, &difmo = W.difmo, &dvmax = W.dvmax,
+           &dvsum = W.dvsum, &deigv = W.deigv, &deigs = W.deigs;
     up(s, dfao, fao, sizeof(double) * L->mpack);
     up(s, dp, p, sizeof(double) * L->mpack);
     daocc.ensure(sizeof(double) * std::max(L->icocc_dim, 1));
@@ -1199,9 +1269,11 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
     const int span_j = span_of(L->nocc, L->ncocc, L->cocc_dim);
     const int span_i = span_of(L->nvir, L->ncvir, L->cvir_dim);
     const int nslots = std::max(1, std::min(widest, 2048));
-    LmoBufs B;
-    upload_lmos(s, L, B);
-    DevBuf deigs, deigv, dfmo, difmo, dorder, dused, dlat, dsj, dsi, dsumb, dnrej;
+    MzWork& W = mz_work();
+    upload_lmos(s, L, W);
+    const LmoBufs& B = W.B;
+    DevBuf &deigs = W.deigs, &deigv = W.deigv, &dfmo = W.dfmo, &difmo = W.difmo, &dorder = W.dorder,
+           &dused = W.dused, &dlat = W.dlat, &dsj = W.dsj, &dsi = W.dsi, &dsumb = W.dsumb, &dnrej = W.dnrej;
     up(s, deigs, eigs, sizeof(double) * std::max(L->nocc, 1));
     up(s, deigv, eigv, sizeof(double) * std::max(L->nvir, 1));
     up(s, dfmo, fmo, sizeof(double) * std::max(nij, 1));
@@ -1256,19 +1328,10 @@ int mhip_mozyme_helecz(mhip_ctx* ctx, int numat, const int* nfirst, const int* n
     hipStream_t s = mz_stream(ctx);
     std::vector<int> iorbs(numat);
     for (int a = 0; a < numat; ++a) iorbs[a] = nlast[a] - nfirst[a] + 1;
-    for (int b = 0; b < numat; ++b)
-      for (int a = 0; a < numat; ++a) {
-        const long long o = nijbo[(size_t)a + (size_t)b * numat];
-        if (o < 0) {
-          MHIP_REQUIRE(a != b, "mhip_mozyme_helecz: every diagonal block must be stored");
-          continue;
-        }
-        const long long blk = (a == b) ? iorbs[a] * (iorbs[a] + 1) / 2 : (long long)iorbs[a] * iorbs[b];
-        MHIP_REQUIRE(o + blk <= mpack, "mhip_mozyme_helecz: nijbo block beyond mpack");
-      }
-    DevBuf di, dn, dp, dh, df, dpart, dout;
-    up(s, di, iorbs.data(), sizeof(int) * numat);
-    up(s, dn, nijbo, sizeof(int) * (size_t)numat * numat);
+    MzWork& W = mz_work();
+    upload_nijbo(s, W, numat, nijbo, mpack, iorbs);
+    DevBuf &di = W.B.iorbs, &dn = W.B.nijbo, &dp = W.dp, &dh = W.dh, &df = W.df, &dpart = W.dpart,
+           &dout = W.dout;
     up(s, dp, p, sizeof(double) * mpack);
     up(s, dh, h, sizeof(double) * mpack);
     up(s, df, f, sizeof(double) * mpack);
@@ -1292,7 +1355,9 @@ int mhip_mozyme_cnvgz(mhip_ctx* ctx, int norbs, long long mpack, double* pnew, d
     for (int i = 0; i < norbs; ++i)
       MHIP_REQUIRE(idiag[i] >= 1 && idiag[i] <= mpack, "mhip_mozyme_cnvgz: idiag out of range");
     hipStream_t s = mz_stream(ctx);
-    DevBuf dpn, dp, d1, d2, d3, di, dpart, dsc;
+    MzWork& W = mz_work();
+    DevBuf &dpn = W.dpn, &dp = W.dp, &d1 = W.d1, &d2 = W.d2, &d3 = W.d3, &di = W.di, &dpart = W.dpart,
+           &dsc = W.dsc;
     up(s, dpn, pnew, sizeof(double) * mpack);
     up(s, dp, p, sizeof(double) * mpack);
     up(s, d1, p1, sizeof(double) * norbs);

@@ -496,6 +496,13 @@ int mhip_mozyme_scf(mhip_ctx* c, mhip_lmo_set* lmo, const mhip_mozyme_opts* o, m
         idiag[l++] = j;
       }
     }
+    // stored off-diagonal blocks (offset, size), in eimp's order (i, then j < i)
+    std::vector<std::pair<int, int>> offdiag;
+    for (int i = 0; i < numat; ++i)
+      for (int j = 0; j < i; ++j) {
+        const int k = nijbo[i + (size_t)j * numat];
+        if (k >= 0) offdiag.emplace_back(k, iorbs[i] * iorbs[j]);
+      }
     std::vector<double> h((size_t)mp), P((size_t)mp), F((size_t)mp), pold((size_t)mp, 0.0), p1(norbs, 0.0),
         p2(norbs, 0.0), p3(norbs, 0.0);
     MHIP_CHECK(hipMemcpy(h.data(), z->hs.ptr, sizeof(double) * mp, hipMemcpyDeviceToHost));
@@ -536,15 +543,11 @@ int mhip_mozyme_scf(mhip_ctx* c, mhip_lmo_set* lmo, const mhip_mozyme_opts* o, m
       MHIP_REQUIRE(check(occ, iorbs) && check(vir, iorbs), "mhip_mozyme_scf: LMO normalization error > 0.1");
       // eimp (src/MOZYME/eimp.F90:16-45): the s-s slot of each stored
       // off-diagonal block of P holds the sum of squares of that F block
-      for (int i = 0; i < numat; ++i)
-        for (int j = 0; j < i; ++j) {
-          const int k = nijbo[i + (size_t)j * numat];
-          if (k < 0) continue;
-          const int l = iorbs[i] * iorbs[j];
-          double sum = 0.0;
-          for (int m = 0; m < l; ++m) sum += F[k + m] * F[k + m];
-          P[k] = sum;
-        }
+      for (const auto& b : offdiag) {
+        double sum = 0.0;
+        for (int m = 0; m < b.second; ++m) sum += F[b.first + m] * F[b.first + m];
+        P[b.first] = sum;
+      }
       // diagg (src/MOZYME/diagg.F90:16-69)
       int nij = 0;
       check_rc(mhip_mozyme_diagg1(c, &L, F.data(), P.data(), eigs.data(), eigv.data(), idiagg, &st1, fmo_dim,
