@@ -85,10 +85,21 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_jac_solve(
   const int I = pairs[2 * pidx], J = pairs[2 * pidx + 1];
   const int tid = threadIdx.x;
   const int k1 = tid >> 5, k2 = tid & 31;
-  for (int e = tid; e < SB * SB; e += SOLVE_THREADS) {
-    const int r = e % SB, c = e / SB;
-    S[r][c] = A[(size_t)gidx_dev(I, J, r) + (size_t)gidx_dev(I, J, c) * lda];
-    Q[r][c] = (r == c) ? 1.0 : 0.0;
+  {
+    // all four loads of this thread in flight before the first LDS store
+    constexpr int PER = SB * SB / SOLVE_THREADS;
+    double v[PER];
+#pragma unroll
+    for (int it = 0; it < PER; ++it) {
+      const int e = tid + SOLVE_THREADS * it;
+      v[it] = A[(size_t)gidx_dev(I, J, e % SB) + (size_t)gidx_dev(I, J, e / SB) * lda];
+    }
+#pragma unroll
+    for (int it = 0; it < PER; ++it) {
+      const int e = tid + SOLVE_THREADS * it, r = e % SB, c = e / SB;
+      S[r][c] = v[it];
+      Q[r][c] = (r == c) ? 1.0 : 0.0;
+    }
   }
   __syncthreads();
   for (int sw = 0; sw < max_inner; ++sw) {
@@ -227,10 +238,21 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_jac_solve_x(
   const bool do_intra = intra[pidx] != 0;
   const int tid = threadIdx.x;
   const int k1 = tid >> 5, k2 = tid & 31;
-  for (int e = tid; e < SB * SB; e += SOLVE_THREADS) {
-    const int r = e % SB, c = e / SB;
-    S[r][c] = A[(size_t)gidx_dev(I, J, r) + (size_t)gidx_dev(I, J, c) * lda];
-    Q[r][c] = (r == c) ? 1.0 : 0.0;
+  {
+    // all four loads of this thread in flight before the first LDS store
+    constexpr int PER = SB * SB / SOLVE_THREADS;
+    double v[PER];
+#pragma unroll
+    for (int it = 0; it < PER; ++it) {
+      const int e = tid + SOLVE_THREADS * it;
+      v[it] = A[(size_t)gidx_dev(I, J, e % SB) + (size_t)gidx_dev(I, J, e / SB) * lda];
+    }
+#pragma unroll
+    for (int it = 0; it < PER; ++it) {
+      const int e = tid + SOLVE_THREADS * it, r = e % SB, c = e / SB;
+      S[r][c] = v[it];
+      Q[r][c] = (r == c) ? 1.0 : 0.0;
+    }
   }
   __syncthreads();
   for (int sw = 0; sw < max_inner; ++sw) {
@@ -475,6 +497,22 @@ __global__ __launch_bounds__(256) void k_jac_apply_c(double* __restrict__ C, int
 }
 
 
+// This thread's 16 elements (e = tid + 256 it) of a 64x64 rotation, all
+// loads issued together; Q = I for a pair marked inactive.
+__device__ __forceinline__ void stage_q(double (&v)[SB * SB / 256], const double* __restrict__ q, bool active,
+                                        int tid) {
+  if (active) {
+#pragma unroll
+    for (int it = 0; it < SB * SB / 256; ++it) v[it] = q[tid + 256 * it];
+  } else {
+#pragma unroll
+    for (int it = 0; it < SB * SB / 256; ++it) {
+      const int e = tid + 256 * it;
+      v[it] = (e % SB == e / SB) ? 1.0 : 0.0;
+    }
+  }
+}
+
 // MFMA versions of the two update kernels (same data flow as above).
 __global__ __launch_bounds__(256) void k_jac_apply_a_mfma(double* __restrict__ A, int lda,
                                                           int npairs, int npo,
@@ -494,10 +532,23 @@ __global__ __launch_bounds__(256) void k_jac_apply_a_mfma(double* __restrict__ A
   const int tid = threadIdx.x;
   const double* q1 = Qbuf + (size_t)P1 * SB * SB;
   const double* q2 = Qbuf + (size_t)P2 * SB * SB;
-  for (int e = tid; e < SB * SB; e += 256) {
-    const int r = e % SB, c = e / SB;
-    At[r][c] = A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda];
-    Qs[r][c] = a1 ? q1[e] : (r == c ? 1.0 : 0.0);
+  // every global load of a stage is issued before the first LDS store, so a
+  // thread waits for memory once per stage, not once per element; Q2 is
+  // fetched into registers now and lands in LDS after the first product
+  constexpr int PER = SB * SB / 256;  // element e = tid + 256 it: row tid % 64, column tid / 64 + 4 it
+  double va[PER], vq[PER], vq2[PER];
+#pragma unroll
+  for (int it = 0; it < PER; ++it) {
+    const int e = tid + 256 * it;
+    va[it] = A[(size_t)gidx(I1, J1, e % SB) + (size_t)gidx(I2, J2, e / SB) * lda];
+  }
+  stage_q(vq, q1, a1, tid);
+  if (P1 != P2) stage_q(vq2, q2, a2, tid);
+#pragma unroll
+  for (int it = 0; it < PER; ++it) {
+    const int e = tid + 256 * it;
+    At[e % SB][e / SB] = va[it];
+    Qs[e % SB][e / SB] = vq[it];
   }
   __syncthreads();
   d4_t acc[4];
@@ -506,9 +557,10 @@ __global__ __launch_bounds__(256) void k_jac_apply_a_mfma(double* __restrict__ A
   __syncthreads();
   mfma_tile64_store(acc, [&](int r, int c, double v) { At[r][c] = v; });
   if (P1 != P2) {
-    for (int e = tid; e < SB * SB; e += 256) {
-      const int r = e % SB, c = e / SB;
-      Qs[r][c] = a2 ? q2[e] : (r == c ? 1.0 : 0.0);
+#pragma unroll
+    for (int it = 0; it < PER; ++it) {
+      const int e = tid + 256 * it;
+      Qs[e % SB][e / SB] = vq2[it];
     }
   }
   __syncthreads();
@@ -549,11 +601,22 @@ __global__ __launch_bounds__(256) void k_jac_apply_c_mfma(double* __restrict__ C
   const int I = pairs[2 * P], J = pairs[2 * P + 1];
   const double* q = Qbuf + (size_t)P * SB * SB;
   const int tid = threadIdx.x;
-  for (int e = tid; e < SB * SB; e += 256) {
-    const int r = e % SB, c = e / SB;
-    const int gr = row0 + r;
-    Ct[r][c] = (gr < nrows) ? C[(size_t)gr + (size_t)gidx(I, J, c) * ldc] : 0.0;
-    Qs[r][c] = q[e];
+  constexpr int PER = SB * SB / 256;
+  double vc[PER], vq[PER];
+  {
+    const int gr = row0 + tid % SB;  // the same row for all of this thread's elements
+#pragma unroll
+    for (int it = 0; it < PER; ++it) {
+      const int e = tid + 256 * it;
+      vc[it] = (gr < nrows) ? C[(size_t)gr + (size_t)gidx(I, J, e / SB) * ldc] : 0.0;
+    }
+  }
+  stage_q(vq, q, true, tid);
+#pragma unroll
+  for (int it = 0; it < PER; ++it) {
+    const int e = tid + 256 * it;
+    Ct[e % SB][e / SB] = vc[it];
+    Qs[e % SB][e / SB] = vq[it];
   }
   __syncthreads();
   d4_t acc[4];
