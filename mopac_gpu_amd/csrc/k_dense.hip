@@ -270,15 +270,18 @@ __device__ __forceinline__ void xcd_tile(int L, int gm, int gn, int& tm, int& tn
 }
 
 constexpr int BK = 16;
+// Eight waves (two per SIMD) on a 4 x 2 grid of sub-tiles, so each SIMD has
+// a second wave to issue MFMAs while one waits on LDS or the barrier.
+constexpr int GB_NT = 512;
 template <int BM, int BN, bool TA, bool TBT>
-__global__ __launch_bounds__(256) void k_gemm_big(int M, int N, int K, double alpha,
-                                                  const double* __restrict__ a, int lda,
-                                                  const double* __restrict__ b, int ldb, double beta,
-                                                  double* __restrict__ c, int ldc) {
+__global__ __launch_bounds__(GB_NT) void k_gemm_big(int M, int N, int K, double alpha,
+                                                    const double* __restrict__ a, int lda,
+                                                    const double* __restrict__ b, int ldb, double beta,
+                                                    double* __restrict__ c, int ldc) {
   constexpr int SA = BM + 16, SBn = BN + 16;
-  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int WM = BM / 4, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
-  constexpr int NA2 = BM * BK / 2 / 256, NB2 = BN * BK / 2 / 256;
+  constexpr int NA2 = BM * BK / 2 / GB_NT, NB2 = BN * BK / 2 / GB_NT;
   static_assert(NA2 >= 1 && NB2 >= 1 && FM >= 1 && FN >= 1, "tile too small");
   __shared__ double As[2][BK][SA];
   __shared__ double Bs[2][BK][SBn];
@@ -287,12 +290,12 @@ __global__ __launch_bounds__(256) void k_gemm_big(int M, int N, int K, double al
   int tm, tn;
   xcd_tile(blockIdx.x, M / BM, N / BN, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
-  const int wm = (w >> 1) * WM, wn = (w & 1) * WN;
+  const int wm = (w >> 1) * WM, wn = (w & 1) * WN;  // w = 0..7
   double2 ra[NA2], rb[NB2];
   auto gload = [&](int k0) {
 #pragma unroll
     for (int r = 0; r < NA2; ++r) {
-      const int idx = tid + 256 * r;
+      const int idx = tid + GB_NT * r;
       const double* src;
       if (!TA) src = a + (size_t)(m0 + 2 * (idx % (BM / 2))) + (size_t)(k0 + idx / (BM / 2)) * lda;
       else src = a + (size_t)(k0 + 2 * (idx % (BK / 2))) + (size_t)(m0 + idx / (BK / 2)) * lda;
@@ -300,7 +303,7 @@ __global__ __launch_bounds__(256) void k_gemm_big(int M, int N, int K, double al
     }
 #pragma unroll
     for (int r = 0; r < NB2; ++r) {
-      const int idx = tid + 256 * r;
+      const int idx = tid + GB_NT * r;
       const double* src;
       if (!TBT) src = b + (size_t)(k0 + 2 * (idx % (BK / 2))) + (size_t)(n0 + idx / (BK / 2)) * ldb;
       else src = b + (size_t)(n0 + 2 * (idx % (BN / 2))) + (size_t)(k0 + idx / (BN / 2)) * ldb;
@@ -310,7 +313,7 @@ __global__ __launch_bounds__(256) void k_gemm_big(int M, int N, int K, double al
   auto lstore = [&](int buf) {
 #pragma unroll
     for (int r = 0; r < NA2; ++r) {
-      const int idx = tid + 256 * r;
+      const int idx = tid + GB_NT * r;
       if (!TA) {
         const int mm = 2 * (idx % (BM / 2)), kk = idx / (BM / 2);
         As[buf][kk][mm] = ra[r].x;
@@ -323,7 +326,7 @@ __global__ __launch_bounds__(256) void k_gemm_big(int M, int N, int K, double al
     }
 #pragma unroll
     for (int r = 0; r < NB2; ++r) {
-      const int idx = tid + 256 * r;
+      const int idx = tid + GB_NT * r;
       if (!TBT) {
         const int kk = 2 * (idx % (BK / 2)), nn = idx / (BK / 2);
         Bs[buf][kk][nn] = rb[r].x;
@@ -381,9 +384,10 @@ __global__ __launch_bounds__(256) void k_gemm_big(int M, int N, int K, double al
 // operands taken from rows of A (column-major, n x k).  The MFMA output keeps
 // lanes along columns, which are contiguous in the packed rows.  Requires
 // n % 128 == 0, k % 16 == 0, k > 0, 16-byte aligned a and even lda.
-__global__ __launch_bounds__(256) void k_syrk_big(int n, int k, double occ, const double* __restrict__ a,
-                                                  int lda, double* __restrict__ p) {
-  constexpr int BM = 128, S = BM + 16, WM = 64, F = 4, NL = BM * BK / 2 / 256;
+__global__ __launch_bounds__(GB_NT) void k_syrk_big(int n, int k, double occ, const double* __restrict__ a,
+                                                    int lda, double* __restrict__ p) {
+  // eight waves on a 4 x 2 grid of 32 x 64 sub-tiles, as in k_gemm_big
+  constexpr int BM = 128, S = BM + 16, WM = 32, WN = 64, FM = 2, FN = 4, NL = BM * BK / 2 / GB_NT;
   __shared__ double As[2][BK][S];
   __shared__ double Bs[2][BK][S];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -396,12 +400,12 @@ __global__ __launch_bounds__(256) void k_syrk_big(int n, int k, double occ, cons
   while (tm * (tm + 1) / 2 > t) --tm;
   const int tn = t - tm * (tm + 1) / 2;
   const int m0 = tm * BM, n0 = tn * BM;
-  const int wm = (w >> 1) * WM, wn = (w & 1) * WM;
+  const int wm = (w >> 1) * WM, wn = (w & 1) * WN;
   double2 ra[NL], rb[NL];
   auto gload = [&](int k0) {
 #pragma unroll
     for (int r = 0; r < NL; ++r) {
-      const int idx = tid + 256 * r;
+      const int idx = tid + GB_NT * r;
       const int mm = 2 * (idx % (BM / 2)), kk = idx / (BM / 2);
       ra[r] = *reinterpret_cast<const double2*>(a + (size_t)(m0 + mm) + (size_t)(k0 + kk) * lda);
       rb[r] = *reinterpret_cast<const double2*>(a + (size_t)(n0 + mm) + (size_t)(k0 + kk) * lda);
@@ -410,7 +414,7 @@ __global__ __launch_bounds__(256) void k_syrk_big(int n, int k, double occ, cons
   auto lstore = [&](int buf) {
 #pragma unroll
     for (int r = 0; r < NL; ++r) {
-      const int idx = tid + 256 * r;
+      const int idx = tid + GB_NT * r;
       const int mm = 2 * (idx % (BM / 2)), kk = idx / (BM / 2);
       As[buf][kk][mm] = ra[r].x;
       As[buf][kk][mm + 1] = ra[r].y;
@@ -418,11 +422,11 @@ __global__ __launch_bounds__(256) void k_syrk_big(int n, int k, double occ, cons
       Bs[buf][kk][mm + 1] = rb[r].y;
     }
   };
-  d4_t acc[F][F];
+  d4_t acc[FM][FN];
 #pragma unroll
-  for (int i = 0; i < F; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < F; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < FN; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
   gload(0);
   lstore(0);
   __syncthreads();
@@ -432,24 +436,24 @@ __global__ __launch_bounds__(256) void k_syrk_big(int n, int k, double occ, cons
     if (kt + 1 < nk) gload((kt + 1) * BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
-      double af[F], bf[F];
+      double af[FM], bf[FN];
 #pragma unroll
-      for (int i = 0; i < F; ++i) af[i] = As[cur][ks + lk][wm + 16 * i + li];
+      for (int i = 0; i < FM; ++i) af[i] = As[cur][ks + lk][wm + 16 * i + li];
 #pragma unroll
-      for (int j = 0; j < F; ++j) bf[j] = Bs[cur][ks + lk][wn + 16 * j + li];
+      for (int j = 0; j < FN; ++j) bf[j] = Bs[cur][ks + lk][wn + 16 * j + li];
 #pragma unroll
-      for (int i = 0; i < F; ++i)
+      for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < F; ++j) acc[i][j] = mfma16x16x4(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < FN; ++j) acc[i][j] = mfma16x16x4(af[i], bf[j], acc[i][j]);
     }
     if (kt + 1 < nk) lstore(cur ^ 1);
     __syncthreads();
   }
   // lane holds (m0 + wm + 16 i + lk + 4 t, n0 + wn + 16 j + li) in acc[i][j][t]
 #pragma unroll
-  for (int i = 0; i < F; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < F; ++j)
+    for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const long long gm = m0 + wm + 16 * i + lk + 4 * q, gn = n0 + wn + 16 * j + li;
@@ -501,10 +505,10 @@ static void gemm_big_dispatch(hipStream_t s, bool TA, bool TBT, int m, int n, in
                               const double* a, int lda, const double* b, int ldb, double beta, double* c,
                               int ldc) {
   const dim3 grid((m / BM) * (n / BN));
-  if (!TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, false>), grid, dim3(256), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
-  else if (TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, true, false>), grid, dim3(256), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
-  else if (!TA && TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, true>), grid, dim3(256), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
-  else hipLaunchKernelGGL((k_gemm_big<BM, BN, true, true>), grid, dim3(256), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  if (!TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, false>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  else if (TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, true, false>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  else if (!TA && TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, true>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  else hipLaunchKernelGGL((k_gemm_big<BM, BN, true, true>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
   MHIP_LAUNCH_CHECK();
 }
 
@@ -571,7 +575,7 @@ static void syrk_launch(hipStream_t s, int n, int k, double occ, const double* a
   if (g_kernel_variant == 1 && n % 128 == 0 && k >= BK && k % BK == 0 && !(lda & 1) &&
       !(reinterpret_cast<uintptr_t>(a) & 15)) {
     const int nb = n / 128;
-    hipLaunchKernelGGL(k_syrk_big, dim3(nb * (nb + 1) / 2), dim3(256), 0, s, n, k, occ, a, lda, p);
+    hipLaunchKernelGGL(k_syrk_big, dim3(nb * (nb + 1) / 2), dim3(GB_NT), 0, s, n, k, occ, a, lda, p);
     MHIP_LAUNCH_CHECK();
     return;
   }

@@ -237,7 +237,11 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_jac_solve_x(
   const int I = pairs[2 * pidx], J = pairs[2 * pidx + 1];
   const bool do_intra = intra[pidx] != 0;
   const int tid = threadIdx.x;
-  const int k1 = tid >> 5, k2 = tid & 31;
+  // blocks by diagonal: thread t owns (k1, k2) = (t % 32, (t % 32 + d) % 32)
+  // with d = (t / 32 + 1) % 32, so the 32 blocks (k, k + 1) whose owners
+  // compute the next round's angles are threads 0..31: one wave runs that
+  // divergent code instead of every wave running it for two lanes
+  const int k1 = tid & 31, k2 = (k1 + (((tid >> 5) + 1) & 31)) & 31;
   {
     // all four loads of this thread in flight before the first LDS store
     constexpr int PER = SB * SB / SOLVE_THREADS;
@@ -321,7 +325,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_jac_solve_x(
       const double2 r1 = rot[b][k1], r2 = rot[b][k2];
       const int q1 = 32 + ((k1 + r) & 31), q2 = 32 + ((k2 + r) & 31);
       const double npq = round_update(S, Q, k1, k1, q1, r1.x, r1.y, k2, q2, r2.x, r2.y);
-      if (r < 31 && k2 == ((k1 + 1) & 31)) {
+      if (r < 31 && tid < 32) {  // thread (k1, k1 + 1)
         // rotation k1 of round r + 1 pairs k1 with q2
         const double dp = dgl[b][k1], dq = dgl[b][q2];
         double c, sn, t;

@@ -66,6 +66,7 @@ inline int cdiv_i(int a, int b) { return (a + b - 1) / b; }
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
 constexpr int PL_BM = 128, PL_BN = 64, PL_BK = 32, PL_KCHUNK = 64;
+constexpr int PL_NT = 512;  // eight waves per product tile
 constexpr int PL_LDK = PL_BK + 8;  // shorts per LDS row (80 bytes)
 
 template <bool KFAST>
@@ -96,7 +97,7 @@ __global__ __launch_bounds__(256) void k_split_planes(int R, int K, int rp, int 
   }
 }
 
-__global__ __launch_bounds__(256) void k_gemm_split_planes(int M, int N, int kp, int mp, int np,
+__global__ __launch_bounds__(PL_NT) void k_gemm_split_planes(int M, int N, int kp, int mp, int np,
                                                            const unsigned short* __restrict__ ap,
                                                            const unsigned short* __restrict__ bp, double alpha,
                                                            double beta, double* __restrict__ c, int ldc) {
@@ -105,22 +106,24 @@ __global__ __launch_bounds__(256) void k_gemm_split_planes(int M, int N, int kp,
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
   const int m0 = blockIdx.y * PL_BM, n0 = blockIdx.x * PL_BN;
-  const int wm = (w >> 1) * 64, wn = (w & 1) * 32;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;  // eight waves, 32 x 32 each
   const size_t aplane = (size_t)mp * kp, bplane = (size_t)np * kp;
-  constexpr int NA = 3 * PL_BM * PL_BK / 8 / 256;  // 16-byte loads per thread: 6
-  constexpr int NB = 3 * PL_BN * PL_BK / 8 / 256;  // 3
+  constexpr int NA = 3 * PL_BM * PL_BK / 8 / PL_NT;  // 16-byte loads per thread: 3
+  constexpr int NB_ALL = 3 * PL_BN * PL_BK / 8;     // 768 over the workgroup
+  constexpr int NB = (NB_ALL + PL_NT - 1) / PL_NT;  // 2, the second for half the threads
   uint4 ra[NA], rb[NB];
   auto gload = [&](int k0) {
 #pragma unroll
     for (int r = 0; r < NA; ++r) {
-      const int idx = tid + 256 * r;
+      const int idx = tid + PL_NT * r;
       const int pl = idx / (PL_BM * 4), rem = idx % (PL_BM * 4);
       const int row = rem / 4, kc = (rem % 4) * 8;
       ra[r] = *reinterpret_cast<const uint4*>(ap + pl * aplane + (size_t)(m0 + row) * kp + k0 + kc);
     }
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
-      const int idx = tid + 256 * r;
+      const int idx = tid + PL_NT * r;
+      if (idx >= NB_ALL) continue;
       const int pl = idx / (PL_BN * 4), rem = idx % (PL_BN * 4);
       const int row = rem / 4, kc = (rem % 4) * 8;
       rb[r] = *reinterpret_cast<const uint4*>(bp + pl * bplane + (size_t)(n0 + row) * kp + k0 + kc);
@@ -129,23 +132,24 @@ __global__ __launch_bounds__(256) void k_gemm_split_planes(int M, int N, int kp,
   auto lstore = [&](int buf) {
 #pragma unroll
     for (int r = 0; r < NA; ++r) {
-      const int idx = tid + 256 * r;
+      const int idx = tid + PL_NT * r;
       const int pl = idx / (PL_BM * 4), rem = idx % (PL_BM * 4);
       *reinterpret_cast<uint4*>(&As[buf][pl][rem / 4][(rem % 4) * 8]) = ra[r];
     }
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
-      const int idx = tid + 256 * r;
+      const int idx = tid + PL_NT * r;
+      if (idx >= NB_ALL) continue;
       const int pl = idx / (PL_BN * 4), rem = idx % (PL_BN * 4);
       *reinterpret_cast<uint4*>(&Bs[buf][pl][rem / 4][(rem % 4) * 8]) = rb[r];
     }
   };
-  f4_t big[2][4], corr[2][4];
-  double tot[2][4][4];
+  f4_t big[2][2], corr[2][2];
+  double tot[2][2][4];
 #pragma unroll
   for (int j = 0; j < 2; ++j)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 2; ++i) {
       big[j][i] = f4_t{0.f, 0.f, 0.f, 0.f};
       corr[j][i] = f4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -158,18 +162,18 @@ __global__ __launch_bounds__(256) void k_gemm_split_planes(int M, int N, int kp,
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) gload((kt + 1) * PL_BK);
-    bf8_t af[4][3], bf[2][3];
+    bf8_t af[2][3], bf[2][3];
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i][p] = *reinterpret_cast<const bf8_t*>(&As[cur][p][wm + 16 * i + li][8 * lg]);
+      for (int i = 0; i < 2; ++i) af[i][p] = *reinterpret_cast<const bf8_t*>(&As[cur][p][wm + 16 * i + li][8 * lg]);
 #pragma unroll
       for (int j = 0; j < 2; ++j) bf[j][p] = *reinterpret_cast<const bf8_t*>(&Bs[cur][p][wn + 16 * j + li][8 * lg]);
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < 2; ++i) {
         big[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[j][0], af[i][0], big[j][i], 0, 0, 0);
         corr[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[j][0], af[i][1], corr[j][i], 0, 0, 0);
         corr[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[j][1], af[i][0], corr[j][i], 0, 0, 0);
@@ -181,7 +185,7 @@ __global__ __launch_bounds__(256) void k_gemm_split_planes(int M, int N, int kp,
 #pragma unroll
       for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < 2; ++i) {
 #pragma unroll
           for (int t = 0; t < 4; ++t) tot[j][i][t] += (double)big[j][i][t] + (double)corr[j][i][t];
           big[j][i] = f4_t{0.f, 0.f, 0.f, 0.f};
@@ -195,7 +199,7 @@ __global__ __launch_bounds__(256) void k_gemm_split_planes(int M, int N, int kp,
 #pragma unroll
   for (int j = 0; j < 2; ++j)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int gm = m0 + wm + 16 * i + li, gn = n0 + wn + 16 * j + 4 * lg + t;
@@ -268,7 +272,7 @@ void gemm_split_bf16(hipStream_t s, char ta, char tb, int m, int n, int k, doubl
     if (!TBT) hipLaunchKernelGGL(k_split_planes<true>, gb, dim3(256), 0, s, n, k, np, kp, b, ldb, bp);
     else hipLaunchKernelGGL(k_split_planes<false>, gb, dim3(256), 0, s, n, k, np, kp, b, ldb, bp);
     MHIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gemm_split_planes, dim3(np / PL_BN, mp / PL_BM), dim3(256), 0, s, m, n, kp, mp, np, ap,
+    hipLaunchKernelGGL(k_gemm_split_planes, dim3(np / PL_BN, mp / PL_BM), dim3(PL_NT), 0, s, m, n, kp, mp, np, ap,
                        bp, alpha, beta, c, ldc);
     MHIP_LAUNCH_CHECK();
   });
