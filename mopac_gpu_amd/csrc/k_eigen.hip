@@ -501,24 +501,25 @@ __global__ __launch_bounds__(256) void k_jac_apply_c(double* __restrict__ C, int
 }
 
 
-// This thread's 16 elements (e = tid + 256 it) of a 64x64 rotation, all
+// This thread's 8 elements (e = tid + 512 it) of a 64x64 rotation, all
 // loads issued together; Q = I for a pair marked inactive.
-__device__ __forceinline__ void stage_q(double (&v)[SB * SB / 256], const double* __restrict__ q, bool active,
+constexpr int UPD_NT = 512;  // eight waves per update workgroup
+__device__ __forceinline__ void stage_q(double (&v)[SB * SB / UPD_NT], const double* __restrict__ q, bool active,
                                         int tid) {
   if (active) {
 #pragma unroll
-    for (int it = 0; it < SB * SB / 256; ++it) v[it] = q[tid + 256 * it];
+    for (int it = 0; it < SB * SB / UPD_NT; ++it) v[it] = q[tid + UPD_NT * it];
   } else {
 #pragma unroll
-    for (int it = 0; it < SB * SB / 256; ++it) {
-      const int e = tid + 256 * it;
+    for (int it = 0; it < SB * SB / UPD_NT; ++it) {
+      const int e = tid + UPD_NT * it;
       v[it] = (e % SB == e / SB) ? 1.0 : 0.0;
     }
   }
 }
 
 // MFMA versions of the two update kernels (same data flow as above).
-__global__ __launch_bounds__(256) void k_jac_apply_a_mfma(double* __restrict__ A, int lda,
+__global__ __launch_bounds__(UPD_NT) void k_jac_apply_a_mfma(double* __restrict__ A, int lda,
                                                           int npairs, int npo,
                                                           const int* __restrict__ pairs,
                                                           const double* __restrict__ Qbuf,
@@ -539,61 +540,61 @@ __global__ __launch_bounds__(256) void k_jac_apply_a_mfma(double* __restrict__ A
   // every global load of a stage is issued before the first LDS store, so a
   // thread waits for memory once per stage, not once per element; Q2 is
   // fetched into registers now and lands in LDS after the first product
-  constexpr int PER = SB * SB / 256;  // element e = tid + 256 it: row tid % 64, column tid / 64 + 4 it
+  constexpr int PER = SB * SB / UPD_NT;  // element e = tid + 512 it: row tid % 64, column tid / 64 + 8 it
   double va[PER], vq[PER], vq2[PER];
 #pragma unroll
   for (int it = 0; it < PER; ++it) {
-    const int e = tid + 256 * it;
+    const int e = tid + UPD_NT * it;
     va[it] = A[(size_t)gidx(I1, J1, e % SB) + (size_t)gidx(I2, J2, e / SB) * lda];
   }
   stage_q(vq, q1, a1, tid);
   if (P1 != P2) stage_q(vq2, q2, a2, tid);
 #pragma unroll
   for (int it = 0; it < PER; ++it) {
-    const int e = tid + 256 * it;
+    const int e = tid + UPD_NT * it;
     At[e % SB][e / SB] = va[it];
     Qs[e % SB][e / SB] = vq[it];
   }
   __syncthreads();
-  d4_t acc[4];
-  for (int i = 0; i < 4; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
-  mfma_tile64([&](int r, int k) { return Qs[k][r]; }, [&](int k, int c) { return At[k][c]; }, SB, acc);
+  d4_t acc[2];
+  for (int i = 0; i < 2; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
+  mfma_tile64_w8([&](int r, int k) { return Qs[k][r]; }, [&](int k, int c) { return At[k][c]; }, SB, acc);
   __syncthreads();
-  mfma_tile64_store(acc, [&](int r, int c, double v) { At[r][c] = v; });
+  mfma_tile64_w8_store(acc, [&](int r, int c, double v) { At[r][c] = v; });
   if (P1 != P2) {
 #pragma unroll
     for (int it = 0; it < PER; ++it) {
-      const int e = tid + 256 * it;
+      const int e = tid + UPD_NT * it;
       Qs[e % SB][e / SB] = vq2[it];
     }
   }
   __syncthreads();
-  for (int i = 0; i < 4; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
-  mfma_tile64([&](int r, int k) { return At[r][k]; }, [&](int k, int c) { return Qs[k][c]; }, SB, acc);
+  for (int i = 0; i < 2; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
+  mfma_tile64_w8([&](int r, int k) { return At[r][k]; }, [&](int k, int c) { return Qs[k][c]; }, SB, acc);
   // stage the result in LDS and write both the tile and its mirror with
   // consecutive lanes on consecutive rows of A
   __syncthreads();
-  mfma_tile64_store(acc, [&](int r, int c, double v) { At[r][c] = v; });
+  mfma_tile64_w8_store(acc, [&](int r, int c, double v) { At[r][c] = v; });
   __syncthreads();
   if (P1 == P2) {
-    for (int e = tid; e < SB * SB; e += 256) {
+    for (int e = tid; e < SB * SB; e += UPD_NT) {
       const int r = e % SB, c = e / SB;
       const double v = (r == c) ? At[r][c] : 0.5 * (At[r][c] + At[c][r]);
       A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda] = v;
     }
     return;
   }
-  for (int e = tid; e < SB * SB; e += 256) {
+  for (int e = tid; e < SB * SB; e += UPD_NT) {
     const int r = e % SB, c = e / SB;
     A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * lda] = At[r][c];
   }
-  for (int e = tid; e < SB * SB; e += 256) {
+  for (int e = tid; e < SB * SB; e += UPD_NT) {
     const int c = e % SB, r = e / SB;
     A[(size_t)gidx(I2, J2, c) + (size_t)gidx(I1, J1, r) * lda] = At[r][c];
   }
 }
 
-__global__ __launch_bounds__(256) void k_jac_apply_c_mfma(double* __restrict__ C, int ldc, int nrows,
+__global__ __launch_bounds__(UPD_NT) void k_jac_apply_c_mfma(double* __restrict__ C, int ldc, int nrows,
                                                           const int* __restrict__ pairs,
                                                           const double* __restrict__ Qbuf,
                                                           const int* __restrict__ active) {
@@ -605,32 +606,32 @@ __global__ __launch_bounds__(256) void k_jac_apply_c_mfma(double* __restrict__ C
   const int I = pairs[2 * P], J = pairs[2 * P + 1];
   const double* q = Qbuf + (size_t)P * SB * SB;
   const int tid = threadIdx.x;
-  constexpr int PER = SB * SB / 256;
+  constexpr int PER = SB * SB / UPD_NT;
   double vc[PER], vq[PER];
   {
     const int gr = row0 + tid % SB;  // the same row for all of this thread's elements
 #pragma unroll
     for (int it = 0; it < PER; ++it) {
-      const int e = tid + 256 * it;
+      const int e = tid + UPD_NT * it;
       vc[it] = (gr < nrows) ? C[(size_t)gr + (size_t)gidx(I, J, e / SB) * ldc] : 0.0;
     }
   }
   stage_q(vq, q, true, tid);
 #pragma unroll
   for (int it = 0; it < PER; ++it) {
-    const int e = tid + 256 * it;
+    const int e = tid + UPD_NT * it;
     Ct[e % SB][e / SB] = vc[it];
     Qs[e % SB][e / SB] = vq[it];
   }
   __syncthreads();
-  d4_t acc[4];
-  for (int i = 0; i < 4; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
-  mfma_tile64([&](int r, int k) { return Ct[r][k]; }, [&](int k, int c) { return Qs[k][c]; }, SB, acc);
+  d4_t acc[2];
+  for (int i = 0; i < 2; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
+  mfma_tile64_w8([&](int r, int k) { return Ct[r][k]; }, [&](int k, int c) { return Qs[k][c]; }, SB, acc);
   // stage in LDS, then store with consecutive lanes on consecutive rows of C
   __syncthreads();
-  mfma_tile64_store(acc, [&](int r, int c, double v) { Ct[r][c] = v; });
+  mfma_tile64_w8_store(acc, [&](int r, int c, double v) { Ct[r][c] = v; });
   __syncthreads();
-  for (int e = tid; e < SB * SB; e += 256) {
+  for (int e = tid; e < SB * SB; e += UPD_NT) {
     const int r = e % SB, c = e / SB;
     const int gr = row0 + r;
     if (gr < nrows) C[(size_t)gr + (size_t)gidx(I, J, c) * ldc] = Ct[r][c];
@@ -703,17 +704,26 @@ __global__ void k_count_active(int npairs, int npo, const int* __restrict__ acti
   }
 }
 
-// Stable ascending rank sort of the diagonal; perm[rank] = index.
-__global__ __launch_bounds__(1024) void k_sort_diag(int npad, const double* __restrict__ A, int lda,
-                                                    int* __restrict__ perm,
-                                                    double* __restrict__ eig) {
-  for (int i = threadIdx.x; i < npad; i += 1024) {
-    const double di = A[(size_t)i + (size_t)i * lda];
-    int rank = 0;
-    for (int j = 0; j < npad; ++j) {
-      const double dj = A[(size_t)j + (size_t)j * lda];
-      rank += (dj < di) || (dj == di && j < i);
+// Stable ascending rank sort of the diagonal; perm[rank] = index.  One
+// thread per entry across the grid; the diagonal goes through LDS in chunks.
+constexpr int SORT_CHUNK = 2048;
+__global__ __launch_bounds__(256) void k_sort_diag(int npad, const double* __restrict__ A, int lda,
+                                                   int* __restrict__ perm, double* __restrict__ eig) {
+  __shared__ double d[SORT_CHUNK];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const double di = (i < npad) ? A[(size_t)i + (size_t)i * lda] : 0.0;
+  int rank = 0;
+  for (int j0 = 0; j0 < npad; j0 += SORT_CHUNK) {
+    const int nj = min(SORT_CHUNK, npad - j0);
+    for (int j = threadIdx.x; j < nj; j += 256) d[j] = A[(size_t)(j0 + j) + (size_t)(j0 + j) * lda];
+    __syncthreads();
+    for (int j = 0; j < nj; ++j) {
+      const double dj = d[j];
+      rank += (dj < di) || (dj == di && j0 + j < i);
     }
+    __syncthreads();
+  }
+  if (i < npad) {
     perm[rank] = i;
     eig[rank] = di;
   }
@@ -905,7 +915,7 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
       }
       if (g_kernel_variant == 1) {
         if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 2], s));
-        hipLaunchKernelGGL(k_jac_apply_a_mfma, dim3(ntiles), dim3(256), 0, s, A, lda, plan.npairs, plan.npo, pr, q,
+        hipLaunchKernelGGL(k_jac_apply_a_mfma, dim3(ntiles), dim3(UPD_NT), 0, s, A, lda, plan.npairs, plan.npo, pr, q,
                            act);
         MHIP_LAUNCH_CHECK();
         if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 3], s));
@@ -914,7 +924,7 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
         MHIP_CHECK(hipEventRecord(plan.ev_solved[step], s));
         MHIP_CHECK(hipStreamWaitEvent(cs, plan.ev_solved[step], 0));
         if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 4], cs));
-        hipLaunchKernelGGL(k_jac_apply_c_mfma, dim3(plan.npad / SB, plan.npairs), dim3(256), 0, cs, C, lda,
+        hipLaunchKernelGGL(k_jac_apply_c_mfma, dim3(plan.npad / SB, plan.npairs), dim3(UPD_NT), 0, cs, C, lda,
                            plan.npad, pr, q, act);
         MHIP_LAUNCH_CHECK();
         if (plan.timing) MHIP_CHECK(hipEventRecord(plan.ev[6 * step + 5], cs));
@@ -974,7 +984,7 @@ int jacobi_sweeps(hipStream_t s, JacobiPlan& plan, double* A, double* C, double 
 void jacobi_sort(hipStream_t s, JacobiPlan& plan, const double* A, const double* C, double* Cout,
                  double* eig_dev) {
   const int npad = plan.npad;
-  hipLaunchKernelGGL(k_sort_diag, dim3(1), dim3(1024), 0, s, npad, A, npad, plan.perm.as<int>(),
+  hipLaunchKernelGGL(k_sort_diag, dim3((npad + 255) / 256), dim3(256), 0, s, npad, A, npad, plan.perm.as<int>(),
                      eig_dev);
   MHIP_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_permute_cols, dim3((npad + 255) / 256, npad), dim3(256), 0, s, npad, npad, C,

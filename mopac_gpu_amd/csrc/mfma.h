@@ -49,4 +49,30 @@ __device__ __forceinline__ void mfma_tile64_store(const d4_t acc[4], const F& fn
       for (int t = 0; t < 4; ++t) fn(qr + 16 * bi + rb + 4 * t, qc + 16 * bj + col, acc[bi * 2 + bj][t]);
 }
 
+// The same 64x64 tile with eight waves: wave w owns the 32x16 strip at rows
+// 32*(w/4), cols 16*(w%4) as two MFMA blocks stacked vertically; acc[bi].
+template <class FA, class FB>
+__device__ __forceinline__ void mfma_tile64_w8(const FA& A, const FB& B, int K, d4_t acc[2]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int qr = 32 * (w >> 2), qc = 16 * (w & 3);
+  const int li = lane & 15, lk = lane >> 4;
+  for (int k = 0; k < K; k += 4) {
+    const double a0 = A(qr + li, k + lk), a1 = A(qr + 16 + li, k + lk);
+    const double b0 = B(k + lk, qc + li);
+    acc[0] = mfma16x16x4(a0, b0, acc[0]);
+    acc[1] = mfma16x16x4(a1, b0, acc[1]);
+  }
+}
+
+template <class F>
+__device__ __forceinline__ void mfma_tile64_w8_store(const d4_t acc[2], const F& fn) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int qr = 32 * (w >> 2), qc = 16 * (w & 3);
+  const int col = lane & 15, rb = lane >> 4;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) fn(qr + 16 * bi + rb + 4 * t, qc + col, acc[bi][t]);
+}
+
 }  // namespace mhip
