@@ -53,13 +53,39 @@ def test_density_for_mozyme_matches_reference(gpu_ctx, helix_lmos):
         assert np.max(np.abs(pg - pr)) < 1e-13 * max(1.0, np.max(np.abs(pr)))
 
 
+@pytest.fixture(scope="module")
+def lewis_helix_lmos():
+    """Lewis-structure LMOs of a 12-residue helix (87 atoms, 6 A stored pairs),
+    F and P of the diagonal starting density from the oracle: a longer rotation
+    list, with many rotations that share no LMO (the level-parallel diagg2),
+    and little room per LMO, so growth is rejected and restored."""
+    nat, xyz = geometry.glycine_helix(12, 3)
+    s = hamiltonian.build(nat, xyz)
+    ms = mozyme.MozymeSystem(s, r_near=6.0, r_dipole=7.0)
+    p0 = ms.diagonal_density()
+    f0 = oracle.mozyme_fock2z(s, ms.nijbo, ms.mpack, ms.h, ms.am, ms.ad, ms.dd, p0, feather=True)
+    L = mozyme.lewis_lmos(ms, f0, pad_atoms=2)
+    L.fao, L.p = f0, p0
+    fd = mozyme.dense_from_sparse(f0, ms.nfirst, ms.nlast - ms.nfirst + 1, ms.nijbo, ms.norbs)
+    L.eigs = np.array([mozyme._energy(L, k, True, fd) for k in range(L.nocc)])
+    L.eigv = np.array([mozyme._energy(L, k, False, fd) for k in range(L.nvir)])
+    return L
+
+
+def test_diagg_sequence_on_lewis_lmos_matches_reference(gpu_ctx, lewis_helix_lmos):
+    _diagg_sequence(gpu_ctx, lewis_helix_lmos, None, 8)
+
+
 @pytest.mark.parametrize("fmo_cap", [None, 60])
 def test_diagg1_diagg2_sequence_matches_reference(gpu_ctx, helix_lmos, fmo_cap):
-    L = helix_lmos
+    _diagg_sequence(gpu_ctx, helix_lmos, fmo_cap, 10)
+
+
+def _diagg_sequence(gpu_ctx, L, fmo_cap, ncalls):
     R = REF.Mozyme(L, fmo_dim=fmo_cap)
     G = mozyme.GpuMozyme(gpu_ctx, L, fmo_dim=fmo_cap)
-    for idiagg in range(1, 11):
-        print(f"idiagg {idiagg} cap {fmo_cap}", flush=True)
+    for idiagg in range(1, ncalls + 1):
+        print(f"idiagg {idiagg} cap {fmo_cap} ({L.numat} atoms)", flush=True)
         r = R.diagg1(L.fao, idiagg)
         g = G.diagg1(L.fao, L.p, idiagg)
         assert g["nij"] == r["nij"]
