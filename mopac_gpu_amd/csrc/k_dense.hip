@@ -291,6 +291,11 @@ __global__ __launch_bounds__(GB_NT) void k_gemm_big(int M, int N, int K, double 
   xcd_tile(blockIdx.x, M / BM, N / BN, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
   const int wm = (w >> 1) * WM, wn = (w & 1) * WN;  // w = 0..7
+  // operands stored transposed (A when TA, B when !TBT): a wave writes 8 k
+  // rows of one column group, rows that share banks; the column index is
+  // XOR-ed with the k pair (bits 2-4), which spreads those stores over the
+  // banks and keeps every aligned 16-column block together for the reads
+  auto swz = [](int k) { return ((k >> 1) & 7) << 2; };
   double2 ra[NA2], rb[NB2];
   auto gload = [&](int k0) {
 #pragma unroll
@@ -320,8 +325,8 @@ __global__ __launch_bounds__(GB_NT) void k_gemm_big(int M, int N, int K, double 
         As[buf][kk][mm + 1] = ra[r].y;
       } else {
         const int kk = 2 * (idx % (BK / 2)), mm = idx / (BK / 2);
-        As[buf][kk][mm] = ra[r].x;
-        As[buf][kk + 1][mm] = ra[r].y;
+        As[buf][kk][mm ^ swz(kk)] = ra[r].x;
+        As[buf][kk + 1][mm ^ swz(kk + 1)] = ra[r].y;
       }
     }
 #pragma unroll
@@ -329,8 +334,8 @@ __global__ __launch_bounds__(GB_NT) void k_gemm_big(int M, int N, int K, double 
       const int idx = tid + GB_NT * r;
       if (!TBT) {
         const int kk = 2 * (idx % (BK / 2)), nn = idx / (BK / 2);
-        Bs[buf][kk][nn] = rb[r].x;
-        Bs[buf][kk + 1][nn] = rb[r].y;
+        Bs[buf][kk][nn ^ swz(kk)] = rb[r].x;
+        Bs[buf][kk + 1][nn ^ swz(kk + 1)] = rb[r].y;
       } else {
         const int nn = 2 * (idx % (BN / 2)), kk = idx / (BN / 2);
         Bs[buf][kk][nn] = rb[r].x;
@@ -354,9 +359,9 @@ __global__ __launch_bounds__(GB_NT) void k_gemm_big(int M, int N, int K, double 
     for (int ks = 0; ks < BK; ks += 4) {
       double af[FM], bf[FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i) af[i] = As[cur][ks + lk][wm + 16 * i + li];
+      for (int i = 0; i < FM; ++i) af[i] = As[cur][ks + lk][(wm + 16 * i + li) ^ (TA ? swz(ks + lk) : 0)];
 #pragma unroll
-      for (int j = 0; j < FN; ++j) bf[j] = Bs[cur][ks + lk][wn + 16 * j + li];
+      for (int j = 0; j < FN; ++j) bf[j] = Bs[cur][ks + lk][(wn + 16 * j + li) ^ (TBT ? 0 : swz(ks + lk))];
 #pragma unroll
       for (int j = 0; j < FN; ++j)
 #pragma unroll
