@@ -307,6 +307,30 @@ def run_helix(args):
         dt, t_start, last = step()
         times.append(dt)
     t_mz = float(np.mean(times))
+    # roofline of the dominant kernel, the diagg2 rotation levels: HIP events
+    # around the level launches, algorithmic bytes counted by the library
+    import ctypes
+    from mopac_gpu_amd import lib as L
+    lib = L.load()
+    sv = [ctypes.c_double(), ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()]
+    lib.mhip_mozyme_stats(1, *[ctypes.byref(v) for v in sv])
+    step()
+    lib.mhip_mozyme_stats(0, *[ctypes.byref(v) for v in sv])
+    d2_ms, d2_rot, d2_lev, d2_launch, d2_bytes = [v.value for v in sv]
+    avg_s = d2_ms * 1e-3 / max(d2_launch, 1)
+    achieved = (d2_bytes / max(d2_launch, 1)) / avg_s / 1e9 if avg_s > 0 else 0.0
+    roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None, "kernel": "k_mz_diagg2",
+            "avg_launch_us": round(avg_s * 1e6, 2), "launches": int(d2_launch),
+            "algorithmic_bytes_per_launch": round(d2_bytes / max(d2_launch, 1)),
+            "rotations_per_scf": int(d2_rot), "levels_per_scf": int(d2_lev), "ms_per_scf": round(d2_ms, 2),
+            "note": "one level per launch; each rotation is a chain of dependent gathers, so the kernel is "
+                    "latency-bound far below the HBM roof"}
+    # CPU baseline: the reference's own MOZYME routines (compiled from its
+    # sources into oracle/_ref), one core, a bounded sample of iterations
+    cpu = None
+    if not args.no_cpu:
+        cpu = helix_cpu_baseline(ms, last["niter"])
     conv = None
     if not args.no_conventional:
         s = hamiltonian.build(nat, xyz)
@@ -326,10 +350,46 @@ def run_helix(args):
                    "dipole_shell_A": 7.0},
         "iterations": last["niter"], "converged": last["converged"], "escf": round(last["escf"], 6),
         "lewis_start_s": round(t_start, 4), "sparse_setup_s": round(t_setup, 3),
-        "conventional_fp64": conv, "device": dev_name, "roofline": None, "cpu_baseline": None,
+        "conventional_fp64": conv, "device": dev_name, "roofline": roof, "cpu_baseline": cpu,
     }
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def helix_cpu_baseline(ms, niter, sample_iters=6):
+    """One core running the reference's density_for_MOZYME, fock2z (fz2n),
+    helecz, diagg1 and diagg2 (oracle/_ref) from the same Lewis start, for
+    `sample_iters` iterations; the per-iteration time is scaled to the GPU
+    SCF's iteration count and the start added."""
+    from oracle import ref as REF
+    from mopac_gpu_amd import mozyme
+    if not REF.available():
+        return None
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    t0 = time.perf_counter()
+    p0 = ms.diagonal_density()
+    f0 = REF.mozyme_fock2z(ms, p0, feather=True)
+    lm = mozyme.lewis_lmos(ms, f0)
+    t_start = time.perf_counter() - t0
+    lm.fao, lm.p = f0, p0
+    lm.eigs, lm.eigv = np.zeros(lm.nocc), np.zeros(lm.nvir)
+    R = REF.Mozyme(mozyme.reference_capacity(lm))
+    lib, P = REF.load(), REF.P
+    xyz, am, dd, ad, w, h = f64(ms.xyz), f64(ms.am), f64(ms.dd), f64(ms.ad), f64(ms.w), f64(ms.h)
+    t1 = time.perf_counter()
+    for it in range(1, sample_iters + 1):
+        p = R.density(0)
+        f = np.zeros(ms.mpack)
+        lib.mz_fock2z(P(xyz), P(am), P(dd), P(ad), len(w), P(w), P(p), P(h), REF.A0, REF.EV, 1, REF.TRUNC1,
+                      REF.TRUNC2, P(f))
+        R.helecz(p, h, f)
+        r = R.diagg1(f, it)
+        R.diagg2(r["nij"], it)
+    per_iter = (time.perf_counter() - t1) / sample_iters
+    return {"value": round(t_start + per_iter * niter, 3), "unit": "s", "cores": 1, "kind": "reference",
+            "sample": f"{sample_iters} iterations of the reference's density_for_MOZYME, fock2z, helecz, diagg1 "
+                      f"and diagg2 on the same helix from the same Lewis start ({per_iter:.3f} s each), scaled "
+                      f"to the GPU SCF's {niter} iterations, plus the start ({t_start:.2f} s)"}
 
 
 if __name__ == "__main__":

@@ -237,6 +237,13 @@ typedef struct {
 int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* lmo, const double* eigs, const double* eigv,
                        const double* fmo, const int* ifmo, int nij, int idiagg,
                        mhip_diagg2_state* st);
+/* No counterpart in the reference.  HIP-event time of the diagg2 rotation
+ * levels on the MOZYME stream, with the rotations, levels, launches and
+ * algorithmic bytes (each rotation reads and writes both LMOs' coefficients,
+ * sized on entry) since the last reset; enable = 1 / 0 resets and switches
+ * the count on / off, -1 only reads. */
+int mhip_mozyme_stats(int enable, double* ms, long long* rotations, long long* levels, long long* launches,
+                      double* bytes);
 
 /* helecz() (src/MOZYME/helecz.F90:16-52): E = sum over stored atom pairs of
  * P (H + F), diagonal elements weighted 1/2, on the sparse nijbo storage. */

@@ -901,6 +901,15 @@ struct LmoBufs {
   LmoDev dev{};
 };
 
+// Optional HIP-event timing and counts of the diagg2 levels (process-wide).
+struct MzStats {
+  bool on = false;
+  double ms = 0.0, bytes = 0.0;
+  long long rotations = 0, levels = 0, launches = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+MzStats g_mz_stats;
+
 // Device copies and scratch of the MOZYME entry points, one set per device,
 // grown as needed and kept until the process ends: no allocation per call.
 struct MzWork {
@@ -1287,6 +1296,26 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
     dsi.ensure(sizeof(double) * (size_t)nslots * span_i);
     dsumb.ensure(sizeof(double) * std::max(nij, 1));
     dnrej.ensure(sizeof(int) * std::max(nij, 1));
+    if (g_mz_stats.on) {
+      // algorithmic bytes: each rotation reads and writes the coefficients of
+      // its two LMOs (their size on entry)
+      auto coefs = [&](const int* nc, const int* nn, const int* idx, int k) {
+        long long c = 0;
+        for (int a = 0; a < nc[k]; ++a) c += L->nlast[idx[nn[k] + a] - 1] - L->nfirst[idx[nn[k] + a] - 1] + 1;
+        return c;
+      };
+      for (int ij : taken) {
+        const int i = ifmo[2 * ij] - 1, j = ifmo[2 * ij + 1] - 1;
+        g_mz_stats.bytes += 16.0 * (coefs(L->nce, L->nnce, L->icvir, i) + coefs(L->ncf, L->nncf, L->icocc, j));
+      }
+      g_mz_stats.rotations += (long long)taken.size();
+      g_mz_stats.levels += nlev;
+      if (!g_mz_stats.e0) {
+        MHIP_CHECK(hipEventCreate(&g_mz_stats.e0));
+        MHIP_CHECK(hipEventCreate(&g_mz_stats.e1));
+      }
+      MHIP_CHECK(hipEventRecord(g_mz_stats.e0, s));
+    }
     for (int l = 1; l <= nlev; ++l)
       for (int b = start[l]; b < start[l + 1]; b += nslots) {
         const int cnt = std::min(nslots, start[l + 1] - b);
@@ -1295,7 +1324,9 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
                            dused.as<int>(), (char*)dlat.ptr, dsj.as<double>(), span_j, dsi.as<double>(), span_i,
                            dsumb.as<double>(), dnrej.as<int>());
         MHIP_LAUNCH_CHECK();
+        if (g_mz_stats.on) ++g_mz_stats.launches;
       }
+    if (g_mz_stats.on) MHIP_CHECK(hipEventRecord(g_mz_stats.e1, s));
     std::vector<double> sb(std::max(nij, 1));
     std::vector<int> nr(std::max(nij, 1));
     if (!taken.empty()) {
@@ -1318,7 +1349,27 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
     st->sumb = sumb;
     st->nrejct[1] = st->nrejct[0];
     st->nrejct[0] = nrej;
+    if (g_mz_stats.on) {
+      float t = 0.f;
+      MHIP_CHECK(hipEventElapsedTime(&t, g_mz_stats.e0, g_mz_stats.e1));
+      g_mz_stats.ms += t;
+    }
   });
+}
+
+int mhip_mozyme_stats(int enable, double* ms, long long* rotations, long long* levels, long long* launches,
+                      double* bytes) {
+  if (ms) *ms = g_mz_stats.ms;
+  if (rotations) *rotations = g_mz_stats.rotations;
+  if (levels) *levels = g_mz_stats.levels;
+  if (launches) *launches = g_mz_stats.launches;
+  if (bytes) *bytes = g_mz_stats.bytes;
+  if (enable >= 0) {
+    g_mz_stats.on = enable != 0;
+    g_mz_stats.ms = g_mz_stats.bytes = 0.0;
+    g_mz_stats.rotations = g_mz_stats.levels = g_mz_stats.launches = 0;
+  }
+  return 0;
 }
 
 int mhip_mozyme_helecz(mhip_ctx* ctx, int numat, const int* nfirst, const int* nlast, const int* nijbo,

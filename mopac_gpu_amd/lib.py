@@ -114,6 +114,8 @@ SIGNATURES = {
     "mhip_destroy_resources": (None, []),
     "mhip_set_tuning": (c_int, [c_void_p, c_int, c_int, c_double]),
     "mhip_set_jacobi_schedule": (c_int, [c_int]),
+    "mhip_mozyme_stats": (c_int, [c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_ll), ctypes.POINTER(c_ll),
+                                  ctypes.POINTER(c_ll), ctypes.POINTER(c_double)]),
     "mhip_last_sweeps": (c_int, [c_void_p]),
     "mhip_mozyme_density": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "mhip_mozyme_diagg1": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
