@@ -43,6 +43,8 @@ struct mhip_ctx {
   mhip::JacobiPlan plan_split;  // occupied / virtual groups, for the exit after block steps
   bool have_c = false;
   double* hscal = nullptr;  // pinned host scalars
+  void* hstage = nullptr;   // pinned staging for result downloads (grow-only)
+  size_t hstage_bytes = 0;
   // pseudo-diagonalization workspace
   mhip::PseudoWork pseudo;
   mhip::OccVirtWork ov;

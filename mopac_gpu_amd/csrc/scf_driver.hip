@@ -11,6 +11,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -205,6 +206,38 @@ FockArgs fock_args(mhip_ctx* c, int mode, double shift) {
   a.shift = shift;
   a.init_mode = mode;
   return a;
+}
+
+// Device -> caller's (pageable) host memory: DMA into the context's pinned
+// staging buffer, then the host copy split over a few threads (a single
+// thread's copy of a 38 MB density took longer than the DMA).
+static void download(mhip_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return;
+  if (c->hstage_bytes < bytes) {
+    if (c->hstage) MHIP_CHECK(hipHostFree(c->hstage));
+    c->hstage = nullptr;
+    c->hstage_bytes = 0;
+    MHIP_CHECK(hipHostMalloc(&c->hstage, bytes));
+    c->hstage_bytes = bytes;
+  }
+  MHIP_CHECK(hipMemcpyAsync(c->hstage, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  MHIP_CHECK(hipStreamSynchronize(c->stream));
+  const char* from = static_cast<const char*>(c->hstage);
+  char* to = static_cast<char*>(dst);
+  const size_t chunk = size_t(4) << 20;
+  const int nthr = (int)std::min<size_t>(8, (bytes + chunk - 1) / chunk);
+  if (nthr <= 1) {
+    std::memcpy(to, from, bytes);
+    return;
+  }
+  const size_t part = (bytes + nthr - 1) / nthr;
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nthr; ++t) {
+    const size_t b0 = t * part, b1 = std::min(bytes, b0 + part);
+    if (b0 < b1) pool.emplace_back([=] { std::memcpy(to + b0, from + b0, b1 - b0); });
+  }
+  std::memcpy(to, from, std::min(bytes, part));
+  for (auto& th : pool) th.join();
 }
 
 // eigenvectors_LAPACK on the device copy of a packed matrix: adds i*1e-10 to
@@ -672,9 +705,9 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
     r->seconds = std::chrono::duration<double>(t1 - t0).count();
     r->jacobi_sweeps = c->total_sweeps;
   }
-  if (p_out) MHIP_CHECK(hipMemcpy(p_out, P, sizeof(double) * mp, hipMemcpyDeviceToHost));
-  if (f_out) MHIP_CHECK(hipMemcpy(f_out, c->f.ptr, sizeof(double) * mp, hipMemcpyDeviceToHost));
-  if (eig_out) MHIP_CHECK(hipMemcpy(eig_out, c->eig.ptr, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (p_out) download(c, p_out, P, sizeof(double) * mp);
+  if (f_out) download(c, f_out, c->f.ptr, sizeof(double) * mp);
+  if (eig_out) download(c, eig_out, c->eig.ptr, sizeof(double) * n);
   if (c_out)
     MHIP_CHECK(hipMemcpy2D(c_out, sizeof(double) * n, c->C.ptr, sizeof(double) * c->npad,
                            sizeof(double) * n, n, hipMemcpyDeviceToHost));
@@ -729,6 +762,7 @@ int mhip_finalize(mhip_ctx* c) {
     c->pseudo.release();
     c->ov.release();
     if (c->hscal) (void)hipHostFree(c->hscal);
+    if (c->hstage) (void)hipHostFree(c->hstage);
     mhip::release_split_scratch(c->stream);
     mhip::mz_release(c->mz);
     c->mz = nullptr;
