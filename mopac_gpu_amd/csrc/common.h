@@ -120,6 +120,8 @@ void gemm_split_bf16(hipStream_t s, char ta, char tb, int m, int n, int k, doubl
 // Frees the BF16 plane buffers gemm_split_bf16 keeps per (device, stream):
 // those of stream s, or all of them when s is null.
 void release_split_scratch(hipStream_t s);
+// Frees the split-K GEMM scratch of stream s (all streams when s is null).
+void release_splitk_scratch(hipStream_t s);
 // Packed lower triangle p (mpack) = occ * A(:,0:k) A(:,0:k)^T over the first n rows.
 void syrk_packed(hipStream_t s, int n, int k, double occ, const double* a, int lda, double* p);
 // Split-BF16 variant (gemm_split_bf16 into a dense n x n scratch, then packed).

@@ -205,6 +205,7 @@ void mhip_destroy_resources(void) {
     b->release();
   g_imp.plan.release();
   release_split_scratch(g_imp.s);
+  release_splitk_scratch(g_imp.s);
   if (g_imp.hs) (void)hipHostFree(g_imp.hs);
   (void)hipStreamDestroy(g_imp.s);
   g_imp = Implicit();

@@ -4,6 +4,8 @@
 #include <cstdint>
 
 #include "common.h"
+#include <mutex>
+#include <vector>
 #include "mfma.h"
 
 namespace mhip {
@@ -277,7 +279,8 @@ template <int BM, int BN, bool TA, bool TBT>
 __global__ __launch_bounds__(GB_NT) void k_gemm_big(int M, int N, int K, double alpha,
                                                     const double* __restrict__ a, int lda,
                                                     const double* __restrict__ b, int ldb, double beta,
-                                                    double* __restrict__ c, int ldc) {
+                                                    double* __restrict__ c, int ldc, int ksplit,
+                                                    double* __restrict__ cpart) {
   constexpr int SA = BM + 16, SBn = BN + 16;
   constexpr int WM = BM / 4, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
@@ -288,7 +291,13 @@ __global__ __launch_bounds__(GB_NT) void k_gemm_big(int M, int N, int K, double 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   int tm, tn;
-  xcd_tile(blockIdx.x, M / BM, N / BN, tm, tn);
+  // ksplit > 1: workgroup kz of a tile takes the kz-th K / ksplit slice; the
+  // first adds beta C, the others write alpha times their part to cpart
+  // (M x N per extra slice), summed into C by k_add_parts
+  const int ntile = (M / BM) * (N / BN);
+  const int kz = blockIdx.x / ntile;
+  const int kbase = kz * (K / ksplit);
+  xcd_tile(blockIdx.x - kz * ntile, M / BM, N / BN, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
   const int wm = (w >> 1) * WM, wn = (w & 1) * WN;  // w = 0..7
   // operands stored transposed (A when TA, B when !TBT): a wave writes 8 k
@@ -302,16 +311,16 @@ __global__ __launch_bounds__(GB_NT) void k_gemm_big(int M, int N, int K, double 
     for (int r = 0; r < NA2; ++r) {
       const int idx = tid + GB_NT * r;
       const double* src;
-      if (!TA) src = a + (size_t)(m0 + 2 * (idx % (BM / 2))) + (size_t)(k0 + idx / (BM / 2)) * lda;
-      else src = a + (size_t)(k0 + 2 * (idx % (BK / 2))) + (size_t)(m0 + idx / (BK / 2)) * lda;
+      if (!TA) src = a + (size_t)(m0 + 2 * (idx % (BM / 2))) + (size_t)(kbase + k0 + idx / (BM / 2)) * lda;
+      else src = a + (size_t)(kbase + k0 + 2 * (idx % (BK / 2))) + (size_t)(m0 + idx / (BK / 2)) * lda;
       ra[r] = *reinterpret_cast<const double2*>(src);
     }
 #pragma unroll
     for (int r = 0; r < NB2; ++r) {
       const int idx = tid + GB_NT * r;
       const double* src;
-      if (!TBT) src = b + (size_t)(k0 + 2 * (idx % (BK / 2))) + (size_t)(n0 + idx / (BK / 2)) * ldb;
-      else src = b + (size_t)(n0 + 2 * (idx % (BN / 2))) + (size_t)(k0 + idx / (BN / 2)) * ldb;
+      if (!TBT) src = b + (size_t)(kbase + k0 + 2 * (idx % (BK / 2))) + (size_t)(n0 + idx / (BK / 2)) * ldb;
+      else src = b + (size_t)(n0 + 2 * (idx % (BN / 2))) + (size_t)(kbase + k0 + idx / (BN / 2)) * ldb;
       rb[r] = *reinterpret_cast<const double2*>(src);
     }
   };
@@ -351,7 +360,7 @@ __global__ __launch_bounds__(GB_NT) void k_gemm_big(int M, int N, int K, double 
   gload(0);
   lstore(0);
   __syncthreads();
-  const int nk = K / BK;
+  const int nk = K / ksplit / BK;
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) gload((kt + 1) * BK);
@@ -378,8 +387,12 @@ __global__ __launch_bounds__(GB_NT) void k_gemm_big(int M, int N, int K, double 
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int gm = m0 + wm + 16 * i + li, gn = n0 + wn + 16 * j + lk + 4 * t;
-        double* dst = c + (size_t)gm + (size_t)gn * ldc;
         const double v = acc[j][i][t];
+        if (kz > 0) {
+          cpart[(size_t)(kz - 1) * M * N + (size_t)gm + (size_t)gn * M] = alpha * v;
+          continue;
+        }
+        double* dst = c + (size_t)gm + (size_t)gn * ldc;
         *dst = (beta == 0.0) ? alpha * v : fma(beta, *dst, alpha * v);
       }
 }
@@ -508,13 +521,50 @@ int g_kernel_variant = 1;  // 1: FP64 MFMA tiles, 0: vector-FMA tiles
 template <int BM, int BN>
 static void gemm_big_dispatch(hipStream_t s, bool TA, bool TBT, int m, int n, int k, double alpha,
                               const double* a, int lda, const double* b, int ldb, double beta, double* c,
-                              int ldc) {
-  const dim3 grid((m / BM) * (n / BN));
-  if (!TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, false>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
-  else if (TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, true, false>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
-  else if (!TA && TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, true>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
-  else hipLaunchKernelGGL((k_gemm_big<BM, BN, true, true>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+                              int ldc, int ks, double* cp) {
+  const dim3 grid((m / BM) * (n / BN) * ks);
+  if (!TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, false>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp);
+  else if (TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, true, false>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp);
+  else if (!TA && TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, true>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp);
+  else hipLaunchKernelGGL((k_gemm_big<BM, BN, true, true>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp);
   MHIP_LAUNCH_CHECK();
+}
+
+// C += the K-slice parts the split workgroups wrote (M x N each, in order)
+__global__ void k_add_parts(int m, int n, int nparts, const double* __restrict__ part, double* __restrict__ c,
+                            int ldc) {
+  const size_t mn = (size_t)m * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < mn; e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % m), j = (int)(e / m);
+    double v = c[i + (size_t)j * ldc];
+    for (int q = 0; q < nparts; ++q) v += part[q * mn + e];
+    c[i + (size_t)j * ldc] = v;
+  }
+}
+
+// K-slice scratch, one grow-only buffer per (device, stream): launches on one
+// stream are ordered, so consecutive products reuse it safely.
+struct SplitKScratch {
+  int device;
+  hipStream_t stream;
+  DevBuf buf;
+};
+std::mutex g_splitk_mu;
+std::vector<SplitKScratch*> g_splitk;
+
+double* splitk_scratch(hipStream_t s, size_t bytes) {
+  int dev = 0;
+  MHIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_splitk_mu);
+  for (SplitKScratch* p : g_splitk)
+    if (p->device == dev && p->stream == s) {
+      p->buf.ensure(bytes);
+      return p->buf.as<double>();
+    }
+  auto* p = new SplitKScratch{dev, s, DevBuf{}};
+  g_splitk.push_back(p);
+  p->buf.ensure(bytes);
+  return p->buf.as<double>();
 }
 
 // The large-tile kernel when the shape divides one of its tiles; the tile is
@@ -540,9 +590,29 @@ static bool gemm_big_launch(hipStream_t s, bool TA, bool TBT, int m, int n, int 
     }
   }
   if (best < 0) return false;
-  if (best == 0) gemm_big_dispatch<128, 128>(s, TA, TBT, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
-  else if (best == 1) gemm_big_dispatch<128, 64>(s, TA, TBT, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
-  else gemm_big_dispatch<64, 64>(s, TA, TBT, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  // K split (each slice keeps 256 or more of it) when it balances the CUs
+  // better: the fewest full-tile equivalents per CU, ceil(tiles ks / 256) / ks,
+  // ties to the smaller split; one-round launches of the smaller tiles always
+  // split in two (their workgroups gain most from a second one on the CU)
+  const long long tiles = (long long)(m / cands[best].bm) * (n / cands[best].bn);
+  int ks = 1;
+  double ks_cost = (double)((tiles + 255) / 256);
+  for (int q : {2, 4}) {
+    if (k % (q * BK) != 0 || k / q < 256) continue;
+    const double cost = (double)((tiles * q + 255) / 256) / q;
+    if (cost < ks_cost - 1e-9 || (q == 2 && best != 0 && tiles <= 256)) {
+      ks = q;
+      ks_cost = cost;
+    }
+  }
+  double* cp = ks > 1 ? splitk_scratch(s, sizeof(double) * (size_t)m * n * (ks - 1)) : nullptr;
+  if (best == 0) gemm_big_dispatch<128, 128>(s, TA, TBT, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp);
+  else if (best == 1) gemm_big_dispatch<128, 64>(s, TA, TBT, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp);
+  else gemm_big_dispatch<64, 64>(s, TA, TBT, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp);
+  if (ks > 1) {
+    hipLaunchKernelGGL(k_add_parts, dim3(2048), dim3(256), 0, s, m, n, ks - 1, cp, c, ldc);
+    MHIP_LAUNCH_CHECK();
+  }
   return true;
 }
 
@@ -647,6 +717,20 @@ void dots(hipStream_t s, long long len, int nvec, const double* a, long long str
 }  // namespace mhip
 
 namespace mhip {
+void release_splitk_scratch(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_splitk_mu);
+  std::vector<SplitKScratch*> keep;
+  for (SplitKScratch* p : g_splitk) {
+    if (s == nullptr || p->stream == s) {
+      p->buf.release();
+      delete p;
+    } else {
+      keep.push_back(p);
+    }
+  }
+  g_splitk.swap(keep);
+}
+
 size_t syrk_split_scratch_bytes(int n, int k) {
   (void)k;
   return sizeof(double) * (size_t)n * n + 8;

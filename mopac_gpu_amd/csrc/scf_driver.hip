@@ -764,6 +764,7 @@ int mhip_finalize(mhip_ctx* c) {
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->hstage) (void)hipHostFree(c->hstage);
     mhip::release_split_scratch(c->stream);
+    mhip::release_splitk_scratch(c->stream);
     mhip::mz_release(c->mz);
     c->mz = nullptr;
     (void)hipStreamDestroy(c->stream);
