@@ -120,6 +120,10 @@ void gemm_split_bf16(hipStream_t s, char ta, char tb, int m, int n, int k, doubl
 // Frees the BF16 plane buffers gemm_split_bf16 keeps per (device, stream):
 // those of stream s, or all of them when s is null.
 void release_split_scratch(hipStream_t s);
+// Grow-only scratch for the K-slice parts of a split GEMM on stream s, and
+// c(m x n, ldc) += the nparts parts (m x n each, in order) it holds.
+double* splitk_scratch(hipStream_t s, size_t bytes);
+void add_parts(hipStream_t s, int m, int n, int nparts, const double* part, double* c, int ldc);
 // Frees the split-K GEMM scratch of stream s (all streams when s is null).
 void release_splitk_scratch(hipStream_t s);
 // Packed lower triangle p (mpack) = occ * A(:,0:k) A(:,0:k)^T over the first n rows.

@@ -542,6 +542,11 @@ __global__ void k_add_parts(int m, int n, int nparts, const double* __restrict__
   }
 }
 
+void add_parts(hipStream_t s, int m, int n, int nparts, const double* part, double* c, int ldc) {
+  hipLaunchKernelGGL(k_add_parts, dim3(2048), dim3(256), 0, s, m, n, nparts, part, c, ldc);
+  MHIP_LAUNCH_CHECK();
+}
+
 // K-slice scratch, one grow-only buffer per (device, stream): launches on one
 // stream are ordered, so consecutive products reuse it safely.
 struct SplitKScratch {
@@ -609,10 +614,7 @@ static bool gemm_big_launch(hipStream_t s, bool TA, bool TBT, int m, int n, int 
   if (best == 0) gemm_big_dispatch<128, 128>(s, TA, TBT, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp);
   else if (best == 1) gemm_big_dispatch<128, 64>(s, TA, TBT, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp);
   else gemm_big_dispatch<64, 64>(s, TA, TBT, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp);
-  if (ks > 1) {
-    hipLaunchKernelGGL(k_add_parts, dim3(2048), dim3(256), 0, s, m, n, ks - 1, cp, c, ldc);
-    MHIP_LAUNCH_CHECK();
-  }
+  if (ks > 1) add_parts(s, m, n, ks - 1, cp, c, ldc);
   return true;
 }
 

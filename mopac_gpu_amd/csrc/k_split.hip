@@ -100,12 +100,16 @@ __global__ __launch_bounds__(256) void k_split_planes(int R, int K, int rp, int 
 __global__ __launch_bounds__(PL_NT) void k_gemm_split_planes(int M, int N, int kp, int mp, int np,
                                                            const unsigned short* __restrict__ ap,
                                                            const unsigned short* __restrict__ bp, double alpha,
-                                                           double beta, double* __restrict__ c, int ldc) {
+                                                           double beta, double* __restrict__ c, int ldc,
+                                                           int ks, double* __restrict__ cpart) {
   __shared__ __attribute__((aligned(16))) unsigned short As[2][3][PL_BM][PL_LDK];
   __shared__ __attribute__((aligned(16))) unsigned short Bs[2][3][PL_BN][PL_LDK];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
   const int m0 = blockIdx.y * PL_BM, n0 = blockIdx.x * PL_BN;
+  // K slice blockIdx.z of ks (whole 64-chunks): the first adds beta C, the
+  // others write alpha times their part to cpart (M x N each)
+  const int kz = blockIdx.z, kslice = kp / ks, kbase = kz * kslice;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;  // eight waves, 32 x 32 each
   const size_t aplane = (size_t)mp * kp, bplane = (size_t)np * kp;
   constexpr int NA = 3 * PL_BM * PL_BK / 8 / PL_NT;  // 16-byte loads per thread: 3
@@ -118,7 +122,7 @@ __global__ __launch_bounds__(PL_NT) void k_gemm_split_planes(int M, int N, int k
       const int idx = tid + PL_NT * r;
       const int pl = idx / (PL_BM * 4), rem = idx % (PL_BM * 4);
       const int row = rem / 4, kc = (rem % 4) * 8;
-      ra[r] = *reinterpret_cast<const uint4*>(ap + pl * aplane + (size_t)(m0 + row) * kp + k0 + kc);
+      ra[r] = *reinterpret_cast<const uint4*>(ap + pl * aplane + (size_t)(m0 + row) * kp + kbase + k0 + kc);
     }
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
@@ -126,7 +130,7 @@ __global__ __launch_bounds__(PL_NT) void k_gemm_split_planes(int M, int N, int k
       if (idx >= NB_ALL) continue;
       const int pl = idx / (PL_BN * 4), rem = idx % (PL_BN * 4);
       const int row = rem / 4, kc = (rem % 4) * 8;
-      rb[r] = *reinterpret_cast<const uint4*>(bp + pl * bplane + (size_t)(n0 + row) * kp + k0 + kc);
+      rb[r] = *reinterpret_cast<const uint4*>(bp + pl * bplane + (size_t)(n0 + row) * kp + kbase + k0 + kc);
     }
   };
   auto lstore = [&](int buf) {
@@ -158,7 +162,7 @@ __global__ __launch_bounds__(PL_NT) void k_gemm_split_planes(int M, int N, int k
   gload(0);
   lstore(0);
   __syncthreads();
-  const int nk = kp / PL_BK;
+  const int nk = kslice / PL_BK;
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) gload((kt + 1) * PL_BK);
@@ -204,8 +208,12 @@ __global__ __launch_bounds__(PL_NT) void k_gemm_split_planes(int M, int N, int k
       for (int t = 0; t < 4; ++t) {
         const int gm = m0 + wm + 16 * i + li, gn = n0 + wn + 16 * j + 4 * lg + t;
         if (gm >= M || gn >= N) continue;
-        double* dst = c + (size_t)gm + (size_t)gn * ldc;
         const double v = tot[j][i][t];
+        if (kz > 0) {
+          cpart[(size_t)(kz - 1) * M * N + (size_t)gm + (size_t)gn * M] = alpha * v;
+          continue;
+        }
+        double* dst = c + (size_t)gm + (size_t)gn * ldc;
         *dst = (beta == 0.0) ? alpha * v : fma(beta, *dst, alpha * v);
       }
 }
@@ -272,9 +280,24 @@ void gemm_split_bf16(hipStream_t s, char ta, char tb, int m, int n, int k, doubl
     if (!TBT) hipLaunchKernelGGL(k_split_planes<true>, gb, dim3(256), 0, s, n, k, np, kp, b, ldb, bp);
     else hipLaunchKernelGGL(k_split_planes<false>, gb, dim3(256), 0, s, n, k, np, kp, b, ldb, bp);
     MHIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gemm_split_planes, dim3(np / PL_BN, mp / PL_BM), dim3(PL_NT), 0, s, m, n, kp, mp, np, ap,
-                       bp, alpha, beta, c, ldc);
+    // one workgroup per CU (LDS): split K (whole 64-chunks, 256 or more per
+    // slice) when it balances the CUs better
+    const long long wgs = (long long)(np / PL_BN) * (mp / PL_BM);
+    int ks = 1;
+    double ks_cost = (double)((wgs + 255) / 256);
+    for (int q : {2, 4}) {
+      if (kp % (q * PL_KCHUNK) != 0 || kp / q < 256) continue;
+      const double cost = (double)((wgs * q + 255) / 256) / q;
+      if (cost < ks_cost - 1e-9) {
+        ks = q;
+        ks_cost = cost;
+      }
+    }
+    double* cp = ks > 1 ? splitk_scratch(s, sizeof(double) * (size_t)m * n * (ks - 1)) : nullptr;
+    hipLaunchKernelGGL(k_gemm_split_planes, dim3(np / PL_BN, mp / PL_BM, ks), dim3(PL_NT), 0, s, m, n, kp, mp, np,
+                       ap, bp, alpha, beta, c, ldc, ks, cp);
     MHIP_LAUNCH_CHECK();
+    if (ks > 1) add_parts(s, m, n, ks - 1, cp, c, ldc);
   });
 }
 
