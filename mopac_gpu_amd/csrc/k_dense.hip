@@ -403,7 +403,8 @@ __global__ __launch_bounds__(GB_NT) void k_gemm_big(int M, int N, int K, double 
 // lanes along columns, which are contiguous in the packed rows.  Requires
 // n % 128 == 0, k % 16 == 0, k > 0, 16-byte aligned a and even lda.
 __global__ __launch_bounds__(GB_NT) void k_syrk_big(int n, int k, double occ, const double* __restrict__ a,
-                                                    int lda, double* __restrict__ p) {
+                                                    int lda, double* __restrict__ p, int ksplit,
+                                                    double* __restrict__ ppart) {
   // eight waves on a 4 x 2 grid of 32 x 64 sub-tiles, as in k_gemm_big
   constexpr int BM = 128, S = BM + 16, WM = 32, WN = 64, FM = 2, FN = 4, NL = BM * BK / 2 / GB_NT;
   __shared__ double As[2][BK][S];
@@ -411,7 +412,10 @@ __global__ __launch_bounds__(GB_NT) void k_syrk_big(int n, int k, double occ, co
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int nt = n / BM, T = nt * (nt + 1) / 2;
-  const int per = T / 8, L = blockIdx.x;
+  // ksplit > 1: workgroup kz of a tile takes the kz-th K / ksplit slice; the
+  // first writes p, the others write their packed part to ppart
+  const int kz = blockIdx.x / T, kbase = kz * (k / ksplit);
+  const int per = T / 8, L = blockIdx.x - kz * T;
   const int t = (L < per * 8) ? (L % 8) * per + L / 8 : L;
   int tm = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
   while ((tm + 1) * (tm + 2) / 2 <= t) ++tm;
@@ -425,8 +429,8 @@ __global__ __launch_bounds__(GB_NT) void k_syrk_big(int n, int k, double occ, co
     for (int r = 0; r < NL; ++r) {
       const int idx = tid + GB_NT * r;
       const int mm = 2 * (idx % (BM / 2)), kk = idx / (BM / 2);
-      ra[r] = *reinterpret_cast<const double2*>(a + (size_t)(m0 + mm) + (size_t)(k0 + kk) * lda);
-      rb[r] = *reinterpret_cast<const double2*>(a + (size_t)(n0 + mm) + (size_t)(k0 + kk) * lda);
+      ra[r] = *reinterpret_cast<const double2*>(a + (size_t)(m0 + mm) + (size_t)(kbase + k0 + kk) * lda);
+      rb[r] = *reinterpret_cast<const double2*>(a + (size_t)(n0 + mm) + (size_t)(kbase + k0 + kk) * lda);
     }
   };
   auto lstore = [&](int buf) {
@@ -448,7 +452,7 @@ __global__ __launch_bounds__(GB_NT) void k_syrk_big(int n, int k, double occ, co
   gload(0);
   lstore(0);
   __syncthreads();
-  const int nk = k / BK;
+  const int nk = k / ksplit / BK;
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) gload((kt + 1) * BK);
@@ -475,7 +479,9 @@ __global__ __launch_bounds__(GB_NT) void k_syrk_big(int n, int k, double occ, co
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const long long gm = m0 + wm + 16 * i + lk + 4 * q, gn = n0 + wn + 16 * j + li;
-        if (gn <= gm) p[gm * (gm + 1) / 2 + gn] = occ * acc[i][j][q];
+        if (gn > gm) continue;
+        if (kz > 0) ppart[(size_t)(kz - 1) * ((size_t)n * (n + 1) / 2) + gm * (gm + 1) / 2 + gn] = occ * acc[i][j][q];
+        else p[gm * (gm + 1) / 2 + gn] = occ * acc[i][j][q];
       }
 }
 
@@ -652,8 +658,23 @@ static void syrk_launch(hipStream_t s, int n, int k, double occ, const double* a
   if (g_kernel_variant == 1 && n % 128 == 0 && k >= BK && k % BK == 0 && !(lda & 1) &&
       !(reinterpret_cast<uintptr_t>(a) & 15)) {
     const int nb = n / 128;
-    hipLaunchKernelGGL(k_syrk_big, dim3(nb * (nb + 1) / 2), dim3(GB_NT), 0, s, n, k, occ, a, lda, p);
+    // K split for CU balance, as in gemm_big_launch (packed parts)
+    const long long T = (long long)nb * (nb + 1) / 2;
+    int ks = 1;
+    double ks_cost = (double)((T + 255) / 256);
+    for (int q : {2, 4}) {
+      if (k % (q * BK) != 0 || k / q < 256) continue;
+      const double cost = (double)((T * q + 255) / 256) / q;
+      if (cost < ks_cost - 1e-9) {
+        ks = q;
+        ks_cost = cost;
+      }
+    }
+    const size_t mp = (size_t)n * (n + 1) / 2;
+    double* cp = ks > 1 ? splitk_scratch(s, sizeof(double) * mp * (ks - 1)) : nullptr;
+    hipLaunchKernelGGL(k_syrk_big, dim3(T * ks), dim3(GB_NT), 0, s, n, k, occ, a, lda, p, ks, cp);
     MHIP_LAUNCH_CHECK();
+    if (ks > 1) add_parts(s, (int)mp, 1, ks - 1, cp, p, (int)mp);
     return;
   }
   const int nt = cdiv(n, TB);

@@ -50,9 +50,10 @@ def test_fock2_matches_oracle(gpu_ctx, which):
     assert np.max(np.abs(f_gpu - f_ref)) < 1e-11 * max(1.0, np.max(np.abs(f_ref)))
 
 
-@pytest.mark.parametrize("nw", [8, 64])
+@pytest.mark.parametrize("nw", [8, 64, 256])
 def test_density_and_energy_match_oracle(gpu_ctx, nw):
-    """(H2O)64 (n = 384, 256 occupied) takes the large-tile packed SYRK."""
+    """(H2O)64 (n = 384, 256 occupied) takes the large-tile packed SYRK, (H2O)256
+    (n = 1536, 1024 occupied) that SYRK with K split in four (packed parts)."""
     s = _water(nw, 5)
     gpu_ctx.upload(s)
     rng = np.random.default_rng(1)
