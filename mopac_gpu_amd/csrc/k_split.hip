@@ -55,11 +55,13 @@ inline int cdiv_i(int a, int b) { return (a + b - 1) / b; }
 // k_split_planes writes op(X) (R x K) as three BF16 planes [plane][r][k] with
 // k contiguous (row stride kp), zero-padded to rp rows and kp columns, so
 // the product kernel needs no bounds checks and reads 16 bytes per lane.
-// k_gemm_split_planes: 128 x 64 output tile per workgroup (four waves of
-// 64 x 32), k in steps of 32 double-buffered in LDS with the next step
+// k_gemm_split_planes: 128 x 64 output tile per workgroup (eight waves of
+// 32 x 32), k in steps of 32 double-buffered in LDS with the next step
 // prefetched into registers, v_mfma_f32_16x16x32_bf16 for the six products,
-// FP32 partial sums flushed to FP64 every 64 values of k.  The product is
-// formed transposed (MFMA A operand from B) so each lane's outputs run down a
+// FP32 partial sums flushed to FP64 every 64 values of k.  blockIdx.z picks
+// one of ks slices of k (whole 64-chunks); slices above the first write
+// their FP64 tile to a parts buffer, which k_add_parts sums in a fixed
+// order.  The product is formed transposed (MFMA A operand from B) so each lane's outputs run down a
 // column of C: coalesced stores.  The k order inside one MFMA is the same
 // for both operands (lane group g holds k = 8 g .. 8 g + 7 of each), which is
 // all the contraction needs.
