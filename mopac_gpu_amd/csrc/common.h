@@ -111,6 +111,12 @@ extern int g_kernel_variant;
 // C = alpha * op(A) * op(B) + beta * C, column-major FP64.
 void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double* a,
           int lda, const double* b, int ldb, double beta, double* c, int ldc);
+// gemm for an m x m result known to be symmetric (a Gram matrix A^T A, or
+// A^T B with A^T B = B^T A): only the tiles on and below the diagonal are
+// multiplied, and the upper triangle is then copied from the lower one.  The
+// input C (beta != 0) is read on the lower triangle only.
+void gemm_sym(hipStream_t s, char ta, char tb, int m, int k, double alpha, const double* a, int lda,
+              const double* b, int ldb, double beta, double* c, int ldc);
 // Same contract as gemm, computed on the BF16 matrix cores from a three-way
 // BF16 split of each FP64 operand (six cross products, FP32 accumulation in
 // 32-term chunks summed in FP64; k_split.hip).

@@ -99,9 +99,9 @@ void mhip_call_syrk(char uplo, char tra, int n, int k, double alpha, const doubl
     const int ar = ta ? k : n, ac = ta ? n : k;
     h2d2(I.a, a, ar, ac, lda, ar, I.s);
     I.d.ensure(sizeof(double) * (size_t)n * n + 8);
-    // D = op(A) op(A)^T
-    if (ta) gemm(I.s, 'T', 'N', n, n, k, alpha, I.a.as<double>(), ar, I.a.as<double>(), ar, 0.0, I.d.as<double>(), n);
-    else gemm(I.s, 'N', 'T', n, n, k, alpha, I.a.as<double>(), ar, I.a.as<double>(), ar, 0.0, I.d.as<double>(), n);
+    // D = op(A) op(A)^T, symmetric: lower-triangle tiles, then mirrored
+    if (ta) gemm_sym(I.s, 'T', 'N', n, k, alpha, I.a.as<double>(), ar, I.a.as<double>(), ar, 0.0, I.d.as<double>(), n);
+    else gemm_sym(I.s, 'N', 'T', n, k, alpha, I.a.as<double>(), ar, I.a.as<double>(), ar, 0.0, I.d.as<double>(), n);
     std::vector<double> d((size_t)n * n);
     MHIP_CHECK(hipMemcpyAsync(d.data(), I.d.ptr, sizeof(double) * d.size(), hipMemcpyDeviceToHost, I.s));
     MHIP_CHECK(hipStreamSynchronize(I.s));

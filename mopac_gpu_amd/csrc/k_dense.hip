@@ -271,6 +271,21 @@ __device__ __forceinline__ void xcd_tile(int L, int gm, int gn, int& tm, int& tn
   tn = r / rows;
 }
 
+// Lower-triangle tile sequence (tile column by tile column, rows tn..T-1),
+// dealt to the XCDs in contiguous chunks as in xcd_tile, so one XCD holds a
+// few whole tile columns that share their B panel.
+__device__ __forceinline__ void tri_tile(int L, int ntile, int T, int& tm, int& tn) {
+  const int per = ntile / 8;
+  int t = (L < per * 8) ? (L % 8) * per + L / 8 : L;
+  int col = 0;
+  while (t >= T - col) {
+    t -= T - col;
+    ++col;
+  }
+  tn = col;
+  tm = col + t;
+}
+
 constexpr int BK = 16;
 // Eight waves (two per SIMD) on a 4 x 2 grid of sub-tiles, so each SIMD has
 // a second wave to issue MFMAs while one waits on LDS or the barrier.
@@ -280,7 +295,7 @@ __global__ __launch_bounds__(GB_NT) void k_gemm_big(int M, int N, int K, double 
                                                     const double* __restrict__ a, int lda,
                                                     const double* __restrict__ b, int ldb, double beta,
                                                     double* __restrict__ c, int ldc, int ksplit,
-                                                    double* __restrict__ cpart) {
+                                                    double* __restrict__ cpart, int tri) {
   constexpr int SA = BM + 16, SBn = BN + 16;
   constexpr int WM = BM / 4, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
@@ -294,10 +309,13 @@ __global__ __launch_bounds__(GB_NT) void k_gemm_big(int M, int N, int K, double 
   // ksplit > 1: workgroup kz of a tile takes the kz-th K / ksplit slice; the
   // first adds beta C, the others write alpha times their part to cpart
   // (M x N per extra slice), summed into C by k_add_parts
-  const int ntile = (M / BM) * (N / BN);
+  // tri (M == N, BM == BN): only the tiles on and below the diagonal
+  const int gtm = M / BM, gtn = N / BN;
+  const int ntile = tri ? gtm * (gtm + 1) / 2 : gtm * gtn;
   const int kz = blockIdx.x / ntile;
   const int kbase = kz * (K / ksplit);
-  xcd_tile(blockIdx.x - kz * ntile, M / BM, N / BN, tm, tn);
+  if (tri) tri_tile(blockIdx.x - kz * ntile, ntile, gtm, tm, tn);
+  else xcd_tile(blockIdx.x - kz * ntile, gtm, gtn, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
   const int wm = (w >> 1) * WM, wn = (w & 1) * WN;  // w = 0..7
   // operands stored transposed (A when TA, B when !TBT): a wave writes 8 k
@@ -527,12 +545,13 @@ int g_kernel_variant = 1;  // 1: FP64 MFMA tiles, 0: vector-FMA tiles
 template <int BM, int BN>
 static void gemm_big_dispatch(hipStream_t s, bool TA, bool TBT, int m, int n, int k, double alpha,
                               const double* a, int lda, const double* b, int ldb, double beta, double* c,
-                              int ldc, int ks, double* cp) {
-  const dim3 grid((m / BM) * (n / BN) * ks);
-  if (!TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, false>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp);
-  else if (TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, true, false>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp);
-  else if (!TA && TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, true>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp);
-  else hipLaunchKernelGGL((k_gemm_big<BM, BN, true, true>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp);
+                              int ldc, int ks, double* cp, int tri = 0) {
+  const int tiles = tri ? (m / BM) * (m / BM + 1) / 2 : (m / BM) * (n / BN);
+  const dim3 grid(tiles * ks);
+  if (!TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, false>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp, tri);
+  else if (TA && !TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, true, false>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp, tri);
+  else if (!TA && TBT) hipLaunchKernelGGL((k_gemm_big<BM, BN, false, true>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp, tri);
+  else hipLaunchKernelGGL((k_gemm_big<BM, BN, true, true>), grid, dim3(GB_NT), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp, tri);
   MHIP_LAUNCH_CHECK();
 }
 
@@ -652,6 +671,89 @@ void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
   if (m <= 0 || n <= 0) return;
   timed(s, K_GEMM_F64, 2.0 * m * n * (double)k, 8.0 * ((double)m * k + (double)k * n + 2.0 * m * n),
         [&] { gemm_launch(s, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); });
+}
+
+// Symmetric m x m result from its lower-triangle tiles: each workgroup takes
+// one 32 x 32 block (bi, bj), bi >= bj, adds the K-slice parts (m x m each,
+// written on the lower tiles only), and stores the block and its transpose,
+// both with lanes down columns.  A diagonal block takes its upper half from
+// its own lower half, so the result is exactly symmetric.
+__global__ __launch_bounds__(256) void k_sym_from_lower(int m, int nparts, const double* __restrict__ part,
+                                                        double* __restrict__ c, int ldc) {
+  __shared__ double t[32][33];
+  int b = blockIdx.x, bj = 0;
+  const int nb = m / 32;
+  while (b >= nb - bj) {
+    b -= nb - bj;
+    ++bj;
+  }
+  const int bi = bj + b;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const size_t mm = (size_t)m * m;
+  for (int r = ty; r < 32; r += 8) {
+    const int i = bi * 32 + tx, j = bj * 32 + r;
+    double v = c[i + (size_t)j * ldc];
+    for (int q = 0; q < nparts; ++q) v += part[q * mm + i + (size_t)j * m];
+    t[tx][r] = v;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = bi * 32 + tx, j = bj * 32 + r;
+    c[i + (size_t)j * ldc] = (bi == bj && tx < r) ? t[r][tx] : t[tx][r];
+    if (bi != bj) c[(size_t)(bj * 32 + tx) + (size_t)(bi * 32 + r) * ldc] = t[r][tx];
+  }
+}
+
+// Shapes and operands the lower-triangle launch takes.
+static bool gemm_sym_fits(int m, int k, const double* a, int lda, const double* b, int ldb) {
+  if (g_kernel_variant != 1 || m % 64 != 0 || k < BK || k % BK != 0 || (lda & 1) || (ldb & 1)) return false;
+  return !(reinterpret_cast<uintptr_t>(a) & 15) && !(reinterpret_cast<uintptr_t>(b) & 15);
+}
+
+// The large-tile kernel on the lower-triangle tiles of a square result.
+static void gemm_sym_launch(hipStream_t s, bool TA, bool TBT, int m, int k, double alpha, const double* a,
+                            int lda, const double* b, int ldb, double beta, double* c, int ldc) {
+  // square tiles only; estimated time as in gemm_big_launch over the triangle
+  int bm = 0;
+  double best_cost = 0.0;
+  for (const auto& q : {std::pair<int, double>{128, 1.0}, std::pair<int, double>{64, 0.6}}) {
+    if (m % q.first) continue;
+    const long long T = m / q.first, tiles = T * (T + 1) / 2;
+    const double cost = (double)((tiles + 255) / 256) * q.first * q.first / q.second;
+    if (bm == 0 || cost < best_cost) {
+      bm = q.first;
+      best_cost = cost;
+    }
+  }
+  const long long T = m / bm, tiles = T * (T + 1) / 2;
+  int ks = 1;
+  double ks_cost = (double)((tiles + 255) / 256);
+  for (int q : {2, 4}) {
+    if (k % (q * BK) != 0 || k / q < 256) continue;
+    const double cost = (double)((tiles * q + 255) / 256) / q;
+    if (cost < ks_cost - 1e-9 || (q == 2 && bm != 128 && tiles <= 256)) {
+      ks = q;
+      ks_cost = cost;
+    }
+  }
+  double* cp = ks > 1 ? splitk_scratch(s, sizeof(double) * (size_t)m * m * (ks - 1)) : nullptr;
+  if (bm == 128) gemm_big_dispatch<128, 128>(s, TA, TBT, m, m, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp, 1);
+  else gemm_big_dispatch<64, 64>(s, TA, TBT, m, m, k, alpha, a, lda, b, ldb, beta, c, ldc, ks, cp, 1);
+  const int nb = m / 32;
+  hipLaunchKernelGGL(k_sym_from_lower, dim3(nb * (nb + 1) / 2), dim3(256), 0, s, m, ks - 1, cp, c, ldc);
+  MHIP_LAUNCH_CHECK();
+}
+
+void gemm_sym(hipStream_t s, char ta, char tb, int m, int k, double alpha, const double* a, int lda,
+              const double* b, int ldb, double beta, double* c, int ldc) {
+  if (m <= 0) return;
+  if (!gemm_sym_fits(m, k, a, lda, b, ldb)) {
+    gemm(s, ta, tb, m, m, k, alpha, a, lda, b, ldb, beta, c, ldc);
+    return;
+  }
+  const bool TA = (ta == 'T' || ta == 't'), TBT = (tb == 'T' || tb == 't');
+  timed(s, K_GEMM_F64, (double)m * (m + 1) * (double)k, 8.0 * (2.0 * m * k + (double)m * m),
+        [&] { gemm_sym_launch(s, TA, TBT, m, k, alpha, a, lda, b, ldb, beta, c, ldc); });
 }
 
 static void syrk_launch(hipStream_t s, int n, int k, double occ, const double* a, int lda, double* p) {

@@ -132,8 +132,8 @@ void bjorck_step(hipStream_t s, OccVirtWork& w, int n, int ld, double* C, int c0
   // the occupied block (c0 == 0) and the virtual block use their own m x m buffer
   double* e = (c0 == 0) ? w.eo.as<double>() : w.ev.as<double>();
   copy_cols(s, n, ld, cb, cw, m);
-  // E = C^T C - I
-  gemm(s, 'T', 'N', m, m, n, 1.0, cw, ld, cw, ld, 0.0, e, m);
+  // E = C^T C - I (symmetric: lower-triangle tiles, then mirrored)
+  gemm_sym(s, 'T', 'N', m, n, 1.0, cw, ld, cw, ld, 0.0, e, m);
   hipLaunchKernelGGL(k_sub_identity, dim3((m + 255) / 256), dim3(256), 0, s, m, e, m);
   MHIP_LAUNCH_CHECK();
   // C <- C - C E / 2 (E is small: the split product's error is second order)

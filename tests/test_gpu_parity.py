@@ -184,6 +184,34 @@ def test_reference_boundary_dropins(gpu_ctx):
     assert np.max(np.abs(cg - ref)) < 1e-12
 
 
+@pytest.mark.parametrize("tra,n,k", [("T", 64, 48), ("N", 640, 256), ("T", 1024, 3072), ("T", 2048, 3072),
+                                     ("N", 1024, 1024)])
+def test_symmetric_result_gemm_matches_numpy(gpu_ctx, tra, n, k):
+    """call_syrk's product op(A) op(A)^T on the lower-triangle tiles: the
+    64- and 128-wide tiles, with and without the K split (the Bjorck Gram
+    shapes of (H2O)512 are n = 1024 and 2048 at k = 3072); both triangles
+    against numpy, and the full product exactly symmetric."""
+    lib = L.load()
+    rng = np.random.default_rng(n + k)
+    x = np.asfortranarray(rng.standard_normal((k, n) if tra == "T" else (n, k)))
+    ref = 0.5 * (x.T @ x if tra == "T" else x @ x.T)
+    tol = 1e-12 * np.sqrt(k) * np.max(np.abs(ref))
+    for uplo in (b"U", b"L"):
+        cc = np.zeros((n, n), order="F")
+        lib.mhip_call_syrk(uplo, tra.encode(), n, k, 0.5, L.ptr(x), x.shape[0], 0.0, L.ptr(cc), n)
+        tri = np.triu if uplo == b"U" else np.tril
+        assert np.max(np.abs(tri(cc) - tri(ref))) < tol
+        # the other triangle is left as it was
+        other = np.tril(cc, -1) if uplo == b"U" else np.triu(cc, 1)
+        assert not np.any(other)
+    # the mirrored upper triangle equals the lower one bit for bit
+    cu = np.zeros((n, n), order="F")
+    cl = np.zeros((n, n), order="F")
+    lib.mhip_call_syrk(b"U", tra.encode(), n, k, 0.5, L.ptr(x), x.shape[0], 0.0, L.ptr(cu), n)
+    lib.mhip_call_syrk(b"L", tra.encode(), n, k, 0.5, L.ptr(x), x.shape[0], 0.0, L.ptr(cl), n)
+    assert np.array_equal(np.triu(cu), np.tril(cl).T)
+
+
 def test_mfma_tiles_match_vector_tiles(gpu_ctx):
     """Checks the FP64 MFMA lane layout: both kernel families against numpy."""
     lib = L.load()
