@@ -620,14 +620,14 @@ static bool gemm_big_launch(hipStream_t s, bool TA, bool TBT, int m, int n, int 
     }
   }
   if (best < 0) return false;
-  // K split (each slice keeps 256 or more of it) when it balances the CUs
+  // K split in 2, 3 or 4 (each slice keeps 256 or more of K) when it balances the CUs
   // better: the fewest full-tile equivalents per CU, ceil(tiles ks / 256) / ks,
   // ties to the smaller split; one-round launches of the smaller tiles always
   // split in two (their workgroups gain most from a second one on the CU)
   const long long tiles = (long long)(m / cands[best].bm) * (n / cands[best].bn);
   int ks = 1;
   double ks_cost = (double)((tiles + 255) / 256);
-  for (int q : {2, 4}) {
+  for (int q : {2, 3, 4}) {
     if (k % (q * BK) != 0 || k / q < 256) continue;
     const double cost = (double)((tiles * q + 255) / 256) / q;
     if (cost < ks_cost - 1e-9 || (q == 2 && best != 0 && tiles <= 256)) {
@@ -728,7 +728,7 @@ static void gemm_sym_launch(hipStream_t s, bool TA, bool TBT, int m, int k, doub
   const long long T = m / bm, tiles = T * (T + 1) / 2;
   int ks = 1;
   double ks_cost = (double)((tiles + 255) / 256);
-  for (int q : {2, 4}) {
+  for (int q : {2, 3, 4}) {
     if (k % (q * BK) != 0 || k / q < 256) continue;
     const double cost = (double)((tiles * q + 255) / 256) / q;
     if (cost < ks_cost - 1e-9 || (q == 2 && bm != 128 && tiles <= 256)) {
