@@ -622,8 +622,10 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
       ++nfragdiag;
     } else if (use_pseudo) {
       if (o->pseudo_diag == 2) {
-        // Bjorck error after k steps ~ |B^T B|^(2^k): more steps while PL is large
-        const int nbj = pl > 0.05 ? 3 : (pl > 1.0e-3 ? 2 : 1);
+        // Bjorck error after k steps ~ |B^T B|^(2^k), and each iteration's
+        // step also squares what the last one left: a second step only
+        // while PL is large (the rotations are big)
+        const int nbj = pl > 0.05 ? 2 : 1;
         occvirt_step(s, c->ov, n, c->npad, c->f.as<double>(), c->C.as<double>(), nclose,
                      c->eig.as<double>(), split_now, nbj);
         block_steps_since_diag += 1;
