@@ -10,7 +10,7 @@ already resident in HBM.
   plain : the reference algorithm (full FP64 diagonalization every iteration)
   fast  : the accelerated form (learned starting density when a trained model
           is present, adaptive eigensolver tolerance, split-BF16 density while
-          far from convergence), judged against plain FP64 by |dHf| and
+          far from convergence), compared with plain FP64 by |dHf| and
           max|dP|.
 
 Prints one JSON line (value = fast-form seconds per SCF, lower is better)
@@ -76,8 +76,8 @@ def copy_calibration(mib):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU path is a design "
-                                   "(see DESIGN.md), so --gpus must be 1"}))
+        print(json.dumps({"error": "one GPU only: the SCF of one molecule runs on a single "
+                                   "device here, so --gpus must be 1"}))
         sys.exit(2)
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
