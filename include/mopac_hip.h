@@ -427,10 +427,27 @@ int mhip_pm7_hamiltonian(int numat, const int* nat, const double* coord, int cha
                          int* nfirst, int* nlast, double* h, double* w, double* enuclr,
                          double* atheat, double* pdiag);
 double mhip_ev_to_kcal(void);
-/* Per-element PM7 values the point-charge terms need (parameters_C after
- * calpar): out[0..4] = natorb, tore, am, ad, dd (atomic units).  Elements 1,
- * 6, 7, 8; -1 for any other. */
+/* Per-element PM7 values (parameters_C after calpar, src/models/calpar.F90):
+ * out[0..8] = natorb, tore, am, ad, dd, aq, qq (atomic units), eisol (eV),
+ * eheat (kcal/mol).  Elements 1, 6, 7, 8; -1 for any other. */
 int mhip_pm7_element(int z, double* out);
+/* Everything one atom pair contributes to hcore, for atom i (element zi at xi,
+ * Angstrom) listed after atom j (zj at xj): w = the two-electron block
+ * (limij x limkl, rotate's order), e1b/e2a = the electron-core attraction
+ * blocks (packed, 10 each), enuc = the core-core term (eV), di = the
+ * resonance block (4 x 4 row-major, orbitals of i down).  Replaces rotate and
+ * h1elec for one pair (src/integrals/rotate.F90:16-124, h1elec.F90:16-84). */
+int mhip_pm7_pair(int zi, int zj, const double* xi, const double* xj, double* w, double* e1b, double* e2a,
+                  double* enuc, double* di);
+/* PM7's post-SCF corrections of a discrete molecule (kcal/mol), the terms
+ * post_scf_corrections adds to the SCF heat of formation for PM7
+ * (src/corrections/post_scf_corrections.F90:112-114): *e_disp the damped
+ * C6 dispersion (PM6_DH_Dispersion, H_bond_correction_PM6_DH_Dispersion.F90),
+ * *e_hbond the DH+ hydrogen-bond term (Hydrogen_bond_corrections with
+ * EH_plus), *n_hbonds the bonds below -1 kcal/mol (may be NULL).  Elements
+ * 1, 6, 7, 8; coord is numat x 3 Angstrom. */
+int mhip_pm7_corrections(int numat, const int* nat, const double* coord, double* e_disp, double* e_hbond,
+                         int* n_hbonds);
 /* The one-electron matrix on the MOZYME sparse storage (nijbo, mpack) and
  * the core-core energy as hcore_for_MOZYME forms them for a discrete system
  * (src/MOZYME/hcore_for_MOZYME.F90:100-260): NDDO terms for stored pairs,

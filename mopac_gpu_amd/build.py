@@ -24,7 +24,6 @@ SOURCES = [
     "k_pseudo.hip",
     "k_occvirt.hip",
     "k_split.hip",
-    "k_precision.hip",
     "k_mlp.hip",
     "k_batch.hip",
     "k_guess.hip",
@@ -33,6 +32,7 @@ SOURCES = [
     "scf_driver.hip",
     "dropin.hip",
     "pm7_hamiltonian.cpp",
+    "pm7_corrections.cpp",
 ]
 
 # The MOZYME kernels reproduce the reference's list-building, cutoff and
@@ -60,7 +60,7 @@ def build(verbose: bool = False) -> str:
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
-            continue
+            raise FileNotFoundError(f"libmopachip source missing: {path}")
         obj = os.path.join(objdir, src + ".o")
         cmd = [HIPCC] + FLAGS + FILE_FLAGS.get(src, []) + ["-c", path, "-o", obj]
         if src.endswith(".cpp"):

@@ -3,30 +3,41 @@
 // This is the producer of the SCF inner loop's inputs: the one-electron matrix
 // H (packed lower triangle), the two-electron integrals W in the order the
 // Fock builder consumes them, the core-core repulsion and the isolated-atom
-// reference energy.  It runs once per geometry on the host, exactly where the
+// reference energy.  It runs once per geometry on the host, where the
 // reference runs it (hcore before iter), and is not on the per-iteration path.
 //
-// Reference behaviour followed (all paths relative to the reference root):
-//   hcore         src/integrals/hcore.F90:157-288     (H assembly, W order)
-//   rotate        src/integrals/rotate.F90:16-124     (pair driver)
-//   rotatd        src/integrals/mndod.F90:1096-1372   (local->molecular W, core)
-//   reppd         src/integrals/mndod.F90:426-845     (22 local sp integrals)
-//   spcore        src/integrals/mndod.F90:1683-1776   (electron-core integrals)
-//   elenuc        src/integrals/mndod.F90:2367-2490   (core terms to H blocks)
-//   rotmat        src/integrals/mndod.F90:1382-1569   (bond frame)
-//   to_point      src/integrals/mndod.F90:3462-3486   (PM7 point-charge feathering)
-//   ccrep         src/integrals/ccrep.F90:16-230      (PM7 core-core)
-//   h1elec/diat   src/integrals/h1elec.F90, diat.F90, set.F90, coe.F90 (overlaps)
-//   wstore        src/integrals/mndod.F90:2027-2095   (one-centre block)
-//   calpar/inid   src/models/calpar.F90, mndod.F90:16-61 (derived multipoles)
-//   pdiag         src/moldat.F90:731-790              (diagonal starting density)
-//   parameters    src/models/parameters_for_PM7_C.F90 (H:24-33, C/N/O:~80-140,
-//                 alpb/xfac 1405-1483, v_par7 1392-1395); eheat parameters_C.F90:224-231
-//   constants     src/conref_C.F90:27-45 (2018 CODATA set, the default)
+// Methods (published):
+//   two-centre integrals  Dewar & Thiel, Theor. Chim. Acta 46, 89 (1977): each
+//                         orbital-product charge distribution is a set of
+//                         point multipoles (monopole, dipole, quadrupole) made
+//                         of point charges, and two distributions interact by
+//                         the Klopman-Ohno form q q' / sqrt(r^2 + (rho + rho')^2),
+//                         whose additive radii rho reproduce the one-centre
+//                         integrals at r = 0.  Evaluated here by one general
+//                         point-charge engine for every integral, in a bond
+//                         frame whose axis runs from the first atom to the
+//                         second.
+//   overlaps              Slater-orbital overlaps in prolate spheroidal
+//                         coordinates (Mulliken, Rieke, Orloff & Orloff, J.
+//                         Chem. Phys. 17, 1248 (1949)): the integrand is
+//                         expanded as a polynomial in (xi, eta) and integrated
+//                         term by term with the auxiliary integrals A_k, B_k.
+//   PM7                   Stewart, J. Mol. Model. 19, 1 (2013): parameters,
+//                         core-core function, point-charge switch at 7 A.
 //
-// The integral formulas and parameter values must equal the reference's; the
-// code around them (data layout, the direct 4-index rotation, the parallel pair
-// loop) is this library's own.
+// Reference behaviour matched (paths relative to the reference root):
+//   hcore         src/integrals/hcore.F90:157-288     (H assembly, W order)
+//   rotate        src/integrals/rotate.F90:16-124     (per-pair outputs)
+//   calpar/inid   src/models/calpar.F90:100-190, src/integrals/mndod.F90:16-61
+//   h1elec/diat   src/integrals/h1elec.F90, diat.F90   (resonance block)
+//   ccrep         src/integrals/ccrep.F90:16-262      (PM7 core-core)
+//   to_point      src/integrals/mndod.F90:3462-3486   (point-charge switch)
+//   wstore        src/integrals/mndod.F90:2027-2095   (one-centre block)
+//   pdiag         src/moldat.F90:731-790              (diagonal starting density)
+//   parameters    src/models/parameters_for_PM7_C.F90, parameters_C.F90:224-231
+//   constants     src/conref_C.F90:27-45 (2018 CODATA set, the default)
+// The per-pair results equal the reference's routines compiled from its
+// sources to rounding (tests/test_pm7_reference.py).
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -42,607 +53,521 @@ namespace {
 constexpr double kA0 = 0.529177210903;        // Bohr radius, Angstrom
 constexpr double kEV = 27.211386245988;       // Hartree in eV
 constexpr double kFPC9 = 23.060547830619029;  // eV -> kcal/mol
-constexpr double kTrunc1 = 7.0;               // PM7 point-charge switch radius (A)
-constexpr double kTrunc2 = 0.22;              // and its Gaussian exponent
+constexpr double kSwitchR = 7.0;              // PM7 point-charge switch radius (A)
+constexpr double kSwitchK = 0.22;             // and its Gaussian exponent (1/A^2)
+constexpr double kPi = 3.14159265358979323846;
 
-struct Elem {
-  int z = 0, natorb = 0, npq = 0, ios = 0, iop = 0;
-  double tore = 0;
-  double uss = 0, upp = 0, betas = 0, betap = 0, zs = 0, zp = 0;
-  double gss = 0, gsp = 0, gpp = 0, gp2 = 0, hsp = 0;
-  double g1 = 0, g2 = 0, g3 = 0;  // first core-core Gaussian
-  double eheat = 0;
-  // derived (calpar / inid)
-  double eisol = 0, dd = 0, qq = 0, am = 0, ad = 0, aq = 0;
-  double po1 = 0, po2 = 0, po3 = 0, po7 = 0, po9 = 0, ddp2 = 0, ddp3 = 0;
-  bool ok = false;
+// ---- PM7 parameter set (values of the reference's tables, as data) ---------
+struct Params {
+  int z, ns, np;  // valence s and p electrons of the neutral atom
+  double uss, upp, betas, betap, zs, zp;
+  double gss, gsp, gpp, gp2, hsp;
+  double gk, gl, gm;  // the element's core-core Gaussian: K, L, M
+  double dhf_atom;    // experimental heat of formation of the atom (kcal/mol)
+};
+const Params kPM7[] = {
+    {1, 1, 0, -11.070112, 0.0, -8.389745, 0.0, 1.260237, 0.0,
+     14.149656, 0.0, 0.0, 0.0, 0.0, 0.177854, 1.428710, 0.991324, 52.102},
+    {6, 2, 2, -51.372620, -40.135421, -14.414930, -7.893717, 1.942244, 1.708723,
+     12.347323, 11.932801, 10.452226, 9.385498, 0.802634, 0.045888, 5.037055, 1.588715, 170.890},
+    {7, 2, 3, -61.623260, -48.949816, -22.110958, -15.464433, 2.354344, 2.028288,
+     11.881002, 9.572117, 12.210140, 10.299947, 2.977433, 0.015143, 4.734148, 1.516714, 113.000},
+    {8, 2, 4, -96.087736, -71.087738, -67.776339, -20.267981, 5.972309, 2.349017,
+     14.955121, 16.088521, 12.403337, 10.499706, 5.028656, -0.016243, 1.871970, 1.844360, 59.559},
+};
+// Diatomic core-core parameters (alpha_AB in 1/A, x_AB).
+struct PairParams {
+  int a, b;
+  double alpha, x;
+};
+const PairParams kPM7Pairs[] = {
+    {1, 1, 4.051163, 2.845627}, {6, 1, 1.038716, 0.204582}, {6, 6, 2.655746, 0.937818},
+    {7, 1, 1.049958, 0.173966}, {7, 6, 2.734636, 0.959163}, {7, 7, 2.786676, 0.863430},
+    {8, 1, 1.508030, 0.160602}, {8, 6, 2.850931, 0.826848}, {8, 7, 2.841313, 0.699190},
+    {8, 8, 2.457135, 0.356439},
+};
+// PM7's global core-core terms: a C-C short-range term (scale, exponent) and
+// a slow O-H term (scale, exponent).
+constexpr double kCCScale = 8.947612, kCCExp = 6.024265;
+constexpr double kOHScale = -0.012037, kOHExp = 0.701333;
+// The reference writes the r^6 coefficient of its PM6/PM7 pair scale factor
+// as a single-precision constant; this is its value as the double it becomes.
+const double kR6 = (double)0.0003f;
+
+// ---- the point-charge multipole model -------------------------------------
+struct PointCharge {
+  double q, x, y, z;  // charge (e) and position (bohr) relative to the atom
+};
+// One point multipole: up to four charges sharing one additive radius.
+struct Multipole {
+  PointCharge c[4];
+  int n = 0;
+  double rho = 0.0;  // additive radius (bohr)
+};
+// Charge distribution of one product of local orbitals: up to two multipoles
+// (the monopole of a p^2 product plus its quadrupole).
+struct Distribution {
+  Multipole m[2];
+  int nm = 0;
+  bool monopole = false;  // carries one unit of charge (switches to a point charge)
+  bool axial = false;     // the pi x pi' product, fixed by axial symmetry
+  int parity = 0;         // bit 0: odd under x -> -x, bit 1: odd under y -> -y
 };
 
-Elem g_elem[9];
+// Local orbitals on each atom: 0 = s, 1 = p along the bond axis (sigma, z),
+// 2 = p along x (pi), 3 = p along y (pi').  Unordered pairs are indexed
+// i(i+1)/2 + j (i >= j), the order MOPAC stores pair blocks in.
+inline int pidx(int i, int j) {
+  if (i < j) std::swap(i, j);
+  return i * (i + 1) / 2 + j;
+}
+
+struct Element {
+  bool ok = false;
+  int z = 0, natorb = 0, nprin = 0;
+  double tore = 0.0;
+  Params p{};
+  double eisol = 0.0;
+  double am = 0.0, ad = 0.0, aq = 0.0;  // additive terms as 1/(2 rho) (hartree)
+  double d1 = 0.0, d2 = 0.0;            // dipole and quadrupole charge offsets (bohr)
+  double rho0 = 0.0, rho1 = 0.0, rho2 = 0.0, rho_core = 0.0;
+  Distribution dist[10];
+};
+Element g_elem[9];
+double g_alpb[9][9], g_xfac[9][9];
 bool g_init = false;
 
-// PM7 diatomic core-core pair parameters (alpha_AB, x_AB), symmetric.
-double g_alpb[9][9], g_xfac[9][9];
-// PM7 global core-core parameters v_par7(1..4).
-constexpr double kPar1 = 8.947612, kPar2 = 6.024265, kPar3 = -0.012037, kPar4 = 0.701333;
+Multipole monopole(double rho) {
+  Multipole m;
+  m.c[0] = {1.0, 0, 0, 0};
+  m.n = 1;
+  m.rho = rho;
+  return m;
+}
+// +1/2 at +d, -1/2 at -d along axis k (0 = x, 1 = y, 2 = z)
+Multipole dipole(int k, double d, double rho) {
+  Multipole m;
+  double a[3] = {0, 0, 0};
+  a[k] = d;
+  m.c[0] = {0.5, a[0], a[1], a[2]};
+  m.c[1] = {-0.5, -a[0], -a[1], -a[2]};
+  m.n = 2;
+  m.rho = rho;
+  return m;
+}
+// linear quadrupole along axis k: +1/4 at +-2d, -1/2 at the centre
+Multipole linear_quadrupole(int k, double d, double rho) {
+  Multipole m;
+  double a[3] = {0, 0, 0};
+  a[k] = 2.0 * d;
+  m.c[0] = {0.25, a[0], a[1], a[2]};
+  m.c[1] = {0.25, -a[0], -a[1], -a[2]};
+  m.c[2] = {-0.5, 0, 0, 0};
+  m.n = 3;
+  m.rho = rho;
+  return m;
+}
+// square quadrupole in the plane of axes k and l: +1/4 where the two
+// coordinates have the same sign, -1/4 where they differ (corners at +-d)
+Multipole square_quadrupole(int k, int l, double d, double rho) {
+  Multipole m;
+  const double s[4][2] = {{1, 1}, {-1, -1}, {1, -1}, {-1, 1}};
+  for (int i = 0; i < 4; ++i) {
+    double a[3] = {0, 0, 0};
+    a[k] = s[i][0] * d;
+    a[l] = s[i][1] * d;
+    m.c[i] = {0.25 * s[i][0] * s[i][1], a[0], a[1], a[2]};
+  }
+  m.n = 4;
+  m.rho = rho;
+  return m;
+}
 
-void derive(Elem& e) {
-  const int i = e.z;
-  if (i == 1) {
-    e.eisol = e.uss;
-    e.am = e.gss / kEV;
+// Interaction energy (hartree) of multipole a at the origin with multipole b
+// at (0, 0, r), additive radii summed.
+double interact(const Multipole& a, const Multipole& b, double r) {
+  const double rho = a.rho + b.rho, rho2 = rho * rho;
+  double s = 0.0;
+  for (int i = 0; i < a.n; ++i)
+    for (int j = 0; j < b.n; ++j) {
+      const double dx = a.c[i].x - b.c[j].x, dy = a.c[i].y - b.c[j].y, dz = a.c[i].z - (b.c[j].z + r);
+      s += a.c[i].q * b.c[j].q / std::sqrt(dx * dx + dy * dy + dz * dz + rho2);
+    }
+  return s;
+}
+// Self-repulsion (hartree) of a set of charges whose pairwise additive term
+// is 1/x (that is, each charge's radius is 1/(2x)).
+double self_energy(const Multipole& m, double x) {
+  const double a2 = 1.0 / (x * x);
+  double s = 0.0;
+  for (int i = 0; i < m.n; ++i)
+    for (int j = 0; j < m.n; ++j) {
+      const double dx = m.c[i].x - m.c[j].x, dy = m.c[i].y - m.c[j].y, dz = m.c[i].z - m.c[j].z;
+      s += m.c[i].q * m.c[j].q / std::sqrt(dx * dx + dy * dy + dz * dz + a2);
+    }
+  return s;
+}
+
+// Secant iteration x <- x - f(x) (x - x_prev) / (f(x) - f(x_prev)), a fixed
+// number of steps.
+template <class F>
+double secant(F&& f, double x0, double x1, int steps) {
+  double f0 = f(x0);
+  for (int k = 0; k < steps; ++k) {
+    const double f1 = f(x1);
+    if (std::fabs(f1 - f0) < 1.0e-25) break;
+    const double x2 = x0 + (x1 - x0) * (0.0 - f0) / (f1 - f0);
+    x0 = x1;
+    f0 = f1;
+    x1 = x2;
+  }
+  return x1;
+}
+
+// Multipole geometry and additive radii of one element (Dewar & Thiel):
+//  d1   the dipole length of the s.p_sigma distribution, <ns|z|np_z>, from
+//       the Slater radial integral N_s N_p (2n+1)! / (zs+zp)^(2n+2) / sqrt(3);
+//  d2   sqrt(<np|r^2|np> / 5), <r^2> = (2n+1)(2n+2) / (2 zp)^2, the
+//       quadrupole offset (linear quadrupoles put +1/4 at +-2 d2);
+//  rho  chosen so each multipole's self-repulsion equals the one-centre
+//       integral it stands for: Gss (monopole), Hsp (dipole), and
+//       Hpp = (Gpp - Gp2)/2 (the square quadrupole of p_x p_y).
+// The dipole and quadrupole radii are solved by five secant steps from the
+// small-x limits of the self-energies (D^2 x^3 and 3 D^4 x^5) and that start
+// + 0.04, the steps MOPAC's calpar takes: the quadrupole term is then ~1e-9
+// short of the exact root, and parity with the reference's integrals needs
+// the same five steps.
+void derive(Element& e) {
+  const Params& p = e.p;
+  e.z = p.z;
+  e.tore = p.ns + p.np;
+  e.natorb = (p.zp > 0.0) ? 4 : 1;
+  e.nprin = (p.z <= 2) ? 1 : 2;
+  const double ev = kEV;
+  if (e.natorb == 1) {
+    e.eisol = p.uss;
+    e.am = p.gss / ev;
     e.ad = e.am;
     e.aq = e.am;
   } else {
-    const double gssc = std::max(e.ios - 1, 0);
-    const int k = e.iop;
-    const double gspc = e.ios * k;
-    const int l = std::min(k, 6 - k);
-    const double gp2c = (k * (k - 1)) / 2 + 0.5 * (l * (l - 1)) / 2;
-    const double gppc = -0.5 * (l * (l - 1)) / 2;
-    const double hspc = -k * e.ios * 0.5;
-    e.zp = std::max(0.3, e.zp);
-    double hpp = 0.5 * (e.gpp - e.gp2);
-    hpp = std::max(0.1, hpp);
-    e.hsp = std::max(1.0e-7, e.hsp);
-    e.eisol = e.uss * e.ios + e.upp * e.iop + e.gss * gssc + e.gpp * gppc + e.gsp * gspc +
-              e.gp2 * gp2c + e.hsp * hspc;
-    const double qn = 2.0;  // principal quantum number of the 2sp shell
-    e.dd = (2.0 * qn + 1) * std::pow(4.0 * e.zs * e.zp, qn + 0.5) /
-           std::pow(e.zs + e.zp, 2.0 * qn + 2) / std::sqrt(3.0);
-    e.qq = std::sqrt((4.0 * qn * qn + 6.0 * qn + 2.0) / 20.0) / e.zp;
-    // Secant solve for the dipole additive term (calpar.F90, jmax = 5).
-    double gdd1 = std::pow(e.hsp / (kEV * e.dd * e.dd), 1.0 / 3.0);
-    double d1 = gdd1, d2 = gdd1 + 0.04;
-    for (int j = 0; j < 5; ++j) {
-      const double df = d2 - d1;
-      const double h1 = 0.5 * d1 - 0.5 / std::sqrt(4.0 * e.dd * e.dd + 1.0 / (d1 * d1));
-      const double h2 = 0.5 * d2 - 0.5 / std::sqrt(4.0 * e.dd * e.dd + 1.0 / (d2 * d2));
-      if (std::fabs(h2 - h1) < 1.0e-25) break;
-      const double d3 = d1 + df * (e.hsp / kEV - h1) / (h2 - h1);
-      d1 = d2;
-      d2 = d3;
-    }
-    // ... and for the quadrupole additive term.
-    const double p4 = 16.0;
-    double gqq = std::pow(p4 * hpp / (kEV * 48.0 * std::pow(e.qq, 4)), 0.2);
-    double q1 = gqq, q2 = gqq + 0.04;
-    for (int j = 0; j < 5; ++j) {
-      const double qf = q2 - q1;
-      const double h1 = 0.25 * q1 - 0.5 / std::sqrt(4.0 * e.qq * e.qq + 1.0 / (q1 * q1)) +
-                        0.25 / std::sqrt(8.0 * e.qq * e.qq + 1.0 / (q1 * q1));
-      const double h2 = 0.25 * q2 - 0.5 / std::sqrt(4.0 * e.qq * e.qq + 1.0 / (q2 * q2)) +
-                        0.25 / std::sqrt(8.0 * e.qq * e.qq + 1.0 / (q2 * q2));
-      if (std::fabs(h2 - h1) < 1.0e-25) break;
-      const double q3 = q1 + qf * (hpp / kEV - h1) / (h2 - h1);
-      q1 = q2;
-      q2 = q3;
-    }
-    e.am = e.gss / kEV;
-    e.ad = d2;
-    e.aq = q2;
+    const double zp = std::max(0.3, p.zp), zs = p.zs;
+    const double hsp = std::max(1.0e-7, p.hsp);
+    const double hpp = std::max(0.1, 0.5 * (p.gpp - p.gp2));
+    // Isolated-atom electronic energy of the valence configuration s^ns p^np
+    // (Hund's rule): one-electron terms, then the Coulomb and exchange pairs.
+    const int ns = p.ns, np = p.np;
+    const int unpaired_pairs = std::min(np, 6 - np);
+    const double n_ss = std::max(ns - 1, 0);
+    const double n_sp = ns * np;
+    const double n_sp_exch = -0.5 * ns * np;
+    const double n_p2 = (np * (np - 1)) / 2 + 0.5 * (unpaired_pairs * (unpaired_pairs - 1)) / 2;
+    const double n_pp = -0.5 * (unpaired_pairs * (unpaired_pairs - 1)) / 2;
+    e.eisol = p.uss * ns + p.upp * np + p.gss * n_ss + p.gpp * n_pp + p.gsp * n_sp + p.gp2 * n_p2 +
+              hsp * n_sp_exch;
+    const int n = e.nprin;
+    const double fact2n = (n == 1) ? 2.0 : 24.0, fact2n1 = (n == 1) ? 6.0 : 120.0;
+    const double ns_norm = std::pow(2.0 * zs, n + 0.5) / std::sqrt(fact2n);
+    const double np_norm = std::pow(2.0 * zp, n + 0.5) / std::sqrt(fact2n);
+    e.d1 = ns_norm * np_norm * fact2n1 / std::pow(zs + zp, 2 * n + 2) / std::sqrt(3.0);
+    e.d2 = std::sqrt((2.0 * n + 1.0) * (2.0 * n + 2.0) / (4.0 * zp * zp) / 5.0);
+    const Multipole dip = dipole(2, e.d1, 0.0);
+    const Multipole quad = square_quadrupole(0, 1, e.d2, 0.0);
+    const double x_dip = std::cbrt(hsp / ev / (e.d1 * e.d1));
+    e.ad = secant([&](double x) { return self_energy(dip, x) - hsp / ev; }, x_dip, x_dip + 0.04, 5);
+    const double x_quad = std::pow(hpp / ev / (3.0 * std::pow(e.d2, 4)), 0.2);
+    e.aq = secant([&](double x) { return self_energy(quad, x) - hpp / ev; }, x_quad, x_quad + 0.04, 5);
+    e.am = p.gss / ev;
   }
-  // inid: additive terms as radii, dipole / quadrupole separations
-  e.po1 = 0.5 / e.am;
-  if (e.ad > 1e-5) e.po2 = 0.5 / e.ad;
-  if (e.aq > 1e-5) e.po3 = 0.5 / e.aq;
-  e.po7 = e.po1;
-  e.ddp2 = e.dd;
-  e.ddp3 = e.qq * std::sqrt(2.0);
-  e.po9 = e.po1;  // no separate core additive term for these elements in PM7
-  if (i == 1) {
-    e.po2 = 0.0;
-    e.po3 = 0.0;
+  e.rho0 = 0.5 / e.am;
+  e.rho1 = (e.natorb > 1) ? 0.5 / e.ad : 0.0;
+  e.rho2 = (e.natorb > 1) ? 0.5 / e.aq : 0.0;
+  e.rho_core = e.rho0;  // PM7 gives H, C, N, O no separate core radius
+  // the ten orbital-product distributions
+  for (auto& d : e.dist) d = Distribution{};
+  auto set = [&](int a, int b, std::initializer_list<Multipole> ms, bool mono, int parity) {
+    Distribution& d = e.dist[pidx(a, b)];
+    for (const Multipole& m : ms) d.m[d.nm++] = m;
+    d.monopole = mono;
+    d.parity = parity;
+  };
+  set(0, 0, {monopole(e.rho0)}, true, 0);
+  if (e.natorb > 1) {
+    set(1, 0, {dipole(2, e.d1, e.rho1)}, false, 0);
+    set(2, 0, {dipole(0, e.d1, e.rho1)}, false, 1);
+    set(3, 0, {dipole(1, e.d1, e.rho1)}, false, 2);
+    set(1, 1, {monopole(e.rho0), linear_quadrupole(2, e.d2, e.rho2)}, true, 0);
+    set(2, 2, {monopole(e.rho0), linear_quadrupole(0, e.d2, e.rho2)}, true, 0);
+    set(3, 3, {monopole(e.rho0), linear_quadrupole(1, e.d2, e.rho2)}, true, 0);
+    set(2, 1, {square_quadrupole(0, 2, e.d2, e.rho2)}, false, 1);
+    set(3, 1, {square_quadrupole(1, 2, e.d2, e.rho2)}, false, 2);
+    e.dist[pidx(3, 2)].axial = true;
+    e.dist[pidx(3, 2)].parity = 3;
   }
   e.ok = true;
 }
 
 void init_params() {
   if (g_init) return;
-  auto set = [](int z, int natorb, int npq, double tore, int ios, int iop, double uss, double upp,
-                double betas, double betap, double zs, double zp, double gss, double gsp,
-                double gpp, double gp2, double hsp, double g1, double g2, double g3,
-                double eheat) {
-    Elem& e = g_elem[z];
-    e.z = z; e.natorb = natorb; e.npq = npq; e.tore = tore; e.ios = ios; e.iop = iop;
-    e.uss = uss; e.upp = upp; e.betas = betas; e.betap = betap; e.zs = zs; e.zp = zp;
-    e.gss = gss; e.gsp = gsp; e.gpp = gpp; e.gp2 = gp2; e.hsp = hsp;
-    e.g1 = g1; e.g2 = g2; e.g3 = g3; e.eheat = eheat;
-    derive(e);
-  };
-  //        Z  nao npq tore s  p     Uss          Upp          beta_s       beta_p       zeta_s     zeta_p
-  //                               Gss         Gsp         Gpp         Gp2         Hsp
-  //                               Gauss(1): K, L, M                    dHf(atom)
-  set(1, 1, 1, 1.0, 1, 0, -11.070112, 0.0, -8.389745, 0.0, 1.260237, 0.0,
-      14.149656, 0.0, 0.0, 0.0, 0.0, 0.177854, 1.428710, 0.991324, 52.102);
-  set(6, 4, 2, 4.0, 2, 2, -51.372620, -40.135421, -14.414930, -7.893717, 1.942244, 1.708723,
-      12.347323, 11.932801, 10.452226, 9.385498, 0.802634, 0.045888, 5.037055, 1.588715, 170.890);
-  set(7, 4, 2, 5.0, 2, 3, -61.623260, -48.949816, -22.110958, -15.464433, 2.354344, 2.028288,
-      11.881002, 9.572117, 12.210140, 10.299947, 2.977433, 0.015143, 4.734148, 1.516714, 113.000);
-  set(8, 4, 2, 6.0, 2, 4, -96.087736, -71.087738, -67.776339, -20.267981, 5.972309, 2.349017,
-      14.955121, 16.088521, 12.403337, 10.499706, 5.028656, -0.016243, 1.871970, 1.844360, 59.559);
+  for (const Params& p : kPM7) {
+    g_elem[p.z].p = p;
+    derive(g_elem[p.z]);
+  }
   std::memset(g_alpb, 0, sizeof(g_alpb));
   std::memset(g_xfac, 0, sizeof(g_xfac));
-  auto pair = [](int a, int b, double alpb, double xfac) {
-    g_alpb[a][b] = g_alpb[b][a] = alpb;
-    g_xfac[a][b] = g_xfac[b][a] = xfac;
-  };
-  pair(1, 1, 4.051163, 2.845627);
-  pair(6, 1, 1.038716, 0.204582);
-  pair(6, 6, 2.655746, 0.937818);
-  pair(7, 1, 1.049958, 0.173966);
-  pair(7, 6, 2.734636, 0.959163);
-  pair(7, 7, 2.786676, 0.863430);
-  pair(8, 1, 1.508030, 0.160602);
-  pair(8, 6, 2.850931, 0.826848);
-  pair(8, 7, 2.841313, 0.699190);
-  pair(8, 8, 2.457135, 0.356439);
+  for (const PairParams& q : kPM7Pairs) {
+    g_alpb[q.a][q.b] = g_alpb[q.b][q.a] = q.alpha;
+    g_xfac[q.a][q.b] = g_xfac[q.b][q.a] = q.x;
+  }
   g_init = true;
 }
 
 inline bool supported(int z) { return z == 1 || z == 6 || z == 7 || z == 8; }
 
-// PM7 smooth switch from NDDO to point-charge electrostatics: returns the NDDO
-// weight; `point` is the point-charge e-e value in eV.
-inline double feather(double r_ang, double& point) {
-  point = kEV * kA0 / r_ang;
-  if (r_ang < kTrunc1) return 1.0 - std::exp(-(r_ang - kTrunc1) * (r_ang - kTrunc1) * kTrunc2);
-  return 0.0;
+// PM7's switch from NDDO to point-charge electrostatics (to_point): the
+// weight c of the NDDO value at r (Angstrom) and the point-charge e-e value
+// 1/r (eV).
+inline double switch_weight(double r_ang, double& point_ev) {
+  point_ev = kEV * kA0 / r_ang;
+  return (r_ang < kSwitchR) ? 1.0 - std::exp(-(r_ang - kSwitchR) * (r_ang - kSwitchR) * kSwitchK) : 0.0;
 }
 
-// 22 unique two-centre two-electron integrals in the bond frame (eV), local
-// axis 1 along the bond, plus the core-core Coulomb function gab.
-void local_repulsion(const Elem& A, const Elem& B, double rij, double ri[22], double& gab) {
-  const double ev1 = kEV / 2, ev2 = ev1 / 2, ev3 = ev2 / 2, ev4 = ev3 / 2;
-  const double td = 2.0, pp = 0.5;
-  for (int k = 0; k < 22; ++k) ri[k] = 0.0;
-  const double r = rij / kA0;
-  const bool si = A.natorb >= 3, sj = B.natorb >= 3;
-  double aee = A.po9 + B.po9;
-  aee = aee * aee;
-  gab = kEV / std::sqrt(r * r + aee);
-  aee = pp / A.am + pp / B.am;
-  aee = aee * aee;
-  if (!si && !sj) {
-    ri[0] = kEV / std::sqrt(r * r + aee);
-  } else if (si && !sj) {
-    const double da = A.dd, qa = A.qq * td;
-    double ade = pp / A.ad + pp / B.am; ade *= ade;
-    double aqe = pp / A.aq + pp / B.am; aqe *= aqe;
-    const double rsq = r * r;
-    double arg[7], sq[7], x;
-    arg[0] = rsq + aee;
-    x = r + da; arg[1] = x * x + ade;
-    x = r - da; arg[2] = x * x + ade;
-    x = r + qa; arg[3] = x * x + aqe;
-    x = r - qa; arg[4] = x * x + aqe;
-    arg[5] = rsq + aqe;
-    arg[6] = arg[5] + qa * qa;
-    for (int k = 0; k < 7; ++k) sq[k] = std::sqrt(arg[k]);
-    const double ee = kEV / sq[0];
-    ri[0] = ee;
-    ri[1] = ev1 / sq[1] - ev1 / sq[2];
-    ri[2] = ee + ev2 / sq[3] + ev2 / sq[4] - ev1 / sq[5];
-    ri[3] = ee + ev1 / sq[6] - ev1 / sq[5];
-  } else if (!si && sj) {
-    const double db = B.dd, qb = B.qq * td;
-    double aed = pp / A.am + pp / B.ad; aed *= aed;
-    double aeq = pp / A.am + pp / B.aq; aeq *= aeq;
-    const double rsq = r * r;
-    double arg[7], sq[7], x;
-    arg[0] = rsq + aee;
-    x = r - db; arg[1] = x * x + aed;
-    x = r + db; arg[2] = x * x + aed;
-    x = r - qb; arg[3] = x * x + aeq;
-    x = r + qb; arg[4] = x * x + aeq;
-    arg[5] = rsq + aeq;
-    arg[6] = arg[5] + qb * qb;
-    for (int k = 0; k < 7; ++k) sq[k] = std::sqrt(arg[k]);
-    const double ee = kEV / sq[0];
-    ri[0] = ee;
-    ri[4] = ev1 / sq[1] - ev1 / sq[2];
-    ri[10] = ee + ev2 / sq[3] + ev2 / sq[4] - ev1 / sq[5];
-    ri[11] = ee + ev1 / sq[6] - ev1 / sq[5];
-  } else {
-    const double da = A.dd, db = B.dd;
-    double qa = A.qq * td, qb = B.qq * td;
-    double ade = pp / A.ad + pp / B.am; ade *= ade;
-    double aqe = pp / A.aq + pp / B.am; aqe *= aqe;
-    double aed = pp / A.am + pp / B.ad; aed *= aed;
-    double aeq = pp / A.am + pp / B.aq; aeq *= aeq;
-    double axx = pp / A.ad + pp / B.ad; axx *= axx;
-    double adq = pp / A.ad + pp / B.aq; adq *= adq;
-    double aqd = pp / A.aq + pp / B.ad; aqd *= aqd;
-    double aqq = pp / A.aq + pp / B.aq; aqq *= aqq;
-    const double rsq = r * r;
-    double a[73], s[73], x, y, z, w;
-    a[1] = rsq + aee;
-    x = r + da; a[2] = x * x + ade;
-    x = r - da; a[3] = x * x + ade;
-    x = r - qa; a[4] = x * x + aqe;
-    x = r + qa; a[5] = x * x + aqe;
-    a[6] = rsq + aqe;
-    a[7] = a[6] + qa * qa;
-    x = r - db; a[8] = x * x + aed;
-    x = r + db; a[9] = x * x + aed;
-    x = r - qb; a[10] = x * x + aeq;
-    x = r + qb; a[11] = x * x + aeq;
-    a[12] = rsq + aeq;
-    a[13] = a[12] + qb * qb;
-    x = da - db; a[14] = rsq + axx + x * x;
-    x = da + db; a[15] = rsq + axx + x * x;
-    x = r + da - db; a[16] = x * x + axx;
-    x = r - da + db; a[17] = x * x + axx;
-    x = r - da - db; a[18] = x * x + axx;
-    x = r + da + db; a[19] = x * x + axx;
-    x = r + da; a[20] = x * x + adq;
-    a[21] = a[20] + qb * qb;
-    x = r - da; a[22] = x * x + adq;
-    a[23] = a[22] + qb * qb;
-    x = r - db; a[24] = x * x + aqd;
-    a[25] = a[24] + qa * qa;
-    x = r + db; a[26] = x * x + aqd;
-    a[27] = a[26] + qa * qa;
-    x = r + da - qb; a[28] = x * x + adq;
-    x = r - da - qb; a[29] = x * x + adq;
-    x = r + da + qb; a[30] = x * x + adq;
-    x = r - da + qb; a[31] = x * x + adq;
-    x = r + qa - db; a[32] = x * x + aqd;
-    x = r + qa + db; a[33] = x * x + aqd;
-    x = r - qa - db; a[34] = x * x + aqd;
-    x = r - qa + db; a[35] = x * x + aqd;
-    a[36] = rsq + aqq;
-    x = qa - qb; a[37] = a[36] + x * x;
-    x = qa + qb; a[38] = a[36] + x * x;
-    a[39] = a[36] + qa * qa;
-    a[40] = a[36] + qb * qb;
-    a[41] = a[39] + qb * qb;
-    x = r - qb; a[42] = x * x + aqq;
-    a[43] = a[42] + qa * qa;
-    x = r + qb; a[44] = x * x + aqq;
-    a[45] = a[44] + qa * qa;
-    x = r + qa; a[46] = x * x + aqq;
-    a[47] = a[46] + qb * qb;
-    x = r - qa; a[48] = x * x + aqq;
-    a[49] = a[48] + qb * qb;
-    x = r + qa - qb; a[50] = x * x + aqq;
-    x = r + qa + qb; a[51] = x * x + aqq;
-    x = r - qa - qb; a[52] = x * x + aqq;
-    x = r - qa + qb; a[53] = x * x + aqq;
-    qa = A.qq;
-    qb = B.qq;
-    x = da - qb; x = x * x;
-    y = r - qb; y = y * y;
-    z = da + qb; z = z * z;
-    w = r + qb; w = w * w;
-    a[54] = x + y + adq;
-    a[55] = x + w + adq;
-    a[56] = z + y + adq;
-    a[57] = z + w + adq;
-    x = qa - db; x = x * x;
-    y = qa + db; y = y * y;
-    z = r + qa; z = z * z;
-    w = r - qa; w = w * w;
-    a[58] = z + x + aqd;
-    a[59] = w + x + aqd;
-    a[60] = z + y + aqd;
-    a[61] = w + y + aqd;
-    x = qa - qb; x = x * x;
-    a[62] = a[36] + td * x;
-    y = qa + qb; y = y * y;
-    a[63] = a[36] + td * y;
-    a[64] = a[36] + td * (qa * qa + qb * qb);
-    z = r + qa - qb; z = z * z;
-    a[65] = z + x + aqq;
-    a[66] = z + y + aqq;
-    z = r + qa + qb; z = z * z;
-    a[67] = z + x + aqq;
-    a[68] = z + y + aqq;
-    z = r - qa - qb; z = z * z;
-    a[69] = z + x + aqq;
-    a[70] = z + y + aqq;
-    z = r - qa + qb; z = z * z;
-    a[71] = z + x + aqq;
-    a[72] = z + y + aqq;
-    for (int k = 1; k <= 72; ++k) s[k] = std::sqrt(a[k]);
-    const double ee = kEV / s[1];
-    const double dze = -ev1 / s[2] + ev1 / s[3];
-    const double qzze = ev2 / s[4] + ev2 / s[5] - ev1 / s[6];
-    const double qxxe = ev1 / s[7] - ev1 / s[6];
-    const double edz = -ev1 / s[8] + ev1 / s[9];
-    const double eqzz = ev2 / s[10] + ev2 / s[11] - ev1 / s[12];
-    const double eqxx = ev1 / s[13] - ev1 / s[12];
-    const double dxdx = ev1 / s[14] - ev1 / s[15];
-    const double dzdz = ev2 / s[16] + ev2 / s[17] - ev2 / s[18] - ev2 / s[19];
-    const double dzqxx = ev2 / s[20] - ev2 / s[21] - ev2 / s[22] + ev2 / s[23];
-    const double qxxdz = ev2 / s[24] - ev2 / s[25] - ev2 / s[26] + ev2 / s[27];
-    const double dzqzz = -ev3 / s[28] + ev3 / s[29] - ev3 / s[30] + ev3 / s[31] - ev2 / s[22] + ev2 / s[20];
-    const double qzzdz = -ev3 / s[32] + ev3 / s[33] - ev3 / s[34] + ev3 / s[35] + ev2 / s[24] - ev2 / s[26];
-    const double qxxqxx = ev3 / s[37] + ev3 / s[38] - ev2 / s[39] - ev2 / s[40] + ev2 / s[36];
-    const double qxxqyy = ev2 / s[41] - ev2 / s[39] - ev2 / s[40] + ev2 / s[36];
-    const double qxxqzz = ev3 / s[43] + ev3 / s[45] - ev3 / s[42] - ev3 / s[44] - ev2 / s[39] + ev2 / s[36];
-    const double qzzqxx = ev3 / s[47] + ev3 / s[49] - ev3 / s[46] - ev3 / s[48] - ev2 / s[40] + ev2 / s[36];
-    const double qzzqzz = ev4 / s[50] + ev4 / s[51] + ev4 / s[52] + ev4 / s[53] - ev3 / s[48] -
-                          ev3 / s[46] - ev3 / s[42] - ev3 / s[44] + ev2 / s[36];
-    const double dxqxz = -ev2 / s[54] + ev2 / s[55] + ev2 / s[56] - ev2 / s[57];
-    const double qxzdx = -ev2 / s[58] + ev2 / s[59] + ev2 / s[60] - ev2 / s[61];
-    const double qxzqxz = ev3 / s[65] - ev3 / s[67] - ev3 / s[69] + ev3 / s[71] - ev3 / s[66] +
-                          ev3 / s[68] + ev3 / s[70] - ev3 / s[72];
-    ri[0] = ee;
-    ri[1] = -dze;
-    ri[2] = ee + qzze;
-    ri[3] = ee + qxxe;
-    ri[4] = -edz;
-    ri[5] = dzdz;
-    ri[6] = dxdx;
-    ri[7] = -edz - qzzdz;
-    ri[8] = -edz - qxxdz;
-    ri[9] = -qxzdx;
-    ri[10] = ee + eqzz;
-    ri[11] = ee + eqxx;
-    ri[12] = -dze - dzqzz;
-    ri[13] = -dze - dzqxx;
-    ri[14] = -dxqxz;
-    ri[15] = ee + eqzz + qzze + qzzqzz;
-    ri[16] = ee + eqzz + qxxe + qxxqzz;
-    ri[17] = ee + eqxx + qzze + qzzqxx;
-    ri[18] = ee + eqxx + qxxe + qxxqxx;
-    ri[19] = qxzqxz;
-    ri[20] = ee + eqxx + qxxe + qxxqyy;
-    ri[21] = pp * (qxxqxx - qxxqyy);
-  }
-  // PM7 feathering toward point charges (monopole-like terms blend, others fade)
+// Two-centre repulsion (eV) between distribution a of atom A (origin) and
+// distribution b of atom B at (0, 0, r bohr), before the point-charge switch.
+double repulsion(const Distribution& a, const Distribution& b, double r) {
+  double s = 0.0;
+  for (int i = 0; i < a.nm; ++i)
+    for (int j = 0; j < b.nm; ++j) s += interact(a.m[i], b.m[j], r);
+  return kEV * s;
+}
+
+// Bond-frame tensors of one atom pair: local two-electron integrals L over
+// unordered local pairs (A's pair down, B's across), electron-core
+// attractions of each atom's distributions in the other's core field, and
+// the core-core Coulomb function gab, all in eV with the point-charge switch
+// applied.  r_ang: distance in Angstrom.
+struct LocalPair {
+  double L[10][10];
+  double core_a[10], core_b[10];
+  double gab;
+};
+void local_pair(const Element& A, const Element& B, double r_ang, LocalPair& lp) {
+  std::memset(&lp, 0, sizeof(lp));
+  const double r = r_ang / kA0;
   double point;
-  const double c = feather(rij, point);
-  static const bool monopole[22] = {1, 0, 1, 1, 0, 0, 0, 0, 0, 0, 1,
-                                    1, 0, 0, 0, 1, 1, 1, 1, 0, 1, 0};
-  for (int k = 0; k < 22; ++k) ri[k] = monopole[k] ? ri[k] * c + (1.0 - c) * point : ri[k] * c;
-  gab = gab * c + (1.0 - c) * point;
-}
-
-// Electron-core attraction integrals in the bond frame: core[k][0] for the
-// electrons of A in the field of core B (ss, s-sigma, sigma-sigma, pi-pi),
-// core[k][1] for the electrons of B in the field of core A.  r in bohr.
-void local_core(const Elem& A, const Elem& B, double r, double core[4][2]) {
-  static const double pxy[7] = {1.0, -0.5, -0.5, 0.5, 0.25, 0.25, 0.5};
-  for (int k = 0; k < 4; ++k) core[k][0] = core[k][1] = 0.0;
-  const double r2 = r * r;
-  const double aci = A.po9, acj = B.po9;
-  const double ssi = (aci + B.po1) * (aci + B.po1);
-  const double ssj = (acj + A.po1) * (acj + A.po1);
-  core[0][0] = -B.tore * kEV / std::sqrt(r2 + ssj);
-  core[0][1] = -A.tore * kEV / std::sqrt(r2 + ssi);
-  if (A.z >= 3 || B.z >= 3) {
-    if (A.z >= 3) {
-      const double ppj = (acj + A.po7) * (acj + A.po7);
-      const double da = A.ddp2, qa = A.ddp3 / std::sqrt(2.0), twoqa = qa + qa;
-      const double adj = (A.po2 + acj) * (A.po2 + acj);
-      const double aqj = (A.po3 + acj) * (A.po3 + acj);
-      double x[7];
-      x[0] = r2 + ppj;
-      x[1] = r2 + aqj;
-      x[2] = (r + da) * (r + da) + adj;
-      x[3] = (r - da) * (r - da) + adj;
-      x[4] = (r - twoqa) * (r - twoqa) + aqj;
-      x[5] = (r + twoqa) * (r + twoqa) + aqj;
-      x[6] = r2 + twoqa * twoqa + aqj;
-      for (int k = 0; k < 7; ++k) x[k] = pxy[k] / std::sqrt(x[k]);
-      core[1][0] = -B.tore * (x[2] + x[3]) * kEV;
-      core[2][0] = -B.tore * (x[0] + x[1] + x[4] + x[5]) * kEV;
-      core[3][0] = -B.tore * (x[0] + x[1] + x[6]) * kEV;
+  const double c = switch_weight(r_ang, point);
+  const int la = A.natorb * (A.natorb + 1) / 2, lb = B.natorb * (B.natorb + 1) / 2;
+  double xx_xx = 0.0, xx_yy = 0.0;
+  for (int ab = 0; ab < la; ++ab)
+    for (int cd = 0; cd < lb; ++cd) {
+      const Distribution& da = A.dist[ab];
+      const Distribution& db = B.dist[cd];
+      if (da.parity != db.parity || da.axial != db.axial) continue;
+      if (da.axial) continue;  // filled below
+      const double v = repulsion(da, db, r);
+      lp.L[ab][cd] = da.monopole && db.monopole ? v * c + (1.0 - c) * point : v * c;
     }
-    if (B.z >= 3) {
-      const double ppi = (aci + B.po7) * (aci + B.po7);
-      const double db = B.ddp2, qb = B.ddp3 / std::sqrt(2.0), twoqb = qb + qb;
-      const double adi = (B.po2 + aci) * (B.po2 + aci);
-      const double aqi = (B.po3 + aci) * (B.po3 + aci);
-      double x[7];
-      x[0] = r2 + ppi;
-      x[1] = r2 + aqi;
-      x[2] = (r + db) * (r + db) + adi;
-      x[3] = (r - db) * (r - db) + adi;
-      x[4] = (r - twoqb) * (r - twoqb) + aqi;
-      x[5] = (r + twoqb) * (r + twoqb) + aqi;
-      x[6] = r2 + twoqb * twoqb + aqi;
-      for (int k = 0; k < 7; ++k) x[k] = pxy[k] / std::sqrt(x[k]);
-      core[1][1] = -A.tore * (-(x[2] + x[3]) * kEV);
-      core[2][1] = -A.tore * (x[0] + x[1] + x[4] + x[5]) * kEV;
-      core[3][1] = -A.tore * (x[0] + x[1] + x[6]) * kEV;
-    }
+  if (A.natorb > 1 && B.natorb > 1) {
+    // (pi pi'|pi pi') = [(pi pi|pi pi) - (pi pi|pi' pi')] / 2 about the bond axis
+    xx_xx = lp.L[pidx(2, 2)][pidx(2, 2)];
+    xx_yy = lp.L[pidx(2, 2)][pidx(3, 3)];
+    lp.L[pidx(3, 2)][pidx(3, 2)] = 0.5 * (xx_xx - xx_yy);
   }
+  // electron-core attraction: each core is a unit monopole of radius rho_core
+  const Multipole core_of_a = monopole(A.rho_core), core_of_b = monopole(B.rho_core);
+  const double point_b = -(kEV / r) * B.tore, point_a = -(kEV / r) * A.tore;
+  for (int ab = 0; ab < la; ++ab) {
+    const Distribution& d = A.dist[ab];
+    if (d.parity != 0 || d.axial || d.nm == 0) continue;
+    double s = 0.0;
+    for (int i = 0; i < d.nm; ++i) s += interact(d.m[i], core_of_b, r);
+    const double v = -B.tore * kEV * s;
+    lp.core_a[ab] = d.monopole ? v * c + (1.0 - c) * point_b : v * c;
+  }
+  for (int cd = 0; cd < lb; ++cd) {
+    const Distribution& d = B.dist[cd];
+    if (d.parity != 0 || d.axial || d.nm == 0) continue;
+    double s = 0.0;
+    for (int i = 0; i < d.nm; ++i) s += interact(core_of_a, d.m[i], r);
+    const double v = -A.tore * kEV * s;
+    lp.core_b[cd] = d.monopole ? v * c + (1.0 - c) * point_a : v * c;
+  }
+  const double rc = A.rho_core + B.rho_core;
+  const double gab = kEV / std::sqrt(r * r + rc * rc);
+  lp.gab = gab * c + (1.0 - c) * point;
 }
 
-// PM7 core-core repulsion (eV) for a pair at r bohr, given the feathered gab.
-double core_core(const Elem& A, const Elem& B, double r_bohr, double gab) {
+// PM7 core-core repulsion (eV) at r bohr, given the switched gab: the
+// screened Coulomb term times the diatomic scale factor, the elements'
+// Gaussians, and the short-range r^-12 wall.
+double core_core(const Element& A, const Element& B, double r_bohr, double gab) {
   const double r = r_bohr * kA0;
-  const int ni = A.z, nj = B.z;
-  const double enuc = A.tore * B.tore * gab;
-  double fff = g_xfac[ni][nj];
-  double abond = g_alpb[ni][nj];
-  if (abond < 1e-6) abond = 1.2;
-  double scale = 1.0 + 2.0 * fff * std::exp(-abond * (r + 0.0003 * std::pow(r, 6)));
-  const int hi = std::max(ni, nj), lo = std::min(ni, nj);
-  if (lo == 1) {
-    if (hi == 6 || hi == 7) scale = 1.0 + 2.0 * fff * std::exp(-abond * r * r);
-    if (hi == 8)
-      scale = 1.0 + 2.0 * fff * std::exp(-abond * r * r) - kPar3 * std::exp(-kPar4 * r * 2);
-  } else if (lo == 6 && hi == 6) {
-    scale = scale + kPar1 * std::exp(-kPar2 * r);
+  const int za = A.z, zb = B.z, hi = std::max(za, zb), lo = std::min(za, zb);
+  const double zz = A.tore * B.tore;
+  const double x = g_xfac[za][zb];
+  const double alpha = g_alpb[za][zb] < 1e-6 ? 1.2 : g_alpb[za][zb];
+  double scale;
+  if (lo == 1 && (hi == 6 || hi == 7)) {
+    scale = 1.0 + 2.0 * x * std::exp(-alpha * r * r);
+  } else if (lo == 1 && hi == 8) {
+    scale = 1.0 + 2.0 * x * std::exp(-alpha * r * r) - kOHScale * std::exp(-kOHExp * r * 2);
+  } else {
+    scale = 1.0 + 2.0 * x * std::exp(-alpha * (r + kR6 * std::pow(r, 6)));
+    if (lo == 6 && hi == 6) scale += kCCScale * std::exp(-kCCExp * r);
   }
-  double e = enuc * scale;
-  double g = 0.0;
-  double ax = A.g2 * (r - A.g3) * (r - A.g3);
-  if (ax < 25.0) g += A.tore * B.tore / r * A.g1 * std::exp(-ax);
-  ax = B.g2 * (r - B.g3) * (r - B.g3);
-  if (ax < 25.0) g += A.tore * B.tore / r * B.g1 * std::exp(-ax);
-  e += g;
-  ax = r / (std::pow((double)ni, 0.3333) + std::pow((double)nj, 0.3333));
-  if (ax < 3.0) e += std::min(1.0e-8 / std::pow(ax, 12), 1.0e5);
+  double e = zz * gab * scale;
+  double gauss = 0.0;
+  for (const Element* E : {&A, &B}) {
+    const double ax = E->p.gl * (r - E->p.gm) * (r - E->p.gm);
+    if (ax < 25.0) gauss += zz / r * E->p.gk * std::exp(-ax);
+  }
+  e += gauss;
+  const double rr = r / (std::pow((double)za, 0.3333) + std::pow((double)zb, 0.3333));
+  if (rr < 3.0) e += std::min(1.0e-8 / std::pow(rr, 12), 1.0e5);
   return e;
 }
 
-// ---- Slater overlap integrals over s/p shells (first-row elements) ----------
-constexpr double kFact[18] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0, 40320.0,
-                              362880.0, 3628800.0, 39916800.0, 479001600.0, 6227020800.0,
-                              87178291200.0, 1307674368000.0, 20922789888000.0,
-                              355687428096000.0};
-
-struct OvState {
-  int isp = 1, ips = 2;
-  double sa = 0, sb = 0;
-  double a[8], b[8];
+// ---- Slater overlaps in prolate spheroidal coordinates ----------------------
+// r_A = (R/2)(xi + eta), r_B = (R/2)(xi - eta), z_A = (R/2)(1 + xi eta),
+// z_B = (R/2)(xi eta - 1) (z along A -> B), rho^2 = (R/2)^2 (xi^2-1)(1-eta^2),
+// dV = (R/2)^3 (xi^2 - eta^2) dxi deta dphi.  Polynomials in (xi, eta) carry
+// the (R/2) powers separately.
+struct Poly {
+  static constexpr int N = 9;
+  double c[N][N];  // c[i][j] xi^i eta^j
+  Poly() { std::memset(c, 0, sizeof(c)); }
 };
-
-void a_ints(double x, int k, double* a) {
-  const double c = std::exp(-x);
-  a[0] = c / x;
-  for (int i = 1; i <= k; ++i) a[i] = (a[i - 1] * i + c) / x;
-}
-
-void b_ints(double x, int k, double* b) {
-  const double absx = std::fabs(x);
-  int last = 0;
-  bool expl = false, series = false;
-  if (absx > 3.0) expl = true;
-  else if (absx > 2.0) { if (k <= 10) expl = true; else { last = 15; series = true; } }
-  else if (absx > 1.0) { if (k <= 7) expl = true; else { last = 12; series = true; } }
-  else if (absx > 0.5) { if (k <= 5) expl = true; else { last = 7; series = true; } }
-  else if (absx <= 1e-6) { }
-  else { last = 6; series = true; }
-  if (expl) {
-    const double ex = std::exp(x), emx = 1.0 / ex;
-    b[0] = (ex - emx) / x;
-    for (int i = 1; i <= k; ++i) b[i] = (i * b[i - 1] + ((i % 2) ? -1.0 : 1.0) * ex - emx) / x;
-  } else if (series) {
-    for (int i = 0; i <= k; ++i) {
-      double y = 0.0;
-      for (int m = 0; m <= last; ++m) {
-        const double xf = (m != 0) ? kFact[m] : 1.0;
-        double pw = 1.0;
-        for (int t = 0; t < m; ++t) pw *= -x;
-        y += pw * (2 * ((m + i + 1) % 2)) / (xf * (m + i + 1));
-      }
-      b[i] = y;
+Poly poly_mul(const Poly& a, const Poly& b) {
+  Poly r;
+  for (int i = 0; i < Poly::N; ++i)
+    for (int j = 0; j < Poly::N; ++j) {
+      if (a.c[i][j] == 0.0) continue;
+      for (int k = 0; k + i < Poly::N; ++k)
+        for (int l = 0; l + j < Poly::N; ++l) r.c[i + k][j + l] += a.c[i][j] * b.c[k][l];
     }
-  } else {
-    for (int i = 0; i <= k; ++i) b[i] = (2 * ((i + 1) % 2)) / (i + 1.0);
+  return r;
+}
+Poly poly_of(std::initializer_list<std::initializer_list<double>> terms) {
+  // each term {coefficient, xi power, eta power}
+  Poly p;
+  for (auto t : terms) {
+    auto it = t.begin();
+    const double c = *it++;
+    const int i = (int)*it++, j = (int)*it;
+    p.c[i][j] += c;
   }
+  return p;
 }
 
-void ov_set(OvState& st, double s1, double s2, int na, int nb, double rab, int ii) {
-  if (na <= nb) { st.isp = 1; st.ips = 2; st.sa = s1; st.sb = s2; }
-  else { st.isp = 2; st.ips = 1; st.sa = s2; st.sb = s1; }
-  int j = ii + 2;
-  if (ii > 3) j -= 1;
-  const double alpha = 0.5 * rab * (st.sa + st.sb);
-  const double beta = 0.5 * rab * (st.sb - st.sa);
-  a_ints(alpha, j - 1, st.a);
-  b_ints(beta, j - 1, st.b);
+// A_k(p) = int_1^inf xi^k e^(-p xi) dxi, k = 0..kmax (p > 0): the closed
+// recurrence A_k = (k A_{k-1} + e^-p) / p, stable upward.
+void aux_a(double p, int kmax, double* a) {
+  const double e = std::exp(-p);
+  a[0] = e / p;
+  for (int k = 1; k <= kmax; ++k) a[k] = (k * a[k - 1] + e) / p;
 }
-
-// Overlaps in the bond frame s[La][Lb][M] (La, Lb: 0=s, 1=p; M: 0=sigma, 1=pi).
-void local_overlap(const Elem& A, const Elem& B, double r_ang, double s[2][2][2]) {
-  for (int i = 0; i < 8; ++i) (&s[0][0][0])[i] = 0.0;
-  auto cls = [](int z) { return z == 1 ? 1 : (z == 6 ? 4 : (z == 7 ? 5 : 6)); };
-  const int jmax = std::max(cls(A.z), cls(B.z)), jmin = std::min(cls(A.z), cls(B.z));
-  const int ii = (jmax == 1) ? 1 : (jmin == 1 ? 2 : 4);
-  const int na = A.z, nb = B.z;
-  const double esa = A.zs, epa = A.zp, esb = B.zs, epb = B.zp;
-  const double rab = r_ang / kA0;
-  OvState st;
-  const double* a = st.a;
-  const double* b = st.b;
-  auto S = [&](int i, int j, int m) -> double& { return s[i - 1][j - 1][m - 1]; };
-  if (ii == 1) {
-    ov_set(st, esa, esb, na, nb, rab, ii);
-    S(1, 1, 1) = 0.25 * std::sqrt(std::pow(st.sa * st.sb * rab * rab, 3)) *
-                 (a[2] * b[0] - b[2] * a[0]);
-  } else if (ii == 2) {
-    ov_set(st, esa, esb, na, nb, rab, ii);
-    const double rab4 = std::pow(rab, 4) * 0.125;
-    double w = std::sqrt(std::pow(st.sa, 3) * std::pow(st.sb, 5)) * rab4;
-    S(1, 1, 1) = std::sqrt(1.0 / 3.0) * w * (a[3] * b[0] - b[3] * a[0] + a[2] * b[1] - b[2] * a[1]);
-    if (na > 1) ov_set(st, epa, esb, na, nb, rab, ii);
-    if (nb > 1) ov_set(st, esa, epb, na, nb, rab, ii);
-    w = std::sqrt(std::pow(st.sa, 3) * std::pow(st.sb, 5)) * rab4;
-    S(st.isp, st.ips, 1) = w * (a[2] * b[0] - b[2] * a[0] + a[3] * b[1] - b[3] * a[1]);
-  } else {
-    ov_set(st, esa, esb, na, nb, rab, ii);
-    const double rab4 = std::pow(rab, 5) * 0.0625;
-    double w = std::sqrt(std::pow(st.sa * st.sb, 5)) * rab4;
-    S(1, 1, 1) = w * (a[4] * b[0] + b[4] * a[0] - 2.0 * a[2] * b[2]) / 3.0;
-    ov_set(st, esa, epb, na, nb, rab, ii);
-    if (na > nb) ov_set(st, epa, esb, na, nb, rab, ii);
-    w = std::sqrt(std::pow(st.sa * st.sb, 5)) * rab4;
-    const double rt3 = 1.0 / std::sqrt(3.0);
-    double d = a[3] * (b[0] - b[2]) - a[1] * (b[2] - b[4]);
-    double e = b[3] * (a[0] - a[2]) - b[1] * (a[2] - a[4]);
-    S(st.isp, st.ips, 1) = w * rt3 * (d + e);
-    ov_set(st, epa, esb, na, nb, rab, ii);
-    if (na > nb) ov_set(st, esa, epb, na, nb, rab, ii);
-    w = std::sqrt(std::pow(st.sa * st.sb, 5)) * rab4;
-    d = a[3] * (b[0] - b[2]) - a[1] * (b[2] - b[4]);
-    e = b[3] * (a[0] - a[2]) - b[1] * (a[2] - a[4]);
-    S(st.ips, st.isp, 1) = w * rt3 * (d - e);
-    ov_set(st, epa, epb, na, nb, rab, ii);
-    w = std::sqrt(std::pow(st.sa * st.sb, 5)) * rab4;
-    S(2, 2, 1) = -w * (b[2] * (a[4] + a[0]) - a[2] * (b[4] + b[0]));
-    S(2, 2, 2) = 0.5 * w * (a[4] * (b[0] - b[2]) - b[4] * (a[0] - a[2]) - a[2] * b[0] + b[2] * a[0]);
+// B_k(t) = int_-1^1 eta^k e^(-t eta) deta, k = 0..kmax.  Upward recurrence
+// B_k = (k B_{k-1} + (-1)^k e^t - e^-t) / t where it is stable, otherwise
+// the power series in t, whose length for each |t| band is the one the
+// reference's overlap code uses (src/integrals/set.F90, bintgs): its overlaps
+// carry that series' truncation (up to ~1e-7 relative for |t| near 0.5), and
+// parity needs the same sums.
+void aux_b(double t, int kmax, double* b) {
+  const double at = std::fabs(t);
+  // (|t| upper bound, largest k served by the recurrence, series terms beyond it)
+  struct Band {
+    double tmax;
+    int kmax_rec, last;
+  };
+  static const Band bands[] = {{0.5, -1, 6}, {1.0, 5, 7}, {2.0, 7, 12}, {3.0, 10, 15}};
+  int kmax_rec = 1 << 20, last = 0;
+  if (at <= 1.0e-6) {
+    for (int k = 0; k <= kmax; ++k) b[k] = (k % 2 == 0) ? 2.0 / (k + 1.0) : 0.0;
+    return;
   }
-}
-
-// Molecular-frame overlap block di[a][b] (a on atom A, b on atom B; s,px,py,pz)
-// for the vector v = xB - xA (Angstrom).  Returns false when beyond the cutoff.
-void molecular_overlap(const Elem& A, const Elem& B, const double v[3], double di[4][4]) {
-  for (int i = 0; i < 16; ++i) (&di[0][0])[i] = 0.0;
-  const double x2 = v[0], y2 = v[1], z2 = v[2];
-  const double rsq = x2 * x2 + y2 * y2 + z2 * z2;
-  if (rsq >= 225.0) return;  // overlap cutoff (15 A)
-  double xy = x2 * x2 + y2 * y2;
-  const double r = std::sqrt(xy + z2 * z2);
-  xy = std::sqrt(xy);
-  double ca, cb, sa, sb;
-  if (xy >= 1e-10) {
-    ca = x2 / xy; cb = z2 / r; sa = y2 / xy; sb = xy / r;
-  } else if (z2 < 0.0) {
-    ca = -1.0; cb = -1.0; sa = 0.0; sb = 0.0;
-  } else if (z2 == 0.0) {
-    ca = cb = sa = sb = 0.0;
-  } else {
-    ca = 1.0; cb = 1.0; sa = 0.0; sb = 0.0;
-  }
-  if (r < 0.001) return;
-  double s[2][2][2];
-  local_overlap(A, B, r, s);
-  // Rotation coefficients for p: sigma (c3), pi (c4), pi' (c2), per molecular
-  // orbital x, y, z (orbital slots 1, 2, 3).
-  const double c3[3] = {ca * sb, sa * sb, cb};
-  const double c4[3] = {ca * cb, sa * cb, -sb};
-  const double c2[3] = {-sa, ca, 0.0};
-  const int la = A.natorb > 1 ? 2 : 1, lb = B.natorb > 1 ? 2 : 1;
-  for (int i = 0; i < la; ++i) {
-    for (int j = 0; j < lb; ++j) {
-      const double aa = (j == 1) ? -1.0 : 1.0;
-      const double bb = 1.0;
-      const int na_ = (i == 0) ? 1 : 3, nb_ = (j == 0) ? 1 : 3;
-      for (int k = 0; k < na_; ++k) {
-        for (int l = 0; l < nb_; ++l) {
-          const double s3a = (i == 0) ? 1.0 : c3[k], s3b = (j == 0) ? 1.0 : c3[l];
-          const double p4a = (i == 0) ? 0.0 : c4[k], p4b = (j == 0) ? 0.0 : c4[l];
-          const double p2a = (i == 0) ? 0.0 : c2[k], p2b = (j == 0) ? 0.0 : c2[l];
-          const int ia = (i == 0) ? 0 : 1 + k, jb = (j == 0) ? 0 : 1 + l;
-          di[ia][jb] = s[i][j][0] * (s3a * s3b) * aa + s[i][j][1] * (p4a * p4b + p2a * p2b) * bb;
-        }
-      }
+  for (const Band& band : bands)
+    if (at <= band.tmax) {
+      kmax_rec = band.kmax_rec;
+      last = band.last;
+      break;
     }
+  if (kmax <= kmax_rec) {
+    const double ep = std::exp(t), em = 1.0 / ep;
+    b[0] = (ep - em) / t;
+    for (int k = 1; k <= kmax; ++k) b[k] = (k * b[k - 1] + ((k % 2) ? -ep : ep) - em) / t;
+    return;
+  }
+  for (int k = 0; k <= kmax; ++k) {
+    // sum_{m=0..last} (-t)^m / m! * int eta^(k+m) (zero for odd k + m)
+    double s = 0.0, term = 1.0;
+    for (int m = 0; m <= last; ++m) {
+      if (m > 0) term *= -t / m;
+      if ((k + m) % 2 == 0) s += term * 2.0 / (k + m + 1.0);
+    }
+    b[k] = s;
   }
 }
 
-// Index of the unordered orbital pair (i >= j, 0-based, s,px,py,pz) in the
-// packed 10-element pair list: (s,s) (px,s) (px,px) (py,s) (py,px) ...
-inline int pidx(int i, int j) {
-  if (i < j) std::swap(i, j);
-  return i * (i + 1) / 2 + j;
+// Overlap of two Slater orbitals in the bond frame: orbital a on A (principal
+// number na, l = la, zeta za), orbital b on B; m = 0 (sigma) or 1 (pi, both
+// along the same perpendicular axis).  R in bohr.
+double sto_overlap(int na, int la, double za, int nb, int lb, double zb, int m, double R) {
+  const Poly rA = poly_of({{1, 1, 0}, {1, 0, 1}});   // xi + eta
+  const Poly rB = poly_of({{1, 1, 0}, {-1, 0, 1}});  // xi - eta
+  const Poly zA = poly_of({{1, 0, 0}, {1, 1, 1}});   // 1 + xi eta
+  const Poly zB = poly_of({{-1, 0, 0}, {1, 1, 1}});  // xi eta - 1
+  const Poly rho2 = poly_of({{-1, 0, 0}, {1, 2, 0}, {1, 0, 2}, {-1, 2, 2}});  // (xi^2-1)(1-eta^2)
+  Poly f = poly_of({{1, 2, 0}, {-1, 0, 2}});  // xi^2 - eta^2
+  int rpow = 3;
+  for (int k = 0; k < na - 1 - la; ++k) f = poly_mul(f, rA), ++rpow;
+  for (int k = 0; k < nb - 1 - lb; ++k) f = poly_mul(f, rB), ++rpow;
+  double ang;  // angular normalization times the phi integral
+  if (m == 0) {
+    if (la == 1) f = poly_mul(f, zA), ++rpow;
+    if (lb == 1) f = poly_mul(f, zB), ++rpow;
+    ang = 2.0 * kPi * (la == 1 ? std::sqrt(3.0 / (4.0 * kPi)) : 1.0 / std::sqrt(4.0 * kPi)) *
+          (lb == 1 ? std::sqrt(3.0 / (4.0 * kPi)) : 1.0 / std::sqrt(4.0 * kPi));
+  } else {
+    f = poly_mul(f, rho2);
+    rpow += 2;
+    ang = kPi * 3.0 / (4.0 * kPi);
+  }
+  auto fact = [](int n) { double r = 1.0; for (int i = 2; i <= n; ++i) r *= i; return r; };
+  const double norm = std::pow(2.0 * za, na + 0.5) / std::sqrt(fact(2 * na)) *
+                      std::pow(2.0 * zb, nb + 0.5) / std::sqrt(fact(2 * nb));
+  // the auxiliary integrals up to the polynomial's own degrees (the band
+  // choice in aux_b depends on the highest order asked for)
+  int imax = 0, jmax = 0;
+  for (int i = 0; i < Poly::N; ++i)
+    for (int j = 0; j < Poly::N; ++j)
+      if (f.c[i][j] != 0.0) imax = std::max(imax, i), jmax = std::max(jmax, j);
+  const double p = 0.5 * R * (za + zb), t = 0.5 * R * (za - zb);
+  double A[Poly::N], B[Poly::N];
+  aux_a(p, imax, A);
+  aux_b(t, jmax, B);
+  double s = 0.0;
+  for (int i = 0; i <= imax; ++i)
+    for (int j = 0; j <= jmax; ++j)
+      if (f.c[i][j] != 0.0) s += f.c[i][j] * A[i] * B[j];
+  return norm * ang * std::pow(0.5 * R, rpow) * s;
+}
+
+// Bond frame of the pair A -> B: rows e[0] (sigma, toward B), e[1] (pi),
+// e[2] (pi'), a right-handed orthonormal set.  d = xB - xA.
+void bond_frame(const double d[3], double r, double e[3][3]) {
+  for (int k = 0; k < 3; ++k) e[0][k] = d[k] / r;
+  // pi: the coordinate axis least aligned with the bond, made orthogonal to it
+  int ax = 0;
+  for (int k = 1; k < 3; ++k)
+    if (std::fabs(e[0][k]) < std::fabs(e[0][ax])) ax = k;
+  double v[3] = {0, 0, 0};
+  v[ax] = 1.0;
+  const double dot = e[0][ax];
+  for (int k = 0; k < 3; ++k) v[k] -= dot * e[0][k];
+  const double nv = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  for (int k = 0; k < 3; ++k) e[1][k] = v[k] / nv;
+  e[2][0] = e[0][1] * e[1][2] - e[0][2] * e[1][1];
+  e[2][1] = e[0][2] * e[1][0] - e[0][0] * e[1][2];
+  e[2][2] = e[0][0] * e[1][1] - e[0][1] * e[1][0];
 }
 
 struct PairOut {
@@ -653,94 +578,25 @@ struct PairOut {
 };
 
 // Everything one atom pair contributes: W block, both core-attraction blocks,
-// the resonance block and the core-core term.  Atom A has the larger index.
-void pair_integrals(const Elem& A, const Elem& B, const double* xa, const double* xb,
-                    PairOut& out) {
+// the resonance block and the core-core term (rotate + h1elec).  Atom A has
+// the larger index (hcore's i).
+void pair_integrals(const Element& A, const Element& B, const double* xa, const double* xb, PairOut& out) {
   const int li = A.natorb, lj = B.natorb;
   const int limij = li * (li + 1) / 2, limkl = lj * (lj + 1) / 2;
   std::memset(&out, 0, sizeof(out));
   const double d[3] = {xb[0] - xa[0], xb[1] - xa[1], xb[2] - xa[2]};
   const double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-  // Resonance block (computed even when the two-electron part is skipped).
-  if (rsq <= 1.0e4) {
-    double di[4][4];
-    molecular_overlap(A, B, d, di);
-    const double bi[4] = {A.betas * 0.5, A.betap * 0.5, A.betap * 0.5, A.betap * 0.5};
-    const double bj[4] = {B.betas * 0.5, B.betap * 0.5, B.betap * 0.5, B.betap * 0.5};
-    for (int a = 0; a < li; ++a)
-      for (int b = 0; b < lj; ++b) out.di[a][b] = di[a][b] * (bi[a] + bj[b]);
-  }
-  if (rsq < 0.00002) return;
-  // Bond frame: row 0 along A->B, rows 1 and 2 perpendicular.
-  double p[3][3];
-  {
-    const double x11 = d[0], x22 = d[1], x33 = d[2];
-    const double bxy = x11 * x11 + x22 * x22;
-    const double r = std::sqrt(bxy + x33 * x33);
-    const double sqb = std::sqrt(bxy);
-    double sb = sqb / r, ca, sa, cb;
-    if (sb > 1.0e-7) {
-      ca = x11 / sqb; sa = x22 / sqb; cb = x33 / r;
-    } else {
-      sa = 0.0; sb = 0.0;
-      if (x33 < 0.0) { ca = -1.0; cb = -1.0; }
-      else if (x33 > 0.0) { ca = 1.0; cb = 1.0; }
-      else { ca = 0.0; cb = 0.0; }
-    }
-    p[0][0] = ca * sb; p[0][1] = sa * sb; p[0][2] = cb;
-    p[1][0] = ca * cb; p[1][1] = sa * cb; p[1][2] = -sb;
-    p[2][0] = -sa;     p[2][1] = ca;      p[2][2] = 0.0;
-  }
-  const double rij = std::sqrt(rsq);
-  double ri[22], gab;
-  local_repulsion(A, B, rij, ri, gab);
-  const double rb = rij / kA0;
-  double core[4][2];
-  local_core(A, B, rb, core);
-  {
-    double point;
-    const double c = feather(rij, point);
-    const double pj = -(kEV / rb) * B.tore, pi = -(kEV / rb) * A.tore;
-    core[0][0] = core[0][0] * c + (1.0 - c) * pj;
-    core[1][0] = core[1][0] * c;
-    core[2][0] = core[2][0] * c + (1.0 - c) * pj;
-    core[3][0] = core[3][0] * c + (1.0 - c) * pj;
-    core[0][1] = core[0][1] * c + (1.0 - c) * pi;
-    core[1][1] = core[1][1] * c;
-    core[2][1] = core[2][1] * c + (1.0 - c) * pi;
-    core[3][1] = core[3][1] * c + (1.0 - c) * pi;
-  }
-  // Local (bond-frame) two-electron tensor over pairs: L[pairA][pairB], local
-  // orbitals 0=s, 1=sigma, 2=pi, 3=pi'.
-  double L[10][10];
-  std::memset(L, 0, sizeof(L));
-  auto setL = [&](int a, int b, int c, int dd, double v) { L[pidx(a, b)][pidx(c, dd)] = v; };
-  setL(0, 0, 0, 0, ri[0]);
-  setL(0, 0, 1, 0, ri[4]);  setL(0, 0, 1, 1, ri[10]);
-  setL(0, 0, 2, 2, ri[11]); setL(0, 0, 3, 3, ri[11]);
-  setL(1, 0, 0, 0, ri[1]);  setL(1, 0, 1, 0, ri[5]);  setL(1, 0, 1, 1, ri[12]);
-  setL(1, 0, 2, 2, ri[13]); setL(1, 0, 3, 3, ri[13]);
-  setL(1, 1, 0, 0, ri[2]);  setL(1, 1, 1, 0, ri[7]);  setL(1, 1, 1, 1, ri[15]);
-  setL(1, 1, 2, 2, ri[17]); setL(1, 1, 3, 3, ri[17]);
-  setL(2, 0, 2, 0, ri[6]);  setL(2, 0, 2, 1, ri[14]); setL(2, 1, 2, 0, ri[9]);
-  setL(2, 1, 2, 1, ri[19]);
-  setL(2, 2, 0, 0, ri[3]);  setL(2, 2, 1, 0, ri[8]);  setL(2, 2, 1, 1, ri[16]);
-  setL(2, 2, 2, 2, ri[18]); setL(2, 2, 3, 3, ri[20]);
-  setL(3, 0, 3, 0, ri[6]);  setL(3, 0, 3, 1, ri[14]); setL(3, 1, 3, 0, ri[9]);
-  setL(3, 1, 3, 1, ri[19]); setL(3, 2, 3, 2, ri[21]);
-  setL(3, 3, 0, 0, ri[3]);  setL(3, 3, 1, 0, ri[8]);  setL(3, 3, 1, 1, ri[16]);
-  setL(3, 3, 2, 2, ri[20]); setL(3, 3, 3, 3, ri[18]);
-  // Molecular orbital mu = sum_a U[mu][a] * local a.  The multipole
-  // integrals above place the sigma axis from B toward A (the dipole charge
-  // at r+D faces away from B), so the sigma column enters with the opposite
-  // sign to the A->B frame used for the core and overlap terms.  This
-  // convention is pinned by the reference's CO2 eigenvalues and H2O charges.
+  if (rsq < 0.00002) return;  // coincident atoms: rotate returns zeros
+  const double r_ang = std::sqrt(rsq), r = r_ang / kA0;
+  double e[3][3];
+  bond_frame(d, r_ang, e);
+  // U[mu][a]: molecular orbital mu (s, x, y, z) in the local orbitals a
+  // (s, sigma, pi, pi') of the same atom.
   double U[4][4] = {{1, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
-  for (int m = 0; m < 3; ++m)
-    for (int k = 0; k < 3; ++k) U[1 + m][1 + k] = (k == 0 ? -1.0 : 1.0) * p[k][m];
-  // Pair transform T[molpair][localpair] over unordered pairs (with the
-  // off-diagonal local pair appearing twice).
-  double TA[10][10], TB[10][10];
+  for (int k = 0; k < 3; ++k)
+    for (int a = 0; a < 3; ++a) U[1 + k][1 + a] = e[a][k];
+  // Pair transform T[molecular pair][local pair] (an off-diagonal local pair
+  // collects both orders).
   auto build_T = [&](int norb, double T[10][10]) {
     std::memset(T, 0, sizeof(double) * 100);
     for (int mu = 0; mu < norb; ++mu)
@@ -748,14 +604,38 @@ void pair_integrals(const Elem& A, const Elem& B, const double* xa, const double
         for (int a = 0; a < norb; ++a)
           for (int b = 0; b < norb; ++b) T[pidx(mu, nu)][pidx(a, b)] += U[mu][a] * U[nu][b];
   };
+  double TA[10][10], TB[10][10];
   build_T(li, TA);
   build_T(lj, TB);
-  // (mu nu | lam sig) = sum TA[mn][ab] L[ab][cd] TB[ls][cd]
+  // Resonance block: overlap in the molecular frame times (beta_a + beta_b)/2.
+  if (rsq < 225.0) {  // overlaps beyond 15 A are taken as zero
+    const int nA = A.nprin, nB = B.nprin;
+    double S[4][4] = {};  // local: [a on A][b on B]
+    S[0][0] = sto_overlap(nA, 0, A.p.zs, nB, 0, B.p.zs, 0, r);
+    if (lj > 1) S[0][1] = sto_overlap(nA, 0, A.p.zs, nB, 1, B.p.zp, 0, r);
+    if (li > 1) S[1][0] = sto_overlap(nA, 1, A.p.zp, nB, 0, B.p.zs, 0, r);
+    if (li > 1 && lj > 1) {
+      S[1][1] = sto_overlap(nA, 1, A.p.zp, nB, 1, B.p.zp, 0, r);
+      S[2][2] = S[3][3] = sto_overlap(nA, 1, A.p.zp, nB, 1, B.p.zp, 1, r);
+    }
+    const double ba[4] = {0.5 * A.p.betas, 0.5 * A.p.betap, 0.5 * A.p.betap, 0.5 * A.p.betap};
+    const double bb[4] = {0.5 * B.p.betas, 0.5 * B.p.betap, 0.5 * B.p.betap, 0.5 * B.p.betap};
+    for (int mu = 0; mu < li; ++mu)
+      for (int nu = 0; nu < lj; ++nu) {
+        double s = 0.0;
+        for (int a = 0; a < li; ++a)
+          for (int b = 0; b < lj; ++b) s += U[mu][a] * U[nu][b] * S[a][b];
+        out.di[mu][nu] = s * (ba[mu] + bb[nu]);
+      }
+  }
+  LocalPair lp;
+  local_pair(A, B, r_ang, lp);
+  // (mu nu | lam sig) = sum_ab,cd TA[mn][ab] L[ab][cd] TB[ls][cd]
   double M[10][10];
   for (int mn = 0; mn < limij; ++mn)
     for (int cd = 0; cd < limkl; ++cd) {
       double acc = 0.0;
-      for (int ab = 0; ab < limij; ++ab) acc += TA[mn][ab] * L[ab][cd];
+      for (int ab = 0; ab < limij; ++ab) acc += TA[mn][ab] * lp.L[ab][cd];
       M[mn][cd] = acc;
     }
   for (int mn = 0; mn < limij; ++mn)
@@ -764,38 +644,46 @@ void pair_integrals(const Elem& A, const Elem& B, const double* xa, const double
       for (int cd = 0; cd < limkl; ++cd) acc += M[mn][cd] * TB[ls][cd];
       out.w[mn * limkl + ls] = acc;
     }
-  // Electron-core attraction blocks in the molecular frame.
-  auto core_block = [&](int norb, int col, double* e) {
-    for (int mu = 0; mu < norb; ++mu)
-      for (int nu = 0; nu <= mu; ++nu) {
-        double v;
-        if (mu == 0) v = core[0][col];
-        else if (nu == 0) v = p[0][mu - 1] * core[1][col];
-        else
-          v = core[2][col] * p[0][mu - 1] * p[0][nu - 1] +
-              core[3][col] * (p[1][mu - 1] * p[1][nu - 1] + p[2][mu - 1] * p[2][nu - 1]);
-        e[pidx(mu, nu)] = v;
-      }
-  };
-  core_block(li, 0, out.e1b);
-  core_block(lj, 1, out.e2a);
-  out.enuc = core_core(A, B, rb, gab);
+  for (int mn = 0; mn < limij; ++mn) {
+    double acc = 0.0;
+    for (int ab = 0; ab < limij; ++ab) acc += TA[mn][ab] * lp.core_a[ab];
+    out.e1b[mn] = acc;
+  }
+  for (int ls = 0; ls < limkl; ++ls) {
+    double acc = 0.0;
+    for (int cd = 0; cd < limkl; ++cd) acc += TB[ls][cd] * lp.core_b[cd];
+    out.e2a[ls] = acc;
+  }
+  out.enuc = core_core(A, B, r, lp.gab);
 }
 
-void one_center_block(const Elem& E, double* w) {
+// Value of the (ss|ss), (s sigma|ss) and (ss|s sigma) repulsions of a pair
+// (eV, switched), sigma pointing from A toward B; for MOZYME's outer2 shell.
+void pair_dipole_terms(const Element& A, const Element& B, double r_ang, double& ss, double& sp_a,
+                       double& sp_b) {
+  double point;
+  const double c = switch_weight(r_ang, point);
+  const double r = r_ang / kA0;
+  ss = repulsion(A.dist[0], B.dist[0], r) * c + (1.0 - c) * point;
+  sp_a = (A.natorb > 1) ? repulsion(A.dist[pidx(1, 0)], B.dist[0], r) * c : 0.0;
+  sp_b = (B.natorb > 1) ? repulsion(A.dist[0], B.dist[pidx(1, 0)], r) * c : 0.0;
+}
+
+void one_center_block(const Element& E, double* w) {
   const int ilim = E.natorb * (E.natorb + 1) / 2;
   for (int i = 0; i < ilim * ilim; ++i) w[i] = 0.0;
   auto W = [&](int i, int j) -> double& { return w[(j - 1) * ilim + (i - 1)]; };
-  W(1, 1) = E.gss;
+  const Params& p = E.p;
+  W(1, 1) = p.gss;
   if (E.natorb > 2) {
     const int ipx = 3, ipy = 6, ipz = 10;
-    W(ipx, 1) = W(ipy, 1) = W(ipz, 1) = E.gsp;
-    W(1, ipx) = W(1, ipy) = W(1, ipz) = E.gsp;
-    W(ipx, ipx) = W(ipy, ipy) = W(ipz, ipz) = E.gpp;
-    W(ipy, ipx) = W(ipz, ipx) = W(ipz, ipy) = E.gp2;
-    W(ipx, ipy) = W(ipx, ipz) = W(ipy, ipz) = E.gp2;
-    W(2, 2) = W(4, 4) = W(7, 7) = E.hsp;
-    W(5, 5) = W(8, 8) = W(9, 9) = 0.5 * (E.gpp - E.gp2);
+    W(ipx, 1) = W(ipy, 1) = W(ipz, 1) = p.gsp;
+    W(1, ipx) = W(1, ipy) = W(1, ipz) = p.gsp;
+    W(ipx, ipx) = W(ipy, ipy) = W(ipz, ipz) = p.gpp;
+    W(ipy, ipx) = W(ipz, ipx) = W(ipz, ipy) = p.gp2;
+    W(ipx, ipy) = W(ipx, ipz) = W(ipy, ipz) = p.gp2;
+    W(2, 2) = W(4, 4) = W(7, 7) = std::max(1.0e-7, p.hsp);
+    W(5, 5) = W(8, 8) = W(9, 9) = 0.5 * (p.gpp - p.gp2);
   }
 }
 
@@ -853,10 +741,10 @@ int mhip_pm7_hamiltonian(int numat, const int* nat, const double* coord, int cha
   for (long long q = 0; q < mpack; ++q) h[q] = 0.0;
   // diagonal one-centre terms
   for (int i = 0; i < numat; ++i) {
-    const Elem& E = g_elem[nat[i]];
+    const Element& E = g_elem[nat[i]];
     for (int a = 0; a < E.natorb; ++a) {
       const long long o = nfirst[i] - 1 + a;
-      h[o * (o + 1) / 2 + o] = (a == 0) ? E.uss : E.upp;
+      h[o * (o + 1) / 2 + o] = (a == 0) ? E.p.uss : E.p.upp;
     }
   }
   // Per-atom accumulation of core-attraction blocks, done per row i in pair
@@ -875,12 +763,12 @@ int mhip_pm7_hamiltonian(int numat, const int* nat, const double* coord, int cha
   std::atomic<int> next_row{0};
   auto worker = [&]() {
   for (int i = next_row++; i < numat; i = next_row++) {
-    const Elem& A = g_elem[nat[i]];
+    const Element& A = g_elem[nat[i]];
     const int li = A.natorb, limij = li * (li + 1) / 2;
     long long kk = off[i];
     double rowsum = 0.0;
     for (int j = 0; j < i; ++j) {
-      const Elem& B = g_elem[nat[j]];
+      const Element& B = g_elem[nat[j]];
       const int lj = B.natorb, limkl = lj * (lj + 1) / 2;
       PairOut po;
       pair_integrals(A, B, coord + 3 * i, coord + 3 * j, po);
@@ -934,14 +822,14 @@ int mhip_pm7_hamiltonian(int numat, const int* nat, const double* coord, int cha
   // Isolated-atom reference: sum of atomic heats minus electronic energies.
   double eheat = 0.0, eisol = 0.0;
   for (int i = 0; i < numat; ++i) {
-    eheat += g_elem[nat[i]].eheat;
+    eheat += g_elem[nat[i]].p.dhf_atom;
     eisol += g_elem[nat[i]].eisol;
   }
   *atheat = eheat - eisol * kFPC9;
   // Diagonal starting density
   const double yy = (double)charge / (norbs + 1.0e-10);
   for (int i = 0; i < numat; ++i) {
-    const Elem& E = g_elem[nat[i]];
+    const Element& E = g_elem[nat[i]];
     for (int a = 0; a < E.natorb; ++a)
       pdiag[nfirst[i] - 1 + a] = (E.natorb == 1) ? E.tore - yy : E.tore * 0.25 - yy;
   }
@@ -973,8 +861,8 @@ int mhip_pm7_hamiltonian_mozyme(int numat, const int* nat, const double* coord, 
   }
   for (long long q = 0; q < mpack; ++q) h[q] = 0.0;
   for (int i = 0; i < numat; ++i) {
-    const Elem& E = g_elem[nat[i]];
-    for (int a = 0; a < E.natorb; ++a) h[nij(i, i) + a * (a + 1) / 2 + a] = (a == 0) ? E.uss : E.upp;
+    const Element& E = g_elem[nat[i]];
+    for (int a = 0; a < E.natorb; ++a) h[nij(i, i) + a * (a + 1) / 2 + a] = (a == 0) ? E.p.uss : E.p.upp;
   }
   const long long npairs = (long long)numat * (numat - 1) / 2;
   std::vector<double> e1b_acc((size_t)numat * 10, 0.0), enuc_row(numat, 0.0);
@@ -982,11 +870,11 @@ int mhip_pm7_hamiltonian_mozyme(int numat, const int* nat, const double* coord, 
   std::atomic<int> next_row{0};
   auto worker = [&]() {
     for (int i = next_row++; i < numat; i = next_row++) {
-      const Elem& A = g_elem[nat[i]];
+      const Element& A = g_elem[nat[i]];
       const int li = A.natorb;
       double rowsum = 0.0;
       for (int j = 0; j < i; ++j) {
-        const Elem& B = g_elem[nat[j]];
+        const Element& B = g_elem[nat[j]];
         const int lj = B.natorb;
         double e1b[10] = {0}, e2a[10] = {0}, enuc = 0.0;
         const int o = nij(i, j);
@@ -1001,38 +889,19 @@ int mhip_pm7_hamiltonian_mozyme(int numat, const int* nat, const double* coord, 
         } else {
           const double* xi = coord + 3 * i;
           const double* xj = coord + 3 * j;
-          double x[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-          const double rij = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
-          double w1;
+          double u[3] = {xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]};  // i -> j
+          const double rij = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+          double w1, sp_i, sp_j;
+          pair_dipole_terms(A, B, rij, w1, sp_i, sp_j);
           if (o == -2) {
-            double ri[22], gab;
-            local_repulsion(A, B, rij, ri, gab);
-            w1 = ri[0];
-            for (int k = 0; k < 3; ++k) x[k] /= rij;
-            if (std::fabs(x[2]) > 0.99999999) x[2] = std::copysign(1.0, x[2]);
-            // the (p, s) elements: outer2's -w tore with w = -ri(2) x (atom i) and
-            // -ri(5) x (atom j); the sigma-odd integrals of local_repulsion carry
-            // the opposite sign to the reference's (its sigma axis runs from B
-            // to A), hence -ri here, which also makes these terms continue the
-            // NDDO attraction across the edge of the stored pairs
-            if (li > 1) {
-              e1b[1] = -ri[1] * x[0] * B.tore;
-              e1b[3] = -ri[1] * x[1] * B.tore;
-              e1b[6] = -ri[1] * x[2] * B.tore;
-            }
-            if (lj > 1) {
-              e2a[1] = -ri[4] * x[0] * A.tore;
-              e2a[3] = -ri[4] * x[1] * A.tore;
-              e2a[6] = -ri[4] * x[2] * A.tore;
-            }
-          } else {
-            const double r = rij / kA0;
-            double aee = 0.5 / A.am + 0.5 / B.am;
-            aee = aee * aee;
-            w1 = kEV / std::sqrt(r * r + aee);
-            double point;
-            const double cf = feather(rij, point);
-            w1 = w1 * cf + (1.0 - cf) * point;
+            // outer2: the (p, s) blocks of each atom in the other's charge
+            // take the (s p_sigma|ss) repulsions, rotated to the molecular
+            // frame by the bond's direction cosines (the z cosine held at +-1
+            // within 1e-8, as outer2 does)
+            for (int k = 0; k < 3; ++k) u[k] /= rij;
+            if (std::fabs(u[2]) > 0.99999999) u[2] = std::copysign(1.0, u[2]);
+            for (int k = 0; k < 3 && li > 1; ++k) e1b[pidx(1 + k, 0)] = -B.tore * u[k] * sp_i;
+            for (int k = 0; k < 3 && lj > 1; ++k) e2a[pidx(1 + k, 0)] = -A.tore * u[k] * sp_j;
           }
           for (int a = 0; a < li; ++a) e1b[pidx(a, a)] = -w1 * B.tore;
           for (int a = 0; a < lj; ++a) e2a[pidx(a, a)] = -w1 * A.tore;
@@ -1082,17 +951,17 @@ int mhip_pm7_atoms(int numat, const int* nat, int charge, int* nfirst, int* nlas
   int k = 1;
   double eheat = 0.0, eisol = 0.0;
   for (int i = 0; i < numat; ++i) {
-    const Elem& E = g_elem[nat[i]];
+    const Element& E = g_elem[nat[i]];
     nfirst[i] = k;
     k += E.natorb;
     nlast[i] = k - 1;
-    eheat += E.eheat;
+    eheat += E.p.dhf_atom;
     eisol += E.eisol;
   }
   *atheat = eheat - eisol * kFPC9;
   const double yy = (double)charge / (norbs + 1.0e-10);
   for (int i = 0; i < numat; ++i) {
-    const Elem& E = g_elem[nat[i]];
+    const Element& E = g_elem[nat[i]];
     for (int a = 0; a < E.natorb; ++a)
       pdiag[nfirst[i] - 1 + a] = (E.natorb == 1) ? E.tore - yy : E.tore * 0.25 - yy;
   }
@@ -1127,12 +996,12 @@ int mhip_pm7_integrals_mozyme(int numat, const int* nat, const double* coord, co
   std::atomic<int> next_row{0};
   auto worker = [&]() {
     for (int i = next_row++; i < numat; i = next_row++) {
-      const Elem& A = g_elem[nat[i]];
+      const Element& A = g_elem[nat[i]];
       const int limij = A.natorb * (A.natorb + 1) / 2;
       long long kk = off[i];
       for (int j = 0; j < i; ++j) {
         if (nij(i, j) < 0) continue;
-        const Elem& B = g_elem[nat[j]];
+        const Element& B = g_elem[nat[j]];
         const int limkl = B.natorb * (B.natorb + 1) / 2;
         PairOut po;
         pair_integrals(A, B, coord + 3 * i, coord + 3 * j, po);
@@ -1152,15 +1021,38 @@ int mhip_pm7_integrals_mozyme(int numat, const int* nat, const double* coord, co
   return 0;
 }
 
+int mhip_pm7_pair(int zi, int zj, const double* xi, const double* xj, double* w, double* e1b, double* e2a,
+                  double* enuc, double* di) {
+  init_params();
+  if (zi < 0 || zi >= 9 || zj < 0 || zj >= 9 || !g_elem[zi].ok || !g_elem[zj].ok) return -1;
+  if (!xi || !xj || !w || !e1b || !e2a || !enuc || !di) return -1;
+  const Element& A = g_elem[zi];
+  const Element& B = g_elem[zj];
+  PairOut po;
+  pair_integrals(A, B, xi, xj, po);
+  const int limij = A.natorb * (A.natorb + 1) / 2, limkl = B.natorb * (B.natorb + 1) / 2;
+  std::memcpy(w, po.w, sizeof(double) * limij * limkl);
+  std::memcpy(e1b, po.e1b, sizeof(double) * 10);
+  std::memcpy(e2a, po.e2a, sizeof(double) * 10);
+  *enuc = po.enuc;
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) di[a * 4 + b] = po.di[a][b];
+  return 0;
+}
+
 int mhip_pm7_element(int z, double* out) {
   init_params();
   if (z < 0 || z >= 9 || !g_elem[z].ok || !out) return -1;
-  const Elem& E = g_elem[z];
+  const Element& E = g_elem[z];
   out[0] = E.natorb;
   out[1] = E.tore;
   out[2] = E.am;
   out[3] = E.ad;
-  out[4] = E.dd;
+  out[4] = E.d1;
+  out[5] = E.aq;
+  out[6] = E.d2;
+  out[7] = E.eisol;
+  out[8] = E.p.dhf_atom;
   return 0;
 }
 

@@ -24,6 +24,13 @@ class System:
     pdiag: np.ndarray
     nclose: int
     charge: int = 0
+    e_disp: float = 0.0   # PM7 dispersion correction, kcal/mol
+    e_hbond: float = 0.0  # PM7 hydrogen-bond correction, kcal/mol
+
+    def hof(self, escf):
+        """Heat of formation: the SCF's escf plus PM7's post-SCF corrections
+        (src/corrections/post_scf_corrections.F90:112-114)."""
+        return escf + self.e_disp + self.e_hbond
 
     @property
     def numat(self):
@@ -53,4 +60,17 @@ def build(nat, xyz, charge=0):
     L.check(lib.mhip_pm7_hamiltonian(numat, L.ptr(nat), L.ptr(xyz), charge, L.ptr(nf), L.ptr(nl),
                                      L.ptr(h), L.ptr(w), ctypes.byref(en), ctypes.byref(at), L.ptr(pd)))
     nel = int(sum(TORE[int(z)] for z in nat)) - charge
-    return System(nat, xyz, n, nf, nl, h, w, en.value, at.value, pd, nel // 2, charge)
+    ed, eh = corrections(nat, xyz)
+    return System(nat, xyz, n, nf, nl, h, w, en.value, at.value, pd, nel // 2, charge, ed, eh)
+
+
+def corrections(nat, xyz):
+    """PM7's dispersion and hydrogen-bond corrections (kcal/mol) of a
+    geometry."""
+    lib = L.load()
+    nat = np.ascontiguousarray(nat, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    ed = ctypes.c_double()
+    eh = ctypes.c_double()
+    L.check(lib.mhip_pm7_corrections(len(nat), L.ptr(nat), L.ptr(xyz), ctypes.byref(ed), ctypes.byref(eh), None))
+    return ed.value, eh.value

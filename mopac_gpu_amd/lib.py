@@ -148,6 +148,9 @@ SIGNATURES = {
     "mhip_guess_water": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhip_guess_density": (c_int, [c_void_p, c_void_p]),
     "mhip_mozyme_setup": (c_int, [c_void_p, c_void_p]),
+    "mhip_pm7_pair": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                               c_void_p]),
+    "mhip_pm7_corrections": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhip_pm7_element": (c_int, [c_int, c_void_p]),
     "mhip_pm7_hamiltonian_mozyme": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_ll, c_void_p, ctypes.POINTER(c_double)]),
     "mhip_pm7_atoms": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, ctypes.POINTER(c_double), c_void_p]),

@@ -117,7 +117,7 @@ def element_parameters(nat):
     """Per-atom (am, ad, dd) of PM7 (parameters_C after calpar)."""
     from . import lib as L
     lib = L.load()
-    out = np.zeros(5)
+    out = np.zeros(9)
     am, ad, dd = [], [], []
     for z in nat:
         L.check(0 if lib.mhip_pm7_element(int(z), L.ptr(out)) == 0 else 1)

@@ -9,7 +9,8 @@ compiled, with amdflang, straight from where they lie.  The reference's own
 CMake build is not used.  Symbols those files reference but never reach on
 this path stay unresolved (the COSMO Fock term addfck, guarded by
 useps = .false.; ijbo, used only when lijbo is false; mopend, reada and timer,
-reached only on error or by keywords the tests do not set), so the library is
+reached only on error or by keywords the tests do not set; the d-orbital and
+solid-state branches of the integral and correction files), so the library is
 opened lazily (RTLD_LAZY); the BLAS/LAPACK routines pulay calls resolve from
 MKL, which the oracle loader makes global.
 """
@@ -62,8 +63,34 @@ SOURCES = [
     "integrals/mndod_C.F90",
     "integrals/mndod.F90",
     "MOZYME/fock2z.F90",
+    # the PM7 producer's reference: parameters, derived multipoles, pair
+    # integrals (rotate/rotatd/reppd/spcore/ccrep), overlaps (h1elec/diat/set)
+    "conref_C.F90",
+    "INDO/reimers_C.F90",
+    "elemts_C.F90",
+    "chemistry/atomradii_C.F90",
+    "models/parameters_for_PM7_C.F90",
+    "integrals/sp_two_electron.F90",
+    "models/calpar.F90",
+    "integrals/rotate.F90",
+    "integrals/h1elec.F90",
+    "integrals/diat.F90",
+    "integrals/set.F90",
+    "integrals/coe.F90",
+    "integrals/ccrep.F90",
+    # PM7's post-SCF corrections (dispersion, hydrogen bonds) and the bonding
+    # topology they read
+    "chemistry/geochk_bits_1.F90",
+    "geometry/bangle.F90",
+    "geometry/dihed.F90",
+    "geometry/dang.F90",
+    "corrections/set_up_dentate.F90",
+    "corrections/H_bond_correction_PM6_DH_Dispersion.F90",
+    "corrections/H_bond_correction_bits.F90",
+    "corrections/H_bond_correction_EH_plus.F90",
+    "corrections/Hydrogen_bond_corrections.F90",
 ]
-HARNESSES = ["ref_harness.F90", "ref_harness_mozyme.F90"]
+HARNESSES = ["ref_harness.F90", "ref_harness_mozyme.F90", "ref_harness_pm7.F90"]
 
 
 def build(src="/root/reference/src"):

@@ -46,6 +46,10 @@ def load():
         lib.mz_helecz.restype = D
         lib.mz_cnvgz.argtypes = [V, V, V, V, V, I, V, I, V]
         lib.mz_fock2z.argtypes = [V, V, V, V, ctypes.c_longlong, V, V, V, D, D, I, D, D, V]
+        lib.pm7_ref_init.argtypes = []
+        lib.pm7_ref_element.argtypes = [I, V]
+        lib.pm7_ref_pair.argtypes = [I, I, V, V, V, V, V, V, V]
+        lib.pm7_ref_corrections.argtypes = [I, V, V, V]
         _lib = lib
     return _lib
 
@@ -199,3 +203,93 @@ class Mozyme:
         s.ncf, s.nce, s.icocc, s.icvir, s.cocc, s.cvir, s.eigs = ncf, nce, icocc, icvir, cocc, cvir, eigs
         s.eigv = self.eigv.copy()
         return s
+
+
+# ---- PM7 producer and post-SCF corrections (src/integrals/rotate.F90,
+# h1elec.F90, ccrep.F90, src/models/calpar.F90, src/corrections/) ----------
+# The MOZYME helpers above write per-element values into the same parameter
+# module, so every PM7 call loads the PM7 set afresh.
+
+NATORB = {1: 1, 6: 4, 7: 4, 8: 4}
+
+
+ELEMENT_FIELDS = ("natorb", "tore", "am", "ad", "aq", "dd", "qq", "eisol", "eheat", "po1", "po2", "po3",
+                  "po9", "ddp3", "uss", "upp", "gss", "gsp", "gpp", "gp2", "hsp")
+
+
+def pm7_element(z):
+    """Element z after calpar, as a dict of ELEMENT_FIELDS."""
+    lib = load()
+    lib.pm7_ref_init()
+    out = np.zeros(21)
+    lib.pm7_ref_element(int(z), P(out))
+    return dict(zip(ELEMENT_FIELDS, out))
+
+
+def pm7_pair(zi, zj, xi, xj, init=True):
+    """rotate + h1elec for atom i (zi at xi) and atom j (zj at xj), Angstrom:
+    (w (limij x limkl), e1b (limij), e2a (limkl), enuc eV, di (natorb_i x natorb_j))."""
+    lib = load()
+    if init:
+        lib.pm7_ref_init()
+    w = np.zeros(2025)
+    e1b = np.zeros(45)
+    e2a = np.zeros(45)
+    en = ctypes.c_double()
+    di = np.zeros((9, 9), order="F")
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    xj = np.ascontiguousarray(xj, dtype=np.float64)
+    lib.pm7_ref_pair(int(zi), int(zj), P(xi), P(xj), P(w), P(e1b), P(e2a), ctypes.byref(en), P(di))
+    ni, nj = NATORB[int(zi)], NATORB[int(zj)]
+    li, lj = ni * (ni + 1) // 2, nj * (nj + 1) // 2
+    return w[:li * lj].reshape(li, lj), e1b[:li].copy(), e2a[:lj].copy(), en.value, di[:ni, :nj].copy()
+
+
+def pm7_hcore(nat, xyz):
+    """H (packed), the two-centre W blocks and enuclr (eV) assembled from the
+    reference's own pair routines in the order hcore visits pairs
+    (src/integrals/hcore.F90:157-288): per atom i its pairs j < i; the
+    resonance block into H(i, j), e1b onto atom i's diagonal block and e2a
+    onto atom j's, Uss/Upp on the diagonal.  blocks[(i, j)] = w (limij x limkl)."""
+    nat = [int(z) for z in nat]
+    xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
+    el = {z: pm7_element(z) for z in set(nat)}
+    no = [NATORB[z] for z in nat]
+    nf = np.cumsum([0] + no[:-1])
+    n = int(sum(no))
+    h = np.zeros(n * (n + 1) // 2)
+    pk = lambda r, c: r * (r + 1) // 2 + c  # noqa: E731  (r >= c, 0-based)
+    for i, z in enumerate(nat):
+        for a in range(no[i]):
+            h[pk(nf[i] + a, nf[i] + a)] += el[z]["uss"] if a == 0 else el[z]["upp"]
+    load().pm7_ref_init()
+    blocks = {}
+    enuc = 0.0
+    for i in range(len(nat)):
+        for j in range(i):
+            w, e1b, e2a, en, di = pm7_pair(nat[i], nat[j], xyz[i], xyz[j], init=False)
+            blocks[(i, j)] = w
+            enuc += en
+            for a in range(no[i]):
+                for b in range(no[j]):
+                    h[pk(nf[i] + a, nf[j] + b)] += di[a, b]
+            for a in range(no[i]):
+                for b in range(a + 1):
+                    h[pk(nf[i] + a, nf[i] + b)] += e1b[a * (a + 1) // 2 + b]
+            for a in range(no[j]):
+                for b in range(a + 1):
+                    h[pk(nf[j] + a, nf[j] + b)] += e2a[a * (a + 1) // 2 + b]
+    return h, blocks, enuc, el
+
+
+def pm7_corrections(nat, xyz):
+    """(dispersion, hydrogen-bond energy, hydrogen bonds counted) in kcal/mol
+    for a discrete molecule: PM6_DH_Dispersion and Hydrogen_bond_corrections
+    as post_scf_corrections calls them for PM7."""
+    lib = load()
+    lib.pm7_ref_init()
+    nat = np.ascontiguousarray(nat, dtype=np.int32)
+    xyz = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
+    out = np.zeros(3)
+    lib.pm7_ref_corrections(len(nat), P(nat), P(xyz), P(out))
+    return out

@@ -151,9 +151,9 @@ def test_scf_pulay_reproduces_api_charges():
     o = oracle.iterate(s, pulay=True, scfcrt=1e-4)
     assert o["converged"]
     assert np.max(np.abs(_charges(s, o["p"], g["nat"]) - np.array(g["charges"]))) < 6e-6
-    # PM7's geometry-only post-SCF corrections are not part of the SCF loop:
-    # the heat of formation agrees to within them
-    assert abs(o["escf"] - g["hof"]) < 0.05
+    # with PM7's post-SCF corrections (dispersion; no hydrogen bond in one
+    # water) the heat of formation is the API's known answer
+    assert abs(o["escf"] + s.e_disp + s.e_hbond - g["hof"]) < 2e-5
 
 
 # ---- 3. against what the mathematics fixes ----------------------------------
