@@ -21,6 +21,9 @@
 #ifndef MOPAC_HIP_H
 #define MOPAC_HIP_H
 
+#include <stdbool.h>
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -70,6 +73,14 @@ int mhip_fock2(mhip_ctx* ctx, double* f, const double* ptot, const double* p, in
  * (src/integrals/helect.F90:16-66). */
 int mhip_helect(mhip_ctx* ctx, int norbs, const double* p, const double* h, const double* f,
                 double* e);
+
+/* `ncalls` Pulay DIIS steps from a fresh start on the uploaded system, as
+ * iter makes them (src/SCF/pulay.F90:16-166, 10-deep circular history,
+ * osinv): call k extrapolates f_seq[k] (mpack) in place from the alpha
+ * density pa_seq[k], and pl_seq[k] receives its PL.  The same device step
+ * mhip_scf takes with PULAY, exposed so a sequence that wraps the history can
+ * be checked call for call. */
+int mhip_pulay_sequence(mhip_ctx* ctx, int ncalls, double* f_seq, const double* pa_seq, double* pl_seq);
 
 /* All eigenpairs of the packed Fock matrix.  Adds i*1e-10 to F(i,i) IN PLACE
  * (as the reference does), returns eigenvectors as columns of c (n x n) in
@@ -318,9 +329,10 @@ int mhip_mozyme_scf(mhip_ctx* ctx, mhip_lmo_set* lmo, const mhip_mozyme_opts* op
 
 /* ---- reference-boundary drop-ins -----------------------------------------
  * Same C signatures as the reference's exported symbols, so its Fortran
- * interfaces (src/gpu/cublas_interfaces.F90, src/matrix/eigenvectors_cuda.F90)
- * bind by changing only the bind(C, name=...) strings.  They use an implicit
- * context on device 0. */
+ * interfaces (src/gpu/cublas_interfaces.F90, src/matrix/eigenvectors_cuda.F90,
+ * src/deprecated/mod_gpu_info.F90) bind by changing only the
+ * bind(C, name=...) strings.  They use an implicit context on the device
+ * mhip_setDevice selected (device 0 by default). */
 
 /* call_gemm_cublas (src/gpu/cuda_wrappers.cu:192-226) */
 void mhip_call_gemm(char tra, char trb, int m, int n, int k, double alpha, const double* a,
@@ -342,8 +354,25 @@ void mhip_call_rot(const double* fmo, const double* eig, double* v, const double
  * FP64 can be measured on its own. */
 void mhip_gemm_split_bf16(char tra, char trb, int m, int n, int k, double alpha, const double* a,
                           int lda, const double* b, int ldb, double beta, double* c, int ldc);
-/* mopac_cuda_destroy_resources (src/gpu/cuda_wrappers.cu:830-856) */
+/* mopac_cuda_destroy_resources (src/gpu/cuda_wrappers.cu:830-856).
+ * Idempotent; the next drop-in call sets the implicit context up again. */
 void mhip_destroy_resources(void);
+/* getGPUInfo (src/gpu/cuda_wrappers.cu:71-112), bound by
+ * src/deprecated/mod_gpu_info.F90:5-15: up to six devices' names (the
+ * marketing name, or the gfx target when the driver reports none), memory,
+ * clock (kHz) and gfx major/minor; hasDouble is true for every device. */
+void mhip_getGPUInfo(bool* hasGpu, bool hasDouble[6], int* nDevices, char name[6][256], int name_size[6],
+                     size_t totalMem[6], int clockRate[6], int major[6], int minor[6]);
+/* setDevice (src/gpu/cuda_wrappers.cu:115-118), bound by
+ * mod_gpu_info.F90:24-28: selects the device the drop-ins above run on;
+ * *stat false for an index that does not exist. */
+void mhip_setDevice(int idevice, bool* stat);
+/* set_mozyme_gpu_pair (src/gpu/cuda_wrappers.cu:58-68): the reference's
+ * two-GPU MOZYME device pair.  Recorded (clamped to existing devices) for
+ * link compatibility; this library runs MOZYME on one GPU. */
+void mhip_set_mozyme_gpu_pair(int dev0, int dev1);
+/* The device mhip_setDevice last selected (0 by default). */
+int mhip_selected_device(void);
 
 /* ---- learned starting density: the device MLP ------------------------------
  * No counterpart in the reference (it replaces nothing; it feeds the SCF a
@@ -439,6 +468,18 @@ int mhip_pm7_element(int z, double* out);
  * h1elec for one pair (src/integrals/rotate.F90:16-124, h1elec.F90:16-84). */
 int mhip_pm7_pair(int zi, int zj, const double* xi, const double* xj, double* w, double* e1b, double* e2a,
                   double* enuc, double* di);
+/* ---- density restart files (MOPAC .den) -----------------------------------
+ * The alpha density (packed, mpack) as den_in_out writes it with DENOUT /
+ * DENOUTF and reads it with OLDENS (src/SCF/iter.F90:1133-1227, :151-159):
+ * one Fortran sequential record (norbs, numat, pa) when formatted == 0,
+ * list-directed text when formatted == 1.  mhip_den_read tries the form
+ * asked for, then the other, and fails when the file's norbs or numat
+ * differ ("Density file read in does not match current data set") or the
+ * file is missing or short.  A restart starts mhip_scf through
+ * mhip_scf_opts.p_init = 2 pa, as OLDENS sets p = 2 pa and pold = p. */
+int mhip_den_write(const char* path, int norbs, int numat, const double* pa, int formatted);
+int mhip_den_read(const char* path, int norbs, int numat, double* pa, int formatted);
+
 /* PM7's post-SCF corrections of a discrete molecule (kcal/mol), the terms
  * post_scf_corrections adds to the SCF heat of formation for PM7
  * (src/corrections/post_scf_corrections.F90:112-114): *e_disp the damped

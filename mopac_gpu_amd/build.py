@@ -33,6 +33,7 @@ SOURCES = [
     "dropin.hip",
     "pm7_hamiltonian.cpp",
     "pm7_corrections.cpp",
+    "den_io.cpp",
 ]
 
 # The MOZYME kernels reproduce the reference's list-building, cutoff and

@@ -1,6 +1,8 @@
 // Reference-boundary drop-ins: the C signatures the reference's Fortran binds
-// (src/gpu/cuda_wrappers.cu), served by this library's kernels on device 0
-// with grow-only device buffers.  Host arrays in, host arrays out, as there.
+// (src/gpu/cuda_wrappers.cu), served by this library's kernels on the device
+// mhip_setDevice selected (device 0 by default) with grow-only device
+// buffers.  Host arrays in, host arrays out, as there.
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -13,6 +15,7 @@ namespace {
 
 struct Implicit {
   bool init = false;
+  int device = 0;
   hipStream_t s = nullptr;
   DevBuf a, b, c, d, cols, rot, cnt;
   JacobiPlan plan;
@@ -21,14 +24,37 @@ struct Implicit {
 };
 Implicit g_imp;
 std::mutex g_mu;
+int g_selected_device = 0;    // set by mhip_setDevice
+int g_pair[2] = {0, 1};       // mhip_set_mozyme_gpu_pair (recorded; one GPU is used)
 
+// Release every buffer of the implicit context (caller holds g_mu).
+void release_implicit() {
+  if (!g_imp.init) return;
+  (void)hipSetDevice(g_imp.device);
+  for (DevBuf* b : {&g_imp.a, &g_imp.b, &g_imp.c, &g_imp.d, &g_imp.cols, &g_imp.rot, &g_imp.cnt,
+                    &g_imp.scal})
+    b->release();
+  g_imp.plan.release();
+  release_split_scratch(g_imp.s);
+  release_splitk_scratch(g_imp.s);
+  if (g_imp.hs) (void)hipHostFree(g_imp.hs);
+  (void)hipStreamDestroy(g_imp.s);
+  g_imp = Implicit();
+}
+
+// The implicit context on the selected device; a new selection releases the
+// buffers held on the old one first.
 Implicit& imp() {
+  if (g_imp.init && g_imp.device != g_selected_device) release_implicit();
   if (!g_imp.init) {
-    MHIP_CHECK(hipSetDevice(0));
+    MHIP_CHECK(hipSetDevice(g_selected_device));
+    g_imp.device = g_selected_device;
     MHIP_CHECK(hipStreamCreateWithFlags(&g_imp.s, hipStreamNonBlocking));
     MHIP_CHECK(hipHostMalloc((void**)&g_imp.hs, sizeof(double) * 64, hipHostMallocDefault));
     g_imp.scal.ensure(sizeof(double) * 64);
     g_imp.init = true;
+  } else {
+    MHIP_CHECK(hipSetDevice(g_imp.device));
   }
   return g_imp;
 }
@@ -199,16 +225,56 @@ void mhip_call_rot(const double* fmo, const double* eig, double* v, const double
 
 void mhip_destroy_resources(void) {
   std::lock_guard<std::mutex> lk(g_mu);
-  if (!g_imp.init) return;
-  for (DevBuf* b : {&g_imp.a, &g_imp.b, &g_imp.c, &g_imp.d, &g_imp.cols, &g_imp.rot, &g_imp.cnt,
-                    &g_imp.scal})
-    b->release();
-  g_imp.plan.release();
-  release_split_scratch(g_imp.s);
-  release_splitk_scratch(g_imp.s);
-  if (g_imp.hs) (void)hipHostFree(g_imp.hs);
-  (void)hipStreamDestroy(g_imp.s);
-  g_imp = Implicit();
+  release_implicit();
 }
+
+void mhip_getGPUInfo(bool* hasGpu, bool hasDouble[6], int* nDevices, char name[6][256], int name_size[6],
+                     size_t totalMem[6], int clockRate[6], int major[6], int minor[6]) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    if (hasGpu) *hasGpu = false;
+    if (nDevices) *nDevices = 0;
+    return;
+  }
+  const int nd = count > 6 ? 6 : count;
+  if (hasGpu) *hasGpu = true;
+  if (nDevices) *nDevices = nd;
+  for (int i = 0; i < 6; ++i) {
+    hipDeviceProp_t prop{};
+    const bool ok = i < nd && hipGetDeviceProperties(&prop, i) == hipSuccess;
+    if (ok) {
+      // the marketing name when the driver has one, else the gfx target
+      std::strncpy(name[i], prop.name[0] ? prop.name : prop.gcnArchName, 255);
+      name[i][255] = '\0';
+    } else {
+      name[i][0] = '\0';
+    }
+    name_size[i] = ok ? (int)std::strlen(name[i]) : 0;
+    totalMem[i] = ok ? prop.totalGlobalMem : 0;
+    clockRate[i] = ok ? prop.clockRate : 0;
+    major[i] = ok ? prop.major : 0;
+    minor[i] = ok ? prop.minor : 0;
+    hasDouble[i] = ok;  // every CDNA device has native FP64
+  }
+}
+
+void mhip_setDevice(int idevice, bool* stat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int count = 0;
+  const bool ok = hipGetDeviceCount(&count) == hipSuccess && idevice >= 0 && idevice < count &&
+                  hipSetDevice(idevice) == hipSuccess;
+  if (ok) g_selected_device = idevice;
+  if (stat) *stat = ok;
+}
+
+void mhip_set_mozyme_gpu_pair(int dev0, int dev1) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return;
+  if (dev0 >= 0 && dev0 < count) g_pair[0] = dev0;
+  if (dev1 >= 0 && dev1 < count) g_pair[1] = dev1;
+}
+
+int mhip_selected_device(void) { return g_selected_device; }
 
 }  // extern "C"

@@ -7,7 +7,7 @@
 namespace mhip {
 
 struct OccVirtWork {
-  DevBuf fd, wv, fmo, b, eo, ev, cw, part;
+  DevBuf fd, wv, fmo, b, b2, eo, ev, cw, part;
   void ensure(int n, int nocc, int ld);
   void release();
 };
@@ -25,9 +25,21 @@ struct OccVirtWork {
 // to the virtual space (diag.F90 keeps the levels of the last diagonalization;
 // here the occupied ones are kept).  Everything stays on the stream.  split = true runs the contractions on the
 // split-BF16 GEMM (the precision form); C^T C - I is always formed in FP64.
+// project = true also removes the virtual columns' overlap with the
+// occupied ones in the same update: C_virt -= C_occ (B^T + C_occ^T C_virt),
+// the overlap formed in FP64 from the old C.  The simultaneous rotations keep
+// C_occ^T C_virt = 0 only to first order in the blocks' own orthonormality
+// error, so without it that overlap accumulates over the iterations and the
+// fixed point FMO = 0 no longer pins the occupied space to FP64 precision.
 constexpr double kMaxStep = 0.3;
 void occvirt_step(hipStream_t s, OccVirtWork& w, int n, int ld, const double* fpacked, double* C,
-                  int nocc, double* eig, bool split, int bjorck_steps);
+                  int nocc, double* eig, bool split, int bjorck_steps, bool project = false);
+
+// The virtual columns made orthogonal to the occupied ones and then
+// orthonormal among themselves: C_virt -= C_occ (C_occ^T C_virt), then
+// `bjorck_steps` Bjorck steps on C_virt.  The occupied columns (and so the
+// density) are left as they are.
+void orthogonalize_virtual(hipStream_t s, OccVirtWork& w, int n, int ld, double* C, int nocc, int bjorck_steps);
 
 // C(:, c0:c0+m) <- C (3I - C^T C) / 2 on the first n rows.
 void bjorck_step(hipStream_t s, OccVirtWork& w, int n, int ld, double* C, int c0, int m,

@@ -112,6 +112,7 @@ void OccVirtWork::ensure(int n, int nocc, int ld) {
   wv.ensure(sizeof(double) * (size_t)ld * nv);
   fmo.ensure(sizeof(double) * no * nv);
   b.ensure(sizeof(double) * no * nv);
+  b2.ensure(sizeof(double) * no * nv);
   // m x m Gram blocks for any column block of C up to ld wide
   eo.ensure(sizeof(double) * (size_t)ld * ld);
   ev.ensure(sizeof(double) * (size_t)ld * ld);
@@ -120,7 +121,7 @@ void OccVirtWork::ensure(int n, int nocc, int ld) {
 }
 
 void OccVirtWork::release() {
-  for (DevBuf* x : {&fd, &wv, &fmo, &b, &eo, &ev, &cw, &part}) x->release();
+  for (DevBuf* x : {&fd, &wv, &fmo, &b, &b2, &eo, &ev, &cw, &part}) x->release();
 }
 
 void bjorck_step(hipStream_t s, OccVirtWork& w, int n, int ld, double* C, int c0, int m, bool split) {
@@ -142,7 +143,7 @@ void bjorck_step(hipStream_t s, OccVirtWork& w, int n, int ld, double* C, int c0
 }
 
 void occvirt_step(hipStream_t s, OccVirtWork& w, int n, int ld, const double* fpacked, double* C,
-                  int nocc, double* eig, bool split, int bjorck_steps) {
+                  int nocc, double* eig, bool split, int bjorck_steps, bool project) {
   const int nvirt = n - nocc;
   if (nocc <= 0 || nvirt <= 0) return;
   MHIP_REQUIRE(nocc <= ld && n <= ld, "occvirt_step: bad sizes");
@@ -174,12 +175,32 @@ void occvirt_step(hipStream_t s, OccVirtWork& w, int n, int ld, const double* fp
   // simultaneous rotations, both from the old C
   double* cw = w.cw.as<double>();
   copy_cols(s, n, ld, C, cw, n);
+  const double* bv = w.b.as<double>();
+  if (project) {
+    // b2 = B + C_virt^T C_occ (nvirt x nocc, FP64): the virtual update then
+    // also takes out the virtual columns' overlap with the occupied ones
+    MHIP_CHECK(hipMemcpyAsync(w.b2.ptr, w.b.ptr, sizeof(double) * (size_t)nocc * nvirt,
+                              hipMemcpyDeviceToDevice, s));
+    gemm(s, 'T', 'N', nvirt, nocc, n, 1.0, cw + (size_t)nocc * ld, ld, cw, ld, 1.0, w.b2.as<double>(), nvirt);
+    bv = w.b2.as<double>();
+  }
   mm('N', 'N', n, nocc, nvirt, 1.0, cw + (size_t)nocc * ld, ld, w.b.as<double>(), nvirt, 1.0, C, ld);
-  mm('N', 'T', n, nvirt, nocc, -1.0, cw, ld, w.b.as<double>(), nvirt, 1.0, cv, ld);
+  mm('N', 'T', n, nvirt, nocc, -1.0, cw, ld, bv, nvirt, 1.0, cv, ld);
   for (int it = 0; it < bjorck_steps; ++it) {
     bjorck_step(s, w, n, ld, C, 0, nocc, split);
     bjorck_step(s, w, n, ld, C, nocc, nvirt, split);
   }
+}
+
+void orthogonalize_virtual(hipStream_t s, OccVirtWork& w, int n, int ld, double* C, int nocc, int bjorck_steps) {
+  const int nvirt = n - nocc;
+  if (nocc <= 0 || nvirt <= 0) return;
+  w.ensure(n, nocc, ld);
+  double* cv = C + (size_t)nocc * ld;
+  // G = C_occ^T C_virt (nocc x nvirt), then C_virt -= C_occ G
+  gemm(s, 'T', 'N', nocc, nvirt, n, 1.0, C, ld, cv, ld, 0.0, w.b2.as<double>(), nocc);
+  gemm(s, 'N', 'N', n, nvirt, nocc, -1.0, C, ld, w.b2.as<double>(), nocc, 1.0, cv, ld);
+  for (int it = 0; it < bjorck_steps; ++it) bjorck_step(s, w, n, ld, C, nocc, nvirt);
 }
 
 }  // namespace mhip

@@ -534,7 +534,7 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
   const int ihomo = std::max(nclose, 1);
   double eig_homo = 0.0, eig_lumo = 0.0;
   PulayState pst;
-  int nfull = 0, npseudo = 0, block_steps_since_diag = 0, nfragdiag = 0, n_split = 0;
+  int nfull = 0, npseudo = 0, block_steps_since_diag = 0, nfragdiag = 0, n_split = 0, fp64_block_steps = 0;
   const int nfrag = setup_fragments(c, o);
   c->frontier.ensure(sizeof(double) * 2);
   for (;;) {
@@ -626,8 +626,14 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
         // step also squares what the last one left: a second step only
         // while PL is large (the rotations are big)
         const int nbj = pl > 0.05 ? 2 : 1;
+        // The FP64 iterations, which decide convergence, also take out the
+        // virtual columns' overlap with the occupied ones: the first of them
+        // (the overlap the large early rotations left) and every fourth after
+        // it (each later step adds only |E||B|, the blocks' Bjorck residue
+        // times the rotation size)
+        const bool project = !split_now && (fp64_block_steps++ % 4 == 0);
         occvirt_step(s, c->ov, n, c->npad, c->f.as<double>(), c->C.as<double>(), nclose,
-                     c->eig.as<double>(), split_now, nbj);
+                     c->eig.as<double>(), split_now, nbj, project);
         block_steps_since_diag += 1;
       } else {
         pseudo_diag_device(s, c->pseudo, n, c->npad, c->f.as<double>(), c->C.as<double>(), nclose,
@@ -638,7 +644,7 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
       if (block_steps_since_diag > 0) {
         // the warm start needs orthonormal columns to working precision
         bjorck_step(s, c->ov, n, c->npad, c->C.as<double>(), 0, nclose);
-        bjorck_step(s, c->ov, n, c->npad, c->C.as<double>(), nclose, n - nclose);
+        orthogonalize_virtual(s, c->ov, n, c->npad, c->C.as<double>(), nclose, 1);
         block_steps_since_diag = 0;
       }
       double tol = c->tol_iter;
@@ -678,8 +684,12 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
   ee = energy(c);
   JacobiPlan* exit_plan = nullptr;
   if (block_steps_since_diag > 0) {
+    // orthonormal to FP64 precision before the exit diagonalization: two
+    // Bjorck steps on the occupied block, the virtual block projected off
+    // the occupied one, two Bjorck steps on it
     bjorck_step(s, c->ov, n, c->npad, c->C.as<double>(), 0, nclose);
-    bjorck_step(s, c->ov, n, c->npad, c->C.as<double>(), nclose, n - nclose);
+    bjorck_step(s, c->ov, n, c->npad, c->C.as<double>(), 0, nclose);
+    orthogonalize_virtual(s, c->ov, n, c->npad, c->C.as<double>(), nclose, 2);
     // C^T F C is block diagonal between the occupied and virtual columns up
     // to the converged FMO: diagonalize the two blocks as independent
     // problems in the same steps
@@ -824,6 +834,26 @@ int mhip_fock2(mhip_ctx* c, double* f, const double* ptot, const double* p, int 
     fock_build(s, fock_args(c, kInitCopy, 0.0));
     MHIP_CHECK(hipMemcpyAsync(f, c->f.ptr, b, hipMemcpyDeviceToHost, s));
     MHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int mhip_pulay_sequence(mhip_ctx* c, int ncalls, double* f_seq, const double* pa_seq, double* pl_seq) {
+  return guarded([&] {
+    MHIP_REQUIRE(c && c->have_system && ncalls >= 0 && (ncalls == 0 || (f_seq && pa_seq && pl_seq)),
+                 "mhip_pulay_sequence: bad arguments");
+    const long long mp = c->mpack;
+    const size_t b = sizeof(double) * mp;
+    hipStream_t s = c->stream;
+    PulayState st;
+    for (int k = 0; k < ncalls; ++k) {
+      MHIP_CHECK(hipMemcpyAsync(c->f.ptr, f_seq + (size_t)k * mp, b, hipMemcpyHostToDevice, s));
+      MHIP_CHECK(hipMemcpyAsync(c->pa.ptr, pa_seq + (size_t)k * mp, b, hipMemcpyHostToDevice, s));
+      double pl = 0.0;
+      pulay_step(c, st, &pl);
+      MHIP_CHECK(hipMemcpyAsync(f_seq + (size_t)k * mp, c->f.ptr, b, hipMemcpyDeviceToHost, s));
+      MHIP_CHECK(hipStreamSynchronize(s));
+      pl_seq[k] = pl;
+    }
   });
 }
 

@@ -103,6 +103,7 @@ SIGNATURES = {
     "mhip_density": (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p]),
     "mhip_fock2": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     "mhip_helect": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double)]),
+    "mhip_pulay_sequence": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "mhip_eigen": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     "mhip_pseudo_diag": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, ctypes.POINTER(c_ll)]),
     "mhip_scf": (c_int, [c_void_p, ctypes.POINTER(ScfOpts), ctypes.POINTER(ScfResult), c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -111,6 +112,10 @@ SIGNATURES = {
     "mhip_call_syrk": (None, [c_char, c_char, c_int, c_int, c_double, c_void_p, c_int, c_double, c_void_p, c_int]),
     "mhip_dsyevd": (None, [c_int, c_void_p, c_int, c_void_p, ctypes.POINTER(c_int)]),
     "mhip_call_rot": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double]),
+    "mhip_getGPUInfo": (None, [c_void_p] * 9),
+    "mhip_setDevice": (None, [c_int, c_void_p]),
+    "mhip_set_mozyme_gpu_pair": (None, [c_int, c_int]),
+    "mhip_selected_device": (c_int, []),
     "mhip_destroy_resources": (None, []),
     "mhip_set_tuning": (c_int, [c_void_p, c_int, c_int, c_double]),
     "mhip_set_jacobi_schedule": (c_int, [c_int]),
@@ -151,6 +156,8 @@ SIGNATURES = {
     "mhip_pm7_pair": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p]),
     "mhip_pm7_corrections": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mhip_den_write": (c_int, [ctypes.c_char_p, c_int, c_int, c_void_p, c_int]),
+    "mhip_den_read": (c_int, [ctypes.c_char_p, c_int, c_int, c_void_p, c_int]),
     "mhip_pm7_element": (c_int, [c_int, c_void_p]),
     "mhip_pm7_hamiltonian_mozyme": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_ll, c_void_p, ctypes.POINTER(c_double)]),
     "mhip_pm7_atoms": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, ctypes.POINTER(c_double), c_void_p]),

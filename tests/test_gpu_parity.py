@@ -129,6 +129,47 @@ def test_scf_h2o_pulay_reproduces_reference_charges(gpu_ctx):
     assert abs(r["escf"] - o["escf"]) < 1e-6
 
 
+@pytest.mark.parametrize("k", [4, 8])
+def test_pulay_sequence_wraps_history(gpu_ctx, k):
+    """14 consecutive Pulay steps on the device (more than the 10 stored Fock
+    matrices, so the circular history wraps and the full 11 x 11 DIIS system
+    is solved) against the oracle's pulay, which equals the reference's
+    Fortran over the same kind of sequence; random densities, Fock matrices
+    built from them."""
+    s = _water(k, 3)
+    gpu_ctx.upload(s)
+    ncall = 14
+    fs = np.stack([oracle.fock2(s, s.h, _rand_density(s, 30 + i), 0.5 * _rand_density(s, 30 + i))
+                   for i in range(ncall)])
+    ps = np.stack([0.5 * _rand_density(s, 50 + i) for i in range(ncall)])
+    f_o, pl_o = oracle.pulay_seq(s.norbs, fs, ps)
+    f_g = np.ascontiguousarray(fs.copy())
+    pl_g = np.zeros(ncall)
+    L.check(L.load().mhip_pulay_sequence(gpu_ctx.h, ncall, L.ptr(f_g), L.ptr(np.ascontiguousarray(ps)),
+                                         L.ptr(pl_g)))
+    assert np.max(np.abs(pl_g - pl_o)) <= 1e-12 * np.max(np.abs(pl_o))
+    assert np.max(np.abs(f_g - f_o)) <= 1e-10 * np.max(np.abs(f_o))
+
+
+def test_scf_pulay_matches_oracle(gpu_ctx):
+    """PULAY on (H2O)32 at SCFCRT = 1e-8, the device eigensolver at its
+    tightest tolerance so the trajectory follows the oracle's (LAPACK)."""
+    nat, xyz = geometry.water_cluster(32, 1, shape=(4, 4, 2))
+    s = hamiltonian.build(nat, xyz)
+    gpu_ctx.upload(s)
+    lib = L.load()
+    lib.mhip_set_tuning(gpu_ctx.h, 1, -1, 1.0e-13)
+    try:
+        r = gpu_ctx.iter(pulay=True, scfcrt=1e-8)
+    finally:
+        lib.mhip_set_tuning(gpu_ctx.h, 1, -1, 1.0e-10)
+    o = oracle.iterate(s, pulay=True, scfcrt=1e-8)
+    assert r["converged"] and o["converged"]
+    assert r["niter"] == o["niter"]
+    assert abs(r["escf"] - o["escf"]) < 1e-6
+    assert np.max(np.abs(r["p"] - o["p"])) < 1e-7
+
+
 def test_scf_water32_matches_oracle(gpu_ctx):
     """Config 1: (H2O)32, 192 basis functions, SCFCRT=1e-6."""
     nat, xyz = geometry.water_cluster(32, 1, shape=(4, 4, 2))
@@ -232,10 +273,14 @@ def test_mfma_tiles_match_vector_tiles(gpu_ctx):
 
 
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
-@pytest.mark.parametrize("m,n,k", [(256, 256, 64), (384, 192, 272), (64, 64, 16), (256, 128, 3072)])
+@pytest.mark.parametrize("m,n,k", [(256, 256, 64), (384, 192, 272), (64, 64, 16), (256, 128, 3072),
+                                   (1024, 1024, 1536), (2048, 1024, 3072)])
 def test_large_tile_gemm_matches_numpy(gpu_ctx, ta, tb, m, n, k):
     """Shapes that divide the 128x128, 128x64 or 64x64 tiles take the
-    double-buffered large-tile kernel; alpha and beta on a random C."""
+    double-buffered large-tile kernel; alpha and beta on a random C.  The
+    launcher splits K for CU balance: (256, 128, 3072) takes 4 slices,
+    (1024, 1024, 1536) 2 and (2048, 1024, 3072) 3, the first slice adding
+    beta C and the others summed in after it."""
     lib = L.load()
     rng = np.random.default_rng(m + 3 * n + 7 * k)
     a = np.asfortranarray(rng.standard_normal((k, m) if ta == "T" else (m, k)))
