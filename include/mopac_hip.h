@@ -74,6 +74,21 @@ int mhip_fock2(mhip_ctx* ctx, double* f, const double* ptot, const double* p, in
 int mhip_helect(mhip_ctx* ctx, int norbs, const double* p, const double* h, const double* f,
                 double* e);
 
+/* Allocation-free variant of mhip_helect.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, hosts that forbid hipMalloc).  p, h, f
+ * (packed, mpack each) and e (one double) are DEVICE pointers owned by the
+ * caller; the work is queued on `stream` (a hipStream_t, NULL = the default
+ * stream) and is not waited for.  The per-row partial sums live in a fixed
+ * device array of mhip_helect_noalloc_max_rows() doubles per GPU (65 536,
+ * 512 KiB, held until the process ends); larger norbs is rejected with an
+ * error.  Calls no hipMalloc/hipFree.  Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (different streams or threads)
+ * race.  Returns 0 on success. */
+int mhip_helect_noalloc(int norbs, const double* p, const double* h, const double* f, double* e,
+                        void* stream);
+int mhip_helect_noalloc_max_rows(void);
+
 /* `ncalls` Pulay DIIS steps from a fresh start on the uploaded system, as
  * iter makes them (src/SCF/pulay.F90:16-166, 10-deep circular history,
  * osinv): call k extrapolates f_seq[k] (mpack) in place from the alpha

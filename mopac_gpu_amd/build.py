@@ -31,6 +31,7 @@ SOURCES = [
     "k_mozyme.hip",
     "scf_driver.hip",
     "dropin.hip",
+    "noalloc.hip",
     "pm7_hamiltonian.cpp",
     "pm7_corrections.cpp",
     "den_io.cpp",

@@ -104,6 +104,8 @@ SIGNATURES = {
     "mhip_fock2": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     "mhip_helect": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double)]),
     "mhip_pulay_sequence": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "mhip_helect_noalloc": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mhip_helect_noalloc_max_rows": (c_int, []),
     "mhip_eigen": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     "mhip_pseudo_diag": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, ctypes.POINTER(c_ll)]),
     "mhip_scf": (c_int, [c_void_p, ctypes.POINTER(ScfOpts), ctypes.POINTER(ScfResult), c_void_p, c_void_p, c_void_p, c_void_p]),
