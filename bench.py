@@ -34,6 +34,12 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
 FAST_PRECISION = 1
+HELIX_SCFCRT = 1.0e-6  # SCFCRT=1.D-6 for the MOZYME helix
+
+
+def progress(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
@@ -301,21 +307,22 @@ def main():
 
 def run_helix(args):
     """The MOZYME workload: a poly-glycine alpha helix (phi -57, psi -47,
-    seeded +-5 degree torsion noise, H / OH caps), PM7 MOZYME 1SCF at MOZYME's
-    default criterion (SELCON 0.01 kcal/mol), FP64.  One step = the
+    seeded +-5 degree torsion noise, H / OH caps), PM7 MOZYME 1SCF with
+    SCFCRT=1.D-6 and MOZYME's default cutoffs (pairs stored within 9.9 A,
+    dipole shell to 10 A, resonance to 7 A), FP64.  One step = the
     Lewis-structure start (hybrids, bonds polarized by the Fock matrix of the
     diagonal density) + the MOZYME SCF, with the sparse H and integrals
     resident.  The conventional FP64 SCF of the same helix on the GPU is run
     once beside it: the difference of the two heats of formation is MOZYME's
-    cutoff approximation (6 A stored pairs, 7 A dipole shell), not an error
-    of either solver."""
+    cutoff approximation, not an error of either solver."""
     from mopac_gpu_amd import geometry, hamiltonian, mozyme
     from mopac_gpu_amd.scf import Context
     nres = args.residues
     nat, xyz = geometry.glycine_helix(nres, 3)
     t0 = time.perf_counter()
-    ms = mozyme.MozymeSystem.from_geometry(nat, xyz, r_near=6.0, r_dipole=7.0)
+    ms = mozyme.MozymeSystem.from_geometry(nat, xyz)
     t_setup = time.perf_counter() - t0
+    progress(f"helix Gly{nres}: {ms.numat} atoms, {ms.norbs} orbitals, mpack {ms.mpack}, setup {t_setup:.2f} s")
     ctx = Context(0)
     dev_name, _, _ = ctx.device_info()
     ms.upload(ctx)
@@ -325,15 +332,17 @@ def run_helix(args):
         f0 = ms.fock2z(ctx, ms.diagonal_density())
         lm = mozyme.lewis_lmos(ms, f0)
         t_start = time.perf_counter() - t
-        r = ms.scf(ctx, lm)
+        r = ms.scf(ctx, lm, scfcrt=HELIX_SCFCRT)
         return time.perf_counter() - t, t_start, r
 
     for _ in range(args.warmup):
-        step()
+        dt, _, r = step()
+        progress(f"warmup step {dt:.3f} s, {r['niter']} iterations")
     times, last = [], None
     for _ in range(args.steps):
         dt, t_start, last = step()
         times.append(dt)
+        progress(f"timed step {dt:.3f} s, {last['niter']} iterations, escf {last['escf']:.6f}")
     t_mz = float(np.mean(times))
     # roofline of the dominant kernel, the diagg2 rotation levels: HIP events
     # around the level launches, algorithmic bytes counted by the library
@@ -361,6 +370,7 @@ def run_helix(args):
         cpu = helix_cpu_baseline(ms, last["niter"])
     conv = None
     if not args.no_conventional:
+        progress("conventional SCF of the same helix")
         s = hamiltonian.build(nat, xyz)
         c2 = Context(0)
         c2.upload(s)
@@ -373,9 +383,10 @@ def run_helix(args):
         "value": round(t_mz, 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(t_mz * 1e3, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": f"synthetic seeded Gly{nres} alpha helix; PM7 integrals generated in-process",
-        "config": {"workload": f"Gly{nres} alpha helix PM7 MOZYME 1SCF (SELCON 0.01)", "numat": ms.numat,
-                   "norbs": ms.norbs, "nocc": int(ms.s.nclose), "stored_pairs_within_A": 6.0,
-                   "dipole_shell_A": 7.0},
+        "config": {"workload": f"Gly{nres} alpha helix PM7 MOZYME 1SCF SCFCRT=1.D-6", "numat": ms.numat,
+                   "norbs": ms.norbs, "nocc": int(ms.s.nclose), "mpack": int(ms.mpack),
+                   "stored_pairs_within_A": mozyme.NDDO_CUTOFF, "dipole_shell_A": mozyme.DIPOLE_CUTOFF,
+                   "resonance_within_A": ms.overlap_cutoff, "scfcrt": HELIX_SCFCRT},
         "iterations": last["niter"], "converged": last["converged"], "escf": round(last["escf"], 6),
         "lewis_start_s": round(t_start, 4), "sparse_setup_s": round(t_setup, 3),
         "conventional_fp64": conv, "device": dev_name, "roofline": roof, "cpu_baseline": cpu,
@@ -406,6 +417,7 @@ def helix_cpu_baseline(ms, niter, sample_iters=6):
     xyz, am, dd, ad, w, h = f64(ms.xyz), f64(ms.am), f64(ms.dd), f64(ms.ad), f64(ms.w), f64(ms.h)
     t1 = time.perf_counter()
     for it in range(1, sample_iters + 1):
+        progress(f"cpu baseline iteration {it}")
         p = R.density(0)
         f = np.zeros(ms.mpack)
         lib.mz_fock2z(P(xyz), P(am), P(dd), P(ad), len(w), P(w), P(p), P(h), REF.A0, REF.EV, 1, REF.TRUNC1,

@@ -507,11 +507,16 @@ int mhip_pm7_corrections(int numat, const int* nat, const double* coord, double*
 /* The one-electron matrix on the MOZYME sparse storage (nijbo, mpack) and
  * the core-core energy as hcore_for_MOZYME forms them for a discrete system
  * (src/MOZYME/hcore_for_MOZYME.F90:100-260): NDDO terms for stored pairs,
- * outer2's point charge + dipole for the nijbo = -2 shell, outer1's point
- * charge for the rest.  The near-pair integrals are the conventional ones
- * (mhip_pm7_hamiltonian's w without the other pairs' blocks). */
+ * with the resonance block zero beyond overlap_cutoff (CUTOFS, Angstrom,
+ * default 7: fillij.F90:120-126, h1elec.F90:53) and from dipole_cutoff
+ * (CUTOF1, diat.F90:97), outer2's point charge + dipole for the nijbo = -2
+ * shell, outer1's point charge for the rest.  The near-pair integrals are the
+ * conventional ones (mhip_pm7_hamiltonian's w without the other pairs'
+ * blocks).  Returns -1 for an unsupported element, a nijbo beyond mpack or a
+ * cutoff <= 0. */
 int mhip_pm7_hamiltonian_mozyme(int numat, const int* nat, const double* coord, const int* nijbo,
-                                long long mpack, double* h, double* enuclr);
+                                long long mpack, double overlap_cutoff, double dipole_cutoff, double* h,
+                                double* enuclr);
 /* mhip_pm7_hamiltonian's atom-level outputs alone (nfirst, nlast, atheat,
  * pdiag), for systems whose pair terms are built sparsely. */
 int mhip_pm7_atoms(int numat, const int* nat, int charge, int* nfirst, int* nlast, double* atheat,

@@ -579,8 +579,12 @@ struct PairOut {
 
 // Everything one atom pair contributes: W block, both core-attraction blocks,
 // the resonance block and the core-core term (rotate + h1elec).  Atom A has
-// the larger index (hcore's i).
-void pair_integrals(const Element& A, const Element& B, const double* xa, const double* xb, PairOut& out) {
+// the larger index (hcore's i).  The resonance block is zero when r^2 >
+// res_cut_sq (h1elec's CUTOFS test) or r^2 >= ovl_cut_sq (diat's overlap
+// cutoff); hcore uses 1e4 and 225 (15 A), hcore_for_MOZYME the 7 A CUTOFS and
+// fillij's CUTOF1.
+void pair_integrals(const Element& A, const Element& B, const double* xa, const double* xb, PairOut& out,
+                    double res_cut_sq = 1.0e4, double ovl_cut_sq = 225.0) {
   const int li = A.natorb, lj = B.natorb;
   const int limij = li * (li + 1) / 2, limkl = lj * (lj + 1) / 2;
   std::memset(&out, 0, sizeof(out));
@@ -608,7 +612,7 @@ void pair_integrals(const Element& A, const Element& B, const double* xa, const 
   build_T(li, TA);
   build_T(lj, TB);
   // Resonance block: overlap in the molecular frame times (beta_a + beta_b)/2.
-  if (rsq < 225.0) {  // overlaps beyond 15 A are taken as zero
+  if (rsq <= res_cut_sq && rsq < ovl_cut_sq) {
     const int nA = A.nprin, nB = B.nprin;
     double S[4][4] = {};  // local: [a on A][b on B]
     S[0][0] = sto_overlap(nA, 0, A.p.zs, nB, 0, B.p.zs, 0, r);
@@ -841,14 +845,19 @@ double mhip_ev_to_kcal(void) { return kFPC9; }
 // The one-electron matrix on the MOZYME sparse storage and the core-core
 // energy, as hcore_for_MOZYME forms them for a discrete system
 // (src/MOZYME/hcore_for_MOZYME.F90:100-260): stored pairs (nijbo >= 0) take
-// the NDDO terms (resonance block, electron-core attraction, core-core);
+// the NDDO terms (electron-core attraction, core-core, and the resonance
+// block up to overlap_cutoff = CUTOFS, fillij.F90:120-126, and below
+// dipole_cutoff = CUTOF1, diat's overlap limit under MOZYME);
 // pairs of the dipole shell (-2) outer2's point charge plus dipole
 // (src/MOZYME/outer2.F90:16-150, from the NDDO (ss|ss), (s sigma|ss) and
 // (ss|s sigma) integrals); the others outer1's point charge
 // (src/MOZYME/outer1.F90:16-72) with PM7's switch to the bare Coulomb.
 int mhip_pm7_hamiltonian_mozyme(int numat, const int* nat, const double* coord, const int* nijbo,
-                                long long mpack, double* h, double* enuclr) {
+                                long long mpack, double overlap_cutoff, double dipole_cutoff, double* h,
+                                double* enuclr) {
   init_params();
+  if (!(overlap_cutoff > 0.0) || !(dipole_cutoff > 0.0)) return -1;
+  const double res_sq = overlap_cutoff * overlap_cutoff, ovl_sq = dipole_cutoff * dipole_cutoff;
   int norbs = 0;
   long long nw = 0;
   if (mhip_pm7_sizes(numat, nat, &norbs, &nw) != 0) return -1;
@@ -880,7 +889,7 @@ int mhip_pm7_hamiltonian_mozyme(int numat, const int* nat, const double* coord, 
         const int o = nij(i, j);
         if (o >= 0) {
           PairOut po;
-          pair_integrals(A, B, coord + 3 * i, coord + 3 * j, po);
+          pair_integrals(A, B, coord + 3 * i, coord + 3 * j, po, res_sq, ovl_sq);
           for (int a = 0; a < li; ++a)
             for (int b = 0; b < lj; ++b) h[o + a * lj + b] += po.di[a][b];
           std::memcpy(e1b, po.e1b, sizeof(e1b));

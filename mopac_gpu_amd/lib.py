@@ -161,7 +161,8 @@ SIGNATURES = {
     "mhip_den_write": (c_int, [ctypes.c_char_p, c_int, c_int, c_void_p, c_int]),
     "mhip_den_read": (c_int, [ctypes.c_char_p, c_int, c_int, c_void_p, c_int]),
     "mhip_pm7_element": (c_int, [c_int, c_void_p]),
-    "mhip_pm7_hamiltonian_mozyme": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_ll, c_void_p, ctypes.POINTER(c_double)]),
+    "mhip_pm7_hamiltonian_mozyme": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_ll, c_double, c_double, c_void_p,
+                                             ctypes.POINTER(c_double)]),
     "mhip_pm7_atoms": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, ctypes.POINTER(c_double), c_void_p]),
     "mhip_pm7_integrals_mozyme": (c_int, [c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_ll), c_void_p]),
     "mhip_mozyme_fock2z": (c_int, [c_void_p, c_void_p, c_void_p]),

@@ -60,16 +60,22 @@ class LmoSet:
         return LmoSet(**{k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in self.__dict__.items()})
 
 
+# fillij.F90:96-126: MOZYME's default cutoffs for a discrete system (CUTOF2,
+# CUTOF1, CUTOFS in Angstrom); with fewer than 30 atoms every pair is stored
+NDDO_CUTOFF, DIPOLE_CUTOFF, OVERLAP_CUTOFF = 9.9, 10.0, 7.0
+
+
 def build_nijbo(xyz, iorbs, rcut):
-    """Offsets of the stored atom-pair blocks: pairs within rcut (Angstrom)
-    and every atom with itself, laid out row by row (a, b <= a), a > b as
-    iorbs[a] x iorbs[b], a == b as a packed triangle (ijbo.F90:16-83)."""
+    """Offsets of the stored atom-pair blocks: pairs closer than rcut
+    (Angstrom, r^2 < rcut^2 as fillij compares) and every atom with itself,
+    laid out row by row (a, b <= a), a > b as iorbs[a] x iorbs[b], a == b as
+    a packed triangle (fillij.F90:150-270, ijbo.F90:16-83)."""
     xyz = np.asarray(xyz, dtype=np.float64)
     iorbs = np.asarray(iorbs, dtype=np.int64)
     numat = len(iorbs)
-    d = np.sqrt(((xyz[:, None, :] - xyz[None, :, :]) ** 2).sum(axis=2))
+    d2 = ((xyz[:, None, :] - xyz[None, :, :]) ** 2).sum(axis=2)
     lower = np.tril(np.ones((numat, numat), dtype=bool))
-    keep = lower & ((d <= rcut) | np.eye(numat, dtype=bool))
+    keep = lower & ((d2 < rcut * rcut) | np.eye(numat, dtype=bool))
     size = np.outer(iorbs, iorbs)
     np.fill_diagonal(size, iorbs * (iorbs + 1) // 2)
     a_idx, b_idx = np.nonzero(keep)          # row-major: a ascending, then b
@@ -81,13 +87,20 @@ def build_nijbo(xyz, iorbs, rcut):
     return nijbo, int(blk.sum())
 
 
-def build_nijbo_shells(xyz, iorbs, r_near, r_dipole):
-    """nijbo with the reference's three classes of pairs (fillij.F90): blocks
-    stored for pairs within r_near, -2 (point charge + dipole) up to r_dipole,
-    -1 (point charge) beyond."""
-    nijbo, size = build_nijbo(xyz, iorbs, r_near)
-    d = np.linalg.norm(xyz[:, None, :] - xyz[None, :, :], axis=2)
-    nijbo[(nijbo < 0) & (d <= r_dipole)] = -2
+def build_nijbo_shells(xyz, iorbs, r_near=None, r_dipole=None):
+    """nijbo with the reference's three classes of pairs (fillij.F90:96-245):
+    blocks stored for r^2 < r_near^2, -2 (point charge + dipole) for
+    r^2 < r_dipole^2, -1 (point charge) beyond.  Left at None, the cutoffs are
+    fillij's defaults: 9.9 and 10 Angstrom, and every pair stored when the
+    system has fewer than 30 atoms; given, they act like CUTOF2= / CUTOF1=."""
+    xyz = np.asarray(xyz, dtype=np.float64)
+    small = len(xyz) < 30 and r_near is None and r_dipole is None
+    r2_near = 1.0e6 if small else (NDDO_CUTOFF if r_near is None else r_near) ** 2
+    r2_dip = 1.0e6 + 10.0 if small else (DIPOLE_CUTOFF if r_dipole is None else r_dipole) ** 2
+    r2_dip = max(r2_dip, r2_near + 1.0e-4)  # fillij.F90:130-134
+    nijbo, size = build_nijbo(xyz, iorbs, np.sqrt(r2_near))
+    d2 = ((xyz[:, None, :] - xyz[None, :, :]) ** 2).sum(axis=2)
+    nijbo[(nijbo < 0) & (d2 < r2_dip)] = -2
     return nijbo, size
 
 
@@ -129,11 +142,14 @@ def element_parameters(nat):
 
 class MozymeSystem:
     """A system in MOZYME storage: sparse H, integrals for SEMIDIRECT, nijbo
-    with near / dipole / point-charge pairs, per-atom parameters."""
+    with near / dipole / point-charge pairs, per-atom parameters.  The
+    cutoffs default to MOZYME's own (build_nijbo_shells; resonance blocks
+    to overlap_cutoff = CUTOFS = 7 Angstrom)."""
 
     TRUNC1, TRUNC2 = 7.0, 0.22  # PM7's switch to point charges (molkst_C trunc_1, trunc_2)
 
-    def __init__(self, system, r_near=6.0, r_dipole=7.0, nijbo=None, mpack=None, feather=True):
+    def __init__(self, system, r_near=None, r_dipole=None, nijbo=None, mpack=None, feather=True,
+                 overlap_cutoff=OVERLAP_CUTOFF):
         self.s = system
         self.numat, self.norbs = system.numat, system.norbs
         self.nat = np.asarray(system.nat, dtype=np.int32)
@@ -149,9 +165,18 @@ class MozymeSystem:
         from . import lib as L
         self.h = np.zeros(self.mpack)
         en = ctypes.c_double()
+        # diat's overlap limit under MOZYME is CUTOF1 (fillij.F90:96-101)
+        if r_dipole is not None:
+            cutof1 = r_dipole
+        elif r_near is None and self.numat < 30:
+            cutof1 = np.sqrt(1.0e6 + 10.0)
+        else:
+            cutof1 = DIPOLE_CUTOFF
+        self.overlap_cutoff = float(overlap_cutoff)
         rc = L.load().mhip_pm7_hamiltonian_mozyme(
             self.numat, L.ptr(np.ascontiguousarray(self.nat)), L.ptr(self.xyz),
-            L.ptr(np.asfortranarray(self.nijbo.astype(np.int32))), self.mpack, L.ptr(self.h), ctypes.byref(en))
+            L.ptr(np.asfortranarray(self.nijbo.astype(np.int32))), self.mpack, self.overlap_cutoff, float(cutof1),
+            L.ptr(self.h), ctypes.byref(en))
         if rc != 0:
             raise ValueError("mhip_pm7_hamiltonian_mozyme: unsupported element or nijbo beyond mpack")
         self.enuclr = en.value
@@ -168,7 +193,8 @@ class MozymeSystem:
         self.feather = bool(feather)
 
     @classmethod
-    def from_geometry(cls, nat, xyz, r_near=6.0, r_dipole=7.0, charge=0, feather=True):
+    def from_geometry(cls, nat, xyz, r_near=None, r_dipole=None, charge=0, feather=True,
+                      overlap_cutoff=OVERLAP_CUTOFF):
         """A MOZYME system straight from the geometry: the atom data and
         the sparse one- and two-electron terms only, without the
         conventional system's full W and packed H (which do not fit a
@@ -191,7 +217,7 @@ class MozymeSystem:
         nel = int(sum(TORE[int(z)] for z in nat)) - charge
         system = SimpleNamespace(nat=nat, xyz=xyz, numat=numat, norbs=norbs.value, nfirst=nf, nlast=nl,
                                  atheat=at.value, pdiag=pd, nclose=nel // 2, charge=charge)
-        return cls(system, r_near=r_near, r_dipole=r_dipole, feather=feather)
+        return cls(system, r_near=r_near, r_dipole=r_dipole, feather=feather, overlap_cutoff=overlap_cutoff)
 
     def diagonal_density(self):
         """The standard starting density (pdiag on the diagonal) in sparse storage."""

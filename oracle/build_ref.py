@@ -63,6 +63,9 @@ SOURCES = [
     "integrals/mndod_C.F90",
     "integrals/mndod.F90",
     "MOZYME/fock2z.F90",
+    "MOZYME/tidy.F90",
+    "MOZYME/check.F90",
+    "MOZYME/eimp.F90",
     # the PM7 producer's reference: parameters, derived multipoles, pair
     # integrals (rotate/rotatd/reppd/spcore/ccrep), overlaps (h1elec/diat/set)
     "conref_C.F90",
@@ -78,6 +81,14 @@ SOURCES = [
     "integrals/set.F90",
     "integrals/coe.F90",
     "integrals/ccrep.F90",
+    "input/digit.F90",
+    "utilities/reada.F90",
+    "integrals/solrot.F90",
+    "MOZYME/outer1.F90",
+    "MOZYME/outer2.F90",
+    "MOZYME/fillij.F90",
+    "MOZYME/add_more_interactions.F90",
+    "MOZYME/hcore_for_MOZYME.F90",
     # PM7's post-SCF corrections (dispersion, hydrogen bonds) and the bonding
     # topology they read
     "chemistry/geochk_bits_1.F90",

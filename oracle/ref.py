@@ -50,6 +50,10 @@ def load():
         lib.pm7_ref_element.argtypes = [I, V]
         lib.pm7_ref_pair.argtypes = [I, I, V, V, V, V, V, V, V]
         lib.pm7_ref_corrections.argtypes = [I, V, V, V]
+        lib.pm7_ref_mozyme_hcore.argtypes = [I, V, V, D, D, D, V, I, V, V, V]
+        lib.mz_tidy.argtypes = [I, V, V, I, V, I, V, V, I, D, V, V, V]
+        lib.mz_check.argtypes = [I, V, V, V, I, V, V, I, V]
+        lib.mz_eimp.argtypes = [V, V]
         _lib = lib
     return _lib
 
@@ -293,3 +297,71 @@ def pm7_corrections(nat, xyz):
     out = np.zeros(3)
     lib.pm7_ref_corrections(len(nat), P(nat), P(xyz), P(out))
     return out
+
+
+def pm7_mozyme_hcore(nat, xyz, cutof2=0.0, cutof1=0.0, cutofs=0.0):
+    """fillij + hcore_for_MOZYME on a discrete molecule (SEMIDIRECT), the
+    cutoffs MOZYME's defaults unless given (Angstrom, as the CUTOF2= /
+    CUTOF1= / CUTOFS= keywords): (nijbo (numat x numat, 0-based offsets, -2,
+    -1), mpack, h (sparse one-electron matrix), enuclr (eV))."""
+    lib = load()
+    lib.pm7_ref_init()
+    nat = np.ascontiguousarray(nat, dtype=np.int32)
+    xyz = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
+    n = len(nat)
+    nij = np.zeros((n, n), dtype=np.int32, order="F")
+    mp = ctypes.c_int()
+    en = ctypes.c_double()
+    one = np.zeros(1)
+    lib.pm7_ref_mozyme_hcore(n, P(nat), P(xyz), cutof2, cutof1, cutofs, P(nij), 0, ctypes.byref(mp), P(one),
+                             ctypes.byref(en))
+    h = np.zeros(mp.value)
+    lib.pm7_ref_mozyme_hcore(n, P(nat), P(xyz), cutof2, cutof1, cutofs, P(nij), mp.value, ctypes.byref(mp), P(h),
+                             ctypes.byref(en))
+    return np.array(nij), mp.value, h, en.value
+
+
+def _mz_setup_atoms(numat, norbs, mpack, nfirst, nlast, nat, nijbo):
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+    keep = [i32(nfirst), i32(nlast), i32(nat), np.asfortranarray(i32(nijbo))]
+    load().mz_setup(numat, norbs, mpack, *[P(a) for a in keep], 1e-13, 0.0)
+    return keep
+
+
+def mozyme_tidy(numat, norbs, iorbs, nc, nnc, ncmo, ic, c, thresh, mode=1):
+    """tidy on one LMO set (copies in, copies out): returns (nc, nnc, ncmo,
+    ic, c, ln, mn, err)."""
+    iorbs = np.asarray(iorbs, dtype=np.int32)
+    nl = np.cumsum(iorbs).astype(np.int32)
+    keep = _mz_setup_atoms(numat, norbs, 1, nl - iorbs + 1, nl, np.ones(numat), np.zeros((numat, numat)))
+    a = [np.array(x, dtype=np.int32) for x in (nc, ic, nnc, ncmo)]
+    cc = np.array(c, dtype=np.float64)
+    ln, mn, err = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    load().mz_tidy(len(a[0]), P(a[0]), P(a[1]), len(a[1]), P(cc), len(cc), P(a[2]), P(a[3]), mode, thresh,
+                   ctypes.byref(ln), ctypes.byref(mn), ctypes.byref(err))
+    del keep
+    return a[0], a[2], a[3], a[1], cc, ln.value, mn.value, err.value
+
+
+def mozyme_check(numat, norbs, iorbs, nc, nnc, ncmo, ic, c):
+    """check on one LMO set: returns (c renormalized, ws = squared norms)."""
+    iorbs = np.asarray(iorbs, dtype=np.int32)
+    nl = np.cumsum(iorbs).astype(np.int32)
+    keep = _mz_setup_atoms(numat, norbs, 1, nl - iorbs + 1, nl, np.ones(numat), np.zeros((numat, numat)))
+    nv = len(nc)
+    pad = lambda x: np.concatenate([np.asarray(x, dtype=np.int32), [0]]).astype(np.int32)  # noqa: E731
+    a = [pad(nnc), pad(nc), np.array(ic, dtype=np.int32), pad(ncmo)]
+    cc = np.array(c, dtype=np.float64)
+    ws = np.zeros(nv)
+    load().mz_check(nv, P(a[0]), P(a[1]), P(a[2]), len(a[2]), P(a[3]), P(cc), len(cc), P(ws))
+    del keep
+    return cc, ws
+
+
+def mozyme_eimp(numat, norbs, nfirst, nlast, nijbo, mpack, f, p):
+    """eimp on sparse F and P: returns the updated P."""
+    keep = _mz_setup_atoms(numat, norbs, mpack, nfirst, nlast, np.ones(numat), nijbo)
+    pp = np.array(p, dtype=np.float64)
+    load().mz_eimp(P(np.ascontiguousarray(f, dtype=np.float64)), P(pp))
+    del keep
+    return pp

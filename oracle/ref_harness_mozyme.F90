@@ -230,4 +230,60 @@ contains
     deallocate (q, qe, ptot2)
   end subroutine mz_fock2z
 
+
+  ! tidy (src/MOZYME/tidy.F90:16-244) on one LMO set, at a single point
+  ! (step_num 1: one compression pass, then the free space spread over the
+  ! LMOs).  mode 1 occupied, 2 virtual.  Needs mz_setup (numat, norbs, iorbs).
+  ! err = 1 when tidy finds the storage too small (moperr).
+  subroutine mz_tidy(nmo, nc, ic, n01, c, n02, nnc, ncmo, mode, thresh_in, ln, mn, err) bind(C, name="mz_tidy")
+    use molkst_C, only: step_num, step_num0, moperr, use_disk
+    use MOZYME_C, only: thresh
+    integer(c_int), value :: nmo, n01, n02, mode
+    integer(c_int) :: nc(nmo), ic(n01), nnc(nmo), ncmo(nmo), ln, mn, err
+    real(c_double) :: c(n02)
+    real(c_double), value :: thresh_in
+    integer :: l1, m1
+    step_num = 1
+    step_num0 = 0
+    use_disk = .false.
+    moperr = .false.
+    thresh = thresh_in
+    call tidy(nmo, nc, ic, n01, c, n02, nnc, ncmo, l1, m1, mode)
+    ln = l1
+    mn = m1
+    err = 0
+    if (moperr) err = 1
+    moperr = .false.
+  end subroutine mz_tidy
+
+  ! check (src/MOZYME/check.F90:16-113): renormalize nvec LMOs; ws_out(i) the
+  ! squared norm found.  Arrays indexed to nvec+1 as check declares them.
+  ! Only for sets whose summed error is <= 0.1 (above that check stops the
+  ! job through mopend, which is not part of this library).
+  subroutine mz_check(nvec, nnc, nc, ic, ic_dim, ncvec, c, c_dim, ws_out) bind(C, name="mz_check")
+    use MOZYME_C, only: iorbs, ws
+    integer(c_int), value :: nvec, ic_dim, c_dim
+    integer(c_int) :: nnc(nvec + 1), nc(nvec + 1), ic(ic_dim), ncvec(nvec + 1)
+    real(c_double) :: c(c_dim), ws_out(nvec)
+    if (allocated(ws)) deallocate (ws)
+    allocate (ws(nvec + 1))
+    call check(nvec, nnc, nc, ic, ic_dim, iorbs, ncvec, c, c_dim)
+    ws_out = ws(1:nvec)
+  end subroutine mz_check
+
+  ! eimp (src/MOZYME/eimp.F90:16-45) on the sparse storage of mz_setup:
+  ! p_io comes back with the first slot of each stored off-diagonal block set
+  ! to the sum of squares of that block of f_in.
+  subroutine mz_eimp(f_in, p_io) bind(C, name="mz_eimp")
+    use molkst_C, only: mpack
+    use common_arrays_C, only: p, f
+    real(c_double) :: f_in(mpack), p_io(mpack)
+    if (allocated(f)) deallocate (f)
+    allocate (f(mpack))
+    f = f_in
+    p = p_io
+    call eimp()
+    p_io = p
+  end subroutine mz_eimp
+
 end module ref_harness_mozyme

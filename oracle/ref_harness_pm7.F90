@@ -18,6 +18,9 @@
 !   pm7_ref_corrections     PM6_DH_Dispersion and Hydrogen_bond_corrections
 !                           for a molecule, as post_scf_corrections calls them
 !                           for PM7 (src/corrections/post_scf_corrections.F90:112-114)
+!   pm7_ref_mozyme_hcore    fillij then hcore_for_MOZYME on a discrete molecule
+!                           (SEMIDIRECT): nijbo, the sparse one-electron matrix
+!                           and the core-core energy
 module ref_harness_pm7
   use iso_c_binding
   implicit none
@@ -175,5 +178,93 @@ contains
     out(2) = Hydrogen_bond_corrections(.false., .false.)
     out(3) = N_Hbonds
   end subroutine pm7_ref_corrections
+
+  ! fillij(.false.) then hcore_for_MOZYME() for a discrete molecule, as
+  ! set_up_MOZYME_arrays and compfg reach them (SEMIDIRECT integrals).  The
+  ! cutoffs are MOZYME's defaults unless cut2 / cut1 / cuts (Angstrom) are > 0,
+  ! which go in as the CUTOF2= / CUTOF1= / CUTOFS= keywords.  nijbo_out gets
+  ! fillij's nijbo (0-based offsets, -2 dipole shell, -1 point charge),
+  ! mpack_out the sparse size; h_out (capacity hcap) the one-electron matrix
+  ! and enuclr_out the core-core energy (eV) when mpack <= hcap.  The
+  ! conventional cutoffs pm7_ref_init sets are restored on return.
+  subroutine pm7_ref_mozyme_hcore(numat_in, nat_in, xyz, cut2, cut1, cuts, nijbo_out, hcap, mpack_out, h_out, &
+      enuclr_out) bind(C, name="pm7_ref_mozyme_hcore")
+    use molkst_C, only: numat, natoms, norbs, id, keywrd, mpack, n2elec, cutofp, ispd, numcal, enuclr
+    use common_arrays_C, only: coord, nat, h, w, uspd
+    use MOZYME_C, only: iorbs, nijbo, lijbo, jopt, numred, mode, cutofs
+    use overlaps_C, only: cutof1, cutof2
+    use parameters_C, only: natorb, uss, upp
+    use cosmo_C, only: useps
+    integer(c_int), value :: numat_in, hcap
+    integer(c_int) :: nat_in(numat_in), nijbo_out(numat_in, numat_in), mpack_out
+    real(c_double) :: xyz(3, numat_in), h_out(*), enuclr_out
+    real(c_double), value :: cut2, cut1, cuts
+    character(len=40) :: word
+    integer :: i, k, m
+    numat = numat_in
+    natoms = numat_in
+    id = 0
+    ispd = 0
+    cutofp = 1.d10
+    useps = .false.
+    keywrd = ' PM7 MOZYME 1SCF SEMIDIRECT'
+    if (cut2 > 0.d0) then
+      write (word, '(A,F0.6)') ' CUTOF2=', cut2
+      keywrd = trim(keywrd) // trim(word)
+    end if
+    if (cut1 > 0.d0) then
+      write (word, '(A,F0.6)') ' CUTOF1=', cut1
+      keywrd = trim(keywrd) // trim(word)
+    end if
+    if (cuts > 0.d0) then
+      write (word, '(A,F0.6)') ' CUTOFS=', cuts
+      keywrd = trim(keywrd) // trim(word)
+    end if
+    numcal = numcal + 1
+    if (allocated(coord)) deallocate (coord)
+    if (allocated(nat)) deallocate (nat)
+    if (allocated(iorbs)) deallocate (iorbs)
+    if (allocated(nijbo)) deallocate (nijbo)
+    if (allocated(jopt)) deallocate (jopt)
+    allocate (coord(3, numat), nat(numat), iorbs(numat), jopt(numat))
+    coord = xyz
+    nat = nat_in
+    norbs = 0
+    do i = 1, numat
+      iorbs(i) = natorb(nat(i))
+      norbs = norbs + iorbs(i)
+    end do
+    jopt = 0
+    numred = numat
+    mode = 0
+    lijbo = .true.
+    call fillij(.false.)
+    mpack_out = mpack
+    nijbo_out = nijbo
+    if (mpack <= hcap) then
+      if (allocated(h)) deallocate (h)
+      if (allocated(w)) deallocate (w)
+      if (allocated(uspd)) deallocate (uspd)
+      allocate (h(mpack), w(n2elec + 2025), uspd(norbs))
+      m = 0
+      do i = 1, numat
+        do k = 1, iorbs(i)
+          m = m + 1
+          if (k == 1) then
+            uspd(m) = uss(nat(i))
+          else
+            uspd(m) = upp(nat(i))
+          end if
+        end do
+      end do
+      call hcore_for_MOZYME()
+      h_out(1:mpack) = h(1:mpack)
+      enuclr_out = enuclr
+    end if
+    deallocate (nijbo)
+    cutofs = 1.d4
+    cutof2 = 1.d10
+    cutof1 = 225.d0
+  end subroutine pm7_ref_mozyme_hcore
 
 end module ref_harness_pm7

@@ -9,6 +9,10 @@ from the reference sources into oracle/_ref (oracle/build_ref.py):
   - whole molecules (glycine, tri-glycine helix, methylamine, water
     clusters): H, W and the core-core energy against the reference's pair
     routines assembled in hcore's order, and the isolated-atom heat;
+  - the MOZYME storage: nijbo of fillij (stored / dipole / point-charge
+    pairs, the under-30-atom rule, CUTOF2 / CUTOF1 / CUTOFS) and the sparse
+    one-electron matrix and core-core energy of hcore_for_MOZYME (outer1,
+    outer2 and the 7 A resonance cutoff);
   - PM6_DH_Dispersion and Hydrogen_bond_corrections on molecules with and
     without hydrogen bonds;
   - heats of formation against the reference's printed known answers.
@@ -194,3 +198,26 @@ def test_heat_of_formation_matches_printed(name):
         # the printed energy split: electronic and core-core energies (eV)
         assert abs(s.enuclr - g["core_core_ev"]) < 6e-6
         assert abs(o["ee"] - g["electronic_ev"]) < 2e-5
+
+
+@pytest.mark.parametrize("nres,cut", [(1, None), (6, None), (12, None), (6, (6.0, 7.5, 5.0)),
+                                      (12, (8.0, 8.5, 9.0))])
+def test_mozyme_storage_and_hamiltonian_match_fillij_hcore(nres, cut):
+    """MozymeSystem's nijbo, sparse H and enuclr against the reference's
+    fillij + hcore_for_MOZYME: Gly1 (under 30 atoms: every pair stored), Gly6
+    and Gly12 helices at MOZYME's default cutoffs (all three pair classes on
+    Gly12), and keyword cutoffs, one with CUTOFS beyond CUTOF1."""
+    from mopac_gpu_amd import mozyme
+    nat, xyz = geometry.glycine_helix(nres, 3)
+    kw = {} if cut is None else dict(cutof2=cut[0], cutof1=cut[1], cutofs=cut[2])
+    nij, mp, h, en = ref.pm7_mozyme_hcore(nat, xyz, **kw)
+    ms = mozyme.MozymeSystem.from_geometry(nat, xyz, r_near=kw.get("cutof2"), r_dipole=kw.get("cutof1"),
+                                           overlap_cutoff=kw.get("cutofs", mozyme.OVERLAP_CUTOFF))
+    assert mp == ms.mpack
+    assert np.array_equal(nij, ms.nijbo)
+    if nres == 1:
+        assert (nij >= 0).all()
+    if nres == 12 and cut is None:
+        assert (nij == -2).any() and (nij == -1).any()
+    assert np.max(np.abs(h - ms.h)) <= 1e-13 * np.max(np.abs(h))
+    assert abs(en - ms.enuclr) <= 1e-13 * abs(en)
