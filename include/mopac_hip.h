@@ -250,11 +250,17 @@ int mhip_mozyme_diagg1(mhip_ctx* ctx, const mhip_lmo_set* lmo, const double* fao
 /* diagg2's saved rejection counts (diagg2.F90:55,338-339) and the MOZYME_C
  * scalars it uses: tiny (from diagg1, updated here), sumb (out), damp (the
  * DAMP constant, 1 for main-group molecules), shift, thresh (1e-13 by
- * default, iter_for_MOZYME.F90:86-97). */
+ * default, iter_for_MOZYME.F90:86-97).  order_free (no counterpart in the
+ * reference): 0 applies the rotations in list order as diagg2 does; 1 applies
+ * the same rotations in rounds of LMO-disjoint pairs (a first-fit colouring
+ * of the list), which changes the order of rotations that share an LMO and
+ * shortens the chain of dependent rotations to about the largest number of
+ * rotations on one LMO. */
 typedef struct {
   int reset;
   double tiny, sumb, damp, shift, thresh;
   int nrejct[2];
+  int order_free;
 } mhip_diagg2_state;
 
 /* diagg2(nocc, nvir, eigv, ..., nij, idiagg, ...) (src/MOZYME/diagg2.F90:
@@ -264,12 +270,33 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* lmo, const double* eigs, con
                        const double* fmo, const int* ifmo, int nij, int idiagg,
                        mhip_diagg2_state* st);
 /* No counterpart in the reference.  HIP-event time of the diagg2 rotation
- * levels on the MOZYME stream, with the rotations, levels, launches and
+ * kernels on the MOZYME stream, with the rotations, levels, launches,
  * algorithmic bytes (each rotation reads and writes both LMOs' coefficients,
- * sized on entry) since the last reset; enable = 1 / 0 resets and switches
- * the count on / off, -1 only reads. */
+ * sized on entry) and, for the one-workgroup kernel, the rounds (a level of
+ * w rotations takes ceil(w / waves) rounds) since the last reset; enable =
+ * 1 / 0 resets and switches the count on / off, -1 only reads. */
 int mhip_mozyme_stats(int enable, double* ms, long long* rotations, long long* levels, long long* launches,
-                      double* bytes);
+                      double* bytes, long long* rounds);
+/* No counterpart in the reference.  Selects how mhip_mozyme_diagg2 runs the
+ * rotation levels: 1 (default) on the LDS rotation kernels, a wave per
+ * rotation with its atom maps and lists in LDS: runs of narrow levels in one
+ * launch of a single workgroup that meets at a barrier between levels, and
+ * each level wider than mhip_mozyme_diagg2_wide_rounds rounds of that
+ * workgroup across the chip, a workgroup per rotation; they apply when
+ * numat and norbs are below 65535, the maps fit in LDS and two LMOs' atom
+ * regions total at most 1024 atoms.  0, or when they do not apply: one
+ * launch per level of the kernel with global maps.  All give identical
+ * results.  enable = -1 only reads; returns the previous setting. */
+int mhip_mozyme_diagg2_one_workgroup(int enable);
+/* No counterpart in the reference.  The width, in rounds of the one-workgroup
+ * kernel, above which a level runs across the chip (default 3); rounds = -1
+ * only reads; returns the previous value. */
+int mhip_mozyme_diagg2_wide_rounds(int rounds);
+/* No counterpart in the reference.  The LDS rotation kernels copy the two
+ * LMOs' atom lists into LDS when they hold at most cap atoms together
+ * (default and maximum 1024) and read them from global memory otherwise;
+ * results are identical.  cap = -1 only reads; returns the previous value. */
+int mhip_mozyme_diagg2_list_cap(int cap);
 
 /* helecz() (src/MOZYME/helecz.F90:16-52): E = sum over stored atom pairs of
  * P (H + F), diagonal elements weighted 1/2, on the sparse nijbo storage. */
@@ -330,6 +357,12 @@ typedef struct {
   int fmo_dim;      /* occupied-virtual list capacity (0: automatic) */
   double enuclr;    /* eV */
   double atheat;    /* kcal/mol */
+  int print_level;  /* 1: one line per iteration on stderr (iteration,
+                       heat, its change, largest occupied-virtual element) */
+  double* heats;    /* if not NULL: heat after iteration k at heats[k - 1], */
+  int heats_cap;    /* for k <= heats_cap */
+  int order_free;   /* diagg2's rotations in LMO-disjoint rounds
+                       (mhip_diagg2_state.order_free); 0: list order */
 } mhip_mozyme_opts;
 typedef struct {
   double escf;      /* heat of formation, kcal/mol (no geometry-only terms) */

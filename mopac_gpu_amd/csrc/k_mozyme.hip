@@ -423,6 +423,7 @@ __global__ void k_mz_occ_energy(LmoDev L, const double* __restrict__ fao, const 
 struct D2Params {
   double tiny, biglim, cnst, shift, thresh;
   int retry, nij;
+  int lcap;  // two LMOs' atoms at most this many: lists in LDS (<= D2W_LCAP)
 };
 
 constexpr int D2_NT = 64;  // one wave: the per-rotation scans run at wave latency
@@ -727,6 +728,327 @@ __global__ __launch_bounds__(D2_NT) void k_mz_diagg2(LmoDev L, const double* __r
   }
 }
 
+// ---------------------------------------------------------------- diagg2, one workgroup
+// The same rotations, every level of a diagg2 call in one launch: a single
+// workgroup of D2W_NW waves on one CU, one wave per rotation, the waves
+// taking the rotations of a level in turn and meeting at a workgroup barrier
+// between levels.  All the waves share the CU's L1 and L2, so the barrier is
+// the only synchronization, and a level costs one rotation's latency instead
+// of a kernel launch.  Each wave keeps its atom maps in LDS (iused: offset of
+// an atom's orbitals in occupied LMO j, 0xffff when absent; latoms: atom in
+// virtual LMO i), entries set only for the atoms of the two LMOs and cleared
+// after the rotation; scans and counts are wave shuffles and ballots.
+constexpr int D2W_NW = 16;        // waves of the workgroup (at most)
+constexpr unsigned short D2W_NONE = 0xffff;
+
+__device__ __forceinline__ int wave_scan_incl(int v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// order of this wave's memory operations kept by the compiler across phases
+// (the hardware keeps a wave's vector memory and LDS operations in order)
+__device__ __forceinline__ void wave_phase() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// Per-wave LDS of the rotation kernels: the atom maps (iused, latoms) and
+// copies of the two LMOs' atom lists (0-based atom numbers) when the two
+// hold at most D2W_LCAP atoms together, which bounds both lists after the
+// rotation (each ends as a subset of the union of the two); larger pairs
+// read their lists from global memory.
+constexpr int D2W_LCAP = 1024;
+struct D2Wave {
+  unsigned short* iused;   // numat
+  unsigned char* latoms;   // numat
+  unsigned short* lj;      // D2W_LCAP
+  unsigned short* li;      // D2W_LCAP
+};
+
+// bytes of LDS a workgroup of nw waves uses: iorbs (numat bytes, shared),
+// then per wave 3 numat + 4 D2W_LCAP
+__host__ __device__ inline size_t d2w_lds_bytes(int numat, int nw) {
+  return (((size_t)numat + 15) & ~(size_t)15) +
+         (size_t)nw * ((((size_t)3 * numat + 15) & ~(size_t)15) + 4 * (size_t)D2W_LCAP);
+}
+
+__device__ __forceinline__ D2Wave d2w_carve(unsigned char* lds, int numat, int nw, int w) {
+  unsigned char* base = lds + (((size_t)numat + 15) & ~(size_t)15) +
+                        (size_t)w * ((((size_t)3 * numat + 15) & ~(size_t)15) + 4 * (size_t)D2W_LCAP);
+  D2Wave v;
+  v.iused = reinterpret_cast<unsigned short*>(base);
+  v.latoms = base + 2 * (size_t)numat;
+  unsigned char* lists = base + (((size_t)3 * numat + 15) & ~(size_t)15);
+  v.lj = reinterpret_cast<unsigned short*>(lists);
+  v.li = v.lj + D2W_LCAP;
+  return v;
+}
+
+// One rotation by one wave.  Atom lists and orbital counts come from LDS,
+// so the dependent global reads are the list entry, the LMO offsets, the
+// atom lists and the coefficients; everything the wave re-reads inside the
+// rotation is in LDS or registers, except the undo log on a rejection (and
+// the lists of an LMO pair too large for the LDS copies).
+__device__ void d2w_rotation(const LmoDev& L, const unsigned char* __restrict__ iorbs_s,
+                             const double* __restrict__ eigs, const double* __restrict__ eigv,
+                             const double* __restrict__ fmo, const int* __restrict__ ifmo, const D2Params& P,
+                             const int ij, const D2Wave& M, double* __restrict__ storej, double* __restrict__ storei,
+                             double* __restrict__ sumb_out, int* __restrict__ nrej_out, int* __restrict__ status,
+                             const int lane) {
+  const unsigned long long below = (1ull << lane) - 1ull;
+  double sumb = 0.0;
+  int nrej = 0;
+  const int i = ifmo[2 * ij] - 1, j = ifmo[2 * ij + 1] - 1;
+  const double c = fmo[ij] * P.cnst;
+  const double d = eigs[j] - eigv[i] - P.shift;
+  const int ncfj = L.ncf[j], ncei = L.nce[i];
+  const int nnj = L.nncf[j], nni = L.nnce[i];
+  const int jlr = L.ncocc[j], ilr = L.ncvir[i];
+  int jur, jncf, iur, incv;
+  if (j != L.nocc - 1) { jur = L.ncocc[j + 1]; jncf = L.nncf[j + 1]; }
+  else { jur = L.cocc_dim; jncf = L.icocc_dim; }
+  jur = min(jlr + L.norbs, jur);
+  if (i != L.nvir - 1) { iur = L.ncvir[i + 1]; incv = L.nnce[i + 1]; }
+  else { iur = L.cvir_dim; incv = L.icvir_dim; }
+  iur = min(ilr + L.norbs, iur);
+  // atom lists into LDS when they fit (every list entry the rotation reads
+  // lies below ncfj + ncei), undo log of the coefficients the LMOs hold now
+  const bool in_lds = ncfj + ncei <= P.lcap;
+  const int* __restrict__ gj = L.icocc + nnj;
+  const int* __restrict__ gi = L.icvir + nni;
+  auto atom_j = [&](int k) { return in_lds ? (int)M.lj[k] : gj[k] - 1; };
+  auto atom_i = [&](int k) { return in_lds ? (int)M.li[k] : gi[k] - 1; };
+  if (in_lds) {
+    for (int k = lane; k < ncfj; k += 64) M.lj[k] = (unsigned short)(gj[k] - 1);
+    for (int k = lane; k < ncei; k += 64) M.li[k] = (unsigned short)(gi[k] - 1);
+  }
+  const int bj = min(4 * ncfj, jur - jlr), bi = min(4 * ncei, iur - ilr);
+  for (int k = lane; k < bj; k += 64) storej[k] = L.cocc[jlr + k];
+  for (int k = lane; k < bi; k += 64) storei[k] = L.cvir[ilr + k];
+  const double e = copysign(sqrt(4.0 * c * c + d * d), d);
+  double alpha = sqrt(0.5 * (1.0 + d / e));
+  wave_phase();
+  for (;;) {
+    const double beta = -copysign(sqrt(1.0 - alpha * alpha), c);
+    sumb += fabs(beta);
+    // offsets of LMO j's atoms
+    int mlf = 0;
+    for (int c0 = 0; c0 < ncfj; c0 += 64) {
+      const int lf = c0 + lane;
+      const int a = (lf < ncfj) ? atom_j(lf) : -1;
+      const int no = (a >= 0) ? (int)iorbs_s[a] : 0;
+      const int incl = wave_scan_incl(no, lane);
+      if (a >= 0) M.iused[a] = (unsigned short)(mlf + incl - no);
+      mlf += __shfl(incl, 63, 64);
+    }
+    wave_phase();
+    int jext = bj, iext = bi;
+    bool fail = false;
+    // virtual LMO atoms, in order: rotate common atoms, grow LMO j
+    int mle_base = 0, mlf_run = mlf, ncf_run = ncfj;
+    for (int c0 = 0; c0 < ncei; c0 += 64) {
+      const int le = c0 + lane;
+      int mie = -1, no = 0;
+      if (le < ncei) {
+        mie = atom_i(le);
+        no = iorbs_s[mie];
+      }
+      const int incl = wave_scan_incl(no, lane);
+      const int mle = mle_base + incl - no;
+      const int chunk_orbs = __shfl(incl, 63, 64);
+      bool grow = false;
+      double cv[4] = {0.0, 0.0, 0.0, 0.0};
+      if (le < ncei) {
+        M.latoms[mie] = 1;
+        const int u = M.iused[mie];
+        for (int x = 0; x < no; ++x) cv[x] = L.cvir[ilr + mle + x];
+        if (u != D2W_NONE) {
+          for (int x = 0; x < no; ++x) {
+            const double a = L.cocc[jlr + u + x], b = cv[x];
+            L.cocc[jlr + u + x] = alpha * a + beta * b;
+            L.cvir[ilr + mle + x] = alpha * b - beta * a;
+          }
+        } else {
+          double s2 = 0.0;
+          for (int x = 0; x < no; ++x) {
+            const double t = beta * cv[x];
+            s2 += t * t;
+          }
+          grow = (s2 > P.thresh);
+        }
+      }
+      const unsigned long long gm = __ballot(grow);
+      if (gm) {
+        const int rank = __popcll(gm & below);
+        const int gincl = wave_scan_incl(grow ? no : 0, lane);
+        const int slot = nnj + ncf_run + rank;
+        const int obase = mlf_run + gincl - no;
+        // the reference stops at the first atom that would not fit
+        fail = __any(grow && (slot >= jncf || jlr + obase + no > jur));
+        if (!fail && grow) {
+          L.icocc[slot] = mie + 1;
+          if (in_lds) M.lj[ncf_run + rank] = (unsigned short)mie;
+          M.iused[mie] = (unsigned short)obase;
+          for (int x = 0; x < no; ++x) {
+            storej[obase + x] = L.cocc[jlr + obase + x];
+            L.cocc[jlr + obase + x] = beta * cv[x];
+            L.cvir[ilr + mle + x] = alpha * cv[x];
+          }
+        }
+        if (fail) break;
+        ncf_run += __popcll(gm);
+        mlf_run += __shfl(gincl, 63, 64);
+        jext = max(jext, mlf_run);
+      }
+      mle_base += chunk_orbs;
+    }
+    if (in_lds) wave_phase();
+    else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // grown atoms re-read from global
+    if (!fail) {
+      if (lane == 0) L.ncf[j] = ncf_run;
+      // occupied LMO atoms not in the virtual LMO: grow LMO i, in order
+      int mle_run = mle_base, nce_run = ncei;
+      for (int c0 = 0; c0 < ncf_run; c0 += 64) {
+        const int lf = c0 + lane;
+        int ii = -1, no = 0, u = 0;
+        bool grow = false;
+        double cj[4] = {0.0, 0.0, 0.0, 0.0};
+        if (lf < ncf_run) {
+          ii = atom_j(lf);
+          no = iorbs_s[ii];
+          if (!M.latoms[ii]) {
+            u = M.iused[ii];
+            double s2 = 0.0;
+            for (int x = 0; x < no; ++x) {
+              cj[x] = L.cocc[jlr + u + x];
+              const double t = beta * cj[x];
+              s2 += t * t;
+            }
+            grow = (s2 > P.thresh);
+          }
+        }
+        const unsigned long long gm = __ballot(grow);
+        if (gm) {
+          const int rank = __popcll(gm & below);
+          const int gincl = wave_scan_incl(grow ? no : 0, lane);
+          const int slot = nni + nce_run + rank;
+          const int obase = mle_run + gincl - no;
+          fail = __any(grow && (slot >= incv || ilr + obase + no > iur));
+          if (!fail && grow) {
+            L.icvir[slot] = ii + 1;
+            if (in_lds) M.li[nce_run + rank] = (unsigned short)ii;
+            M.latoms[ii] = 1;
+            for (int x = 0; x < no; ++x) {
+              storei[obase + x] = L.cvir[ilr + obase + x];
+              L.cvir[ilr + obase + x] = -beta * cj[x];
+              L.cocc[jlr + u + x] = alpha * cj[x];
+            }
+          }
+          if (fail) break;
+          nce_run += __popcll(gm);
+          mle_run += __shfl(gincl, 63, 64);
+          iext = max(iext, mle_run);
+        }
+      }
+      if (in_lds) wave_phase();
+      else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      if (!fail) {
+        if (lane == 0) L.nce[i] = nce_run;
+        // clear the map entries of the atoms now in the two LMOs
+        for (int le = lane; le < nce_run; le += 64) M.latoms[atom_i(le)] = 0;
+        for (int lf = lane; lf < ncf_run; lf += 64) M.iused[atom_j(lf)] = D2W_NONE;
+        wave_phase();
+        break;
+      }
+    }
+    // rejection (label 1000): restore both LMOs from the undo log (written
+    // by other lanes: wait for the wave's stores first), clear the maps
+    ++nrej;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    const int rj = max(bj, jext), ri = max(bi, iext);
+    for (int k = lane; k < rj; k += 64) L.cocc[jlr + k] = storej[k];
+    for (int k = lane; k < ri; k += 64) L.cvir[ilr + k] = storei[k];
+    for (int k = lane; k < L.numat; k += 64) {
+      M.iused[k] = D2W_NONE;
+      M.latoms[k] = 0;
+    }
+    if (lane == 0) {
+      L.ncf[j] = ncfj;
+      L.nce[i] = ncei;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    if (P.retry) {
+      alpha = 0.5 * (alpha + 1.0);
+      continue;
+    }
+    break;
+  }
+  if (lane == 0) {
+    sumb_out[ij] = sumb;
+    nrej_out[ij] = nrej;
+  }
+}
+
+__device__ void d2w_init_lds(const LmoDev& L, unsigned char* lds, int nw) {
+  for (int a = threadIdx.x; a < L.numat; a += blockDim.x) lds[a] = (unsigned char)L.iorbs[a];
+  for (int w = 0; w < nw; ++w) {
+    D2Wave M = d2w_carve(lds, L.numat, nw, w);
+    for (int a = threadIdx.x; a < L.numat; a += blockDim.x) {
+      M.iused[a] = D2W_NONE;
+      M.latoms[a] = 0;
+    }
+  }
+  __syncthreads();
+}
+
+// Every level of the list in one workgroup: the waves take a level's
+// rotations in turn and meet at the barrier before the next level.
+__global__ __launch_bounds__(64 * D2W_NW) void k_mz_diagg2_wg(LmoDev L, const double* __restrict__ eigs,
+                                                            const double* __restrict__ eigv,
+                                                            const double* __restrict__ fmo,
+                                                            const int* __restrict__ ifmo, D2Params P,
+                                                            const int* __restrict__ order,
+                                                            const int* __restrict__ level_start, int lev0,
+                                                            int lev1, double* __restrict__ storej_all, int span_j,
+                                                            double* __restrict__ storei_all, int span_i,
+                                                            double* __restrict__ sumb_out,
+                                                            int* __restrict__ nrej_out, int* __restrict__ status) {
+  extern __shared__ __align__(16) unsigned char d2w_lds[];
+  const int nw = blockDim.x >> 6;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  d2w_init_lds(L, d2w_lds, nw);
+  const D2Wave M = d2w_carve(d2w_lds, L.numat, nw, w);
+  double* storej = storej_all + (size_t)w * span_j;
+  double* storei = storei_all + (size_t)w * span_i;
+  for (int l = lev0; l < lev1; ++l) {
+    const int b1 = level_start[l + 1];
+    for (int b = level_start[l] + w; b < b1; b += nw)
+      d2w_rotation(L, d2w_lds, eigs, eigv, fmo, ifmo, P, order[b], M, storej, storei, sumb_out, nrej_out, status,
+                   lane);
+    __syncthreads();  // every wave reaches it once per level
+  }
+}
+
+// One level across the chip: a one-wave workgroup per rotation.
+__global__ __launch_bounds__(64) void k_mz_diagg2_lvl(LmoDev L, const double* __restrict__ eigs,
+                                                      const double* __restrict__ eigv,
+                                                      const double* __restrict__ fmo,
+                                                      const int* __restrict__ ifmo, D2Params P,
+                                                      const int* __restrict__ order, int base,
+                                                      double* __restrict__ storej_all, int span_j,
+                                                      double* __restrict__ storei_all, int span_i,
+                                                      double* __restrict__ sumb_out, int* __restrict__ nrej_out,
+                                                      int* __restrict__ status) {
+  extern __shared__ __align__(16) unsigned char d2w_lds[];
+  d2w_init_lds(L, d2w_lds, 1);
+  const D2Wave M = d2w_carve(d2w_lds, L.numat, 1, 0);
+  const int slot = blockIdx.x;
+  d2w_rotation(L, d2w_lds, eigs, eigv, fmo, ifmo, P, order[base + slot], M, storej_all + (size_t)slot * span_j,
+               storei_all + (size_t)slot * span_i, sumb_out, nrej_out, status, threadIdx.x);
+}
+
 // ---------------------------------------------------------------- helecz
 // One workgroup per atom i: sum over its stored pairs (j < i) and its
 // diagonal block; partial sums combined in a fixed order by k_mz_sum2.
@@ -905,10 +1227,16 @@ struct LmoBufs {
 struct MzStats {
   bool on = false;
   double ms = 0.0, bytes = 0.0;
-  long long rotations = 0, levels = 0, launches = 0;
+  long long rotations = 0, levels = 0, launches = 0, rounds = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 MzStats g_mz_stats;
+// diagg2 on the LDS rotation kernels (runs of narrow levels in one workgroup,
+// k_mz_diagg2_wg; wider levels across the chip, k_mz_diagg2_lvl) when they
+// fit, else the global-map kernel one launch per level (k_mz_diagg2)
+bool g_mz_one_wg = true;
+int g_mz_wide_rounds = 3;
+int g_mz_lcap = 1024;  // D2W_LCAP
 
 // Device copies and scratch of the MOZYME entry points, one set per device,
 // grown as needed and kept until the process ends: no allocation per call.
@@ -917,7 +1245,7 @@ struct MzWork {
   std::vector<int> nijbo_seen, iorbs_seen;  // the nijbo (and sizes) last checked and uploaded
   long long nijbo_mpack = -1;
   DevBuf dp, dpart, dfao, daocc, dcounts, doffs, dnfmo, dold, dfmo, difmo, dvmax, dvsum, deigv, deigs, dorder,
-      dused, dlat, dsj, dsi, dsumb, dnrej, dh, df, dout, dpn, d1, d2, d3, di, dsc;
+      dused, dlat, dsj, dsi, dsumb, dnrej, dh, df, dout, dpn, d1, d2, d3, di, dsc, dlevel, dstatus;
 };
 MzWork& mz_work() {
   static std::map<int, MzWork> work;
@@ -1236,6 +1564,7 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
     P.thresh = st->thresh;
     P.retry = retry ? 1 : 0;
     P.nij = nij;
+    P.lcap = std::min(g_mz_lcap, D2W_LCAP);
     // the rotations diagg2.F90 takes (same tests, same arithmetic as the
     // reference's loop), each at one level past the last earlier rotation
     // on its virtual or its occupied LMO
@@ -1244,14 +1573,48 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
     taken.reserve(nij);
     level.reserve(nij);
     int nlev = 0;
+    // order-free: round = the first round in which neither LMO rotates yet
+    // (bit r of used_v[i] / used_o[j], words per LMO grown as needed)
+    int words = 4;
+    std::vector<uint64_t> used_v, used_o;
+    if (st->order_free) {
+      used_v.assign((size_t)std::max(L->nvir, 1) * words, 0);
+      used_o.assign((size_t)std::max(L->nocc, 1) * words, 0);
+    }
+    auto first_free = [&](int i, int j) {
+      for (;;) {
+        for (int w = 0; w < words; ++w) {
+          const uint64_t f = ~(used_v[(size_t)i * words + w] | used_o[(size_t)j * words + w]);
+          if (f) return w * 64 + __builtin_ctzll(f);
+        }
+        const int nw2 = 2 * words;
+        auto grow = [&](std::vector<uint64_t>& u, int n) {
+          std::vector<uint64_t> g((size_t)std::max(n, 1) * nw2, 0);
+          for (int k = 0; k < n; ++k)
+            for (int w = 0; w < words; ++w) g[(size_t)k * nw2 + w] = u[(size_t)k * words + w];
+          u.swap(g);
+        };
+        grow(used_v, L->nvir);
+        grow(used_o, L->nocc);
+        words = nw2;
+      }
+    };
     for (int ij = 0; ij < nij; ++ij) {
       if (!(std::fabs(fmo[ij]) >= P.tiny)) continue;
       const int i = ifmo[2 * ij] - 1, j = ifmo[2 * ij + 1] - 1;
       const double c = fmo[ij] * P.cnst;
       const double d = eigs[j] - eigv[i] - P.shift;
       if (!(std::fabs(c / d) >= P.biglim)) continue;
-      const int l = std::max(lev_v[i], lev_o[j]) + 1;
-      lev_v[i] = lev_o[j] = l;
+      int l;
+      if (st->order_free) {
+        const int r = first_free(i, j);
+        used_v[(size_t)i * words + r / 64] |= 1ull << (r % 64);
+        used_o[(size_t)j * words + r / 64] |= 1ull << (r % 64);
+        l = r + 1;
+      } else {
+        l = std::max(lev_v[i], lev_o[j]) + 1;
+        lev_v[i] = lev_o[j] = l;
+      }
       taken.push_back(ij);
       level.push_back(l);
       nlev = std::max(nlev, l);
@@ -1277,7 +1640,23 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
     };
     const int span_j = span_of(L->nocc, L->ncocc, L->cocc_dim);
     const int span_i = span_of(L->nvir, L->ncvir, L->cvir_dim);
-    const int nslots = std::max(1, std::min(widest, 2048));
+    // The LDS kernels need 16-bit atom numbers and orbital offsets and their
+    // maps in LDS.
+    constexpr size_t kLds = 160 * 1024;
+    const bool lds_ok = g_mz_one_wg && L->numat < (int)D2W_NONE && L->norbs < (int)D2W_NONE &&
+                        d2w_lds_bytes(L->numat, 1) <= kLds;
+    int nw = 0;
+    while (lds_ok && nw < D2W_NW && d2w_lds_bytes(L->numat, nw + 1) <= kLds) ++nw;
+    nw = std::min(nw, std::max(widest, 1));
+    // a level wider than g_mz_wide_rounds rounds of the workgroup runs across
+    // the chip, one workgroup per rotation; runs of narrower levels run in
+    // one workgroup
+    const int wide = lds_ok ? (int)std::min<long long>((long long)nw * g_mz_wide_rounds, 1LL << 30) : 0;
+    int widest_wide = 0;
+    if (lds_ok)
+      for (int l = 1; l <= nlev; ++l)
+        if (start[l + 1] - start[l] > wide) widest_wide = std::max(widest_wide, start[l + 1] - start[l]);
+    const int nslots = lds_ok ? std::max(nw, std::min(widest_wide, 2048)) : std::max(1, std::min(widest, 2048));
     MzWork& W = mz_work();
     upload_lmos(s, L, W);
     const LmoBufs& B = W.B;
@@ -1288,10 +1667,17 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
     up(s, dfmo, fmo, sizeof(double) * std::max(nij, 1));
     up(s, difmo, ifmo, sizeof(int) * 2 * std::max(nij, 1));
     up(s, dorder, order.data(), sizeof(int) * order.size());
-    dused.ensure(sizeof(int) * (size_t)nslots * L->numat);
-    MHIP_CHECK(hipMemsetAsync(dused.ptr, 0xff, sizeof(int) * (size_t)nslots * L->numat, s));  // -1
-    dlat.ensure((size_t)nslots * L->numat);
-    MHIP_CHECK(hipMemsetAsync(dlat.ptr, 0, (size_t)nslots * L->numat, s));
+    if (!lds_ok) {
+      dused.ensure(sizeof(int) * (size_t)nslots * L->numat);
+      MHIP_CHECK(hipMemsetAsync(dused.ptr, 0xff, sizeof(int) * (size_t)nslots * L->numat, s));  // -1
+      dlat.ensure((size_t)nslots * L->numat);
+      MHIP_CHECK(hipMemsetAsync(dlat.ptr, 0, (size_t)nslots * L->numat, s));
+    } else {
+      // level l (0-based) holds order[level[l] .. level[l + 1])
+      up(s, W.dlevel, start.data() + 1, sizeof(int) * (size_t)(nlev + 1));
+      W.dstatus.ensure(sizeof(int));
+      MHIP_CHECK(hipMemsetAsync(W.dstatus.ptr, 0, sizeof(int), s));
+    }
     dsj.ensure(sizeof(double) * (size_t)nslots * span_j);
     dsi.ensure(sizeof(double) * (size_t)nslots * span_i);
     dsumb.ensure(sizeof(double) * std::max(nij, 1));
@@ -1316,7 +1702,50 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
       }
       MHIP_CHECK(hipEventRecord(g_mz_stats.e0, s));
     }
-    for (int l = 1; l <= nlev; ++l)
+    if (lds_ok && nlev > 0) {
+      static bool attr_set = false;
+      if (!attr_set) {
+        MHIP_CHECK(hipFuncSetAttribute((const void*)k_mz_diagg2_wg, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)kLds));
+        MHIP_CHECK(hipFuncSetAttribute((const void*)k_mz_diagg2_lvl, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)kLds));
+        attr_set = true;
+      }
+      const size_t lds_wg = d2w_lds_bytes(L->numat, nw), lds_lvl = d2w_lds_bytes(L->numat, 1);
+      int l = 1;
+      while (l <= nlev) {
+        const int width = start[l + 1] - start[l];
+        if (width > wide) {
+          for (int b = start[l]; b < start[l + 1]; b += nslots) {
+            const int cnt = std::min(nslots, start[l + 1] - b);
+            hipLaunchKernelGGL(k_mz_diagg2_lvl, dim3(cnt), dim3(64), lds_lvl, s, B.dev, deigs.as<double>(),
+                               deigv.as<double>(), dfmo.as<double>(), difmo.as<int>(), P, dorder.as<int>(), b,
+                               dsj.as<double>(), span_j, dsi.as<double>(), span_i, dsumb.as<double>(),
+                               dnrej.as<int>(), W.dstatus.as<int>());
+            MHIP_LAUNCH_CHECK();
+            if (g_mz_stats.on) {
+              ++g_mz_stats.launches;
+              ++g_mz_stats.rounds;
+            }
+          }
+          ++l;
+          continue;
+        }
+        int l1 = l;  // the run of narrow levels l .. l1 - 1
+        while (l1 <= nlev && start[l1 + 1] - start[l1] <= wide) ++l1;
+        hipLaunchKernelGGL(k_mz_diagg2_wg, dim3(1), dim3(64 * nw), lds_wg, s, B.dev, deigs.as<double>(),
+                           deigv.as<double>(), dfmo.as<double>(), difmo.as<int>(), P, dorder.as<int>(),
+                           W.dlevel.as<int>(), l - 1, l1 - 1, dsj.as<double>(), span_j, dsi.as<double>(), span_i,
+                           dsumb.as<double>(), dnrej.as<int>(), W.dstatus.as<int>());
+        MHIP_LAUNCH_CHECK();
+        if (g_mz_stats.on) {
+          ++g_mz_stats.launches;
+          for (int q = l; q < l1; ++q) g_mz_stats.rounds += (start[q + 1] - start[q] + nw - 1) / nw;
+        }
+        l = l1;
+      }
+    }
+    for (int l = 1; l <= nlev && !lds_ok; ++l)
       for (int b = start[l]; b < start[l + 1]; b += nslots) {
         const int cnt = std::min(nslots, start[l + 1] - b);
         hipLaunchKernelGGL(k_mz_diagg2, dim3(cnt), dim3(D2_NT), 0, s, B.dev, deigs.as<double>(),
@@ -1324,7 +1753,10 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
                            dused.as<int>(), (char*)dlat.ptr, dsj.as<double>(), span_j, dsi.as<double>(), span_i,
                            dsumb.as<double>(), dnrej.as<int>());
         MHIP_LAUNCH_CHECK();
-        if (g_mz_stats.on) ++g_mz_stats.launches;
+        if (g_mz_stats.on) {
+          ++g_mz_stats.launches;
+          ++g_mz_stats.rounds;
+        }
       }
     if (g_mz_stats.on) MHIP_CHECK(hipEventRecord(g_mz_stats.e1, s));
     std::vector<double> sb(std::max(nij, 1));
@@ -1339,7 +1771,10 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
     down(s, L->nce, B.nce, sizeof(int) * L->nvir);
     down(s, L->icvir, B.icvir, sizeof(int) * L->icvir_dim);
     down(s, L->cvir, B.cvir, sizeof(double) * L->cvir_dim);
+    int status = 0;
+    if (lds_ok) down(s, &status, W.dstatus, sizeof(int));
     MHIP_CHECK(hipStreamSynchronize(s));
+    MHIP_REQUIRE(status == 0, "mhip_mozyme_diagg2: rotation kernel status " + std::to_string(status));
     double sumb = 0.0;
     int nrej = 0;
     for (int ij : taken) {  // list order
@@ -1358,18 +1793,37 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
 }
 
 int mhip_mozyme_stats(int enable, double* ms, long long* rotations, long long* levels, long long* launches,
-                      double* bytes) {
+                      double* bytes, long long* rounds) {
   if (ms) *ms = g_mz_stats.ms;
   if (rotations) *rotations = g_mz_stats.rotations;
   if (levels) *levels = g_mz_stats.levels;
   if (launches) *launches = g_mz_stats.launches;
   if (bytes) *bytes = g_mz_stats.bytes;
+  if (rounds) *rounds = g_mz_stats.rounds;
   if (enable >= 0) {
     g_mz_stats.on = enable != 0;
     g_mz_stats.ms = g_mz_stats.bytes = 0.0;
-    g_mz_stats.rotations = g_mz_stats.levels = g_mz_stats.launches = 0;
+    g_mz_stats.rotations = g_mz_stats.levels = g_mz_stats.launches = g_mz_stats.rounds = 0;
   }
   return 0;
+}
+
+int mhip_mozyme_diagg2_one_workgroup(int enable) {
+  const int was = g_mz_one_wg ? 1 : 0;
+  if (enable >= 0) g_mz_one_wg = enable != 0;
+  return was;
+}
+
+int mhip_mozyme_diagg2_wide_rounds(int rounds) {
+  const int was = g_mz_wide_rounds;
+  if (rounds >= 0) g_mz_wide_rounds = rounds;
+  return was;
+}
+
+int mhip_mozyme_diagg2_list_cap(int cap) {
+  const int was = g_mz_lcap;
+  if (cap >= 0) g_mz_lcap = std::min(cap, D2W_LCAP);
+  return was;
 }
 
 int mhip_mozyme_helecz(mhip_ctx* ctx, int numat, const int* nfirst, const int* nlast, const int* nijbo,

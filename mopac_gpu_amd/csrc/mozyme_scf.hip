@@ -19,6 +19,7 @@
 // block, so no two workgroups write the same element.
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <vector>
 
@@ -518,6 +519,7 @@ int mhip_mozyme_scf(mhip_ctx* c, mhip_lmo_set* lmo, const mhip_mozyme_opts* o, m
     st2.reset = 1;
     st2.damp = o->damp > 0 ? o->damp : 1.0;
     st2.thresh = o->thresh > 0 ? o->thresh : 1.0e-13;
+    st2.order_free = o->order_free ? 1 : 0;
     const double selcon = o->scfcrt > 0 ? o->scfcrt : 1.0e-2;  // scfcri (src/MOZYME/scfcri.F90)
     const int itrmax = o->itrmax > 0 ? o->itrmax : 2000;
     const double fpc9 = mhip_ev_to_kcal();
@@ -570,6 +572,13 @@ int mhip_mozyme_scf(mhip_ctx* c, mhip_lmo_set* lmo, const mhip_mozyme_opts* o, m
       eold = escf;
       // isitsc (src/MOZYME/isitsc.F90:16-116), single point (emin = 0)
       const double ovmax = st1.ovmax;
+      if (o->heats && niter <= o->heats_cap) o->heats[niter - 1] = escf;
+      if (o->print_level > 0) {
+        std::fprintf(stderr, "mozyme iteration %4d  heat %16.6f  change %12.6f  ovmax %10.3e  %.3f s\n", niter,
+                     escf, ediff, ovmax,
+                     std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+        std::fflush(stderr);
+      }
       const bool ok_now = ovmax < 5.0 * selcon && std::fabs(ediff) < selcon;
       if ((ok_now && scf1) || niter > itrmax) okscf = true;
       else scf1 = ok_now;

@@ -74,7 +74,8 @@ class LmoSetC(ctypes.Structure):
 
 class MozymeOptsC(ctypes.Structure):
     _fields_ = [("scfcrt", c_double), ("itrmax", c_int), ("thresh", c_double), ("damp", c_double),
-                ("fmo_dim", c_int), ("enuclr", c_double), ("atheat", c_double)]
+                ("fmo_dim", c_int), ("enuclr", c_double), ("atheat", c_double), ("print_level", c_int),
+                ("heats", c_void_p), ("heats_cap", c_int), ("order_free", c_int)]
 
 
 class MozymeResultC(ctypes.Structure):
@@ -90,7 +91,7 @@ class Diagg1State(ctypes.Structure):
 
 class Diagg2State(ctypes.Structure):
     _fields_ = [("reset", c_int), ("tiny", c_double), ("sumb", c_double), ("damp", c_double),
-                ("shift", c_double), ("thresh", c_double), ("nrejct", c_int * 2)]
+                ("shift", c_double), ("thresh", c_double), ("nrejct", c_int * 2), ("order_free", c_int)]
 
 
 # name -> (restype, argtypes)
@@ -122,7 +123,10 @@ SIGNATURES = {
     "mhip_set_tuning": (c_int, [c_void_p, c_int, c_int, c_double]),
     "mhip_set_jacobi_schedule": (c_int, [c_int]),
     "mhip_mozyme_stats": (c_int, [c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_ll), ctypes.POINTER(c_ll),
-                                  ctypes.POINTER(c_ll), ctypes.POINTER(c_double)]),
+                                  ctypes.POINTER(c_ll), ctypes.POINTER(c_double), ctypes.POINTER(c_ll)]),
+    "mhip_mozyme_diagg2_one_workgroup": (c_int, [c_int]),
+    "mhip_mozyme_diagg2_wide_rounds": (c_int, [c_int]),
+    "mhip_mozyme_diagg2_list_cap": (c_int, [c_int]),
     "mhip_last_sweeps": (c_int, [c_void_p]),
     "mhip_mozyme_density": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "mhip_mozyme_diagg1": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,

@@ -241,7 +241,8 @@ class MozymeSystem:
                                h.ctypes.data, w.ctypes.data, len(w), int(self.feather), self.TRUNC1, self.TRUNC2)
         L.check(ctx.lib.mhip_mozyme_setup(ctx.h, ctypes.byref(sysc)))
 
-    def scf(self, ctx, lmos, scfcrt=1e-2, itrmax=2000, thresh=1e-13, fmo_dim=0):
+    def scf(self, ctx, lmos, scfcrt=1e-2, itrmax=2000, thresh=1e-13, fmo_dim=0, print_level=0, heats_cap=0,
+            order_free=False):
         """iter_for_MOZYME on the device from the LMOs of `lmos` (an LmoSet on
         this system's nijbo); returns the result and the updated LMOs."""
         from . import lib as L
@@ -256,8 +257,11 @@ class MozymeSystem:
                       cvir_dim=len(a["cvir"]))
         for k, v in a.items():
             setattr(c, k, v.ctypes.data)
+        heats = np.zeros(max(heats_cap, 1))
         o = L.MozymeOptsC(scfcrt=scfcrt, itrmax=itrmax, thresh=thresh, damp=1.0, fmo_dim=fmo_dim,
-                          enuclr=self.enuclr, atheat=self.s.atheat)
+                          enuclr=self.enuclr, atheat=self.s.atheat, print_level=print_level,
+                          heats=heats.ctypes.data if heats_cap > 0 else None, heats_cap=heats_cap,
+                          order_free=int(order_free))
         r = L.MozymeResultC()
         p = np.zeros(self.mpack)
         f = np.zeros(self.mpack)
@@ -268,7 +272,7 @@ class MozymeSystem:
             setattr(out, k, a[k].copy())
         out.p, out.fao = p, f
         return dict(escf=r.escf, ee=r.ee, niter=r.niter, converged=bool(r.converged), ovmax=r.ovmax,
-                    seconds=r.seconds, p=p, f=f, lmos=out)
+                    seconds=r.seconds, p=p, f=f, lmos=out, heats=heats[:min(r.niter, heats_cap)].copy())
 
     def fock2z(self, ctx, p):
         from . import lib as L

@@ -54,6 +54,9 @@ def load():
         lib.mz_tidy.argtypes = [I, V, V, I, V, I, V, V, I, D, V, V, V]
         lib.mz_check.argtypes = [I, V, V, V, I, V, V, I, V]
         lib.mz_eimp.argtypes = [V, V]
+        lib.mz_iter_prologue.argtypes = [I, I, V, V, V]
+        lib.mz_get_offsets.argtypes = [I, I, V, V, V, V]
+        lib.mz_set_shift.argtypes = [D]
         _lib = lib
     return _lib
 
@@ -194,6 +197,16 @@ class Mozyme:
                         ctypes.byref(pm))
         return pm.value
 
+    def prologue(self, p, f):
+        """tidy, check and eimp as each iteration of iter_for_MOZYME starts;
+        the module's p becomes eimp's array for the next diagg1."""
+        s = self.s
+        err = ctypes.c_int()
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+        load().mz_iter_prologue(s.nocc, s.nvir, P(c(p)), P(c(f)), ctypes.byref(err))
+        if err.value:
+            raise RuntimeError("tidy: LMO storage too small")
+
     def get(self):
         s = self.s.copy()
         eigs = np.zeros(s.nocc)
@@ -205,6 +218,10 @@ class Mozyme:
         cvir = np.zeros(len(s.cvir))
         load().mz_get_lmos(s.nocc, s.nvir, P(ncf), P(icocc), P(cocc), P(nce), P(icvir), P(cvir), P(eigs))
         s.ncf, s.nce, s.icocc, s.icvir, s.cocc, s.cvir, s.eigs = ncf, nce, icocc, icvir, cocc, cvir, eigs
+        offs = [np.zeros(s.nocc, np.int32), np.zeros(s.nocc, np.int32), np.zeros(s.nvir, np.int32),
+                np.zeros(s.nvir, np.int32)]
+        load().mz_get_offsets(s.nocc, s.nvir, *[P(a) for a in offs])
+        s.nncf, s.ncocc, s.nnce, s.ncvir = offs
         s.eigv = self.eigv.copy()
         return s
 
@@ -365,3 +382,79 @@ def mozyme_eimp(numat, norbs, nfirst, nlast, nijbo, mpack, f, p):
     load().mz_eimp(P(np.ascontiguousarray(f, dtype=np.float64)), P(pp))
     del keep
     return pp
+
+
+def mozyme_scf(ms, lmos, scfcrt=1e-2, itrmax=2000, fmo_dim=None, on_iteration=None):
+    """iter_for_MOZYME's single-point loop driven over the reference's own
+    routines (tidy, check, eimp, diagg1, diagg2, density_for_MOZYME, cnvgz,
+    fz2n, helecz), in the order of src/MOZYME/iter_for_MOZYME.F90:220-640,
+    with isitsc's test (isitsc.F90:31-46, emin = 0) and the loop's
+    three-point extrapolation and level-shift schedule (:617-630), from the
+    LmoSet lmos (taken with the reference's capacities) on the MozymeSystem
+    ms.  Returns dict(escf, niter, converged, heats, ovmax)."""
+    from mopac_gpu_amd import mozyme
+    lib = load()
+    L = mozyme.reference_capacity(lmos)
+    L.eigs, L.eigv = np.zeros(L.nocc), np.zeros(L.nvir)
+    L.p, L.fao = np.zeros(L.mpack), np.zeros(L.mpack)
+    if fmo_dim is None:
+        fmo_dim = min(L.nocc * L.nvir, 50 * max(L.nocc, L.nvir) * 64)
+    R = Mozyme(L, fmo_dim=fmo_dim)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    xyz, am, dd, ad, w, h = f64(ms.xyz), f64(ms.am), f64(ms.dd), f64(ms.ad), f64(ms.w), f64(ms.h)
+    fpc9 = 23.060547830619029  # eV -> kcal/mol (fpc(9) of the 2018 CODATA set)
+
+    def fock(p):
+        f = np.zeros(ms.mpack)
+        lib.mz_fock2z(P(xyz), P(am), P(dd), P(ad), len(w), P(w), P(f64(p)), P(h), A0, EV, 1, TRUNC1, TRUNC2,
+                      P(f))
+        return f
+
+    def heat(p, f):
+        return (R.helecz(p, h, f) + ms.enuclr) * fpc9 + ms.s.atheat
+
+    iorbs = (L.nlast - L.nfirst + 1).astype(np.int64)
+    idiag = np.array([L.nijbo[a, a] + k * (k + 1) // 2 + k + 1 for a in range(L.numat) for k in range(iorbs[a])],
+                     dtype=np.int32)
+    p = R.density(0)
+    f = fock(p)
+    escf = heat(p, f)
+    pold = np.zeros(L.mpack)
+    p1, p2, p3 = np.zeros(L.norbs), np.zeros(L.norbs), np.zeros(L.norbs)
+    eold, shift, niter, idiagg, lstart = 0.0, 0.0, 0, 0, 0
+    three, scf1, okscf = True, False, False
+    heats, ovmaxes = [], []
+    while True:
+        R.prologue(p, f)
+        r = R.diagg1(f, idiagg)
+        lib.mz_set_shift(shift)
+        R.diagg2(r["nij"], idiagg)
+        p = R.density(0)
+        idiagg += 1
+        niter += 1
+        if three:
+            R.cnvgz(p, pold, p1, p2, p3, niter, idiag, True)
+        f = fock(p)
+        escf = heat(p, f)
+        ediff = escf - eold
+        eold = escf
+        ovmax = r["ovmax"]
+        heats.append(escf)
+        ovmaxes.append(ovmax)
+        if on_iteration:
+            on_iteration(niter, escf, ediff, ovmax)
+        ok_now = ovmax < 5.0 * scfcrt and abs(ediff) < scfcrt
+        if (ok_now and scf1) or niter > itrmax:
+            okscf = True
+        else:
+            scf1 = ok_now
+        if okscf and niter > 3:
+            break
+        if three:
+            if niter % 3 == 2 and abs(ediff) < 0.1:
+                three, lstart, shift = False, niter, 0.0
+        elif ediff > 0.0 and shift < 11.0 and niter > lstart + 2:
+            shift += 2.0
+            lstart = niter
+    return dict(escf=escf, niter=niter, converged=niter <= itrmax, heats=np.array(heats),
+                ovmax=np.array(ovmaxes))

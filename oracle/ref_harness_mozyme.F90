@@ -286,4 +286,61 @@ contains
     p_io = p
   end subroutine mz_eimp
 
+
+  ! The start of an iter_for_MOZYME iteration on the loaded LMOs
+  ! (iter_for_MOZYME.F90:221, 326, 482-484, 495): tidy on the occupied and
+  ! the virtual set, check on both, then eimp on p_in with f_in, leaving the
+  ! result in the module's p (what diagg1 reads).  err = 1 when tidy found
+  ! the storage too small, 2 when check found an error above 0.1 (not
+  ! reached here: check would stop the job).
+  subroutine mz_iter_prologue(nocc, nvir, p_in, f_in, err) bind(C, name="mz_iter_prologue")
+    use molkst_C, only: mpack, step_num, step_num0, moperr, use_disk
+    use common_arrays_C, only: p, f
+    use MOZYME_C, only: ncf, nncf, ncocc, icocc, cocc, nce, nnce, ncvir, icvir, cvir, icocc_dim, cocc_dim, &
+      icvir_dim, cvir_dim, iorbs, ws
+    integer(c_int), value :: nocc, nvir
+    real(c_double) :: p_in(mpack), f_in(mpack)
+    integer(c_int) :: err
+    integer :: lno, lnv, mn
+    step_num = 1
+    step_num0 = 0
+    use_disk = .false.
+    moperr = .false.
+    err = 0
+    call tidy(nocc, ncf, icocc, icocc_dim, cocc, cocc_dim, nncf, ncocc, lno, mn, 1)
+    if (moperr) err = 1
+    call tidy(nvir, nce, icvir, icvir_dim, cvir, cvir_dim, nnce, ncvir, lnv, mn, 2)
+    if (moperr) err = 1
+    moperr = .false.
+    if (err /= 0) return
+    if (allocated(ws)) deallocate (ws)
+    allocate (ws(max(nocc, nvir) + 1))
+    call check(nocc, nncf, ncf, icocc, icocc_dim, iorbs, ncocc, cocc, cocc_dim)
+    call check(nvir, nnce, nce, icvir, icvir_dim, iorbs, ncvir, cvir, cvir_dim)
+    if (allocated(f)) deallocate (f)
+    allocate (f(mpack))
+    f = f_in
+    p = p_in
+    call eimp()
+  end subroutine mz_iter_prologue
+
+  ! The LMO offsets tidy may have moved (nncf, ncocc, nnce, ncvir)
+  subroutine mz_get_offsets(nocc, nvir, nncf_out, ncocc_out, nnce_out, ncvir_out) bind(C, name="mz_get_offsets")
+    use MOZYME_C, only: nncf, ncocc, nnce, ncvir
+    integer(c_int), value :: nocc, nvir
+    integer(c_int) :: nncf_out(nocc), ncocc_out(nocc), nnce_out(nvir), ncvir_out(nvir)
+    nncf_out = nncf(1:nocc)
+    ncocc_out = ncocc(1:nocc)
+    nnce_out = nnce(1:nvir)
+    ncvir_out = ncvir(1:nvir)
+  end subroutine mz_get_offsets
+
+
+  ! the level shift diagg2 reads (MOZYME_C shift)
+  subroutine mz_set_shift(shift_in) bind(C, name="mz_set_shift")
+    use MOZYME_C, only: shift
+    real(c_double), value :: shift_in
+    shift = shift_in
+  end subroutine mz_set_shift
+
 end module ref_harness_mozyme

@@ -72,20 +72,39 @@ def lewis_helix_lmos():
     return L
 
 
-def test_diagg_sequence_on_lewis_lmos_matches_reference(gpu_ctx, lewis_helix_lmos):
-    _diagg_sequence(gpu_ctx, lewis_helix_lmos, None, 8)
+@pytest.fixture(params=[(1, 3, 1024), (1, 1 << 30, 1024), (1, 0, 1024), (1, 3, 0), (0, 3, 1024)],
+                ids=["lds_hybrid", "lds_one_workgroup", "lds_across_chip", "lds_lists_in_global",
+                     "global_per_level"])
+def diagg2_path(request):
+    """Every way of running diagg2's levels: the LDS kernels with the default
+    split, with every level in the one workgroup, with every level across the
+    chip, with the atom lists read from global memory, and the global-map
+    kernel one launch per level."""
+    from mopac_gpu_amd import lib as L_
+    lib = L_.load()
+    was = lib.mhip_mozyme_diagg2_one_workgroup(request.param[0])
+    was_w = lib.mhip_mozyme_diagg2_wide_rounds(request.param[1])
+    was_c = lib.mhip_mozyme_diagg2_list_cap(request.param[2])
+    yield request.param
+    lib.mhip_mozyme_diagg2_one_workgroup(was)
+    lib.mhip_mozyme_diagg2_wide_rounds(was_w)
+    lib.mhip_mozyme_diagg2_list_cap(was_c)
+
+
+def test_diagg_sequence_on_lewis_lmos_matches_reference(gpu_ctx, lewis_helix_lmos, diagg2_path):
+    _diagg_sequence(gpu_ctx, lewis_helix_lmos, None, 8, diagg2_path)
 
 
 @pytest.mark.parametrize("fmo_cap", [None, 60])
-def test_diagg1_diagg2_sequence_matches_reference(gpu_ctx, helix_lmos, fmo_cap):
-    _diagg_sequence(gpu_ctx, helix_lmos, fmo_cap, 10)
+def test_diagg1_diagg2_sequence_matches_reference(gpu_ctx, helix_lmos, fmo_cap, diagg2_path):
+    _diagg_sequence(gpu_ctx, helix_lmos, fmo_cap, 10, diagg2_path)
 
 
-def _diagg_sequence(gpu_ctx, L, fmo_cap, ncalls):
+def _diagg_sequence(gpu_ctx, L, fmo_cap, ncalls, path=1):
     R = REF.Mozyme(L, fmo_dim=fmo_cap)
     G = mozyme.GpuMozyme(gpu_ctx, L, fmo_dim=fmo_cap)
     for idiagg in range(1, ncalls + 1):
-        print(f"idiagg {idiagg} cap {fmo_cap} ({L.numat} atoms)", flush=True)
+        print(f"idiagg {idiagg} cap {fmo_cap} ({L.numat} atoms) diagg2 path {path}", flush=True)
         r = R.diagg1(L.fao, idiagg)
         g = G.diagg1(L.fao, L.p, idiagg)
         assert g["nij"] == r["nij"]
@@ -217,3 +236,58 @@ def test_lewis_start_reaches_the_conventional_energy(gpu_ctx):
     print(f"escf {r['escf']:.6f} conventional {conv['escf']:.6f} niter {r['niter']}", flush=True)
     assert r["converged"] and 3 < r["niter"] < 60
     assert abs(r["escf"] - conv["escf"]) < 0.05
+
+
+@pytest.mark.parametrize("nres,scfcrt", [(1, 1e-2), (12, 1e-2), (30, 1e-3)])
+def test_mozyme_loop_follows_the_reference_routines(gpu_ctx, nres, scfcrt):
+    """The whole MOZYME SCF on the device against iter_for_MOZYME's loop
+    driven over the reference's own routines (tidy, check, eimp, diagg1,
+    diagg2, density_for_MOZYME, cnvgz, fz2n, helecz; oracle/ref.py
+    mozyme_scf) from the same Lewis start at MOZYME's default cutoffs: Gly1
+    (under 30 atoms, every pair stored), Gly12 and Gly30 (stored, dipole and
+    point-charge pairs).  The heat after every iteration agrees to 1e-6
+    kcal/mol and both stop at the same iteration."""
+    nat, xyz = geometry.glycine_helix(nres, 3)
+    ms = mozyme.MozymeSystem.from_geometry(nat, xyz)
+    ms.upload(gpu_ctx)
+    f0 = ms.fock2z(gpu_ctx, ms.diagonal_density())
+    lmos = mozyme.lewis_lmos(ms, f0)
+    print(f"MOZYME loop Gly{nres}: {ms.numat} atoms, SCFCRT {scfcrt}", flush=True)
+    r = ms.scf(gpu_ctx, lmos, scfcrt=scfcrt, heats_cap=500)
+    ref = REF.mozyme_scf(ms, lmos, scfcrt=scfcrt)
+    print(f"gpu {r['escf']:.6f} ({r['niter']} it), reference routines {ref['escf']:.6f} ({ref['niter']} it)",
+          flush=True)
+    assert r["converged"] and ref["converged"]
+    assert r["niter"] == ref["niter"]
+    assert np.max(np.abs(r["heats"] - ref["heats"])) < 1e-6
+
+
+@pytest.mark.parametrize("nres", [12, 30])
+def test_order_free_sweep_converges_to_the_reference_order_heat(gpu_ctx, nres):
+    """The order-free form (diagg2's rotations in LMO-disjoint rounds) ends at
+    the reference-order form's heat within the metric's 1e-3 kcal/mol, at
+    SCFCRT 1e-3, with far fewer dependent rounds than the reference order's
+    levels."""
+    import ctypes
+    from mopac_gpu_amd import lib as L_
+    lib = L_.load()
+    nat, xyz = geometry.glycine_helix(nres, 3)
+    ms = mozyme.MozymeSystem.from_geometry(nat, xyz)
+    ms.upload(gpu_ctx)
+    f0 = ms.fock2z(gpu_ctx, ms.diagonal_density())
+    lmos = mozyme.lewis_lmos(ms, f0)
+    out = {}
+    for of in (False, True):
+        sv = [ctypes.c_double(), ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double(),
+              ctypes.c_longlong()]
+        lib.mhip_mozyme_stats(1, *[ctypes.byref(v) for v in sv])
+        r = ms.scf(gpu_ctx, lmos, scfcrt=1e-3, order_free=of)
+        lib.mhip_mozyme_stats(0, *[ctypes.byref(v) for v in sv])
+        out[of] = (r, sv[2].value)
+        print(f"Gly{nres} order_free={of}: escf {r['escf']:.6f}, {r['niter']} iterations, levels {sv[2].value}",
+              flush=True)
+    (r0, lev0), (r1, lev1) = out[False], out[True]
+    assert r0["converged"] and r1["converged"]
+    assert abs(r1["escf"] - r0["escf"]) < 1e-3
+    assert abs(r1["niter"] - r0["niter"]) <= 5
+    assert lev1 < lev0

@@ -289,7 +289,8 @@ def test_mozyme_energy_at_the_conventional_density(r_near, r_dipole, tol):
     """The MOZYME Hamiltonian (mhip_pm7_hamiltonian_mozyme) and sparse Fock
     build (the restated fz2n with PM7's point-charge switch) evaluated at the
     converged conventional density give the conventional heat of formation:
-    exactly when every pair is stored; to 1e-4 kcal/mol when the approximated
+    to 1e-6 kcal/mol when every pair is stored (only the resonance beyond
+    MOZYME's 7 A CUTOFS differs); to 1e-4 kcal/mol when the approximated
     pairs lie beyond the 7 A switch (PM7's NDDO terms are point charges there
     already; what remains is the dropped resonance and core-core exponentials);
     within 0.15 kcal/mol when the dipole shell starts at 6 A."""
