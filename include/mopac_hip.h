@@ -269,6 +269,23 @@ typedef struct {
 int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* lmo, const double* eigs, const double* eigv,
                        const double* fmo, const int* ifmo, int nij, int idiagg,
                        mhip_diagg2_state* st);
+/* tidy (src/MOZYME/tidy.F90:16-244) at a single point on the occupied
+ * (occupied = 1) or the virtual set of lmo, on the device: atoms whose
+ * squared coefficient sum is <= thresh leave their LMO, and the free space
+ * is spread over the LMOs as the reference spreads it (LMO 1 stays at 0).
+ * Updates the set's counts, offsets, atom lists and coefficients in place
+ * (the free space between LMOs is left undefined); fails when the storage
+ * is too small for the next annihilation step. */
+int mhip_mozyme_tidy(mhip_ctx* ctx, mhip_lmo_set* lmo, int occupied, double thresh);
+/* check (src/MOZYME/check.F90:16-113) on the occupied or the virtual set, on
+ * the device: every LMO scaled to unit norm; *error = the summed |1 - norm|
+ * before the scaling (the reference stops the job above 0.1). */
+int mhip_mozyme_check(mhip_ctx* ctx, mhip_lmo_set* lmo, int occupied, double* error);
+/* eimp (src/MOZYME/eimp.F90:16-45) on the device: the first element of each
+ * stored off-diagonal block of p becomes the sum of squares of that block of
+ * f. */
+int mhip_mozyme_eimp(mhip_ctx* ctx, int numat, const int* nfirst, const int* nlast, const int* nijbo,
+                     long long mpack, const double* f, double* p);
 /* No counterpart in the reference.  HIP-event time of the diagg2 rotation
  * kernels on the MOZYME stream, with the rotations, levels, launches,
  * algorithmic bytes (each rotation reads and writes both LMOs' coefficients,
@@ -278,15 +295,17 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* lmo, const double* eigs, con
 int mhip_mozyme_stats(int enable, double* ms, long long* rotations, long long* levels, long long* launches,
                       double* bytes, long long* rounds);
 /* No counterpart in the reference.  Selects how mhip_mozyme_diagg2 runs the
- * rotation levels: 1 (default) on the LDS rotation kernels, a wave per
+ * rotation levels: 1 on the LDS rotation kernels, a wave per
  * rotation with its atom maps and lists in LDS: runs of narrow levels in one
  * launch of a single workgroup that meets at a barrier between levels, and
  * each level wider than mhip_mozyme_diagg2_wide_rounds rounds of that
  * workgroup across the chip, a workgroup per rotation; they apply when
- * numat and norbs are below 65535, the maps fit in LDS and two LMOs' atom
- * regions total at most 1024 atoms.  0, or when they do not apply: one
- * launch per level of the kernel with global maps.  All give identical
- * results.  enable = -1 only reads; returns the previous setting. */
+ * numat and norbs are below 65535 and the maps fit in LDS (two LMOs with
+ * more than mhip_mozyme_diagg2_list_cap atoms keep their lists in global
+ * memory).  0, or when they do not apply: one
+ * launch per level of the kernel with global maps (the default, the fastest
+ * measured).  All give identical results.  enable = -1 only reads; returns
+ * the previous setting. */
 int mhip_mozyme_diagg2_one_workgroup(int enable);
 /* No counterpart in the reference.  The width, in rounds of the one-workgroup
  * kernel, above which a level runs across the chip (default 3); rounds = -1
@@ -343,12 +362,12 @@ int mhip_mozyme_fock2z(mhip_ctx* ctx, const double* p, double* f);
 
 /* The MOZYME SCF (iter_for_MOZYME, src/MOZYME/iter_for_MOZYME.F90:16-691)
  * for a single point of a discrete system, after mhip_mozyme_setup: starting
- * from the LMOs in lmo (updated in place; their capacities must leave room to
- * grow), each iteration runs tidy and check on the host, eimp, diagg
+ * from the LMOs in lmo (updated in place, offsets included; their capacities
+ * must leave room to grow), each iteration runs tidy, check, eimp, diagg
  * (diagg1, diagg2, density_for_MOZYME), cnvgz while the three-point
  * extrapolation is on, buildf/fock2z and helecz, with isitsc's test and the
- * reference's level-shift schedule.  The hydrogen-bond search addhb is not
- * run. */
+ * reference's level-shift schedule, all on the device with the LMOs, P and F
+ * resident.  The hydrogen-bond search addhb is not run. */
 typedef struct {
   double scfcrt;    /* SELCON, kcal/mol (scfcri: 1e-2 when <= 0) */
   int itrmax;       /* 2000 when <= 0 */

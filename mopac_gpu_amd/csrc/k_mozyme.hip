@@ -1208,6 +1208,88 @@ __global__ void k_mz_damp(int norbs, const int* __restrict__ idiag, const double
   p1[i] = p2[i];
 }
 
+
+// ---------------------------------------------------------------- tidy, check, eimp
+// tidy (src/MOZYME/tidy.F90:16-244), first pass, one thread per LMO in the
+// reference's order: an atom stays when the squared sum of its coefficients
+// exceeds thresh; its flag, and the LMO's kept atom and coefficient counts.
+__global__ void k_mz_tidy_count(int nmo, const int* __restrict__ nc, const int* __restrict__ nnc,
+                                const int* __restrict__ ncmo, const int* __restrict__ ic,
+                                const double* __restrict__ c, const int* __restrict__ iorbs, double thresh,
+                                unsigned char* __restrict__ keep, int* __restrict__ kc, int* __restrict__ kq) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nmo) return;
+  int m = ncmo[i], kept = 0, q = 0;
+  for (int k = 0; k < nc[i]; ++k) {
+    const int no = iorbs[ic[nnc[i] + k] - 1];
+    double sum = 0.0;
+    for (int x = 0; x < no; ++x) sum += c[m + x] * c[m + x];
+    const bool kk = sum > thresh;
+    keep[nnc[i] + k] = kk ? 1 : 0;
+    if (kk) {
+      ++kept;
+      q += no;
+    }
+    m += no;
+  }
+  kc[i] = kept;
+  kq[i] = q;
+}
+
+// tidy's moves: the kept atoms and coefficients of each LMO, in order, from
+// its old place in (ic, c) to its new one in (ic2, c2); the count becomes
+// the kept count.  Only each LMO's own extent is written.
+__global__ void k_mz_tidy_move(int nmo, int* __restrict__ nc, const int* __restrict__ nnc,
+                               const int* __restrict__ ncmo, const int* __restrict__ ic,
+                               const double* __restrict__ c, const int* __restrict__ iorbs,
+                               const unsigned char* __restrict__ keep, const int* __restrict__ nnc_new,
+                               const int* __restrict__ ncmo_new, int* __restrict__ ic2, double* __restrict__ c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nmo) return;
+  int m = ncmo[i], r = 0, q = ncmo_new[i];
+  for (int k = 0; k < nc[i]; ++k) {
+    const int a = ic[nnc[i] + k];
+    const int no = iorbs[a - 1];
+    if (keep[nnc[i] + k]) {
+      ic2[nnc_new[i] + r++] = a;
+      for (int x = 0; x < no; ++x) c2[q + x] = c[m + x];
+      q += no;
+    }
+    m += no;
+  }
+  nc[i] = r;
+}
+
+// check (src/MOZYME/check.F90:16-113): each LMO's squared norm, summed in the
+// reference's order by one thread, |1 - norm| into err, then the LMO scaled
+// to unit norm.
+__global__ void k_mz_check(int nmo, const int* __restrict__ nc, const int* __restrict__ nnc,
+                           const int* __restrict__ ncmo, const int* __restrict__ ic, double* __restrict__ c,
+                           const int* __restrict__ iorbs, double* __restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nmo) return;
+  int n = 0;
+  double sum = 0.0;
+  for (int k = 0; k < nc[i]; ++k) {
+    const int no = iorbs[ic[nnc[i] + k] - 1];
+    for (int x = 0; x < no; ++x, ++n) sum += c[ncmo[i] + n] * c[ncmo[i] + n];
+  }
+  err[i] = fabs(1.0 - sum);
+  const double sc = 1.0 / sqrt(sum);
+  for (int x = 0; x < n; ++x) c[ncmo[i] + x] *= sc;
+}
+
+// eimp (src/MOZYME/eimp.F90:16-45): the first element of each stored
+// off-diagonal block of p becomes the sum of squares of that block of f.
+__global__ void k_mz_eimp(int nblk, const int* __restrict__ off, const int* __restrict__ len,
+                          const double* __restrict__ f, double* __restrict__ p) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  double sum = 0.0;
+  for (int m = 0; m < len[b]; ++m) sum += f[off[b] + m] * f[off[b] + m];
+  p[off[b]] = sum;
+}
+
 inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace
@@ -1231,10 +1313,12 @@ struct MzStats {
   hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 MzStats g_mz_stats;
-// diagg2 on the LDS rotation kernels (runs of narrow levels in one workgroup,
-// k_mz_diagg2_wg; wider levels across the chip, k_mz_diagg2_lvl) when they
-// fit, else the global-map kernel one launch per level (k_mz_diagg2)
-bool g_mz_one_wg = true;
+// diagg2 on the global-map kernel one launch per level (k_mz_diagg2, the
+// default: fastest measured on the 400-residue helix, 1.20 s of rotations per
+// SCF against 1.25-1.56 s for the LDS kernels), or on the LDS rotation kernels
+// (runs of narrow levels in one workgroup, k_mz_diagg2_wg; wider levels
+// across the chip, k_mz_diagg2_lvl)
+bool g_mz_one_wg = false;
 int g_mz_wide_rounds = 3;
 int g_mz_lcap = 1024;  // D2W_LCAP
 
@@ -1244,6 +1328,10 @@ struct MzWork {
   LmoBufs B;
   std::vector<int> nijbo_seen, iorbs_seen;  // the nijbo (and sizes) last checked and uploaded
   long long nijbo_mpack = -1;
+  bool resident = false;                 // a resident MOZYME loop owns B
+  std::vector<int> nq_occ, nq_vir;       // its LMOs' coefficient counts (last tidy)
+  DevBuf icocc2, cocc2, icvir2, cvir2;   // tidy's second copies (swapped with B's)
+  DevBuf dkeep, dkc, dkq, dnn_new, dnq_new, derr, deoff, delen;
   DevBuf dp, dpart, dfao, daocc, dcounts, doffs, dnfmo, dold, dfmo, difmo, dvmax, dvsum, deigv, deigs, dorder,
       dused, dlat, dsj, dsi, dsumb, dnrej, dh, df, dout, dpn, d1, d2, d3, di, dsc, dlevel, dstatus;
 };
@@ -1376,44 +1464,71 @@ hipStream_t mz_stream(mhip_ctx* ctx) {
 
 }  // namespace
 
-extern "C" {
 
-int mhip_mozyme_density(mhip_ctx* ctx, const mhip_lmo_set* L, int mode, const double* partp, double* p) {
-  return mz_guard([&] {
-    hipStream_t s = mz_stream(ctx);
-    MzWork& W = mz_work();
-    upload_lmos(s, L, W);
-    const LmoBufs& B = W.B;
-    DevBuf &dp = W.dp, &dpart = W.dpart;
-    dp.ensure(sizeof(double) * L->mpack);
-    if (mode != 0) {
-      MHIP_REQUIRE(partp != nullptr, "mhip_mozyme_density: partp required for mode != 0");
-      up(s, dpart, partp, sizeof(double) * L->mpack);
-    }
-    const int nb = std::min(4096, cdiv(L->mpack, 256));
-    hipLaunchKernelGGL(k_mz_density_init, dim3(nb), dim3(256), 0, s, L->mpack, mode,
-                       mode != 0 ? dpart.as<double>() : nullptr, dp.as<double>());
+// ---- device-pointer cores of the entry points (the resident MOZYME loop
+// calls them directly; the entry points add the host transfers) --------------
+namespace mhip {
+
+void mz_density_dev(hipStream_t s, long long mpack, int nocc, int mode, const double* partp_d, double* p_d) {
+  const LmoBufs& B = mz_work().B;
+  const int nb = std::min(4096, cdiv(mpack, 256));
+  hipLaunchKernelGGL(k_mz_density_init, dim3(nb), dim3(256), 0, s, mpack, mode, partp_d, p_d);
+  MHIP_LAUNCH_CHECK();
+  if (nocc > 0) {
+    hipLaunchKernelGGL(k_mz_density, dim3(nocc), dim3(256), 0, s, B.dev, p_d);
     MHIP_LAUNCH_CHECK();
-    if (L->nocc > 0) {
-      hipLaunchKernelGGL(k_mz_density, dim3(L->nocc), dim3(256), 0, s, B.dev, dp.as<double>());
-      MHIP_LAUNCH_CHECK();
-    }
-    const double spinfa = (mode == 0 || mode == 1) ? 2.0 : (mode == -1 ? -2.0 : 1.0);
-    if (std::fabs(spinfa - 1.0) > 1e-10) {
-      hipLaunchKernelGGL(k_scale, dim3(nb), dim3(256), 0, s, L->mpack, spinfa, dp.as<double>());
-      MHIP_LAUNCH_CHECK();
-    }
-    down(s, p, dp, sizeof(double) * L->mpack);
-    MHIP_CHECK(hipStreamSynchronize(s));
-  });
+  }
+  const double spinfa = (mode == 0 || mode == 1) ? 2.0 : (mode == -1 ? -2.0 : 1.0);
+  if (std::fabs(spinfa - 1.0) > 1e-10) {
+    hipLaunchKernelGGL(k_scale, dim3(nb), dim3(256), 0, s, mpack, spinfa, p_d);
+    MHIP_LAUNCH_CHECK();
+  }
 }
 
-int mhip_mozyme_diagg1(mhip_ctx* ctx, const mhip_lmo_set* L, const double* fao, const double* p,
-                       double* eigs, double* eigv, int idiagg, mhip_diagg1_state* st, int fmo_dim,
-                       double* fmo, int* ifmo, int* nij) {
-  return mz_guard([&] {
+double mz_helecz_dev(hipStream_t s, int numat, const double* p_d, const double* h_d, const double* f_d) {
+  MzWork& W = mz_work();
+  W.dpart.ensure(sizeof(double) * 2 * numat);
+  W.dout.ensure(sizeof(double));
+  hipLaunchKernelGGL(k_mz_helecz, dim3(numat), dim3(256), 0, s, numat, W.B.iorbs.as<int>(), W.B.nijbo.as<int>(),
+                     p_d, h_d, f_d, W.dpart.as<double>());
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mz_sum2, dim3(1), dim3(1024), 0, s, numat, W.dpart.as<double>(), W.dout.as<double>());
+  MHIP_LAUNCH_CHECK();
+  double e = 0.0;
+  down(s, &e, W.dout, sizeof(double));
+  MHIP_CHECK(hipStreamSynchronize(s));
+  return e;
+}
+
+// pnew and p (device) end equal, as cnvgz leaves them
+double mz_cnvgz_dev(hipStream_t s, int norbs, long long mpack, double* pn_d, double* p_d, double* p1_d,
+                    double* p2_d, double* p3_d, int niter, const int* idiag_d, int three_point) {
+  MzWork& W = mz_work();
+  const int nparts = std::min(1024, cdiv(mpack, 256));
+  W.dpart.ensure(sizeof(double) * nparts);
+  W.dsc.ensure(sizeof(double) * 2);
+  hipLaunchKernelGGL(k_mz_absdiff_max, dim3(nparts), dim3(256), 0, s, mpack, pn_d, p_d, W.dpart.as<double>());
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mz_cnvgz_diag, dim3(1), dim3(1024), 0, s, norbs, nparts, W.dpart.as<double>(), idiag_d, pn_d,
+                     p_d, p1_d, p2_d, p3_d, niter, three_point, W.dsc.as<double>());
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mz_extrap, dim3(std::min(4096, cdiv(mpack, 256))), dim3(256), 0, s, mpack,
+                     W.dsc.as<double>(), pn_d, p_d);
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mz_damp, dim3(cdiv(norbs, 256)), dim3(256), 0, s, norbs, idiag_d, W.dsc.as<double>(), p2_d,
+                     p3_d, niter, three_point, pn_d, p1_d);
+  MHIP_LAUNCH_CHECK();
+  MHIP_CHECK(hipMemcpyAsync(p_d, pn_d, sizeof(double) * mpack, hipMemcpyDeviceToDevice, s));
+  double sc[2];
+  down(s, sc, W.dsc, sizeof(sc));
+  MHIP_CHECK(hipStreamSynchronize(s));
+  return sc[0];
+}
+
+void mz_diagg1_dev(hipStream_t s, const mhip_lmo_set* L, const double* fao_d, const double* p_d, double* eigs,
+                   double* eigv, int idiagg, mhip_diagg1_state* st, int fmo_dim, double* fmo, int* ifmo, int* nij) {
+  {
     MHIP_REQUIRE(st && fmo && ifmo && nij && eigs && eigv && fmo_dim > 0, "mhip_mozyme_diagg1: bad arguments");
-    hipStream_t s = mz_stream(ctx);
     // saved state of diagg1.F90:61-71 (reset when st->reset is set)
     if (st->reset) {
       st->fref = st->oldens ? 0.0 : 10.0;
@@ -1424,13 +1539,10 @@ int mhip_mozyme_diagg1(mhip_ctx* ctx, const mhip_lmo_set* L, const double* fao, 
     const bool full = (idiagg <= 5 || idiagg % 2 == 0);
     MHIP_REQUIRE(full || st->nfmo != nullptr, "mhip_mozyme_diagg1: nfmo needed to reuse the list");
     MzWork& W = mz_work();
-    upload_lmos(s, L, W);
     const LmoBufs& B = W.B;
-    DevBuf &dfao = W.dfao, &dp = W.dp, &daocc = W.daocc, &dcounts = W.dcounts, &doffs = W.doffs,
+    DevBuf &daocc = W.daocc, &dcounts = W.dcounts, &doffs = W.doffs,
            &dnfmo = W.dnfmo, &dold = W.dold, &dfmo = W.dfmo, &difmo = W.difmo, &dvmax = W.dvmax,
            &dvsum = W.dvsum, &deigv = W.deigv, &deigs = W.deigs;
-    up(s, dfao, fao, sizeof(double) * L->mpack);
-    up(s, dp, p, sizeof(double) * L->mpack);
     daocc.ensure(sizeof(double) * std::max(L->icocc_dim, 1));
     MHIP_CHECK(hipMemsetAsync(daocc.ptr, 0, sizeof(double) * std::max(L->icocc_dim, 1), s));
     up(s, deigv, eigv, sizeof(double) * std::max(L->nvir, 1));
@@ -1459,8 +1571,8 @@ int mhip_mozyme_diagg1(mhip_ctx* ctx, const mhip_lmo_set* L, const double* fao, 
     std::vector<int> offs(nv, 0), counts(nv, 0);
     if (full) {
       if (L->nvir > 0) {
-        hipLaunchKernelGGL(k_mz_diagg1_count, dim3(L->nvir), dim3(256), 0, s, B.dev, dfao.as<double>(),
-                           dp.as<double>(), daocc.as<double>(), P, dcounts.as<int>(), deigv.as<double>());
+        hipLaunchKernelGGL(k_mz_diagg1_count, dim3(L->nvir), dim3(256), 0, s, B.dev, fao_d,
+                           p_d, daocc.as<double>(), P, dcounts.as<int>(), deigv.as<double>());
         MHIP_LAUNCH_CHECK();
         down(s, counts.data(), dcounts, sizeof(int) * L->nvir);
         MHIP_CHECK(hipStreamSynchronize(s));
@@ -1486,8 +1598,8 @@ int mhip_mozyme_diagg1(mhip_ctx* ctx, const mhip_lmo_set* L, const double* fao, 
     }
     up(s, doffs, offs.data(), sizeof(int) * nv);
     if (L->nvir > 0) {
-      hipLaunchKernelGGL(k_mz_diagg1_write, dim3(L->nvir), dim3(256), 0, s, B.dev, dfao.as<double>(),
-                         dp.as<double>(), daocc.as<double>(), P, doffs.as<int>(), dnfmo.as<int>(),
+      hipLaunchKernelGGL(k_mz_diagg1_write, dim3(L->nvir), dim3(256), 0, s, B.dev, fao_d,
+                         p_d, daocc.as<double>(), P, doffs.as<int>(), dnfmo.as<int>(),
                          full ? nullptr : dold.as<int>(), dfmo.as<double>(), difmo.as<int>(),
                          dvmax.as<double>(), dvsum.as<double>(), deigv.as<double>());
       MHIP_LAUNCH_CHECK();
@@ -1519,8 +1631,8 @@ int mhip_mozyme_diagg1(mhip_ctx* ctx, const mhip_lmo_set* L, const double* fao, 
       st->fref = (tiny * tiny) * (tiny * tiny);  // tiny**4 as the reference's powi
     } else {
       if (L->nocc > 0) {
-        hipLaunchKernelGGL(k_mz_occ_energy, dim3(cdiv(L->nocc, 64)), dim3(64), 0, s, B.dev, dfao.as<double>(),
-                           dp.as<double>(), daocc.as<double>(), P.cutoff, deigs.as<double>());
+        hipLaunchKernelGGL(k_mz_occ_energy, dim3(cdiv(L->nocc, 64)), dim3(64), 0, s, B.dev, fao_d,
+                           p_d, daocc.as<double>(), P.cutoff, deigs.as<double>());
         MHIP_LAUNCH_CHECK();
         down(s, eigs, deigs, sizeof(double) * L->nocc);
       }
@@ -1531,14 +1643,13 @@ int mhip_mozyme_diagg1(mhip_ctx* ctx, const mhip_lmo_set* L, const double* fao, 
     st->tiny = tiny;
     st->sumt = sumt;
     st->ovmax = tiny;
-  });
+  }
 }
 
-int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const double* eigv,
-                       const double* fmo, const int* ifmo, int nij, int idiagg, mhip_diagg2_state* st) {
-  return mz_guard([&] {
+void mz_diagg2_dev(hipStream_t s, const mhip_lmo_set* L, const double* eigs, const double* eigv, const double* fmo,
+                   const int* ifmo, int nij, int idiagg, mhip_diagg2_state* st) {
+  {
     MHIP_REQUIRE(st && eigs && eigv && nij >= 0 && (nij == 0 || (fmo && ifmo)), "mhip_mozyme_diagg2: bad arguments");
-    hipStream_t s = mz_stream(ctx);
     for (int k = 0; k < nij; ++k)
       MHIP_REQUIRE(ifmo[2 * k] >= 1 && ifmo[2 * k] <= L->nvir && ifmo[2 * k + 1] >= 1 &&
                        ifmo[2 * k + 1] <= L->nocc,
@@ -1658,7 +1769,6 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
         if (start[l + 1] - start[l] > wide) widest_wide = std::max(widest_wide, start[l + 1] - start[l]);
     const int nslots = lds_ok ? std::max(nw, std::min(widest_wide, 2048)) : std::max(1, std::min(widest, 2048));
     MzWork& W = mz_work();
-    upload_lmos(s, L, W);
     const LmoBufs& B = W.B;
     DevBuf &deigs = W.deigs, &deigv = W.deigv, &dfmo = W.dfmo, &difmo = W.difmo, &dorder = W.dorder,
            &dused = W.dused, &dlat = W.dlat, &dsj = W.dsj, &dsi = W.dsi, &dsumb = W.dsumb, &dnrej = W.dnrej;
@@ -1692,7 +1802,10 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
       };
       for (int ij : taken) {
         const int i = ifmo[2 * ij] - 1, j = ifmo[2 * ij + 1] - 1;
-        g_mz_stats.bytes += 16.0 * (coefs(L->nce, L->nnce, L->icvir, i) + coefs(L->ncf, L->nncf, L->icocc, j));
+        // a resident loop keeps each LMO's coefficient count from its last tidy
+        g_mz_stats.bytes += 16.0 * (W.resident ? (double)W.nq_vir[i] + W.nq_occ[j]
+                                               : (double)(coefs(L->nce, L->nnce, L->icvir, i) +
+                                                          coefs(L->ncf, L->nncf, L->icocc, j)));
       }
       g_mz_stats.rotations += (long long)taken.size();
       g_mz_stats.levels += nlev;
@@ -1765,12 +1878,6 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
       down(s, sb.data(), dsumb, sizeof(double) * nij);
       down(s, nr.data(), dnrej, sizeof(int) * nij);
     }
-    down(s, L->ncf, B.ncf, sizeof(int) * L->nocc);
-    down(s, L->icocc, B.icocc, sizeof(int) * L->icocc_dim);
-    down(s, L->cocc, B.cocc, sizeof(double) * L->cocc_dim);
-    down(s, L->nce, B.nce, sizeof(int) * L->nvir);
-    down(s, L->icvir, B.icvir, sizeof(int) * L->icvir_dim);
-    down(s, L->cvir, B.cvir, sizeof(double) * L->cvir_dim);
     int status = 0;
     if (lds_ok) down(s, &status, W.dstatus, sizeof(int));
     MHIP_CHECK(hipStreamSynchronize(s));
@@ -1789,6 +1896,244 @@ int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const
       MHIP_CHECK(hipEventElapsedTime(&t, g_mz_stats.e0, g_mz_stats.e1));
       g_mz_stats.ms += t;
     }
+  }
+}
+
+
+hipStream_t mz_stream_of(mhip_ctx* ctx) { return mz_stream(ctx); }
+
+// The LMO set uploaded once (checked as every entry point checks it) and
+// owned by the device until mz_resident_end copies it back.
+void mz_resident_begin(hipStream_t s, const mhip_lmo_set* L) {
+  MzWork& W = mz_work();
+  upload_lmos(s, L, W);
+  W.resident = true;
+  auto counts = [&](int n, const int* nc, const int* nn, const int* idx, std::vector<int>& out) {
+    out.assign(std::max(n, 1), 0);
+    for (int k = 0; k < n; ++k)
+      for (int a = 0; a < nc[k]; ++a) out[k] += L->nlast[idx[nn[k] + a] - 1] - L->nfirst[idx[nn[k] + a] - 1] + 1;
+  };
+  counts(L->nocc, L->ncf, L->nncf, L->icocc, W.nq_occ);
+  counts(L->nvir, L->nce, L->nnce, L->icvir, W.nq_vir);
+}
+
+// Everything tidy and diagg2 change, back into the caller's arrays.
+void mz_resident_end(hipStream_t s, mhip_lmo_set* L) {
+  MzWork& W = mz_work();
+  const LmoBufs& B = W.B;
+  W.resident = false;
+  down(s, L->ncf, B.ncf, sizeof(int) * L->nocc);
+  down(s, const_cast<int*>(L->nncf), B.nncf, sizeof(int) * L->nocc);
+  down(s, const_cast<int*>(L->ncocc), B.ncocc, sizeof(int) * L->nocc);
+  down(s, L->icocc, B.icocc, sizeof(int) * L->icocc_dim);
+  down(s, L->cocc, B.cocc, sizeof(double) * L->cocc_dim);
+  down(s, L->nce, B.nce, sizeof(int) * L->nvir);
+  down(s, const_cast<int*>(L->nnce), B.nnce, sizeof(int) * L->nvir);
+  down(s, const_cast<int*>(L->ncvir), B.ncvir, sizeof(int) * L->nvir);
+  down(s, L->icvir, B.icvir, sizeof(int) * L->icvir_dim);
+  down(s, L->cvir, B.cvir, sizeof(double) * L->cvir_dim);
+  MHIP_CHECK(hipStreamSynchronize(s));
+}
+
+// tidy (src/MOZYME/tidy.F90:16-244) at a single point on the resident
+// occupied (occ) or virtual set: the per-LMO passes on the device, the
+// free-space spreading (an order-dependent recurrence over the LMO counts)
+// on the host, which also keeps the new offsets in nn_host / no_host.
+// Returns false when the storage is too small for the next annihilation
+// step (the reference then re-allocates 60% larger arrays).
+bool mz_tidy_dev(hipStream_t s, int numat, int norbs, bool occ, int nmo, int* nn_host, int* no_host,
+                 double thresh) {
+  MzWork& W = mz_work();
+  LmoBufs& B = W.B;
+  if (nmo == 0) return true;
+  DevBuf& nc = occ ? B.ncf : B.nce;
+  DevBuf& nnc = occ ? B.nncf : B.nnce;
+  DevBuf& ncmo = occ ? B.ncocc : B.ncvir;
+  DevBuf& ic = occ ? B.icocc : B.icvir;
+  DevBuf& c = occ ? B.cocc : B.cvir;
+  DevBuf& ic2 = occ ? W.icocc2 : W.icvir2;
+  DevBuf& c2 = occ ? W.cocc2 : W.cvir2;
+  const int n01 = occ ? B.dev.icocc_dim : B.dev.icvir_dim;
+  const int n02 = occ ? B.dev.cocc_dim : B.dev.cvir_dim;
+  W.dkeep.ensure((size_t)n01);
+  W.dkc.ensure(sizeof(int) * nmo);
+  W.dkq.ensure(sizeof(int) * nmo);
+  hipLaunchKernelGGL(k_mz_tidy_count, dim3(cdiv(nmo, 128)), dim3(128), 0, s, nmo, nc.as<int>(), nnc.as<int>(),
+                     ncmo.as<int>(), ic.as<int>(), c.as<double>(), B.iorbs.as<int>(), thresh,
+                     (unsigned char*)W.dkeep.ptr, W.dkc.as<int>(), W.dkq.as<int>());
+  MHIP_LAUNCH_CHECK();
+  std::vector<int> kc(nmo), kq(nmo);
+  down(s, kc.data(), W.dkc, sizeof(int) * nmo);
+  down(s, kq.data(), W.dkq, sizeof(int) * nmo);
+  MHIP_CHECK(hipStreamSynchronize(s));
+  long long ln = 0, mn = 0;
+  for (int i = 0; i < nmo; ++i) {
+    ln += kc[i];
+    mn += kq[i];
+  }
+  int ispace = (int)((n01 - ln) / nmo), jspace = (int)((n02 - mn) / nmo);
+  const int per = n01 / nmo;
+  if (per + ispace < numat && ispace < std::max(per / 5, 20)) return false;
+  if (ispace > numat && jspace > nmo) ispace = jspace = 0;
+  std::vector<int> nn_new(nmo), no_new(nmo);
+  int jtop = n01, mtop = n02;
+  int jsav = (int)(n01 - ln - (long long)ispace * nmo), msav = (int)(n02 - mn - (long long)jspace * nmo);
+  long long lc = ln - kc[nmo - 1], mc = mn - kq[nmo - 1];  // compacted starts of LMO i
+  for (int i = nmo - 1; i >= 1; --i) {
+    int j = std::min(kc[i] + ispace, numat);
+    if (j < kc[i] + ispace) {
+      jsav += kc[i] + ispace - j;
+      jtop -= j;
+    } else {
+      const int jdash = std::min(numat, std::max(jsav, 0) + j);
+      jtop -= jdash;
+      jsav = jsav - jdash + j;
+    }
+    // the moved LMO lands at or above its compacted place (never over an
+    // LMO below it); the reference's bookkeeping keeps it so
+    if (jtop < lc) return false;
+    nn_new[i] = jtop;
+    const int n = kq[i];
+    j = std::min(n + jspace, norbs);
+    if (j < n + jspace) {
+      msav += n + jspace - j;
+      mtop -= j;
+    } else {
+      const int mdash = std::min(norbs, std::max(msav, 0) + j);
+      mtop -= mdash;
+      msav = msav - mdash + j;
+    }
+    if (mtop < mc) return false;
+    no_new[i] = mtop;
+    lc -= kc[i - 1];
+    mc -= kq[i - 1];
+  }
+  nn_new[0] = 0;
+  no_new[0] = 0;
+  up(s, W.dnn_new, nn_new.data(), sizeof(int) * nmo);
+  up(s, W.dnq_new, no_new.data(), sizeof(int) * nmo);
+  ic2.ensure(sizeof(int) * (size_t)n01);
+  c2.ensure(sizeof(double) * (size_t)n02);
+  hipLaunchKernelGGL(k_mz_tidy_move, dim3(cdiv(nmo, 128)), dim3(128), 0, s, nmo, nc.as<int>(), nnc.as<int>(),
+                     ncmo.as<int>(), ic.as<int>(), c.as<double>(), B.iorbs.as<int>(),
+                     (const unsigned char*)W.dkeep.ptr, W.dnn_new.as<int>(), W.dnq_new.as<int>(), ic2.as<int>(),
+                     c2.as<double>());
+  MHIP_LAUNCH_CHECK();
+  MHIP_CHECK(hipMemcpyAsync(nnc.ptr, W.dnn_new.ptr, sizeof(int) * nmo, hipMemcpyDeviceToDevice, s));
+  MHIP_CHECK(hipMemcpyAsync(ncmo.ptr, W.dnq_new.ptr, sizeof(int) * nmo, hipMemcpyDeviceToDevice, s));
+  std::swap(ic, ic2);
+  std::swap(c, c2);
+  LmoDev& d = B.dev;
+  if (occ) {
+    d.icocc = B.icocc.as<int>();
+    d.cocc = B.cocc.as<double>();
+    W.nq_occ = kq;
+  } else {
+    d.icvir = B.icvir.as<int>();
+    d.cvir = B.cvir.as<double>();
+    W.nq_vir = kq;
+  }
+  std::copy(nn_new.begin(), nn_new.end(), nn_host);
+  std::copy(no_new.begin(), no_new.end(), no_host);
+  return true;
+}
+
+// check (src/MOZYME/check.F90:16-113) on the resident set: renormalizes every
+// LMO and returns the summed normalization error (the reference stops the
+// job above 0.1).
+double mz_check_dev(hipStream_t s, bool occ, int nmo) {
+  MzWork& W = mz_work();
+  LmoBufs& B = W.B;
+  if (nmo == 0) return 0.0;
+  W.derr.ensure(sizeof(double) * nmo);
+  hipLaunchKernelGGL(k_mz_check, dim3(cdiv(nmo, 128)), dim3(128), 0, s, nmo,
+                     (occ ? B.ncf : B.nce).as<int>(), (occ ? B.nncf : B.nnce).as<int>(),
+                     (occ ? B.ncocc : B.ncvir).as<int>(), (occ ? B.icocc : B.icvir).as<int>(),
+                     (occ ? B.cocc : B.cvir).as<double>(), B.iorbs.as<int>(), W.derr.as<double>());
+  MHIP_LAUNCH_CHECK();
+  std::vector<double> err(nmo);
+  down(s, err.data(), W.derr, sizeof(double) * nmo);
+  MHIP_CHECK(hipStreamSynchronize(s));
+  double e = 0.0;
+  for (double x : err) e += x;
+  return e;
+}
+
+// The stored off-diagonal blocks of nijbo (offset, length), for eimp.
+void mz_eimp_blocks(hipStream_t s, int numat, const int* nijbo, const int* iorbs, int* nblk) {
+  MzWork& W = mz_work();
+  std::vector<int> off, len;
+  for (int i = 0; i < numat; ++i)
+    for (int j = 0; j < i; ++j) {
+      const int k = nijbo[i + (size_t)j * numat];
+      if (k >= 0) {
+        off.push_back(k);
+        len.push_back(iorbs[i] * iorbs[j]);
+      }
+    }
+  *nblk = (int)off.size();
+  up(s, W.deoff, off.data(), sizeof(int) * off.size());
+  up(s, W.delen, len.data(), sizeof(int) * len.size());
+}
+
+void mz_eimp_dev(hipStream_t s, int nblk, const double* f_d, double* p_d) {
+  MzWork& W = mz_work();
+  if (nblk == 0) return;
+  hipLaunchKernelGGL(k_mz_eimp, dim3(cdiv(nblk, 256)), dim3(256), 0, s, nblk, W.deoff.as<int>(),
+                     W.delen.as<int>(), f_d, p_d);
+  MHIP_LAUNCH_CHECK();
+}
+
+}  // namespace mhip
+
+extern "C" {
+
+int mhip_mozyme_density(mhip_ctx* ctx, const mhip_lmo_set* L, int mode, const double* partp, double* p) {
+  return mz_guard([&] {
+    hipStream_t s = mz_stream(ctx);
+    MzWork& W = mz_work();
+    upload_lmos(s, L, W);
+    DevBuf &dp = W.dp, &dpart = W.dpart;
+    dp.ensure(sizeof(double) * L->mpack);
+    if (mode != 0) {
+      MHIP_REQUIRE(partp != nullptr, "mhip_mozyme_density: partp required for mode != 0");
+      up(s, dpart, partp, sizeof(double) * L->mpack);
+    }
+    mhip::mz_density_dev(s, L->mpack, L->nocc, mode, mode != 0 ? dpart.as<double>() : nullptr, dp.as<double>());
+    down(s, p, dp, sizeof(double) * L->mpack);
+    MHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int mhip_mozyme_diagg1(mhip_ctx* ctx, const mhip_lmo_set* L, const double* fao, const double* p,
+                       double* eigs, double* eigv, int idiagg, mhip_diagg1_state* st, int fmo_dim,
+                       double* fmo, int* ifmo, int* nij) {
+  return mz_guard([&] {
+    hipStream_t s = mz_stream(ctx);
+    MzWork& W = mz_work();
+    upload_lmos(s, L, W);
+    up(s, W.dfao, fao, sizeof(double) * L->mpack);
+    up(s, W.dp, p, sizeof(double) * L->mpack);
+    mhip::mz_diagg1_dev(s, L, W.dfao.as<double>(), W.dp.as<double>(), eigs, eigv, idiagg, st, fmo_dim, fmo, ifmo,
+                        nij);
+  });
+}
+
+int mhip_mozyme_diagg2(mhip_ctx* ctx, mhip_lmo_set* L, const double* eigs, const double* eigv,
+                       const double* fmo, const int* ifmo, int nij, int idiagg, mhip_diagg2_state* st) {
+  return mz_guard([&] {
+    hipStream_t s = mz_stream(ctx);
+    MzWork& W = mz_work();
+    upload_lmos(s, L, W);
+    mhip::mz_diagg2_dev(s, L, eigs, eigv, fmo, ifmo, nij, idiagg, st);
+    const LmoBufs& B = W.B;
+    down(s, L->ncf, B.ncf, sizeof(int) * L->nocc);
+    down(s, L->icocc, B.icocc, sizeof(int) * L->icocc_dim);
+    down(s, L->cocc, B.cocc, sizeof(double) * L->cocc_dim);
+    down(s, L->nce, B.nce, sizeof(int) * L->nvir);
+    down(s, L->icvir, B.icvir, sizeof(int) * L->icvir_dim);
+    down(s, L->cvir, B.cvir, sizeof(double) * L->cvir_dim);
+    MHIP_CHECK(hipStreamSynchronize(s));
   });
 }
 
@@ -1806,6 +2151,49 @@ int mhip_mozyme_stats(int enable, double* ms, long long* rotations, long long* l
     g_mz_stats.rotations = g_mz_stats.levels = g_mz_stats.launches = g_mz_stats.rounds = 0;
   }
   return 0;
+}
+
+int mhip_mozyme_tidy(mhip_ctx* ctx, mhip_lmo_set* L, int occupied, double thresh) {
+  return mz_guard([&] {
+    MHIP_REQUIRE(L, "mhip_mozyme_tidy: null LMO set");
+    hipStream_t s = mz_stream(ctx);
+    mhip::mz_resident_begin(s, L);
+    const bool occ = occupied != 0;
+    const int nmo = occ ? L->nocc : L->nvir;
+    std::vector<int> nn(std::max(nmo, 1)), no(std::max(nmo, 1));
+    const bool ok = mhip::mz_tidy_dev(s, L->numat, L->norbs, occ, nmo, nn.data(), no.data(), thresh);
+    mhip::mz_resident_end(s, L);
+    MHIP_REQUIRE(ok, "mhip_mozyme_tidy: LMO storage too small for the next annihilation step");
+  });
+}
+
+int mhip_mozyme_check(mhip_ctx* ctx, mhip_lmo_set* L, int occupied, double* error) {
+  return mz_guard([&] {
+    MHIP_REQUIRE(L && error, "mhip_mozyme_check: bad arguments");
+    hipStream_t s = mz_stream(ctx);
+    mhip::mz_resident_begin(s, L);
+    *error = mhip::mz_check_dev(s, occupied != 0, occupied ? L->nocc : L->nvir);
+    mhip::mz_resident_end(s, L);
+  });
+}
+
+int mhip_mozyme_eimp(mhip_ctx* ctx, int numat, const int* nfirst, const int* nlast, const int* nijbo,
+                     long long mpack, const double* f, double* p) {
+  return mz_guard([&] {
+    MHIP_REQUIRE(numat > 0 && nfirst && nlast && nijbo && f && p, "mhip_mozyme_eimp: bad arguments");
+    hipStream_t s = mz_stream(ctx);
+    std::vector<int> iorbs(numat);
+    for (int a = 0; a < numat; ++a) iorbs[a] = nlast[a] - nfirst[a] + 1;
+    MzWork& W = mz_work();
+    upload_nijbo(s, W, numat, nijbo, mpack, iorbs);
+    int nblk = 0;
+    mhip::mz_eimp_blocks(s, numat, nijbo, iorbs.data(), &nblk);
+    up(s, W.df, f, sizeof(double) * mpack);
+    up(s, W.dp, p, sizeof(double) * mpack);
+    mhip::mz_eimp_dev(s, nblk, W.df.as<double>(), W.dp.as<double>());
+    down(s, p, W.dp, sizeof(double) * mpack);
+    MHIP_CHECK(hipStreamSynchronize(s));
+  });
 }
 
 int mhip_mozyme_diagg2_one_workgroup(int enable) {
@@ -1835,20 +2223,10 @@ int mhip_mozyme_helecz(mhip_ctx* ctx, int numat, const int* nfirst, const int* n
     for (int a = 0; a < numat; ++a) iorbs[a] = nlast[a] - nfirst[a] + 1;
     MzWork& W = mz_work();
     upload_nijbo(s, W, numat, nijbo, mpack, iorbs);
-    DevBuf &di = W.B.iorbs, &dn = W.B.nijbo, &dp = W.dp, &dh = W.dh, &df = W.df, &dpart = W.dpart,
-           &dout = W.dout;
-    up(s, dp, p, sizeof(double) * mpack);
-    up(s, dh, h, sizeof(double) * mpack);
-    up(s, df, f, sizeof(double) * mpack);
-    dpart.ensure(sizeof(double) * 2 * numat);
-    dout.ensure(sizeof(double));
-    hipLaunchKernelGGL(k_mz_helecz, dim3(numat), dim3(256), 0, s, numat, di.as<int>(), dn.as<int>(),
-                       dp.as<double>(), dh.as<double>(), df.as<double>(), dpart.as<double>());
-    MHIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_mz_sum2, dim3(1), dim3(1024), 0, s, numat, dpart.as<double>(), dout.as<double>());
-    MHIP_LAUNCH_CHECK();
-    down(s, e, dout, sizeof(double));
-    MHIP_CHECK(hipStreamSynchronize(s));
+    up(s, W.dp, p, sizeof(double) * mpack);
+    up(s, W.dh, h, sizeof(double) * mpack);
+    up(s, W.df, f, sizeof(double) * mpack);
+    *e = mhip::mz_helecz_dev(s, numat, W.dp.as<double>(), W.dh.as<double>(), W.df.as<double>());
   });
 }
 
@@ -1861,39 +2239,21 @@ int mhip_mozyme_cnvgz(mhip_ctx* ctx, int norbs, long long mpack, double* pnew, d
       MHIP_REQUIRE(idiag[i] >= 1 && idiag[i] <= mpack, "mhip_mozyme_cnvgz: idiag out of range");
     hipStream_t s = mz_stream(ctx);
     MzWork& W = mz_work();
-    DevBuf &dpn = W.dpn, &dp = W.dp, &d1 = W.d1, &d2 = W.d2, &d3 = W.d3, &di = W.di, &dpart = W.dpart,
-           &dsc = W.dsc;
+    DevBuf &dpn = W.dpn, &dp = W.dp, &d1 = W.d1, &d2 = W.d2, &d3 = W.d3, &di = W.di;
     up(s, dpn, pnew, sizeof(double) * mpack);
     up(s, dp, p, sizeof(double) * mpack);
     up(s, d1, p1, sizeof(double) * norbs);
     up(s, d2, p2, sizeof(double) * norbs);
     up(s, d3, p3, sizeof(double) * norbs);
     up(s, di, idiag, sizeof(int) * norbs);
-    const int nparts = std::min(1024, cdiv(mpack, 256));
-    dpart.ensure(sizeof(double) * nparts);
-    dsc.ensure(sizeof(double) * 2);
-    hipLaunchKernelGGL(k_mz_absdiff_max, dim3(nparts), dim3(256), 0, s, mpack, dpn.as<double>(), dp.as<double>(),
-                       dpart.as<double>());
-    MHIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_mz_cnvgz_diag, dim3(1), dim3(1024), 0, s, norbs, nparts, dpart.as<double>(), di.as<int>(),
-                       dpn.as<double>(), dp.as<double>(), d1.as<double>(), d2.as<double>(), d3.as<double>(),
-                       niter, three_point, dsc.as<double>());
-    MHIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_mz_extrap, dim3(std::min(4096, cdiv(mpack, 256))), dim3(256), 0, s, mpack,
-                       dsc.as<double>(), dpn.as<double>(), dp.as<double>());
-    MHIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_mz_damp, dim3(cdiv(norbs, 256)), dim3(256), 0, s, norbs, di.as<int>(), dsc.as<double>(),
-                       d2.as<double>(), d3.as<double>(), niter, three_point, dpn.as<double>(), d1.as<double>());
-    MHIP_LAUNCH_CHECK();
-    double sc[2];
-    down(s, sc, dsc, sizeof(sc));
+    *pmax = mhip::mz_cnvgz_dev(s, norbs, mpack, dpn.as<double>(), dp.as<double>(), d1.as<double>(),
+                               d2.as<double>(), d3.as<double>(), niter, di.as<int>(), three_point);
     down(s, pnew, dpn, sizeof(double) * mpack);
     down(s, p, dpn, sizeof(double) * mpack);  // p = pnew
     down(s, p1, d1, sizeof(double) * norbs);
     down(s, p2, d2, sizeof(double) * norbs);
     down(s, p3, d3, sizeof(double) * norbs);
     MHIP_CHECK(hipStreamSynchronize(s));
-    *pmax = sc[0];
   });
 }
 

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "k_mozyme.h"
 
 namespace mhip {
 
@@ -186,10 +187,9 @@ void mz_release(MzFock* z) {
   delete z;
 }
 
-// f (device, sparse) = h + G(p) for p (device, sparse)
-void mz_fock2z(mhip_ctx* c, const double* ps, double* fs) {
+// f (device, sparse) = h + G(p) for p (device, sparse), on stream s
+void mz_fock2z_on(mhip_ctx* c, hipStream_t s, const double* ps, double* fs) {
   MzFock* z = c->mz;
-  hipStream_t s = c->stream;
   hipLaunchKernelGGL(k_mzf_scatter, dim3(nblk(z->mpack)), dim3(256), 0, s, z->mpack, z->map.as<int>(), ps,
                      z->pd.as<double>(), z->pad.as<double>());
   MHIP_LAUNCH_CHECK();
@@ -210,6 +210,8 @@ void mz_fock2z(mhip_ctx* c, const double* ps, double* fs) {
                      z->ad.as<double>(), z->qe.as<double>(), ps, z->feather, z->trunc1, z->trunc2, fs);
   MHIP_LAUNCH_CHECK();
 }
+
+void mz_fock2z(mhip_ctx* c, const double* ps, double* fs) { mz_fock2z_on(c, c->stream, ps, fs); }
 
 }  // namespace mhip
 
@@ -334,115 +336,10 @@ int mhip_mozyme_fock2z(mhip_ctx* c, const double* p, double* f) {
 }  // extern "C"
 
 // ---- the MOZYME SCF loop (iter_for_MOZYME) ---------------------------------
-
-namespace {
-
-// LMO arrays the loop owns (tidy moves LMOs inside them)
-struct LmoStore {
-  int nmo = 0;
-  std::vector<int> nc, nnc, ncmo, ic;
-  std::vector<double> c;
-};
-
-// tidy (src/MOZYME/tidy.F90:16-244) for a single point: drop every atom whose
-// coefficients in an LMO have a squared sum <= thresh, compress, then spread
-// the free space over the LMOs from the last one down (LMO 1 stays at 0).
-// Returns false when the storage is too small for the next annihilation
-// step (the reference then re-allocates 60% larger arrays).
-bool tidy(LmoStore& L, const std::vector<int>& iorbs, int numat, int norbs, double thresh) {
-  const int nmo = L.nmo;
-  if (nmo == 0) return true;
-  const int n01 = (int)L.ic.size(), n02 = (int)L.c.size();
-  std::vector<int> iused(nmo + 1, 0);
-  int ln = 0, mn = 0, in = 0;
-  for (int i = 0; i < nmo; ++i) {
-    int ll = L.nnc[i];
-    int mm = L.ncmo[i];
-    L.ncmo[i] = mn;
-    L.nnc[i] = (i == 0) ? 0 : L.nnc[i - 1] + in;
-    in = 0;
-    for (int j1 = 0; j1 < L.nc[i]; ++j1) {
-      const int atom = L.ic[ll++];
-      const int no = iorbs[atom - 1];
-      double sum = 0.0;
-      for (int k = 0; k < no; ++k) sum += L.c[mm + k] * L.c[mm + k];
-      if (sum > thresh) {
-        for (int k = 0; k < no; ++k) L.c[mn + k] = L.c[mm + k];
-        mn += no;
-        ++in;
-        L.ic[ln++] = atom;
-      }
-      mm += no;
-    }
-    L.nc[i] = in;
-    iused[i + 1] = mn;
-  }
-  int ispace = (n01 - ln) / nmo, jspace = (n02 - mn) / nmo;
-  const int per = n01 / nmo;
-  if (per + ispace < numat && ispace < std::max(per / 5, 20)) return false;
-  if (ispace > numat && jspace > nmo) ispace = jspace = 0;
-  int jtop = n01, mtop = n02;
-  int jsav = n01 - ln - ispace * nmo, msav = n02 - mn - jspace * nmo;
-  for (int i = nmo - 1; i >= 1; --i) {
-    int j = std::min(L.nc[i] + ispace, numat);
-    if (j < L.nc[i] + ispace) {
-      jsav += L.nc[i] + ispace - j;
-      jtop -= j;
-    } else {
-      const int jdash = std::min(numat, std::max(jsav, 0) + j);
-      jtop -= jdash;
-      jsav = jsav - jdash + j;
-    }
-    const int l = L.nnc[i];
-    // the moved LMO must land at or above its compressed position (and so
-    // never over an LMO below it); the reference's bookkeeping keeps it so
-    if (jtop < l) return false;
-    L.nnc[i] = jtop;
-    for (int k = L.nc[i] - 1; k >= 0; --k) L.ic[jtop + k] = L.ic[l + k];
-    const int n = iused[i + 1] - iused[i];
-    j = std::min(n + jspace, norbs);
-    if (j < n + jspace) {
-      msav += n + jspace - j;
-      mtop -= j;
-    } else {
-      const int mdash = std::min(norbs, std::max(msav, 0) + j);
-      mtop -= mdash;
-      msav = msav - mdash + j;
-    }
-    if (mtop < iused[i]) return false;
-    L.ncmo[i] = mtop;
-    for (int k = n - 1; k >= 0; --k) L.c[mtop + k] = L.c[iused[i] + k];
-  }
-  return true;
-}
-
-// check (src/MOZYME/check.F90:16-113): renormalize every LMO; false when the
-// summed normalization error exceeds 0.1 (the reference stops the job)
-bool check(LmoStore& L, const std::vector<int>& iorbs) {
-  double error = 0.0;
-  std::vector<double> ws(L.nmo);
-  for (int i = 0; i < L.nmo; ++i) {
-    int n = 0;
-    double sum = 0.0;
-    for (int j = 0; j < L.nc[i]; ++j) {
-      const int atom = L.ic[L.nnc[i] + j];
-      for (int mm = 0; mm < iorbs[atom - 1]; ++mm, ++n) sum += L.c[L.ncmo[i] + n] * L.c[L.ncmo[i] + n];
-    }
-    error += std::fabs(1.0 - sum);
-    ws[i] = sum;
-  }
-  for (int i = 0; i < L.nmo; ++i) {
-    const double sc = 1.0 / std::sqrt(ws[i]);
-    int n = 0;
-    for (int j = 0; j < L.nc[i]; ++j) {
-      const int atom = L.ic[L.nnc[i] + j];
-      for (int mm = 0; mm < iorbs[atom - 1]; ++mm, ++n) L.c[L.ncmo[i] + n] *= sc;
-    }
-  }
-  return error <= 0.1;
-}
-
-}  // namespace
+// The LMOs, P, F and the extrapolation arrays stay on the device for the
+// whole SCF; each iteration sends the host the scalars the loop's control
+// needs, tidy's per-LMO counts and diagg1's list (diagg2 orders its rotations
+// on the host), and nothing else.
 
 extern "C" {
 
@@ -457,56 +354,55 @@ int mhip_mozyme_scf(mhip_ctx* c, mhip_lmo_set* lmo, const mhip_mozyme_opts* o, m
     MHIP_REQUIRE(lmo->numat == numat && lmo->norbs == norbs && lmo->mpack == mp,
                  "mhip_mozyme_scf: LMO set and system differ");
     const auto t0 = std::chrono::steady_clock::now();
-    std::vector<int> iorbs(numat), nijbo((size_t)numat * numat);
+    hipStream_t s = mz_stream_of(c);
+    MHIP_CHECK(hipStreamSynchronize(c->stream));  // the system upload, on the context's stream
+    std::vector<int> iorbs(numat);
     for (int a = 0; a < numat; ++a) iorbs[a] = lmo->nlast[a] - lmo->nfirst[a] + 1;
-    std::copy(lmo->nijbo, lmo->nijbo + (size_t)numat * numat, nijbo.begin());
-    // working copies of the LMOs
-    LmoStore occ, vir;
-    occ.nmo = lmo->nocc;
-    vir.nmo = lmo->nvir;
-    occ.nc.assign(lmo->ncf, lmo->ncf + lmo->nocc);
-    occ.nnc.assign(lmo->nncf, lmo->nncf + lmo->nocc);
-    occ.ncmo.assign(lmo->ncocc, lmo->ncocc + lmo->nocc);
-    occ.ic.assign(lmo->icocc, lmo->icocc + lmo->icocc_dim);
-    occ.c.assign(lmo->cocc, lmo->cocc + lmo->cocc_dim);
-    vir.nc.assign(lmo->nce, lmo->nce + lmo->nvir);
-    vir.nnc.assign(lmo->nnce, lmo->nnce + lmo->nvir);
-    vir.ncmo.assign(lmo->ncvir, lmo->ncvir + lmo->nvir);
-    vir.ic.assign(lmo->icvir, lmo->icvir + lmo->icvir_dim);
-    vir.c.assign(lmo->cvir, lmo->cvir + lmo->cvir_dim);
-    mhip_lmo_set L = *lmo;
-    auto bind = [&] {
-      L.ncf = occ.nc.data();
-      L.nncf = occ.nnc.data();
-      L.ncocc = occ.ncmo.data();
-      L.icocc = occ.ic.data();
-      L.cocc = occ.c.data();
-      L.nce = vir.nc.data();
-      L.nnce = vir.nnc.data();
-      L.ncvir = vir.ncmo.data();
-      L.icvir = vir.ic.data();
-      L.cvir = vir.c.data();
-    };
-    bind();
+    // the host side of the resident set: sizes and the LMO offsets tidy moves
+    std::vector<int> nncf(lmo->nncf, lmo->nncf + lmo->nocc), ncocc(lmo->ncocc, lmo->ncocc + lmo->nocc),
+        nnce(lmo->nnce, lmo->nnce + lmo->nvir), ncvir(lmo->ncvir, lmo->ncvir + lmo->nvir);
+    mhip_lmo_set meta = *lmo;
+    meta.nncf = nncf.data();
+    meta.ncocc = ncocc.data();
+    meta.nnce = nnce.data();
+    meta.ncvir = ncvir.data();
+    mz_resident_begin(s, lmo);
+    struct EndResident {  // the LMOs go back to the caller on every exit
+      hipStream_t s;
+      mhip_lmo_set* L;
+      bool done = false;
+      ~EndResident() {
+        if (!done) try { mz_resident_end(s, L); } catch (...) {}
+      }
+    } end_resident{s, lmo};
+    // device arrays of the loop
+    DevBuf P, F, pold, p1, p2, p3, idiag_d;
+    P.ensure(sizeof(double) * mp);
+    F.ensure(sizeof(double) * mp);
+    pold.ensure(sizeof(double) * mp);
+    for (DevBuf* b : {&p1, &p2, &p3}) b->ensure(sizeof(double) * norbs);
+    MHIP_CHECK(hipMemsetAsync(pold.ptr, 0, sizeof(double) * mp, s));
+    for (DevBuf* b : {&p1, &p2, &p3}) MHIP_CHECK(hipMemsetAsync(b->ptr, 0, sizeof(double) * norbs, s));
+    struct FreeAll {
+      DevBuf* bufs[7];
+      ~FreeAll() {
+        for (DevBuf* b : bufs) b->release();
+      }
+    } free_all{{&P, &F, &pold, &p1, &p2, &p3, &idiag_d}};
     // idiag: 1-based position of P(i, i) in the sparse storage
-    std::vector<int> idiag(norbs), diag(numat);
+    std::vector<int> idiag(norbs);
     for (int a = 0, l = 0; a < numat; ++a) {
-      int j = nijbo[a + (size_t)a * numat];
+      int j = lmo->nijbo[a + (size_t)a * numat];
       for (int k = 1; k <= iorbs[a]; ++k) {
         j += k;
         idiag[l++] = j;
       }
     }
-    // stored off-diagonal blocks (offset, size), in eimp's order (i, then j < i)
-    std::vector<std::pair<int, int>> offdiag;
-    for (int i = 0; i < numat; ++i)
-      for (int j = 0; j < i; ++j) {
-        const int k = nijbo[i + (size_t)j * numat];
-        if (k >= 0) offdiag.emplace_back(k, iorbs[i] * iorbs[j]);
-      }
-    std::vector<double> h((size_t)mp), P((size_t)mp), F((size_t)mp), pold((size_t)mp, 0.0), p1(norbs, 0.0),
-        p2(norbs, 0.0), p3(norbs, 0.0);
-    MHIP_CHECK(hipMemcpy(h.data(), z->hs.ptr, sizeof(double) * mp, hipMemcpyDeviceToHost));
+    idiag_d.ensure(sizeof(int) * norbs);
+    MHIP_CHECK(hipMemcpyAsync(idiag_d.ptr, idiag.data(), sizeof(int) * norbs, hipMemcpyHostToDevice, s));
+    int nblk = 0;
+    mz_eimp_blocks(s, numat, lmo->nijbo, iorbs.data(), &nblk);
+    const double* H = z->hs.as<double>();
     std::vector<double> eigs(std::max(lmo->nocc, 1), 0.0), eigv(std::max(lmo->nvir, 1), 0.0);
     const int fmo_dim = o->fmo_dim > 0 ? o->fmo_dim : (int)std::min<long long>(
         (long long)lmo->nocc * lmo->nvir, 50LL * std::max(lmo->nocc, lmo->nvir) * 64);
@@ -523,51 +419,43 @@ int mhip_mozyme_scf(mhip_ctx* c, mhip_lmo_set* lmo, const mhip_mozyme_opts* o, m
     const double selcon = o->scfcrt > 0 ? o->scfcrt : 1.0e-2;  // scfcri (src/MOZYME/scfcri.F90)
     const int itrmax = o->itrmax > 0 ? o->itrmax : 2000;
     const double fpc9 = mhip_ev_to_kcal();
-    auto check_rc = [](int rc) {
-      if (rc != 0) throw HipError{get_error()};
-    };
-    auto energy = [&](double& ee) {
-      check_rc(mhip_mozyme_helecz(c, numat, lmo->nfirst, lmo->nlast, nijbo.data(), mp, P.data(), h.data(),
-                                  F.data(), &ee));
+    double ee = 0.0;
+    auto energy = [&]() {
+      ee = mz_helecz_dev(s, numat, P.as<double>(), H, F.as<double>());
       return (ee + o->enuclr) * fpc9 + o->atheat;
     };
     // iter_for_MOZYME.F90:433-478: density from the LMOs, Fock, energy
-    check_rc(mhip_mozyme_density(c, &L, 0, nullptr, P.data()));
-    check_rc(mhip_mozyme_fock2z(c, P.data(), F.data()));
-    double ee = 0.0;
-    double escf = energy(ee);
+    mz_density_dev(s, mp, lmo->nocc, 0, nullptr, P.as<double>());
+    mz_fock2z_on(c, s, P.as<double>(), F.as<double>());
+    double escf = energy();
     double eold = 0.0, ediff = 0.0, shift = 0.0;
     int niter = 0, idiagg = 0, lstart = 0;
     bool three = true, scf1 = false, okscf = false;
     for (;;) {
-      MHIP_REQUIRE(tidy(occ, iorbs, numat, norbs, st2.thresh) && tidy(vir, iorbs, numat, norbs, st2.thresh),
+      MHIP_REQUIRE(mz_tidy_dev(s, numat, norbs, true, lmo->nocc, nncf.data(), ncocc.data(), st2.thresh) &&
+                       mz_tidy_dev(s, numat, norbs, false, lmo->nvir, nnce.data(), ncvir.data(), st2.thresh),
                    "mhip_mozyme_scf: LMO storage too small for the next step (enlarge icocc/cocc/icvir/cvir)");
-      MHIP_REQUIRE(check(occ, iorbs) && check(vir, iorbs), "mhip_mozyme_scf: LMO normalization error > 0.1");
+      MHIP_REQUIRE(mz_check_dev(s, true, lmo->nocc) <= 0.1 && mz_check_dev(s, false, lmo->nvir) <= 0.1,
+                   "mhip_mozyme_scf: LMO normalization error > 0.1");
       // eimp (src/MOZYME/eimp.F90:16-45): the s-s slot of each stored
       // off-diagonal block of P holds the sum of squares of that F block
-      for (const auto& b : offdiag) {
-        double sum = 0.0;
-        for (int m = 0; m < b.second; ++m) sum += F[b.first + m] * F[b.first + m];
-        P[b.first] = sum;
-      }
+      mz_eimp_dev(s, nblk, F.as<double>(), P.as<double>());
       // diagg (src/MOZYME/diagg.F90:16-69)
       int nij = 0;
-      check_rc(mhip_mozyme_diagg1(c, &L, F.data(), P.data(), eigs.data(), eigv.data(), idiagg, &st1, fmo_dim,
-                                  fmo.data(), ifmo.data(), &nij));
+      mz_diagg1_dev(s, &meta, F.as<double>(), P.as<double>(), eigs.data(), eigv.data(), idiagg, &st1, fmo_dim,
+                    fmo.data(), ifmo.data(), &nij);
       st2.tiny = st1.tiny;
       st2.shift = shift;
-      check_rc(mhip_mozyme_diagg2(c, &L, eigs.data(), eigv.data(), fmo.data(), ifmo.data(), nij, idiagg, &st2));
+      mz_diagg2_dev(s, &meta, eigs.data(), eigv.data(), fmo.data(), ifmo.data(), nij, idiagg, &st2);
       st1.tiny = st2.tiny;
-      check_rc(mhip_mozyme_density(c, &L, 0, nullptr, P.data()));
+      mz_density_dev(s, mp, lmo->nocc, 0, nullptr, P.as<double>());
       ++idiagg;
       ++niter;
-      if (three) {
-        double pmax = 0.0;
-        check_rc(mhip_mozyme_cnvgz(c, norbs, mp, P.data(), pold.data(), p1.data(), p2.data(), p3.data(), niter,
-                                   idiag.data(), 1, &pmax));
-      }
-      check_rc(mhip_mozyme_fock2z(c, P.data(), F.data()));
-      escf = energy(ee);
+      if (three)
+        mz_cnvgz_dev(s, norbs, mp, P.as<double>(), pold.as<double>(), p1.as<double>(), p2.as<double>(),
+                     p3.as<double>(), niter, idiag_d.as<int>(), 1);
+      mz_fock2z_on(c, s, P.as<double>(), F.as<double>());
+      escf = energy();
       ediff = escf - eold;
       eold = escf;
       // isitsc (src/MOZYME/isitsc.F90:16-116), single point (emin = 0)
@@ -595,19 +483,12 @@ int mhip_mozyme_scf(mhip_ctx* c, mhip_lmo_set* lmo, const mhip_mozyme_opts* o, m
         lstart = niter;
       }
     }
-    // LMOs back to the caller's arrays (same capacities)
-    std::copy(occ.nc.begin(), occ.nc.end(), lmo->ncf);
-    std::copy(occ.ic.begin(), occ.ic.end(), lmo->icocc);
-    std::copy(occ.c.begin(), occ.c.end(), lmo->cocc);
-    std::copy(vir.nc.begin(), vir.nc.end(), lmo->nce);
-    std::copy(vir.ic.begin(), vir.ic.end(), lmo->icvir);
-    std::copy(vir.c.begin(), vir.c.end(), lmo->cvir);
-    std::copy(occ.nnc.begin(), occ.nnc.end(), const_cast<int*>(lmo->nncf));
-    std::copy(occ.ncmo.begin(), occ.ncmo.end(), const_cast<int*>(lmo->ncocc));
-    std::copy(vir.nnc.begin(), vir.nnc.end(), const_cast<int*>(lmo->nnce));
-    std::copy(vir.ncmo.begin(), vir.ncmo.end(), const_cast<int*>(lmo->ncvir));
-    if (p_out) std::copy(P.begin(), P.end(), p_out);
-    if (f_out) std::copy(F.begin(), F.end(), f_out);
+    // LMOs (with the offsets tidy moved) back to the caller's arrays
+    end_resident.done = true;
+    mz_resident_end(s, lmo);
+    if (p_out) MHIP_CHECK(hipMemcpyAsync(p_out, P.ptr, sizeof(double) * mp, hipMemcpyDeviceToHost, s));
+    if (f_out) MHIP_CHECK(hipMemcpyAsync(f_out, F.ptr, sizeof(double) * mp, hipMemcpyDeviceToHost, s));
+    MHIP_CHECK(hipStreamSynchronize(s));
     res->escf = escf;
     res->ee = ee;
     res->niter = niter;

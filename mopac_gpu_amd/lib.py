@@ -125,6 +125,9 @@ SIGNATURES = {
     "mhip_mozyme_stats": (c_int, [c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_ll), ctypes.POINTER(c_ll),
                                   ctypes.POINTER(c_ll), ctypes.POINTER(c_double), ctypes.POINTER(c_ll)]),
     "mhip_mozyme_diagg2_one_workgroup": (c_int, [c_int]),
+    "mhip_mozyme_tidy": (c_int, [c_void_p, c_void_p, c_int, c_double]),
+    "mhip_mozyme_check": (c_int, [c_void_p, c_void_p, c_int, ctypes.POINTER(c_double)]),
+    "mhip_mozyme_eimp": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p]),
     "mhip_mozyme_diagg2_wide_rounds": (c_int, [c_int]),
     "mhip_mozyme_diagg2_list_cap": (c_int, [c_int]),
     "mhip_last_sweeps": (c_int, [c_void_p]),

@@ -664,10 +664,20 @@ class GpuMozyme:
         self.st1.tiny = self.st2.tiny
         return self.st2.sumb
 
+    def tidy(self, occupied, thresh=1e-13):
+        self.L.check(self.L.load().mhip_mozyme_tidy(self.ctx.h, self.ctypes.byref(self.c), int(occupied), thresh))
+
+    def check(self, occupied):
+        e = self.ctypes.c_double()
+        self.L.check(self.L.load().mhip_mozyme_check(self.ctx.h, self.ctypes.byref(self.c), int(occupied),
+                                                     self.ctypes.byref(e)))
+        return e.value
+
     def get(self):
         s = self.s.copy()
         a = self.a
         s.ncf, s.icocc, s.cocc = a["ncf"].copy(), a["icocc"].copy(), a["cocc"].copy()
         s.nce, s.icvir, s.cvir = a["nce"].copy(), a["icvir"].copy(), a["cvir"].copy()
+        s.nncf, s.ncocc, s.nnce, s.ncvir = a["nncf"].copy(), a["ncocc"].copy(), a["nnce"].copy(), a["ncvir"].copy()
         s.eigs, s.eigv = self.eigs.copy(), self.eigv.copy()
         return s

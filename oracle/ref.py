@@ -366,6 +366,13 @@ def mozyme_check(numat, norbs, iorbs, nc, nnc, ncmo, ic, c):
     nl = np.cumsum(iorbs).astype(np.int32)
     keep = _mz_setup_atoms(numat, norbs, 1, nl - iorbs + 1, nl, np.ones(numat), np.zeros((numat, numat)))
     nv = len(nc)
+    # check stops the job through mopend (not in this library) above 0.1
+    err = 0.0
+    for k in range(nv):
+        n = int(sum(iorbs[a - 1] for a in ic[nnc[k]:nnc[k] + nc[k]]))
+        err += abs(1.0 - float(np.sum(np.asarray(c[ncmo[k]:ncmo[k] + n]) ** 2)))
+    if err > 0.1:
+        raise ValueError(f"check: normalization error {err:.3g} > 0.1 would stop the reference")
     pad = lambda x: np.concatenate([np.asarray(x, dtype=np.int32), [0]]).astype(np.int32)  # noqa: E731
     a = [pad(nnc), pad(nc), np.array(ic, dtype=np.int32), pad(ncmo)]
     cc = np.array(c, dtype=np.float64)

@@ -291,3 +291,92 @@ def test_order_free_sweep_converges_to_the_reference_order_heat(gpu_ctx, nres):
     assert abs(r1["escf"] - r0["escf"]) < 1e-3
     assert abs(r1["niter"] - r0["niter"]) <= 5
     assert lev1 < lev0
+
+
+def _lmo_extents(nc, nn, no, ic, c, iorbs):
+    """Each LMO's atoms and coefficients (the parts tidy defines)."""
+    atoms, coefs = [], []
+    for k in range(len(nc)):
+        a = ic[nn[k]:nn[k] + nc[k]]
+        atoms.append(a.copy())
+        coefs.append(c[no[k]:no[k] + int(sum(iorbs[x - 1] for x in a))].copy())
+    return atoms, coefs
+
+
+@pytest.fixture(scope="module")
+def tidy_lmos(gpu_ctx):
+    """LMOs of a Gly12 helix at MOZYME's cutoffs after four iterations of the
+    device MOZYME SCF from the Lewis start (grown LMOs with small tails, in
+    the reference's capacities), every LMO's norm moved off 1 by ~1e-5 (so
+    check has something to restore while staying below the reference's 0.1
+    stop)."""
+    nat, xyz = geometry.glycine_helix(12, 3)
+    ms = mozyme.MozymeSystem.from_geometry(nat, xyz)
+    ms.upload(gpu_ctx)
+    f0 = ms.fock2z(gpu_ctx, ms.diagonal_density())
+    L = ms.scf(gpu_ctx, mozyme.lewis_lmos(ms, f0), itrmax=4)["lmos"]
+    rng = np.random.default_rng(5)
+    iorbs = L.iorbs
+    for nc, nn, no, ic, c in ((L.ncf, L.nncf, L.ncocc, L.icocc, L.cocc), (L.nce, L.nnce, L.ncvir, L.icvir, L.cvir)):
+        for k in range(len(nc)):
+            n = int(sum(iorbs[a - 1] for a in ic[nn[k]:nn[k] + nc[k]]))
+            c[no[k]:no[k] + n] *= 1.0 + 1e-5 * rng.standard_normal()
+    L.eigs, L.eigv = np.zeros(L.nocc), np.zeros(L.nvir)
+    return ms, L
+
+
+@pytest.mark.parametrize("occupied", [1, 0])
+def test_tidy_and_check_match_reference(gpu_ctx, tidy_lmos, occupied):
+    """mhip_mozyme_tidy and mhip_mozyme_check on the device against the
+    reference's tidy.F90 and check.F90 (oracle/_ref) on the same LMO set:
+    identical counts, offsets, atom lists and coefficients after tidy; after
+    check the same normalization error and coefficients to 1e-15."""
+    ms, L = tidy_lmos
+    iorbs = L.iorbs
+    if occupied:
+        args = (L.ncf, L.nncf, L.ncocc, L.icocc, L.cocc)
+    else:
+        args = (L.nce, L.nnce, L.ncvir, L.icvir, L.cvir)
+    thresh = 1e-6  # tidy's THRESH: drops the grown LMOs' small tails
+    nc, nn, no, ic, c, ln, mn, err = REF.mozyme_tidy(L.numat, L.norbs, iorbs, *args, thresh,
+                                                       mode=1 if occupied else 2)
+    assert err == 0
+    G = mozyme.GpuMozyme(gpu_ctx, L)
+    G.tidy(occupied, thresh)
+    g = G.get()
+    gnc, gnn, gno, gic, gc = ((g.ncf, g.nncf, g.ncocc, g.icocc, g.cocc) if occupied else
+                              (g.nce, g.nnce, g.ncvir, g.icvir, g.cvir))
+    print(f"tidy occupied={occupied}: {int(np.sum(args[0]) - np.sum(nc))} atoms dropped", flush=True)
+    assert np.sum(nc) < np.sum(args[0])
+    assert np.array_equal(gnc, nc) and np.array_equal(gnn, nn) and np.array_equal(gno, no)
+    ra, rc = _lmo_extents(nc, nn, no, ic, c, iorbs)
+    ga, gcf = _lmo_extents(gnc, gnn, gno, gic, gc, iorbs)
+    for k in range(len(nc)):
+        assert np.array_equal(ra[k], ga[k])
+        assert np.array_equal(rc[k], gcf[k])
+    # check on the tidied set
+    cr, ws = REF.mozyme_check(L.numat, L.norbs, iorbs, nc, nn, no, ic, c)
+    e = G.check(occupied)
+    assert e == pytest.approx(np.sum(np.abs(1.0 - ws)), rel=1e-12)
+    g = G.get()
+    gc = g.cocc if occupied else g.cvir
+    _, rc2 = _lmo_extents(nc, nn, no, ic, cr, iorbs)
+    _, gc2 = _lmo_extents(nc, nn, no, ic, gc, iorbs)
+    for k in range(len(nc)):
+        assert np.max(np.abs(rc2[k] - gc2[k]), initial=0.0) < 1e-15
+
+
+def test_eimp_matches_reference(gpu_ctx, tidy_lmos):
+    from mopac_gpu_amd import lib as L_
+    ms, L = tidy_lmos
+    rng = np.random.default_rng(9)
+    f = rng.standard_normal(ms.mpack)
+    p = rng.standard_normal(ms.mpack)
+    pr = REF.mozyme_eimp(ms.numat, ms.norbs, ms.nfirst, ms.nlast, ms.nijbo, ms.mpack, f, p)
+    pg = p.copy()
+    L_.check(L_.load().mhip_mozyme_eimp(gpu_ctx.h, ms.numat, L_.ptr(ms.nfirst.astype(np.int32)),
+                                        L_.ptr(ms.nlast.astype(np.int32)),
+                                        L_.ptr(np.asfortranarray(ms.nijbo.astype(np.int32))), ms.mpack,
+                                        L_.ptr(f), L_.ptr(pg)))
+    assert not np.array_equal(pr, p)
+    assert np.array_equal(pg, pr)
