@@ -295,7 +295,7 @@ int mhip_mozyme_eimp(mhip_ctx* ctx, int numat, const int* nfirst, const int* nla
 int mhip_mozyme_stats(int enable, double* ms, long long* rotations, long long* levels, long long* launches,
                       double* bytes, long long* rounds);
 /* No counterpart in the reference.  Selects how mhip_mozyme_diagg2 runs the
- * rotation levels: 1 on the LDS rotation kernels, a wave per
+ * rotation levels: 1 (default) on the LDS rotation kernels, a wave per
  * rotation with its atom maps and lists in LDS: runs of narrow levels in one
  * launch of a single workgroup that meets at a barrier between levels, and
  * each level wider than mhip_mozyme_diagg2_wide_rounds rounds of that
@@ -303,12 +303,11 @@ int mhip_mozyme_stats(int enable, double* ms, long long* rotations, long long* l
  * numat and norbs are below 65535 and the maps fit in LDS (two LMOs with
  * more than mhip_mozyme_diagg2_list_cap atoms keep their lists in global
  * memory).  0, or when they do not apply: one
- * launch per level of the kernel with global maps (the default, the fastest
- * measured).  All give identical results.  enable = -1 only reads; returns
- * the previous setting. */
+ * launch per level of the kernel with global maps.  All give identical
+ * results.  enable = -1 only reads; returns the previous setting. */
 int mhip_mozyme_diagg2_one_workgroup(int enable);
 /* No counterpart in the reference.  The width, in rounds of the one-workgroup
- * kernel, above which a level runs across the chip (default 3); rounds = -1
+ * kernel, above which a level runs across the chip (default 1); rounds = -1
  * only reads; returns the previous value. */
 int mhip_mozyme_diagg2_wide_rounds(int rounds);
 /* No counterpart in the reference.  The LDS rotation kernels copy the two

@@ -754,75 +754,85 @@ __device__ __forceinline__ int wave_scan_incl(int v, int lane) {
 // (the hardware keeps a wave's vector memory and LDS operations in order)
 __device__ __forceinline__ void wave_phase() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
+// A rotation of the list with everything about it that no rotation of the
+// call changes (its LMOs, their offsets and storage limits, the numerator
+// and denominator of its angle), laid out by the host in level order so a
+// rotation starts with one load.  Only the atom counts ncf[j] / nce[i], the
+// lists and the coefficients are read as the earlier rotations left them.
+struct D2Rot {
+  double c, d;                  // fmo * damp, eigs[j] - eigv[i] - shift
+  int ij, i, j;                 // list entry (0-based), virtual and occupied LMO
+  int nnj, nni, jlr, ilr;       // atom and coefficient offsets of j and i
+  int jur, iur, jncf, incv;     // their coefficient and atom limits (exclusive)
+  int pad;
+};
+
 // Per-wave LDS of the rotation kernels: the atom maps (iused, latoms) and
-// copies of the two LMOs' atom lists (0-based atom numbers) when the two
-// hold at most D2W_LCAP atoms together, which bounds both lists after the
-// rotation (each ends as a subset of the union of the two); larger pairs
-// read their lists from global memory.
+// copies of the two LMOs' atom lists (0-based atom numbers) and orbital
+// counts when the two hold at most D2W_LCAP atoms together, which bounds both
+// lists after the rotation (each ends as a subset of the union of the two);
+// larger pairs read their lists from global memory.  Map entries are set for
+// the atoms of the two LMOs when a rotation starts, so no clearing is needed.
 constexpr int D2W_LCAP = 1024;
 struct D2Wave {
   unsigned short* iused;   // numat
   unsigned char* latoms;   // numat
   unsigned short* lj;      // D2W_LCAP
   unsigned short* li;      // D2W_LCAP
+  unsigned char* oj;       // D2W_LCAP
+  unsigned char* oi;       // D2W_LCAP
 };
 
-// bytes of LDS a workgroup of nw waves uses: iorbs (numat bytes, shared),
-// then per wave 3 numat + 4 D2W_LCAP
-__host__ __device__ inline size_t d2w_lds_bytes(int numat, int nw) {
-  return (((size_t)numat + 15) & ~(size_t)15) +
-         (size_t)nw * ((((size_t)3 * numat + 15) & ~(size_t)15) + 4 * (size_t)D2W_LCAP);
+__host__ __device__ inline size_t d2w_wave_bytes(int numat) {
+  return (((size_t)3 * numat + 15) & ~(size_t)15) + 6 * (size_t)D2W_LCAP;
 }
+// LDS bytes of a workgroup of nw waves
+__host__ __device__ inline size_t d2w_lds_bytes(int numat, int nw) { return (size_t)nw * d2w_wave_bytes(numat); }
 
-__device__ __forceinline__ D2Wave d2w_carve(unsigned char* lds, int numat, int nw, int w) {
-  unsigned char* base = lds + (((size_t)numat + 15) & ~(size_t)15) +
-                        (size_t)w * ((((size_t)3 * numat + 15) & ~(size_t)15) + 4 * (size_t)D2W_LCAP);
+__device__ __forceinline__ D2Wave d2w_carve(unsigned char* lds, int numat, int w) {
+  unsigned char* base = lds + (size_t)w * d2w_wave_bytes(numat);
   D2Wave v;
   v.iused = reinterpret_cast<unsigned short*>(base);
   v.latoms = base + 2 * (size_t)numat;
   unsigned char* lists = base + (((size_t)3 * numat + 15) & ~(size_t)15);
   v.lj = reinterpret_cast<unsigned short*>(lists);
   v.li = v.lj + D2W_LCAP;
+  v.oj = lists + 4 * (size_t)D2W_LCAP;
+  v.oi = v.oj + D2W_LCAP;
   return v;
 }
 
-// One rotation by one wave.  Atom lists and orbital counts come from LDS,
-// so the dependent global reads are the list entry, the LMO offsets, the
-// atom lists and the coefficients; everything the wave re-reads inside the
-// rotation is in LDS or registers, except the undo log on a rejection (and
-// the lists of an LMO pair too large for the LDS copies).
-__device__ void d2w_rotation(const LmoDev& L, const unsigned char* __restrict__ iorbs_s,
-                             const double* __restrict__ eigs, const double* __restrict__ eigv,
-                             const double* __restrict__ fmo, const int* __restrict__ ifmo, const D2Params& P,
-                             const int ij, const D2Wave& M, double* __restrict__ storej, double* __restrict__ storei,
-                             double* __restrict__ sumb_out, int* __restrict__ nrej_out, int* __restrict__ status,
-                             const int lane) {
+// One rotation by one wave.  The dependent global reads are the record, the
+// two atom counts, the atom lists, their orbital counts and the
+// coefficients; everything the wave re-reads inside the rotation is in LDS
+// or registers, except the undo log on a rejection (and the lists of an LMO
+// pair too large for the LDS copies).
+__device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P, const D2Wave& M,
+                             double* __restrict__ storej, double* __restrict__ storei, double* __restrict__ sumb_out,
+                             int* __restrict__ nrej_out, const int lane) {
   const unsigned long long below = (1ull << lane) - 1ull;
   double sumb = 0.0;
   int nrej = 0;
-  const int i = ifmo[2 * ij] - 1, j = ifmo[2 * ij + 1] - 1;
-  const double c = fmo[ij] * P.cnst;
-  const double d = eigs[j] - eigv[i] - P.shift;
+  const int i = R.i, j = R.j;
+  const double c = R.c, d = R.d;
+  const int nnj = R.nnj, nni = R.nni, jlr = R.jlr, ilr = R.ilr;
+  const int jur = R.jur, iur = R.iur, jncf = R.jncf, incv = R.incv;
   const int ncfj = L.ncf[j], ncei = L.nce[i];
-  const int nnj = L.nncf[j], nni = L.nnce[i];
-  const int jlr = L.ncocc[j], ilr = L.ncvir[i];
-  int jur, jncf, iur, incv;
-  if (j != L.nocc - 1) { jur = L.ncocc[j + 1]; jncf = L.nncf[j + 1]; }
-  else { jur = L.cocc_dim; jncf = L.icocc_dim; }
-  jur = min(jlr + L.norbs, jur);
-  if (i != L.nvir - 1) { iur = L.ncvir[i + 1]; incv = L.nnce[i + 1]; }
-  else { iur = L.cvir_dim; incv = L.icvir_dim; }
-  iur = min(ilr + L.norbs, iur);
-  // atom lists into LDS when they fit (every list entry the rotation reads
-  // lies below ncfj + ncei), undo log of the coefficients the LMOs hold now
+  // atom lists (and orbital counts) into LDS when they fit (every list entry
+  // the rotation reads lies below ncfj + ncei), undo log of the coefficients
+  // the LMOs hold now
   const bool in_lds = ncfj + ncei <= P.lcap;
   const int* __restrict__ gj = L.icocc + nnj;
   const int* __restrict__ gi = L.icvir + nni;
   auto atom_j = [&](int k) { return in_lds ? (int)M.lj[k] : gj[k] - 1; };
   auto atom_i = [&](int k) { return in_lds ? (int)M.li[k] : gi[k] - 1; };
+  auto orbs_j = [&](int k, int a) { return in_lds ? (int)M.oj[k] : L.iorbs[a]; };
+  auto orbs_i = [&](int k, int a) { return in_lds ? (int)M.oi[k] : L.iorbs[a]; };
   if (in_lds) {
     for (int k = lane; k < ncfj; k += 64) M.lj[k] = (unsigned short)(gj[k] - 1);
     for (int k = lane; k < ncei; k += 64) M.li[k] = (unsigned short)(gi[k] - 1);
+    for (int k = lane; k < ncfj; k += 64) M.oj[k] = (unsigned char)L.iorbs[M.lj[k]];
+    for (int k = lane; k < ncei; k += 64) M.oi[k] = (unsigned char)L.iorbs[M.li[k]];
   }
   const int bj = min(4 * ncfj, jur - jlr), bi = min(4 * ncei, iur - ilr);
   for (int k = lane; k < bj; k += 64) storej[k] = L.cocc[jlr + k];
@@ -833,12 +843,17 @@ __device__ void d2w_rotation(const LmoDev& L, const unsigned char* __restrict__ 
   for (;;) {
     const double beta = -copysign(sqrt(1.0 - alpha * alpha), c);
     sumb += fabs(beta);
+    // the maps for the atoms of the two LMOs: none of i's atoms in j yet,
+    // none of j's in i
+    for (int k = lane; k < ncei; k += 64) M.iused[atom_i(k)] = D2W_NONE;
+    for (int k = lane; k < ncfj; k += 64) M.latoms[atom_j(k)] = 0;
+    wave_phase();
     // offsets of LMO j's atoms
     int mlf = 0;
     for (int c0 = 0; c0 < ncfj; c0 += 64) {
       const int lf = c0 + lane;
       const int a = (lf < ncfj) ? atom_j(lf) : -1;
-      const int no = (a >= 0) ? (int)iorbs_s[a] : 0;
+      const int no = (a >= 0) ? orbs_j(lf, a) : 0;
       const int incl = wave_scan_incl(no, lane);
       if (a >= 0) M.iused[a] = (unsigned short)(mlf + incl - no);
       mlf += __shfl(incl, 63, 64);
@@ -853,7 +868,7 @@ __device__ void d2w_rotation(const LmoDev& L, const unsigned char* __restrict__ 
       int mie = -1, no = 0;
       if (le < ncei) {
         mie = atom_i(le);
-        no = iorbs_s[mie];
+        no = orbs_i(le, mie);
       }
       const int incl = wave_scan_incl(no, lane);
       const int mle = mle_base + incl - no;
@@ -889,7 +904,10 @@ __device__ void d2w_rotation(const LmoDev& L, const unsigned char* __restrict__ 
         fail = __any(grow && (slot >= jncf || jlr + obase + no > jur));
         if (!fail && grow) {
           L.icocc[slot] = mie + 1;
-          if (in_lds) M.lj[ncf_run + rank] = (unsigned short)mie;
+          if (in_lds) {
+            M.lj[ncf_run + rank] = (unsigned short)mie;
+            M.oj[ncf_run + rank] = (unsigned char)no;
+          }
           M.iused[mie] = (unsigned short)obase;
           for (int x = 0; x < no; ++x) {
             storej[obase + x] = L.cocc[jlr + obase + x];
@@ -917,7 +935,7 @@ __device__ void d2w_rotation(const LmoDev& L, const unsigned char* __restrict__ 
         double cj[4] = {0.0, 0.0, 0.0, 0.0};
         if (lf < ncf_run) {
           ii = atom_j(lf);
-          no = iorbs_s[ii];
+          no = orbs_j(lf, ii);
           if (!M.latoms[ii]) {
             u = M.iused[ii];
             double s2 = 0.0;
@@ -952,28 +970,20 @@ __device__ void d2w_rotation(const LmoDev& L, const unsigned char* __restrict__ 
           iext = max(iext, mle_run);
         }
       }
-      if (in_lds) wave_phase();
-      else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
       if (!fail) {
         if (lane == 0) L.nce[i] = nce_run;
-        // clear the map entries of the atoms now in the two LMOs
-        for (int le = lane; le < nce_run; le += 64) M.latoms[atom_i(le)] = 0;
-        for (int lf = lane; lf < ncf_run; lf += 64) M.iused[atom_j(lf)] = D2W_NONE;
         wave_phase();
         break;
       }
     }
     // rejection (label 1000): restore both LMOs from the undo log (written
-    // by other lanes: wait for the wave's stores first), clear the maps
+    // by other lanes: wait for the wave's stores first); the maps are set
+    // again at the retry
     ++nrej;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     const int rj = max(bj, jext), ri = max(bi, iext);
     for (int k = lane; k < rj; k += 64) L.cocc[jlr + k] = storej[k];
     for (int k = lane; k < ri; k += 64) L.cvir[ilr + k] = storei[k];
-    for (int k = lane; k < L.numat; k += 64) {
-      M.iused[k] = D2W_NONE;
-      M.latoms[k] = 0;
-    }
     if (lane == 0) {
       L.ncf[j] = ncfj;
       L.nce[i] = ncei;
@@ -986,67 +996,42 @@ __device__ void d2w_rotation(const LmoDev& L, const unsigned char* __restrict__ 
     break;
   }
   if (lane == 0) {
-    sumb_out[ij] = sumb;
-    nrej_out[ij] = nrej;
+    sumb_out[R.ij] = sumb;
+    nrej_out[R.ij] = nrej;
   }
 }
 
-__device__ void d2w_init_lds(const LmoDev& L, unsigned char* lds, int nw) {
-  for (int a = threadIdx.x; a < L.numat; a += blockDim.x) lds[a] = (unsigned char)L.iorbs[a];
-  for (int w = 0; w < nw; ++w) {
-    D2Wave M = d2w_carve(lds, L.numat, nw, w);
-    for (int a = threadIdx.x; a < L.numat; a += blockDim.x) {
-      M.iused[a] = D2W_NONE;
-      M.latoms[a] = 0;
-    }
-  }
-  __syncthreads();
-}
-
-// Every level of the list in one workgroup: the waves take a level's
-// rotations in turn and meet at the barrier before the next level.
-__global__ __launch_bounds__(64 * D2W_NW) void k_mz_diagg2_wg(LmoDev L, const double* __restrict__ eigs,
-                                                            const double* __restrict__ eigv,
-                                                            const double* __restrict__ fmo,
-                                                            const int* __restrict__ ifmo, D2Params P,
-                                                            const int* __restrict__ order,
+// Every level of a run in one workgroup: the waves take a level's rotations
+// in turn and meet at the barrier before the next level.
+__global__ __launch_bounds__(64 * D2W_NW) void k_mz_diagg2_wg(LmoDev L, const D2Rot* __restrict__ rec, D2Params P,
                                                             const int* __restrict__ level_start, int lev0,
                                                             int lev1, double* __restrict__ storej_all, int span_j,
                                                             double* __restrict__ storei_all, int span_i,
                                                             double* __restrict__ sumb_out,
-                                                            int* __restrict__ nrej_out, int* __restrict__ status) {
+                                                            int* __restrict__ nrej_out) {
   extern __shared__ __align__(16) unsigned char d2w_lds[];
   const int nw = blockDim.x >> 6;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  d2w_init_lds(L, d2w_lds, nw);
-  const D2Wave M = d2w_carve(d2w_lds, L.numat, nw, w);
+  const D2Wave M = d2w_carve(d2w_lds, L.numat, w);
   double* storej = storej_all + (size_t)w * span_j;
   double* storei = storei_all + (size_t)w * span_i;
   for (int l = lev0; l < lev1; ++l) {
     const int b1 = level_start[l + 1];
-    for (int b = level_start[l] + w; b < b1; b += nw)
-      d2w_rotation(L, d2w_lds, eigs, eigv, fmo, ifmo, P, order[b], M, storej, storei, sumb_out, nrej_out, status,
-                   lane);
+    for (int b = level_start[l] + w; b < b1; b += nw) d2w_rotation(L, rec[b], P, M, storej, storei, sumb_out, nrej_out, lane);
     __syncthreads();  // every wave reaches it once per level
   }
 }
 
 // One level across the chip: a one-wave workgroup per rotation.
-__global__ __launch_bounds__(64) void k_mz_diagg2_lvl(LmoDev L, const double* __restrict__ eigs,
-                                                      const double* __restrict__ eigv,
-                                                      const double* __restrict__ fmo,
-                                                      const int* __restrict__ ifmo, D2Params P,
-                                                      const int* __restrict__ order, int base,
-                                                      double* __restrict__ storej_all, int span_j,
+__global__ __launch_bounds__(64) void k_mz_diagg2_lvl(LmoDev L, const D2Rot* __restrict__ rec, D2Params P,
+                                                      int base, double* __restrict__ storej_all, int span_j,
                                                       double* __restrict__ storei_all, int span_i,
-                                                      double* __restrict__ sumb_out, int* __restrict__ nrej_out,
-                                                      int* __restrict__ status) {
+                                                      double* __restrict__ sumb_out, int* __restrict__ nrej_out) {
   extern __shared__ __align__(16) unsigned char d2w_lds[];
-  d2w_init_lds(L, d2w_lds, 1);
-  const D2Wave M = d2w_carve(d2w_lds, L.numat, 1, 0);
+  const D2Wave M = d2w_carve(d2w_lds, L.numat, 0);
   const int slot = blockIdx.x;
-  d2w_rotation(L, d2w_lds, eigs, eigv, fmo, ifmo, P, order[base + slot], M, storej_all + (size_t)slot * span_j,
-               storei_all + (size_t)slot * span_i, sumb_out, nrej_out, status, threadIdx.x);
+  d2w_rotation(L, rec[base + slot], P, M, storej_all + (size_t)slot * span_j, storei_all + (size_t)slot * span_i,
+               sumb_out, nrej_out, threadIdx.x);
 }
 
 // ---------------------------------------------------------------- helecz
@@ -1313,13 +1298,13 @@ struct MzStats {
   hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 MzStats g_mz_stats;
-// diagg2 on the global-map kernel one launch per level (k_mz_diagg2, the
-// default: fastest measured on the 400-residue helix, 1.20 s of rotations per
-// SCF against 1.25-1.56 s for the LDS kernels), or on the LDS rotation kernels
-// (runs of narrow levels in one workgroup, k_mz_diagg2_wg; wider levels
-// across the chip, k_mz_diagg2_lvl)
-bool g_mz_one_wg = false;
-int g_mz_wide_rounds = 3;
+// diagg2 on the LDS rotation kernels (the default: levels of more than one
+// round of the workgroup across the chip, k_mz_diagg2_lvl; runs of narrower
+// levels in one workgroup, k_mz_diagg2_wg) or on the global-map kernel one
+// launch per level (k_mz_diagg2).  Measured on the 400-residue helix in the
+// reference order: 0.86 s of rotations per SCF against 1.18 s.
+bool g_mz_one_wg = true;
+int g_mz_wide_rounds = 1;
 int g_mz_lcap = 1024;  // D2W_LCAP
 
 // Device copies and scratch of the MOZYME entry points, one set per device,
@@ -1333,7 +1318,7 @@ struct MzWork {
   DevBuf icocc2, cocc2, icvir2, cvir2;   // tidy's second copies (swapped with B's)
   DevBuf dkeep, dkc, dkq, dnn_new, dnq_new, derr, deoff, delen;
   DevBuf dp, dpart, dfao, daocc, dcounts, doffs, dnfmo, dold, dfmo, difmo, dvmax, dvsum, deigv, deigs, dorder,
-      dused, dlat, dsj, dsi, dsumb, dnrej, dh, df, dout, dpn, d1, d2, d3, di, dsc, dlevel, dstatus;
+      dused, dlat, dsj, dsi, dsumb, dnrej, dh, df, dout, dpn, d1, d2, d3, di, dsc, dlevel, drec;
 };
 MzWork& mz_work() {
   static std::map<int, MzWork> work;
@@ -1783,10 +1768,28 @@ void mz_diagg2_dev(hipStream_t s, const mhip_lmo_set* L, const double* eigs, con
       dlat.ensure((size_t)nslots * L->numat);
       MHIP_CHECK(hipMemsetAsync(dlat.ptr, 0, (size_t)nslots * L->numat, s));
     } else {
-      // level l (0-based) holds order[level[l] .. level[l + 1])
+      // level l (0-based) holds the records [level[l], level[l + 1])
       up(s, W.dlevel, start.data() + 1, sizeof(int) * (size_t)(nlev + 1));
-      W.dstatus.ensure(sizeof(int));
-      MHIP_CHECK(hipMemsetAsync(W.dstatus.ptr, 0, sizeof(int), s));
+      std::vector<D2Rot> recs(order.size());
+      for (size_t k = 0; k < order.size(); ++k) {
+        const int ij = order[k], i = ifmo[2 * ij] - 1, j = ifmo[2 * ij + 1] - 1;
+        D2Rot& r = recs[k];
+        r.c = fmo[ij] * P.cnst;
+        r.d = eigs[j] - eigv[i] - P.shift;
+        r.ij = ij;
+        r.i = i;
+        r.j = j;
+        r.nnj = L->nncf[j];
+        r.nni = L->nnce[i];
+        r.jlr = L->ncocc[j];
+        r.ilr = L->ncvir[i];
+        r.jur = std::min(r.jlr + L->norbs, (j + 1 < L->nocc) ? L->ncocc[j + 1] : L->cocc_dim);
+        r.jncf = (j + 1 < L->nocc) ? L->nncf[j + 1] : L->icocc_dim;
+        r.iur = std::min(r.ilr + L->norbs, (i + 1 < L->nvir) ? L->ncvir[i + 1] : L->cvir_dim);
+        r.incv = (i + 1 < L->nvir) ? L->nnce[i + 1] : L->icvir_dim;
+        r.pad = 0;
+      }
+      up(s, W.drec, recs.data(), sizeof(D2Rot) * recs.size());
     }
     dsj.ensure(sizeof(double) * (size_t)nslots * span_j);
     dsi.ensure(sizeof(double) * (size_t)nslots * span_i);
@@ -1831,10 +1834,9 @@ void mz_diagg2_dev(hipStream_t s, const mhip_lmo_set* L, const double* eigs, con
         if (width > wide) {
           for (int b = start[l]; b < start[l + 1]; b += nslots) {
             const int cnt = std::min(nslots, start[l + 1] - b);
-            hipLaunchKernelGGL(k_mz_diagg2_lvl, dim3(cnt), dim3(64), lds_lvl, s, B.dev, deigs.as<double>(),
-                               deigv.as<double>(), dfmo.as<double>(), difmo.as<int>(), P, dorder.as<int>(), b,
+            hipLaunchKernelGGL(k_mz_diagg2_lvl, dim3(cnt), dim3(64), lds_lvl, s, B.dev, W.drec.as<D2Rot>(), P, b,
                                dsj.as<double>(), span_j, dsi.as<double>(), span_i, dsumb.as<double>(),
-                               dnrej.as<int>(), W.dstatus.as<int>());
+                               dnrej.as<int>());
             MHIP_LAUNCH_CHECK();
             if (g_mz_stats.on) {
               ++g_mz_stats.launches;
@@ -1846,10 +1848,9 @@ void mz_diagg2_dev(hipStream_t s, const mhip_lmo_set* L, const double* eigs, con
         }
         int l1 = l;  // the run of narrow levels l .. l1 - 1
         while (l1 <= nlev && start[l1 + 1] - start[l1] <= wide) ++l1;
-        hipLaunchKernelGGL(k_mz_diagg2_wg, dim3(1), dim3(64 * nw), lds_wg, s, B.dev, deigs.as<double>(),
-                           deigv.as<double>(), dfmo.as<double>(), difmo.as<int>(), P, dorder.as<int>(),
+        hipLaunchKernelGGL(k_mz_diagg2_wg, dim3(1), dim3(64 * nw), lds_wg, s, B.dev, W.drec.as<D2Rot>(), P,
                            W.dlevel.as<int>(), l - 1, l1 - 1, dsj.as<double>(), span_j, dsi.as<double>(), span_i,
-                           dsumb.as<double>(), dnrej.as<int>(), W.dstatus.as<int>());
+                           dsumb.as<double>(), dnrej.as<int>());
         MHIP_LAUNCH_CHECK();
         if (g_mz_stats.on) {
           ++g_mz_stats.launches;
@@ -1878,10 +1879,7 @@ void mz_diagg2_dev(hipStream_t s, const mhip_lmo_set* L, const double* eigs, con
       down(s, sb.data(), dsumb, sizeof(double) * nij);
       down(s, nr.data(), dnrej, sizeof(int) * nij);
     }
-    int status = 0;
-    if (lds_ok) down(s, &status, W.dstatus, sizeof(int));
     MHIP_CHECK(hipStreamSynchronize(s));
-    MHIP_REQUIRE(status == 0, "mhip_mozyme_diagg2: rotation kernel status " + std::to_string(status));
     double sumb = 0.0;
     int nrej = 0;
     for (int ij : taken) {  // list order

@@ -72,7 +72,7 @@ def lewis_helix_lmos():
     return L
 
 
-@pytest.fixture(params=[(1, 3, 1024), (1, 1 << 30, 1024), (1, 0, 1024), (1, 3, 0), (0, 3, 1024)],
+@pytest.fixture(params=[(1, 1, 1024), (1, 1 << 30, 1024), (1, 0, 1024), (1, 1, 0), (0, 1, 1024)],
                 ids=["lds_hybrid", "lds_one_workgroup", "lds_across_chip", "lds_lists_in_global",
                      "global_per_level"])
 def diagg2_path(request):
