@@ -319,11 +319,13 @@ def run_helix(args):
     cutoffs (pairs stored within 9.9 A, dipole shell to 10 A, resonance to
     7 A), FP64.  One step = the Lewis-structure start (hybrids, bonds
     polarized by the Fock matrix of the diagonal density) + the MOZYME SCF,
-    with the sparse H and integrals resident.  Two forms: the reference's
+    with the sparse H and integrals resident.  Three forms: the reference's
     rotation order (the parity form, whose every iteration matches the
-    reference's routines) and the order-free sweep (diagg2's rotations in
-    LMO-disjoint rounds); `value` is the order-free form's time when its heat
-    is within 1e-3 kcal/mol of the reference-order form's.  The conventional
+    reference's routines), the same with diagg1's occupied-virtual elements
+    in FP32 (the mixed form, FP32 accumulation), and the order-free sweep
+    (diagg2's rotations in LMO-disjoint rounds); `value` is the order-free
+    form's time when its heat is within 1e-3 kcal/mol of the reference-order
+    form's.  The conventional
     FP64 SCF of the same helix is run once beside them (--no-conventional
     skips it): the difference of the heats is MOZYME's cutoff approximation,
     not an error of either solver."""
@@ -342,26 +344,28 @@ def run_helix(args):
     ms.upload(ctx)
     lib = L.load()
 
-    def step(order_free):
+    def step(form):
+        order_free, fp32 = form
         t = time.perf_counter()
         f0 = ms.fock2z(ctx, ms.diagonal_density())
         lm = mozyme.lewis_lmos(ms, f0)
         t_start = time.perf_counter() - t
         r = ms.scf(ctx, lm, scfcrt=args.helix_scfcrt, itrmax=args.helix_itrmax, print_level=1,
-                   order_free=order_free)
+                   order_free=order_free, fp32_fmo=fp32)
         return time.perf_counter() - t, t_start, r
 
-    def stats_of(order_free):
+    def stats_of(form):
         """diagg2's HIP-event time and counts over one more step"""
         sv = [ctypes.c_double(), ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong(),
               ctypes.c_double(), ctypes.c_longlong()]
         lib.mhip_mozyme_stats(1, *[ctypes.byref(v) for v in sv])
-        step(order_free)
+        step(form)
         lib.mhip_mozyme_stats(0, *[ctypes.byref(v) for v in sv])
         return dict(zip(("ms", "rotations", "levels", "launches", "bytes", "rounds"), [v.value for v in sv]))
 
     forms = {}
-    for name, of in (("reference_order", False), ("order_free", True)):
+    for name, of in (("reference_order", (False, False)), ("reference_order_mixed", (False, True)),
+                     ("order_free", (True, False))):
         for _ in range(args.warmup):
             dt, _, r = step(of)
             progress(f"{name} warmup step {dt:.3f} s, {r['niter']} iterations")
@@ -376,8 +380,9 @@ def run_helix(args):
                        "lewis_start_s": round(t_start, 4), "diagg2_ms": round(st["ms"], 2),
                        "diagg2_rotations": int(st["rotations"]), "diagg2_levels": int(st["levels"]),
                        "diagg2_launches": int(st["launches"]), "_stats": st, "_escf": last["escf"]}
-    ro, of = forms["reference_order"], forms["order_free"]
+    ro, of, mx = forms["reference_order"], forms["order_free"], forms["reference_order_mixed"]
     of["minus_reference_order_kcal"] = round(of["_escf"] - ro["_escf"], 6)
+    mx["minus_fp64_kcal"] = round(mx["_escf"] - ro["_escf"], 6)
     use_of = of["converged"] and abs(of["_escf"] - ro["_escf"]) < 1e-3
     best = of if use_of else ro
     # roofline of the dominant kernel of the timed form, the diagg2 rotations:

@@ -231,12 +231,19 @@ int mhip_mozyme_density(mhip_ctx* ctx, const mhip_lmo_set* lmo, int mode, const 
 /* The values diagg1 keeps between calls (SAVE, diagg1.F90:61-71) and the
  * MOZYME_C scalars it reads and writes.  Set reset = 1 for a new SCF (oldens
  * as the OLDENS keyword); nfmo (nvir ints, caller-owned) carries each
- * virtual LMO's list length to the calls that reuse the list. */
+ * virtual LMO's list length to the calls that reuse the list.
+ * fp32_accumulate (no counterpart in the reference): 1 forms the
+ * occupied-virtual elements (the F c products of each virtual LMO and their
+ * dot products with the occupied LMOs) in FP32 from the FP64 data, the
+ * mixed-precision form; the virtual LMO energies follow from those products,
+ * while the occupied LMO energies, the tests' weights and everything after
+ * diagg1 stay FP64. */
 typedef struct {
   int reset, oldens;
   double fref, safety, oldlim;
   double tiny, sumt, ovmax;
   int* nfmo;
+  int fp32_accumulate;
 } mhip_diagg1_state;
 
 /* diagg1(fao, nocc, nvir, eigv, ...) (src/MOZYME/diagg1.F90:16-649): the
@@ -381,6 +388,8 @@ typedef struct {
   int heats_cap;    /* for k <= heats_cap */
   int order_free;   /* diagg2's rotations in LMO-disjoint rounds
                        (mhip_diagg2_state.order_free); 0: list order */
+  int fp32_fmo;     /* diagg1's occupied-virtual elements in FP32
+                       (mhip_diagg1_state.fp32_accumulate) */
 } mhip_mozyme_opts;
 typedef struct {
   double escf;      /* heat of formation, kcal/mol (no geometry-only terms) */

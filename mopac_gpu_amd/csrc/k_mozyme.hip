@@ -114,6 +114,7 @@ struct D1Params {
   double cutoff, flim, oldlim;
   int full;        // idiagg <= 5 or even: fast test + new list; else reuse the old list
   int fmo_dim;
+  int fp32;        // the elements' products and sums in FP32 (the mixed form)
 };
 
 // Per virtual LMO: ws on its atoms (position-indexed), aov, avir, eigv.
@@ -135,8 +136,13 @@ __device__ __forceinline__ int vpos(const VirtLds& V, int a) {
   return -1;
 }
 
+template <class T>
+__device__ void ws_accumulate(const LmoDev& L, VirtLds& V, const double* __restrict__ fao,
+                              const double* __restrict__ p, double cutoff, int loopi);
+
 __device__ void virt_setup(const LmoDev& L, int i, const double* __restrict__ fao,
-                           const double* __restrict__ p, double cutoff, VirtLds& V, double* eigv) {
+                           const double* __restrict__ p, double cutoff, VirtLds& V, double* eigv,
+                           bool fp32) {
   const int tid = threadIdx.x;
   const int loopi = L.ncvir[i];
   if (tid == 0) {
@@ -162,35 +168,12 @@ __device__ void virt_setup(const LmoDev& L, int i, const double* __restrict__ fa
   }
   __syncthreads();
   // ws(j1 orbitals) = sum over atoms k1 of the LMO of F(j1, k1) c_vir(k1), as
-  // diagg1.F90:142-195 accumulates it (k1 in LMO order)
-  for (int jj = tid; jj < V.nat; jj += blockDim.x) {
-    const int j1 = V.atoms[jj], nj = L.iorbs[j1 - 1];
-    double w[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int kk = 0; kk < V.nat; ++kk) {
-      const int k1 = V.atoms[kk], nk = L.iorbs[k1 - 1];
-      const int kj = nij_of(L, k1, j1);
-      if (kj < 0) continue;
-      if (!(V.avir[kk] * p[kj] > cutoff)) continue;
-      const double* cv = L.cvir + loopi + V.woff[kk];
-      if (nk == 1 && nj == 1) {
-        w[0] += fao[kj] * cv[0];
-      } else if (k1 > j1) {
-        int ii = kj;
-        for (int i4 = 0; i4 < nk; ++i4)
-          for (int jx = 0; jx < nj; ++jx) w[jx] += fao[ii++] * cv[i4];
-      } else if (k1 < j1) {
-        int ii = kj;
-        for (int jx = 0; jx < nj; ++jx)
-          for (int i4 = 0; i4 < nk; ++i4) w[jx] += fao[ii++] * cv[i4];
-      } else {
-        for (int jx = 1; jx <= nj; ++jx)
-          for (int i4 = 1; i4 <= nj; ++i4) {
-            const int ii = (i4 > jx) ? kj + (i4 * (i4 - 1)) / 2 + jx : kj + (jx * (jx - 1)) / 2 + i4;
-            w[jx - 1] += fao[ii - 1] * cv[i4 - 1];
-          }
-      }
-    }
-    for (int x = 0; x < nj; ++x) V.ws[V.woff[jj] + x] = w[x];
+  // diagg1.F90:142-195 accumulates it (k1 in LMO order); in FP32 for the
+  // mixed form
+  if (fp32) {
+    ws_accumulate<float>(L, V, fao, p, cutoff, loopi);
+  } else {
+    ws_accumulate<double>(L, V, fao, p, cutoff, loopi);
   }
   __syncthreads();
   for (int k = tid; k < V.nat; k += blockDim.x) {
@@ -211,6 +194,40 @@ __device__ void virt_setup(const LmoDev& L, int i, const double* __restrict__ fa
     if (eigv) eigv[i] = s;
   }
   __syncthreads();
+}
+
+template <class T>
+__device__ void ws_accumulate(const LmoDev& L, VirtLds& V, const double* __restrict__ fao,
+                              const double* __restrict__ p, double cutoff, int loopi) {
+  for (int jj = threadIdx.x; jj < V.nat; jj += blockDim.x) {
+    const int j1 = V.atoms[jj], nj = L.iorbs[j1 - 1];
+    T w[4] = {T(0), T(0), T(0), T(0)};
+    for (int kk = 0; kk < V.nat; ++kk) {
+      const int k1 = V.atoms[kk], nk = L.iorbs[k1 - 1];
+      const int kj = nij_of(L, k1, j1);
+      if (kj < 0) continue;
+      if (!(V.avir[kk] * p[kj] > cutoff)) continue;
+      const double* cv = L.cvir + loopi + V.woff[kk];
+      if (nk == 1 && nj == 1) {
+        w[0] += T(fao[kj]) * T(cv[0]);
+      } else if (k1 > j1) {
+        int ii = kj;
+        for (int i4 = 0; i4 < nk; ++i4)
+          for (int jx = 0; jx < nj; ++jx) w[jx] += T(fao[ii++]) * T(cv[i4]);
+      } else if (k1 < j1) {
+        int ii = kj;
+        for (int jx = 0; jx < nj; ++jx)
+          for (int i4 = 0; i4 < nk; ++i4) w[jx] += T(fao[ii++]) * T(cv[i4]);
+      } else {
+        for (int jx = 1; jx <= nj; ++jx)
+          for (int i4 = 1; i4 <= nj; ++i4) {
+            const int ii = (i4 > jx) ? kj + (i4 * (i4 - 1)) / 2 + jx : kj + (jx * (jx - 1)) / 2 + i4;
+            w[jx - 1] += T(fao[ii - 1]) * T(cv[i4 - 1]);
+          }
+      }
+    }
+    for (int x = 0; x < nj; ++x) V.ws[V.woff[jj] + x] = (double)w[x];
+  }
 }
 
 // Occupied LMO j against the virtual LMO in V: the fast test (full pass only)
@@ -234,6 +251,7 @@ __device__ int occ_virt_element(const LmoDev& L, const VirtLds& V, const double*
   const int loopj = L.ncocc[j];
   bool lij = false;
   double sum = 0.0;
+  float sum32 = 0.0f;
   int kl = 0;
   for (int kk = L.nncf[j]; kk < L.nncf[j] + L.ncf[j]; ++kk) {
     const int k1 = L.icocc[kk], nk = L.iorbs[k1 - 1];
@@ -242,10 +260,15 @@ __device__ int occ_virt_element(const LmoDev& L, const VirtLds& V, const double*
       kl += nk;
     } else {
       lij = true;
-      for (int x = 0; x < nk; ++x) sum += V.ws[V.woff[vp] + x] * L.cocc[loopj + kl + x];
+      if (P.fp32) {
+        for (int x = 0; x < nk; ++x) sum32 += (float)V.ws[V.woff[vp] + x] * (float)L.cocc[loopj + kl + x];
+      } else {
+        for (int x = 0; x < nk; ++x) sum += V.ws[V.woff[vp] + x] * L.cocc[loopj + kl + x];
+      }
       kl += nk;
     }
   }
+  if (P.fp32) sum = (double)sum32;
   *sum_out = sum;
   if (!fast_test) return 2;  // the reuse path stores every listed element
   return (lij && fabs(sum) > P.oldlim) ? 2 : 1;
@@ -274,7 +297,7 @@ __global__ __launch_bounds__(256) void k_mz_diagg1_count(LmoDev L, const double*
                                                          double* __restrict__ eigv) {
   __shared__ VirtLds V;
   const int i = blockIdx.x;
-  virt_setup(L, i, fao, p, P.cutoff, V, eigv);
+  virt_setup(L, i, fao, p, P.cutoff, V, eigv, P.fp32 != 0);
   int cnt = 0;
   for (int j = threadIdx.x; j < L.nocc; j += blockDim.x) {
     double s;
@@ -308,7 +331,7 @@ __global__ __launch_bounds__(256) void k_mz_diagg1_write(LmoDev L, const double*
   __shared__ int scan[256];
   __shared__ int base_s;
   const int i = blockIdx.x, tid = threadIdx.x;
-  virt_setup(L, i, fao, p, P.cutoff, V, eigv);
+  virt_setup(L, i, fao, p, P.cutoff, V, eigv, P.fp32 != 0);
   double mx = 0.0, sm = 0.0;
   if (P.full) {
     if (tid == 0) base_s = offs[i];
@@ -1541,6 +1564,7 @@ void mz_diagg1_dev(hipStream_t s, const mhip_lmo_set* L, const double* fao_d, co
     P.oldlim = st->oldlim;
     P.full = full ? 1 : 0;
     P.fmo_dim = fmo_dim;
+    P.fp32 = st->fp32_accumulate ? 1 : 0;
     const int nv = std::max(L->nvir, 1);
     dcounts.ensure(sizeof(int) * nv);
     doffs.ensure(sizeof(int) * nv);

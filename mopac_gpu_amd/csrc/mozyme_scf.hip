@@ -411,6 +411,7 @@ int mhip_mozyme_scf(mhip_ctx* c, mhip_lmo_set* lmo, const mhip_mozyme_opts* o, m
     mhip_diagg1_state st1{};
     st1.reset = 1;
     st1.nfmo = nfmo.data();
+    st1.fp32_accumulate = o->fp32_fmo ? 1 : 0;
     mhip_diagg2_state st2{};
     st2.reset = 1;
     st2.damp = o->damp > 0 ? o->damp : 1.0;

@@ -75,7 +75,7 @@ class LmoSetC(ctypes.Structure):
 class MozymeOptsC(ctypes.Structure):
     _fields_ = [("scfcrt", c_double), ("itrmax", c_int), ("thresh", c_double), ("damp", c_double),
                 ("fmo_dim", c_int), ("enuclr", c_double), ("atheat", c_double), ("print_level", c_int),
-                ("heats", c_void_p), ("heats_cap", c_int), ("order_free", c_int)]
+                ("heats", c_void_p), ("heats_cap", c_int), ("order_free", c_int), ("fp32_fmo", c_int)]
 
 
 class MozymeResultC(ctypes.Structure):
@@ -86,7 +86,7 @@ class MozymeResultC(ctypes.Structure):
 class Diagg1State(ctypes.Structure):
     _fields_ = [("reset", c_int), ("oldens", c_int), ("fref", c_double), ("safety", c_double),
                 ("oldlim", c_double), ("tiny", c_double), ("sumt", c_double), ("ovmax", c_double),
-                ("nfmo", c_void_p)]
+                ("nfmo", c_void_p), ("fp32_accumulate", c_int)]
 
 
 class Diagg2State(ctypes.Structure):

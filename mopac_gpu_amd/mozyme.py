@@ -242,7 +242,7 @@ class MozymeSystem:
         L.check(ctx.lib.mhip_mozyme_setup(ctx.h, ctypes.byref(sysc)))
 
     def scf(self, ctx, lmos, scfcrt=1e-2, itrmax=2000, thresh=1e-13, fmo_dim=0, print_level=0, heats_cap=0,
-            order_free=False):
+            order_free=False, fp32_fmo=False):
         """iter_for_MOZYME on the device from the LMOs of `lmos` (an LmoSet on
         this system's nijbo); returns the result and the updated LMOs."""
         from . import lib as L
@@ -261,7 +261,7 @@ class MozymeSystem:
         o = L.MozymeOptsC(scfcrt=scfcrt, itrmax=itrmax, thresh=thresh, damp=1.0, fmo_dim=fmo_dim,
                           enuclr=self.enuclr, atheat=self.s.atheat, print_level=print_level,
                           heats=heats.ctypes.data if heats_cap > 0 else None, heats_cap=heats_cap,
-                          order_free=int(order_free))
+                          order_free=int(order_free), fp32_fmo=int(fp32_fmo))
         r = L.MozymeResultC()
         p = np.zeros(self.mpack)
         f = np.zeros(self.mpack)
@@ -604,7 +604,7 @@ class GpuMozyme:
     """The MOZYME kernels of libmopachip on an LmoSet, with the state the
     reference keeps between calls (mhip_diagg1_state / mhip_diagg2_state)."""
 
-    def __init__(self, ctx, s, fmo_dim=None, thresh=1e-13, shift=0.0, damp=1.0, oldens=False):
+    def __init__(self, ctx, s, fmo_dim=None, thresh=1e-13, shift=0.0, damp=1.0, oldens=False, fp32=False):
         import ctypes
         from . import lib as L
         self.ctypes, self.L = ctypes, L
@@ -612,7 +612,7 @@ class GpuMozyme:
         self.s = s.copy()
         self.fmo_dim = int(fmo_dim or max(1, s.nocc * s.nvir))
         self.nfmo = np.zeros(max(s.nvir, 1), dtype=np.int32)
-        self.st1 = L.Diagg1State(reset=1, oldens=int(oldens))
+        self.st1 = L.Diagg1State(reset=1, oldens=int(oldens), fp32_accumulate=int(fp32))
         self.st1.nfmo = self.nfmo.ctypes.data
         self.st2 = L.Diagg2State(reset=1, damp=damp, shift=shift, thresh=thresh)
         self.fmo = np.zeros(self.fmo_dim)

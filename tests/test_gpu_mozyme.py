@@ -380,3 +380,39 @@ def test_eimp_matches_reference(gpu_ctx, tidy_lmos):
                                         L_.ptr(f), L_.ptr(pg)))
     assert not np.array_equal(pr, p)
     assert np.array_equal(pg, pr)
+
+
+def test_fp32_elements_track_fp64(gpu_ctx, lewis_helix_lmos):
+    """The mixed form's diagg1 (occupied-virtual elements from FP32 products
+    and sums of the FP64 data): on a fresh state the same list as FP64, each
+    element and each virtual LMO energy within FP32 rounding of the FP64
+    one."""
+    L = lewis_helix_lmos
+    g64 = mozyme.GpuMozyme(gpu_ctx, L)
+    g32 = mozyme.GpuMozyme(gpu_ctx, L, fp32=True)
+    r64 = g64.diagg1(L.fao, L.p, 1)
+    r32 = g32.diagg1(L.fao, L.p, 1)
+    assert r32["nij"] == r64["nij"] and np.array_equal(r32["ifmo"], r64["ifmo"])
+    scale = np.max(np.abs(r64["fmo"]))
+    err = np.max(np.abs(r32["fmo"] - r64["fmo"]))
+    print(f"nij {r64['nij']}, max |fmo| {scale:.3g}, FP32 error {err:.3g}", flush=True)
+    assert 0.0 < err < 1e-5 * scale
+    # the virtual LMO energies come from the same FP32 products
+    assert np.max(np.abs(r32["eigv"] - r64["eigv"])) < 1e-5 * np.max(np.abs(r64["eigv"]))
+
+
+@pytest.mark.parametrize("nres", [12, 30])
+def test_mixed_form_converges_to_the_fp64_heat(gpu_ctx, nres):
+    """The whole MOZYME SCF with diagg1's elements in FP32 ends within the
+    metric's 1e-3 kcal/mol of the FP64 SCF at SCFCRT 1e-3."""
+    nat, xyz = geometry.glycine_helix(nres, 3)
+    ms = mozyme.MozymeSystem.from_geometry(nat, xyz)
+    ms.upload(gpu_ctx)
+    f0 = ms.fock2z(gpu_ctx, ms.diagonal_density())
+    lmos = mozyme.lewis_lmos(ms, f0)
+    r64 = ms.scf(gpu_ctx, lmos, scfcrt=1e-3)
+    r32 = ms.scf(gpu_ctx, lmos, scfcrt=1e-3, fp32_fmo=True)
+    print(f"Gly{nres}: FP64 {r64['escf']:.6f} ({r64['niter']} it), mixed {r32['escf']:.6f} ({r32['niter']} it)",
+          flush=True)
+    assert r64["converged"] and r32["converged"]
+    assert abs(r32["escf"] - r64["escf"]) < 1e-3
