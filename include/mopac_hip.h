@@ -179,8 +179,12 @@ typedef struct {
 typedef struct mhip_batch mhip_batch;
 int mhip_batch_create(int nmol, const mhip_batch_molecule* mols, mhip_batch** out);
 /* Eigensolver workgroup size per size class (n <= 32, 64, 128, 256):
- * 64, 128, 256 or 512 (chosen by the autotuner, mopac_gpu_amd/batch.py). */
+ * 64, 128, 256, 512 or 1024 (chosen by the autotuner, mopac_gpu_amd/batch.py). */
 int mhip_batch_set_tiles(mhip_batch* b, const int* tiles);
+/* Where the eigensolver keeps its matrices: lds = 1 (default) puts the
+ * packed A = C^T F C, and C where both fit, in LDS; lds = 0 keeps both in
+ * global memory (the same rotations, for comparison). */
+int mhip_batch_set_eig_lds(mhip_batch* b, int lds);
 /* Capture iters_per_graph iterations of the whole batch into one hipGraph
  * and replay it (at most max_launches times) until every molecule is done. */
 int mhip_batch_run(mhip_batch* b, int iters_per_graph, int max_launches, double* seconds,

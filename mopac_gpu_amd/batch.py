@@ -25,7 +25,7 @@ from . import lib as L
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TILES_JSON = os.path.join(HERE, "tuning", "batch_tiles.json")
-CANDIDATES = (64, 128, 256, 512)
+CANDIDATES = (64, 128, 256, 512, 1024)
 
 
 class MoleculeC(ctypes.Structure):
@@ -84,6 +84,10 @@ class Batch:
         L.check(self.lib.mhip_batch_set_tiles(self.h, t))
         self.tiles = list(tiles)
 
+    def set_eig_lds(self, on=True):
+        """Eigensolver matrices in LDS (default) or in global memory."""
+        L.check(self.lib.mhip_batch_set_eig_lds(self.h, 1 if on else 0))
+
     def run(self, iters_per_graph=40, max_launches=50):
         sec = ctypes.c_double()
         nconv = ctypes.c_int()
@@ -128,6 +132,7 @@ def autotune(batch, repeats=2, write=True):
             batch.set_tiles(trial)
             t = min(batch.run()[0] for _ in range(repeats))
             scores[cand] = t
+            print(f"[tune] class {cls} tile {cand}: {t:.4f} s", flush=True)
         best[cls] = min(scores, key=scores.get)
         timings[f"class{cls}"] = scores
     batch.set_tiles(best)
@@ -138,3 +143,35 @@ def autotune(batch, repeats=2, write=True):
                        "seconds_per_candidate": timings, "molecules": len(batch.systems),
                        "when": time.strftime("%Y-%m-%d %H:%M:%S")}, fh, indent=1)
     return best, timings
+
+
+def main(argv=None):
+    """python -m mopac_gpu_amd.batch [--out FILE]: tune the eigensolver
+    workgroup sizes on the 4096-molecule workload (seed 4) and write the
+    table (default: the package's tuning file)."""
+    import argparse
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--out", default=TILES_JSON)
+    ap.add_argument("--molecules", type=int, default=4096)
+    ap.add_argument("--repeats", type=int, default=2)
+    args = ap.parse_args(argv)
+    systems = build_systems(workload(args.molecules, seed=4))
+    b = Batch(systems, scfcrt=1e-6, tiles=[64, 128, 256, 256])
+    try:
+        best, timings = autotune(b, repeats=args.repeats, write=False)
+        tuned = min(b.run()[0] for _ in range(args.repeats))
+    finally:
+        b.close()
+    table = {"tiles": best, "classes": ["n<=32", "n<=64", "n<=128", "n<=256"],
+             "seconds_per_candidate": timings, "molecules": len(systems), "tuned_seconds": tuned,
+             "workload": f"batch.workload({len(systems)}, seed=4): alkanes CnH2n+2 (n=1..20) and water "
+                         "clusters (1..30 H2O), SCFCRT=1e-6",
+             "device": "MI355X (gfx950)"}
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    with open(args.out, "w") as fh:
+        json.dump(table, fh, indent=1)
+    print(json.dumps({"tiles": best, "tuned_seconds": tuned}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -8,8 +8,8 @@
 //
 // Per iteration: control (shift, mode) -> fock2 (batched) -> helect ->
 // control (energy, test) -> eigensolve + density (one workgroup per
-// molecule: warm-started cyclic Jacobi in parallel order, matrix in global
-// memory/L2) -> cnvg (batched) -> control (PL, frontier levels, newdg) ->
+// molecule: warm-started cyclic Jacobi in parallel order, the packed matrix
+// in LDS, and the eigenvectors too where they fit) -> cnvg (batched) -> control (PL, frontier levels, newdg) ->
 // Pa = P / 2.  The eigensolve's workgroup size per size class comes from a
 // measured table (mhip_batch_tune).
 #include <algorithm>
@@ -162,6 +162,30 @@ __device__ double wg_max(double v, double* red) {
   return r;
 }
 
+// maxima of two values over the workgroup: within each wave by lane
+// exchange, then across the NT / 64 waves through red (2 NT / 64 doubles)
+template <int NT>
+__device__ void wg_max2(double& a, double& b, double* red) {
+  for (int o = 32; o > 0; o >>= 1) {
+    a = fmax(a, __shfl_xor(a, o));
+    b = fmax(b, __shfl_xor(b, o));
+  }
+  constexpr int NW = NT / 64;
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[w] = a;
+    red[NW + w] = b;
+  }
+  __syncthreads();
+  a = red[0];
+  b = red[NW];
+  for (int i = 1; i < NW; ++i) {
+    a = fmax(a, red[i]);
+    b = fmax(b, red[NW + i]);
+  }
+  __syncthreads();
+}
+
 constexpr int kMaxBatchN = 256;
 
 template <int NT>
@@ -301,6 +325,313 @@ __global__ __launch_bounds__(NT) void k_beig(const BatchMol* __restrict__ mols, 
   if (tid == 0) st[m].have_c = 1;
 }
 
+// LDS form of the batched eigensolver: the same warm-started cyclic Jacobi in
+// parallel (round-robin) order, with the working matrix A = C^T F C kept as
+// its lower triangle, packed, in LDS (np (np+1) / 2 doubles, np = n rounded
+// up to even; row n of an odd n is a zero row that no rotation touches), and
+// C in LDS too when both fit (CLDS).  One round is two phases: the angles of
+// its np / 2 disjoint pairs, then every 2x2 block (pair k1, pair k2) of A
+// turned by both rotations together (rows by k1, columns by k2), and, with
+// CLDS, the columns of C.  Without CLDS the rounds log their (c, s) in global
+// memory and C is turned once per sweep: A is parked in global memory, C
+// passes through the freed LDS in chunks of rows, each chunk taking the
+// sweep's rounds in order (the same operations on every element as turning
+// C round by round), then A comes back.
+// A rotation whose |a_pq| is under 1e-2 tol max(|diag|, 1) is skipped: the
+// sweep loop stops once every off-diagonal element is under tol max(|diag|,
+// 1), so such an element already meets it.  The warm start stages F in the
+// LDS that A later takes: T' = (F C)^T in global memory (lanes along rows of
+// F, C read as wave-uniform columns), then A = C^T T' with lanes along the
+// columns of T' (contiguous) and four rows per thread.
+__device__ __forceinline__ int tri(int i) { return (i * (i + 1)) >> 1; }
+// rr_index for idx, r < np - 1 (the sum wraps at most once)
+__device__ __forceinline__ int rr_pos(int idx, int r, int np) {
+  if (idx == 0) return 0;
+  const int t = r + idx - 1;
+  return 1 + (t >= np - 1 ? t - (np - 1) : t);
+}
+__device__ __forceinline__ int sym_at(int a, int b) { return a >= b ? tri(a) + b : tri(b) + a; }
+// row of packed lower-triangle index e (exact for e < 2^22)
+__device__ __forceinline__ int tri_row(int e) {
+  int i = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+  if (tri(i + 1) <= e) ++i;
+  if (tri(i) > e) --i;
+  return i;
+}
+
+// C <- C R^T for the logged rounds of one sweep, in chunks of rows staged in
+// LDS (ch, room doubles); the pairs of round r + 1 are staged while round r
+// is applied.
+template <int NT>
+__device__ void beig_flush_c(double* __restrict__ C, const double* __restrict__ lg, int n, int np, double* ch,
+                             int room, double (*cs2)[kMaxBatchN / 2], double (*sn2)[kMaxBatchN / 2],
+                             int (*pq2)[kMaxBatchN / 2]) {
+  const int tid = threadIdx.x, npair = np / 2, nround = np - 1;
+  const int R = min(n, room / n);
+  auto stage = [&](int r) {
+    const int b = r & 1;
+    for (int k = tid; k < npair; k += NT) {
+      int p = rr_pos(k, r, np), q = rr_pos(np - 1 - k, r, np);
+      if (p > q) { const int t = p; p = q; q = t; }
+      const double c = lg[2 * (r * npair + k)], s = lg[2 * (r * npair + k) + 1];
+      cs2[b][k] = c;
+      sn2[b][k] = q < n ? s : 0.0;
+      pq2[b][k] = p | (q << 16);
+    }
+  };
+  for (int r0 = 0; r0 < n; r0 += R) {
+    const int rows = min(R, n - r0);
+    for (int e = tid; e < rows * n; e += NT) {
+      const int row = e % rows, col = e / rows;
+      ch[row + col * rows] = C[r0 + row + (size_t)col * n];
+    }
+    stage(0);
+    __syncthreads();
+    const int stepk = NT / rows, stepr = NT % rows;
+    for (int r = 0; r < nround; ++r) {
+      if (r + 1 < nround) stage(r + 1);
+      const int b = r & 1;
+      int k = tid / rows, row = tid % rows;
+      for (int e = tid; e < npair * rows; e += NT) {
+        const double s = sn2[b][k];
+        if (s != 0.0) {
+          const double c = cs2[b][k];
+          const int pq = pq2[b][k];
+          double* cp = ch + (pq & 0xffff) * rows + row;
+          double* cq = ch + (pq >> 16) * rows + row;
+          const double u = *cp, v = *cq;
+          *cp = c * u - s * v;
+          *cq = s * u + c * v;
+        }
+        k += stepk;
+        row += stepr;
+        if (row >= rows) { row -= rows; ++k; }
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < rows * n; e += NT) {
+      const int row = e % rows, col = e / rows;
+      C[r0 + row + (size_t)col * n] = ch[row + col * rows];
+    }
+    __syncthreads();
+  }
+}
+
+template <int NT, bool CLDS>
+__global__ __launch_bounds__(NT) void k_beig_lds(const BatchMol* __restrict__ mols, const int* __restrict__ do_eig,
+                                                 BatchState* __restrict__ st, const int* __restrict__ list,
+                                                 double tol, int room) {
+  const int m = list[blockIdx.x];
+  if (!do_eig[m]) return;
+  const BatchMol& M = mols[m];
+  const int n = M.n, tid = threadIdx.x;
+  const int np = (n + 1) & ~1;
+  const int npair = np / 2;
+  __shared__ double red[2 * (NT / 64)];
+  __shared__ double cs2[2][kMaxBatchN / 2], sn2[2][kMaxBatchN / 2];
+  __shared__ int pp[kMaxBatchN / 2], qq[kMaxBatchN / 2], pq2[2][kMaxBatchN / 2];
+  __shared__ int rank[kMaxBatchN];
+  __shared__ int rotated;
+  double* cs = cs2[0];
+  double* sn = sn2[0];
+  extern __shared__ double dyn[];
+  double* A = dyn;                               // tri(np) packed lower triangle
+  double* C = CLDS ? dyn + tri(np) : M.C;        // n x n, column-major
+  double* Tt = M.T;                              // n x n scratch, global
+  const bool warm = st[m].have_c != 0;
+  const int na = tri(np);
+  // without CLDS: the sweep's (c, s) log, np (np-1) doubles, then A parked
+  double* lg = M.A;
+  double* park = M.A + (size_t)np * (np - 1);
+  // eigenvectors_LAPACK: F(i,i) += i * 1e-10, in place
+  for (int i = tid; i < n; i += NT) M.f[tri(i) + i] += (double)(i + 1) * 1.0e-10;
+  __syncthreads();
+  // F (packed, same layout as A's first n rows) into LDS, zero row n
+  for (int e = tid; e < na; e += NT) A[e] = e < tri(n) ? M.f[e] : 0.0;
+  if (CLDS && warm)
+    for (int e = tid; e < n * n; e += NT) C[e] = M.C[e];
+  __syncthreads();
+  if (warm) {
+    // T'(c, r) = sum_k F(r, k) C(k, c): four columns c per thread
+    const int ncb = (n + 3) >> 2;
+    for (int w = tid; w < n * ncb; w += NT) {
+      const int r = w % n, c0 = (w / n) * 4;
+      const int rb = tri(r);
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      const double* c_0 = C + (size_t)c0 * n;
+      const int nc = min(4, n - c0);
+      int kb = 0;  // tri(k)
+      for (int k = 0; k < n; ++k) {
+        const double f = A[k <= r ? rb + k : kb + r];
+        kb += k + 1;
+        s0 += f * c_0[k];
+        if (nc > 1) s1 += f * c_0[k + n];
+        if (nc > 2) s2 += f * c_0[k + 2 * n];
+        if (nc > 3) s3 += f * c_0[k + 3 * n];
+      }
+      Tt[c0 + (size_t)r * n] = s0;
+      if (nc > 1) Tt[c0 + 1 + (size_t)r * n] = s1;
+      if (nc > 2) Tt[c0 + 2 + (size_t)r * n] = s2;
+      if (nc > 3) Tt[c0 + 3 + (size_t)r * n] = s3;
+    }
+    __syncthreads();
+    // A(i, j) = sum_k C(k, i) T'(j, k), j <= i: rows i0..i0+3 per thread
+    const int nib = (n + 3) >> 2;
+    for (int w = tid; w < n * nib; w += NT) {
+      const int j = w % n, i0 = (w / n) * 4;
+      if (j > i0 + 3) continue;
+      const int ni = min(4, n - i0);
+      const double* ci = C + (size_t)i0 * n;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      for (int k = 0; k < n; ++k) {
+        const double t = Tt[j + (size_t)k * n];
+        s0 += ci[k] * t;
+        if (ni > 1) s1 += ci[k + n] * t;
+        if (ni > 2) s2 += ci[k + 2 * n] * t;
+        if (ni > 3) s3 += ci[k + 3 * n] * t;
+      }
+      if (j <= i0) A[tri(i0) + j] = s0;
+      if (ni > 1 && j <= i0 + 1) A[tri(i0 + 1) + j] = s1;
+      if (ni > 2 && j <= i0 + 2) A[tri(i0 + 2) + j] = s2;
+      if (ni > 3 && j <= i0 + 3) A[tri(i0 + 3) + j] = s3;
+    }
+  } else {
+    for (int e = tid; e < n * n; e += NT) {
+      const int r = e % n, c = e / n;
+      C[e] = (r == c) ? 1.0 : 0.0;
+    }
+  }
+  __syncthreads();
+  const int nblk = tri(npair);  // 2x2 blocks (k1 >= k2)
+  const int stepk = NT / n, stepr = NT % n;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, dg = 0.0;
+    for (int e = tid; e < tri(n); e += NT) {
+      const int i = tri_row(e);
+      const double v = fabs(A[e]);
+      if (e == tri(i) + i) dg = fmax(dg, v);
+      else off = fmax(off, v);
+    }
+    if (tid == 0) rotated = 0;
+    wg_max2<NT>(off, dg, red);
+    const double lim = tol * fmax(dg, 1.0);
+    if (off <= lim) break;
+    const double skip = 1.0e-2 * lim;
+    for (int r = 0; r < np - 1; ++r) {
+      for (int k = tid; k < npair; k += NT) {
+        int p = rr_pos(k, r, np), q = rr_pos(np - 1 - k, r, np);
+        if (p > q) { const int t = p; p = q; q = t; }
+        double c = 1.0, s = 0.0;
+        if (q < n) {
+          const double app = A[tri(p) + p], aqq = A[tri(q) + q], apq = A[tri(q) + p];
+          if (fabs(apq) > skip && fabs(apq) > 1.0e-18 * sqrt(fabs(app * aqq))) {
+            const double theta = (aqq - app) / (2.0 * apq);
+            const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            s = t * c;
+          }
+        }
+        pp[k] = p;
+        qq[k] = q;
+        cs[k] = c;
+        sn[k] = s;
+        if (!CLDS) {
+          lg[2 * (r * npair + k)] = c;
+          lg[2 * (r * npair + k) + 1] = s;
+          if (s != 0.0) rotated = 1;
+        }
+      }
+      __syncthreads();
+      // A <- R A R^T, R = [c -s; s c] on rows/columns (p, q) of every pair
+      for (int b = tid; b < nblk; b += NT) {
+        const int k1 = tri_row(b), k2 = b - tri(k1);
+        const double s1 = sn[k1], s2 = sn[k2];
+        if (s1 == 0.0 && s2 == 0.0) continue;
+        const double c1 = cs[k1], c2 = cs[k2];
+        const int p1 = pp[k1], q1 = qq[k1], p2 = pp[k2], q2 = qq[k2];
+        if (k1 == k2) {
+          const int ip = tri(p1) + p1, iq = tri(q1) + q1, ipq = tri(q1) + p1;
+          const double x = A[ip], y = A[ipq], v = A[iq];
+          // rows, then columns
+          const double x1 = c1 * x - s1 * y, y1 = s1 * x + c1 * y;
+          const double u1 = c1 * y - s1 * v, v1 = s1 * y + c1 * v;
+          A[ip] = c1 * x1 - s1 * u1;
+          A[iq] = s1 * y1 + c1 * v1;
+          A[ipq] = 0.0;
+        } else {
+          const int ixx = sym_at(p1, p2), iyx = sym_at(q1, p2), ixu = sym_at(p1, q2), iyu = sym_at(q1, q2);
+          const double x = A[ixx], y = A[iyx], u = A[ixu], v = A[iyu];
+          const double x1 = c1 * x - s1 * y, y1 = s1 * x + c1 * y;
+          const double u1 = c1 * u - s1 * v, v1 = s1 * u + c1 * v;
+          A[ixx] = c2 * x1 - s2 * u1;
+          A[ixu] = s2 * x1 + c2 * u1;
+          A[iyx] = c2 * y1 - s2 * v1;
+          A[iyu] = s2 * y1 + c2 * v1;
+        }
+      }
+      // C <- C R^T on columns (p, q)
+      if (CLDS) {
+        int k = tid / n, row = tid % n;
+        for (int e = tid; e < npair * n; e += NT) {
+          const double s = sn[k];
+          if (s != 0.0) {
+            const double c = cs[k];
+            double* cp = C + (size_t)pp[k] * n + row;
+            double* cq = C + (size_t)qq[k] * n + row;
+            const double u = *cp, v = *cq;
+            *cp = c * u - s * v;
+            *cq = s * u + c * v;
+          }
+          k += stepk;
+          row += stepr;
+          if (row >= n) { row -= n; ++k; }
+        }
+      }
+      __syncthreads();
+    }
+    // every thread reads the flag before thread 0 may clear it for the next sweep
+    const bool pending = !CLDS && rotated != 0;
+    __syncthreads();
+    if (pending) {
+      for (int e = tid; e < na; e += NT) park[e] = A[e];
+      __syncthreads();
+      beig_flush_c<NT>(C, lg, n, np, dyn, room, cs2, sn2, pq2);
+      for (int e = tid; e < na; e += NT) A[e] = park[e];
+      __syncthreads();
+    }
+  }
+  // ascending order, ties by index
+  for (int i = tid; i < n; i += NT) {
+    const double ai = A[tri(i) + i];
+    int rk = 0;
+    for (int j = 0; j < n; ++j) {
+      const double aj = A[tri(j) + j];
+      rk += (aj < ai || (aj == ai && j < i)) ? 1 : 0;
+    }
+    rank[i] = rk;
+  }
+  __syncthreads();
+  double* sorted = CLDS ? M.C : Tt;
+  for (int e = tid; e < n * n; e += NT) {
+    const int r = e % n, c = e / n;
+    sorted[r + rank[c] * n] = C[e];
+  }
+  for (int i = tid; i < n; i += NT) M.eig[rank[i]] = A[tri(i) + i];
+  __syncthreads();
+  if (!CLDS)
+    for (int e = tid; e < n * n; e += NT) M.C[e] = Tt[e];
+  __syncthreads();
+  // P = 2 C_occ C_occ^T, packed
+  const double* Cg = M.C;
+  for (int q = tid; q < tri(n); q += NT) {
+    const int r = tri_row(q), c = q - tri(r);
+    double s = 0.0;
+    for (int k = 0; k < M.nocc; ++k) s += Cg[r + k * n] * Cg[c + k * n];
+    M.p[q] = 2.0 * s;
+  }
+  if (tid == 0) st[m].have_c = 1;
+}
+
 // ---- control, part 3: PL, frontier levels, newdg (iter.F90:1030-1041) -------
 __global__ void k_bctl_post(int nmol, const BatchMol* __restrict__ mols, BatchState* __restrict__ st,
                             const int* __restrict__ do_eig, const double* __restrict__ scal) {
@@ -353,7 +684,10 @@ struct mhip_batch {
   DevBuf atom_mol, atom_idx, row_mol, row_idx;
   int n_atom_blk = 0, n_row_blk = 0;
   // molecules by eigensolver class (workgroup size), in launch order
-  std::vector<int> cls_count;
+  std::vector<int> cls_count, cls_nmax;
+  int eig_form[4] = {0, 0, 0, 0};  // 0 global, 1 A in LDS, 2 A and C in LDS
+  size_t eig_lds[4] = {0, 0, 0, 0};
+  bool force_global_eig = false;
   DevBuf cls_list;
   int tile[4] = {64, 128, 256, 256};   // workgroup size per class (n <= 32, 64, 128, 256)
   double tol = 1.0e-10;
@@ -367,11 +701,69 @@ namespace {
 
 int size_class(int n) { return n <= 32 ? 0 : (n <= 64 ? 1 : (n <= 128 ? 2 : 3)); }
 
+// LDS bytes a workgroup can take beyond the kernel's static arrays
+size_t lds_room(const void* fn) {
+  int dev = 0, lds_max = 0;
+  MHIP_CHECK(hipGetDevice(&dev));
+  MHIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  hipFuncAttributes fa{};
+  MHIP_CHECK(hipFuncGetAttributes(&fa, fn));
+  return lds_max > (int)fa.sharedSizeBytes ? (size_t)lds_max - fa.sharedSizeBytes : 0;
+}
+
+// Eigensolver form of a size class (its largest n, workgroup size NT): A and
+// C in LDS when both fit (2), else A in LDS (1), else the global-memory
+// kernel (0); the LDS forms get their dynamic LDS limit raised here, before
+// any capture.  Returns the form and sets the dynamic LDS bytes.
 template <int NT>
-void launch_beig(hipStream_t s, int nblk, const BatchMol* mols, const int* doeig, BatchState* st, const int* list,
-                 double tol) {
-  hipLaunchKernelGGL(k_beig<NT>, dim3(nblk), dim3(NT), 0, s, mols, doeig, st, list, tol);
+int plan_beig(int nmax, size_t* lds) {
+  const size_t np = (size_t)((nmax + 1) & ~1);
+  const size_t a = sizeof(double) * (np * (np + 1) / 2), c = sizeof(double) * (size_t)nmax * nmax;
+  const void* f2 = (const void*)k_beig_lds<NT, true>;
+  const void* f1 = (const void*)k_beig_lds<NT, false>;
+  if (a + c <= lds_room(f2)) {
+    MHIP_CHECK(hipFuncSetAttribute(f2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(a + c)));
+    *lds = a + c;
+    return 2;
+  }
+  if (a <= lds_room(f1)) {
+    MHIP_CHECK(hipFuncSetAttribute(f1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)a));
+    *lds = a;
+    return 1;
+  }
+  *lds = 0;
+  return 0;
+}
+
+template <int NT>
+void launch_beig(hipStream_t s, int form, size_t lds, int nblk, const BatchMol* mols, const int* doeig,
+                 BatchState* st, const int* list, double tol) {
+  const int room = (int)(lds / sizeof(double));
+  if (form == 2)
+    hipLaunchKernelGGL((k_beig_lds<NT, true>), dim3(nblk), dim3(NT), lds, s, mols, doeig, st, list, tol, room);
+  else if (form == 1)
+    hipLaunchKernelGGL((k_beig_lds<NT, false>), dim3(nblk), dim3(NT), lds, s, mols, doeig, st, list, tol, room);
+  else
+    hipLaunchKernelGGL(k_beig<NT>, dim3(nblk), dim3(NT), 0, s, mols, doeig, st, list, tol);
   MHIP_LAUNCH_CHECK();
+}
+
+void plan_eig_classes(mhip_batch* b) {
+  for (int c = 0; c < 4; ++c) {
+    size_t lds = 0;
+    int form = 0;
+    if (b->cls_nmax[c] > 0) {
+      switch (b->tile[c]) {
+        case 64: form = plan_beig<64>(b->cls_nmax[c], &lds); break;
+        case 128: form = plan_beig<128>(b->cls_nmax[c], &lds); break;
+        case 512: form = plan_beig<512>(b->cls_nmax[c], &lds); break;
+        case 1024: form = plan_beig<1024>(b->cls_nmax[c], &lds); break;
+        default: form = plan_beig<256>(b->cls_nmax[c], &lds); break;
+      }
+    }
+    b->eig_form[c] = b->force_global_eig ? 0 : form;
+    b->eig_lds[c] = b->force_global_eig ? 0 : lds;
+  }
 }
 
 void launch_eig_class(mhip_batch* b, int cls, int offset, int count) {
@@ -380,11 +772,14 @@ void launch_eig_class(mhip_batch* b, int cls, int offset, int count) {
   auto* mols = b->mols.as<BatchMol>();
   auto* st = b->st.as<BatchState>();
   const int* de = b->doeig.as<int>();
+  const int f = b->eig_form[cls];
+  const size_t l = b->eig_lds[cls];
   switch (b->tile[cls]) {
-    case 64: launch_beig<64>(b->s, count, mols, de, st, list, b->tol); break;
-    case 128: launch_beig<128>(b->s, count, mols, de, st, list, b->tol); break;
-    case 512: launch_beig<512>(b->s, count, mols, de, st, list, b->tol); break;
-    default: launch_beig<256>(b->s, count, mols, de, st, list, b->tol); break;
+    case 64: launch_beig<64>(b->s, f, l, count, mols, de, st, list, b->tol); break;
+    case 128: launch_beig<128>(b->s, f, l, count, mols, de, st, list, b->tol); break;
+    case 512: launch_beig<512>(b->s, f, l, count, mols, de, st, list, b->tol); break;
+    case 1024: launch_beig<1024>(b->s, f, l, count, mols, de, st, list, b->tol); break;
+    default: launch_beig<256>(b->s, f, l, count, mols, de, st, list, b->tol); break;
   }
 }
 
@@ -439,8 +834,8 @@ int mhip_batch_create(int nmol, const mhip_batch_molecule* mols, mhip_batch** ou
     b->nmol = nmol;
     MHIP_CHECK(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
     // ---- host tables, then one upload per array
-    std::vector<long long> o_w(nmol), o_mp(nmol), o_nn(nmol), o_n(nmol), o_at(nmol), o_cc(nmol);
-    long long tw = 0, tmp = 0, tnn = 0, tn = 0, tat = 0, tcc = 0;
+    std::vector<long long> o_w(nmol), o_mp(nmol), o_nn(nmol), o_n(nmol), o_at(nmol), o_cc(nmol), o_a(nmol);
+    long long tw = 0, tmp = 0, tnn = 0, tn = 0, tat = 0, tcc = 0, ta = 0;
     std::vector<int> nf0_all, norb_all, heavy_all, light_all, nhb_all, nlb_all;
     std::vector<long long> woff_all, wcum_all, w1off_all, ccol_all;
     for (int m = 0; m < nmol; ++m) {
@@ -485,6 +880,13 @@ int mhip_batch_create(int nmol, const mhip_batch_molecule* mols, mhip_batch** ou
       tw += kr;
       tmp += (long long)n * (n + 1) / 2;
       tnn += (long long)n * n;
+      {
+        // the eigensolver's A: n x n (global form), or the LDS form's
+        // rotation log of one sweep and parked A
+        const long long np = (n + 1) & ~1;
+        o_a[m] = ta;
+        ta += std::max((long long)n * n, np * (np - 1) + np * (np + 1) / 2);
+      }
       tn += n;
       tat += na;
       tcc += std::max(cc, 1LL);
@@ -507,7 +909,8 @@ int mhip_batch_create(int nmol, const mhip_batch_molecule* mols, mhip_batch** ou
     for (DevBuf* x : {&b->f, &b->p, &b->pa, &b->pold}) x->ensure(sizeof(double) * tmp);
     b->p1.ensure(sizeof(double) * tn);
     b->eig.ensure(sizeof(double) * tn);
-    for (DevBuf* x : {&b->C, &b->A, &b->T}) x->ensure(sizeof(double) * tnn);
+    for (DevBuf* x : {&b->C, &b->T}) x->ensure(sizeof(double) * tnn);
+    b->A.ensure(sizeof(double) * ta);
     b->jrow.ensure(sizeof(double) * 10 * tat);
     b->jcol.ensure(sizeof(double) * tcc);
     for (int m = 0; m < nmol; ++m) {
@@ -576,7 +979,7 @@ int mhip_batch_create(int nmol, const mhip_batch_molecule* mols, mhip_batch** ou
       q.pold = b->pold.as<double>() + o_mp[m];
       q.p1 = b->p1.as<double>() + o_n[m];
       q.C = b->C.as<double>() + o_nn[m];
-      q.A = b->A.as<double>() + o_nn[m];
+      q.A = b->A.as<double>() + o_a[m];
       q.T = b->T.as<double>() + o_nn[m];
       q.eig = b->eig.as<double>() + o_n[m];
       cv[m] = BatchCnvg{n, q.p, q.pold, q.p1};
@@ -633,13 +1036,16 @@ int mhip_batch_create(int nmol, const mhip_batch_molecule* mols, mhip_batch** ou
     // molecules grouped by eigensolver class
     std::vector<int> order;
     b->cls_count.assign(4, 0);
+    b->cls_nmax.assign(4, 0);
     for (int c = 0; c < 4; ++c)
       for (int m = 0; m < nmol; ++m)
         if (size_class(b->n_h[m]) == c) {
           order.push_back(m);
           b->cls_count[c]++;
+          b->cls_nmax[c] = std::max(b->cls_nmax[c], b->n_h[m]);
         }
     upv(b->cls_list, order.data(), sizeof(int) * nmol);
+    plan_eig_classes(b);
     MHIP_CHECK(hipStreamSynchronize(s));
     *out = b;
   });
@@ -649,10 +1055,25 @@ int mhip_batch_set_tiles(mhip_batch* b, const int* tiles) {
   return bguard([&] {
     MHIP_REQUIRE(b && tiles, "mhip_batch_set_tiles: bad arguments");
     for (int c = 0; c < 4; ++c) {
-      MHIP_REQUIRE(tiles[c] == 64 || tiles[c] == 128 || tiles[c] == 256 || tiles[c] == 512,
-                   "mhip_batch_set_tiles: workgroup sizes are 64, 128, 256 or 512");
+      MHIP_REQUIRE(tiles[c] == 64 || tiles[c] == 128 || tiles[c] == 256 || tiles[c] == 512 || tiles[c] == 1024,
+                   "mhip_batch_set_tiles: workgroup sizes are 64, 128, 256, 512 or 1024");
       b->tile[c] = tiles[c];
     }
+    plan_eig_classes(b);
+    if (b->exec) {
+      (void)hipGraphExecDestroy(b->exec);
+      (void)hipGraphDestroy(b->graph);
+      b->exec = nullptr;
+      b->graph = nullptr;
+    }
+  });
+}
+
+int mhip_batch_set_eig_lds(mhip_batch* b, int lds) {
+  return bguard([&] {
+    MHIP_REQUIRE(b, "mhip_batch_set_eig_lds: bad arguments");
+    b->force_global_eig = lds == 0;
+    plan_eig_classes(b);
     if (b->exec) {
       (void)hipGraphExecDestroy(b->exec);
       (void)hipGraphDestroy(b->graph);

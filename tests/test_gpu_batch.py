@@ -44,3 +44,25 @@ def test_batch_is_replayable_and_tiles_do_not_change_results(small_batch):
     e2, n2, _ = b.results()
     assert np.max(np.abs(e1 - e2)) < 1e-6
     assert np.array_equal(n1, n2)
+
+
+def test_lds_and_global_eigensolver_forms_agree(small_batch):
+    """The packed-LDS eigensolver (A in LDS; C too for the classes where both
+    fit) against the global-memory form of the same rotations: the same
+    iterations, heats within 1e-6 kcal/mol, densities within 1e-7."""
+    systems, b = small_batch
+    norbs = np.array([s.norbs for s in systems])
+    assert norbs.max() > 128 and norbs.min() <= 64  # both LDS forms run
+    b.set_tiles([64, 128, 256, 256])
+    b.set_eig_lds(False)
+    b.run(iters_per_graph=20, max_launches=20)
+    eg, ng, cg = b.results()
+    pg = [b.density(m) for m in range(len(systems))]
+    b.set_eig_lds(True)
+    b.run(iters_per_graph=20, max_launches=20)
+    el, nl, cl = b.results()
+    assert cg.all() and cl.all()
+    assert np.array_equal(ng, nl)
+    assert np.max(np.abs(eg - el)) < 1e-6
+    for m in range(len(systems)):
+        assert np.max(np.abs(b.density(m) - pg[m])) < 1e-7, (m, norbs[m])
