@@ -483,13 +483,31 @@ def run_batch(args):
         times.append(sec)
         progress(f"timed {sec:.3f} s, {nconv} converged")
     escf, niter, conv = b.results()
+    # roofline of the eigensolver (the dominant kernel): the same iterations
+    # as the timed run's graph launches, launched one by one with HIP events
+    # around each size class's eigensolver launch, flops counted on the device
+    iters = 20 * max(1, -(-(int(niter.max()) + 1) // 20))
+    eig_ms, eig_fl, eig_rot = b.eig_profile(iters)
     b.close()
+    classes = []
+    for c, name in enumerate(["n<=32", "n<=64", "n<=128", "n<=256"]):
+        if eig_ms[c] > 0:
+            classes.append({"class": name, "workgroup": b.tiles[c], "ms": round(float(eig_ms[c]), 2),
+                            "gflop": round(float(eig_fl[c]) / 1e9, 2), "rotations": int(eig_rot[c]),
+                            "tflops": round(float(eig_fl[c]) / (eig_ms[c] * 1e-3) / 1e12, 3)})
+    ach = float(eig_fl.sum()) / (float(eig_ms.sum()) * 1e-3) / 1e12
+    roof = {"bound": "mfma", "pipe": "fp64 vector (no MFMA form: 2x2 rotations)", "achieved": round(ach, 3),
+            "peak": 78.6, "unit": "TFLOP/s", "frac": round(ach / 78.6, 4), "traffic": None,
+            "kernel": "k_beig_lds (every size class)", "launch_ms_total": round(float(eig_ms.sum()), 2),
+            "launches": 4 * iters, "classes": classes}
+    progress(f"eigensolver: {eig_ms.sum():.1f} ms over {iters} iterations, {eig_fl.sum() / 1e9:.1f} GFLOP, "
+             f"{ach:.3f} TFLOP/s")
     norbs = np.array([x.norbs for x in systems])
     cpu, check = None, None
     if not args.no_cpu:
         rng = np.random.default_rng(4)
         sample = sorted(rng.choice(len(systems), 256, replace=False).tolist())
-        nproc = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        nproc = args.cpu_threads or len(os.sched_getaffinity(0))
         chunks = [[(m, systems[m]) for m in sample[k::nproc]] for k in range(nproc)]
         os.environ["MKL_NUM_THREADS"] = "1"
         os.environ["OMP_NUM_THREADS"] = "1"
@@ -515,7 +533,7 @@ def run_batch(args):
                    "molecules": len(systems), "norbs_min": int(norbs.min()), "norbs_max": int(norbs.max()),
                    "iters_per_graph": 20},
         "converged": int(conv.sum()), "mean_iterations": round(float(niter.mean()), 2),
-        "oracle_check": check, "roofline": None, "cpu_baseline": cpu,
+        "oracle_check": check, "roofline": roof, "cpu_baseline": cpu,
         "note": "one captured hipGraph per 20 iterations of every molecule; per-kernel times in "
                 "profiles/batch4096_kernels.txt",
     }

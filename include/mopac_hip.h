@@ -185,6 +185,13 @@ int mhip_batch_set_tiles(mhip_batch* b, const int* tiles);
  * packed A = C^T F C, and C where both fit, in LDS; lds = 0 keeps both in
  * global memory (the same rotations, for comparison). */
 int mhip_batch_set_eig_lds(mhip_batch* b, int lds);
+/* Eigensolver profile: `iters` iterations of the batch from its starting
+ * densities, launched one by one (no graph) with HIP events around the
+ * eigensolver launch of each size class (n <= 32, 64, 128, 256).  ms[4]: the
+ * classes' summed launch times; flops[4]: their FP64 flops (12 n per applied
+ * rotation, 2 n^3 + n^2 (n+1) for a warm start, n (n+1) nocc for the
+ * density); rotations[4] (may be NULL): applied rotations. */
+int mhip_batch_eig_profile(mhip_batch* b, int iters, double* ms, double* flops, long long* rotations);
 /* Capture iters_per_graph iterations of the whole batch into one hipGraph
  * and replay it (at most max_launches times) until every molecule is done. */
 int mhip_batch_run(mhip_batch* b, int iters_per_graph, int max_launches, double* seconds,

@@ -88,6 +88,15 @@ class Batch:
         """Eigensolver matrices in LDS (default) or in global memory."""
         L.check(self.lib.mhip_batch_set_eig_lds(self.h, 1 if on else 0))
 
+    def eig_profile(self, iters):
+        """Per size class: eigensolver launch time (ms, HIP events, eager
+        launches), FP64 flops and applied rotations over `iters` iterations."""
+        ms = np.zeros(4)
+        fl = np.zeros(4)
+        rot = np.zeros(4, dtype=np.int64)
+        L.check(self.lib.mhip_batch_eig_profile(self.h, iters, L.ptr(ms), L.ptr(fl), L.ptr(rot)))
+        return ms, fl, rot
+
     def run(self, iters_per_graph=40, max_launches=50):
         sec = ctypes.c_double()
         nconv = ctypes.c_int()

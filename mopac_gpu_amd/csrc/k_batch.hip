@@ -40,7 +40,18 @@ struct BatchState {
   int niter, iredy, ready, done, converged, newdg, oknewd, recompute, have_c, do_eig, mode, itrmax;
   double ten, shift, shfmax, shfto, shftbo, shiftb, pl, eold, diff, sellim, escf, ee;
   double eig_homo, eig_lumo, enuclr, atheat, selcon, pltest;
+  // eigensolver work of this molecule since the batch was reset: applied
+  // rotations, and FP64 flops of the method (12 n per rotation: 6 n on the
+  // symmetric A, 6 n on C; warm start 2 n^3 + 2 n tri(n); density
+  // 2 tri(n) nocc)
+  long long eig_rot;
+  double eig_flops;
 };
+
+__device__ __forceinline__ double beig_flops(int n, int nocc, long long nrot, bool warm) {
+  const double dn = n, tn = 0.5 * dn * (dn + 1.0);
+  return 12.0 * dn * (double)nrot + (warm ? 2.0 * dn * dn * dn + 2.0 * dn * tn : 0.0) + 2.0 * tn * nocc;
+}
 
 constexpr double kEnrgy = 23.060547830619029;  // fpc_9, eV -> kcal/mol
 
@@ -200,6 +211,8 @@ __global__ __launch_bounds__(NT) void k_beig(const BatchMol* __restrict__ mols, 
   __shared__ double cs[kMaxBatchN / 2], sn[kMaxBatchN / 2];
   __shared__ int pp[kMaxBatchN / 2], qq[kMaxBatchN / 2];
   __shared__ int rank[kMaxBatchN];
+  __shared__ int nrot;
+  if (tid == 0) nrot = 0;
   double* A = M.A;
   double* C = M.C;
   double* T = M.T;
@@ -257,6 +270,7 @@ __global__ __launch_bounds__(NT) void k_beig(const BatchMol* __restrict__ mols, 
             const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
             c = 1.0 / sqrt(t * t + 1.0);
             s = t * c;
+            atomicAdd(&nrot, 1);
           }
         }
         pp[k] = p;
@@ -322,7 +336,11 @@ __global__ __launch_bounds__(NT) void k_beig(const BatchMol* __restrict__ mols, 
     for (int k = 0; k < M.nocc; ++k) s += C[r + k * n] * C[c + k * n];
     M.p[q] = 2.0 * s;
   }
-  if (tid == 0) st[m].have_c = 1;
+  if (tid == 0) {
+    st[m].eig_rot += nrot;
+    st[m].eig_flops += beig_flops(n, M.nocc, nrot, warm);
+    st[m].have_c = 1;
+  }
 }
 
 // LDS form of the batched eigensolver: the same warm-started cyclic Jacobi in
@@ -359,59 +377,75 @@ __device__ __forceinline__ int tri_row(int e) {
   return i;
 }
 
-// C <- C R^T for the logged rounds of one sweep, in chunks of rows staged in
-// LDS (ch, room doubles); the pairs of round r + 1 are staged while round r
-// is applied.
+// C <- C R^T for the logged rounds of one sweep, in chunks of at most NT rows
+// staged in LDS (ch, room doubles, column-major per chunk); thread (row,
+// slot) turns its row for the pairs slot, slot + NT / rows, ... of every
+// round, while the pairs of the next round are staged.
 template <int NT>
 __device__ void beig_flush_c(double* __restrict__ C, const double* __restrict__ lg, int n, int np, double* ch,
-                             int room, double (*cs2)[kMaxBatchN / 2], double (*sn2)[kMaxBatchN / 2],
-                             int (*pq2)[kMaxBatchN / 2]) {
+                             int room, double2 (*csn2)[kMaxBatchN / 2], int2 (*off2)[kMaxBatchN / 2]) {
   const int tid = threadIdx.x, npair = np / 2, nround = np - 1;
-  const int R = min(n, room / n);
-  auto stage = [&](int r) {
-    const int b = r & 1;
-    for (int k = tid; k < npair; k += NT) {
-      int p = rr_pos(k, r, np), q = rr_pos(np - 1 - k, r, np);
-      if (p > q) { const int t = p; p = q; q = t; }
-      const double c = lg[2 * (r * npair + k)], s = lg[2 * (r * npair + k) + 1];
-      cs2[b][k] = c;
-      sn2[b][k] = q < n ? s : 0.0;
-      pq2[b][k] = p | (q << 16);
-    }
-  };
+  const int R = min(min(n, room / n), NT);
   for (int r0 = 0; r0 < n; r0 += R) {
     const int rows = min(R, n - r0);
+    // round r's (c, s) for the pairs k = tid + j NT are fetched into
+    // registers two rounds ahead and put into LDS one round ahead
+    constexpr int KP = (kMaxBatchN / 2 + NT - 1) / NT;
+    double fc[KP], fs[KP];
+    auto fetch = [&](int r) {
+#pragma unroll
+      for (int j = 0; j < KP; ++j) {
+        const int k = tid + j * NT;
+        if (k < npair) {
+          fc[j] = lg[2 * (r * npair + k)];
+          fs[j] = lg[2 * (r * npair + k) + 1];
+        }
+      }
+    };
+    auto put = [&](int r) {
+      const int b = r & 1;
+#pragma unroll
+      for (int j = 0; j < KP; ++j) {
+        const int k = tid + j * NT;
+        if (k < npair) {
+          int p = rr_pos(k, r, np), q = rr_pos(np - 1 - k, r, np);
+          if (p > q) { const int t = p; p = q; q = t; }
+          csn2[b][k] = make_double2(fc[j], q < n ? fs[j] : 0.0);
+          off2[b][k] = make_int2(p * rows, q * rows);
+        }
+      }
+    };
     for (int e = tid; e < rows * n; e += NT) {
       const int row = e % rows, col = e / rows;
       ch[row + col * rows] = C[r0 + row + (size_t)col * n];
     }
-    stage(0);
+    fetch(0);
+    put(0);
+    if (nround > 1) fetch(1);
     __syncthreads();
-    const int stepk = NT / rows, stepr = NT % rows;
+    const int kstep = NT / rows, row = tid % rows, slot = tid / rows;
     for (int r = 0; r < nround; ++r) {
-      if (r + 1 < nround) stage(r + 1);
+      if (r + 1 < nround) put(r + 1);
+      if (r + 2 < nround) fetch(r + 2);
       const int b = r & 1;
-      int k = tid / rows, row = tid % rows;
-      for (int e = tid; e < npair * rows; e += NT) {
-        const double s = sn2[b][k];
-        if (s != 0.0) {
-          const double c = cs2[b][k];
-          const int pq = pq2[b][k];
-          double* cp = ch + (pq & 0xffff) * rows + row;
-          double* cq = ch + (pq >> 16) * rows + row;
-          const double u = *cp, v = *cq;
-          *cp = c * u - s * v;
-          *cq = s * u + c * v;
+      if (slot < kstep) {
+        for (int k = slot; k < npair; k += kstep) {
+          const double2 a = csn2[b][k];
+          if (a.y != 0.0) {
+            const int2 o = off2[b][k];
+            double* cp = ch + o.x + row;
+            double* cq = ch + o.y + row;
+            const double u = *cp, v = *cq;
+            *cp = a.x * u - a.y * v;
+            *cq = a.y * u + a.x * v;
+          }
         }
-        k += stepk;
-        row += stepr;
-        if (row >= rows) { row -= rows; ++k; }
       }
       __syncthreads();
     }
     for (int e = tid; e < rows * n; e += NT) {
-      const int row = e % rows, col = e / rows;
-      C[r0 + row + (size_t)col * n] = ch[row + col * rows];
+      const int row2 = e % rows, col = e / rows;
+      C[r0 + row2 + (size_t)col * n] = ch[row2 + col * rows];
     }
     __syncthreads();
   }
@@ -428,12 +462,13 @@ __global__ __launch_bounds__(NT) void k_beig_lds(const BatchMol* __restrict__ mo
   const int np = (n + 1) & ~1;
   const int npair = np / 2;
   __shared__ double red[2 * (NT / 64)];
-  __shared__ double cs2[2][kMaxBatchN / 2], sn2[2][kMaxBatchN / 2];
-  __shared__ int pp[kMaxBatchN / 2], qq[kMaxBatchN / 2], pq2[2][kMaxBatchN / 2];
+  __shared__ double2 csn2[2][kMaxBatchN / 2];  // (c, s) of every pair; two buffers for the flush
+  __shared__ int4 pq4[kMaxBatchN / 2];         // p, q, tri(p), tri(q)
+  __shared__ int2 off2[2][kMaxBatchN / 2];     // the flush's column offsets
   __shared__ int rank[kMaxBatchN];
-  __shared__ int rotated;
-  double* cs = cs2[0];
-  double* sn = sn2[0];
+  __shared__ int rotated, nrot;
+  if (tid == 0) nrot = 0;
+  double2* csn = csn2[0];
   extern __shared__ double dyn[];
   double* A = dyn;                               // tri(np) packed lower triangle
   double* C = CLDS ? dyn + tri(np) : M.C;        // n x n, column-major
@@ -452,27 +487,30 @@ __global__ __launch_bounds__(NT) void k_beig_lds(const BatchMol* __restrict__ mo
     for (int e = tid; e < n * n; e += NT) C[e] = M.C[e];
   __syncthreads();
   if (warm) {
-    // T'(c, r) = sum_k F(r, k) C(k, c): four columns c per thread
+    // T'(c, r) = sum_k F(r, k) C(k, c): four columns c per thread (the
+    // columns past n repeat column n - 1 and are not stored)
     const int ncb = (n + 3) >> 2;
     for (int w = tid; w < n * ncb; w += NT) {
       const int r = w % n, c0 = (w / n) * 4;
       const int rb = tri(r);
+      const double* q0 = C + (size_t)c0 * n;
+      const double* q1 = C + (size_t)min(c0 + 1, n - 1) * n;
+      const double* q2 = C + (size_t)min(c0 + 2, n - 1) * n;
+      const double* q3 = C + (size_t)min(c0 + 3, n - 1) * n;
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-      const double* c_0 = C + (size_t)c0 * n;
-      const int nc = min(4, n - c0);
-      int kb = 0;  // tri(k)
+#pragma unroll 4
       for (int k = 0; k < n; ++k) {
-        const double f = A[k <= r ? rb + k : kb + r];
-        kb += k + 1;
-        s0 += f * c_0[k];
-        if (nc > 1) s1 += f * c_0[k + n];
-        if (nc > 2) s2 += f * c_0[k + 2 * n];
-        if (nc > 3) s3 += f * c_0[k + 3 * n];
+        const double f = A[k <= r ? rb + k : tri(k) + r];
+        s0 += f * q0[k];
+        s1 += f * q1[k];
+        s2 += f * q2[k];
+        s3 += f * q3[k];
       }
-      Tt[c0 + (size_t)r * n] = s0;
-      if (nc > 1) Tt[c0 + 1 + (size_t)r * n] = s1;
-      if (nc > 2) Tt[c0 + 2 + (size_t)r * n] = s2;
-      if (nc > 3) Tt[c0 + 3 + (size_t)r * n] = s3;
+      double* t = Tt + c0 + (size_t)r * n;
+      t[0] = s0;
+      if (c0 + 1 < n) t[1] = s1;
+      if (c0 + 2 < n) t[2] = s2;
+      if (c0 + 3 < n) t[3] = s3;
     }
     __syncthreads();
     // A(i, j) = sum_k C(k, i) T'(j, k), j <= i: rows i0..i0+3 per thread
@@ -480,20 +518,23 @@ __global__ __launch_bounds__(NT) void k_beig_lds(const BatchMol* __restrict__ mo
     for (int w = tid; w < n * nib; w += NT) {
       const int j = w % n, i0 = (w / n) * 4;
       if (j > i0 + 3) continue;
-      const int ni = min(4, n - i0);
-      const double* ci = C + (size_t)i0 * n;
+      const double* c0 = C + (size_t)i0 * n;
+      const double* c1 = C + (size_t)min(i0 + 1, n - 1) * n;
+      const double* c2 = C + (size_t)min(i0 + 2, n - 1) * n;
+      const double* c3 = C + (size_t)min(i0 + 3, n - 1) * n;
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll 4
       for (int k = 0; k < n; ++k) {
         const double t = Tt[j + (size_t)k * n];
-        s0 += ci[k] * t;
-        if (ni > 1) s1 += ci[k + n] * t;
-        if (ni > 2) s2 += ci[k + 2 * n] * t;
-        if (ni > 3) s3 += ci[k + 3 * n] * t;
+        s0 += c0[k] * t;
+        s1 += c1[k] * t;
+        s2 += c2[k] * t;
+        s3 += c3[k] * t;
       }
       if (j <= i0) A[tri(i0) + j] = s0;
-      if (ni > 1 && j <= i0 + 1) A[tri(i0 + 1) + j] = s1;
-      if (ni > 2 && j <= i0 + 2) A[tri(i0 + 2) + j] = s2;
-      if (ni > 3 && j <= i0 + 3) A[tri(i0 + 3) + j] = s3;
+      if (i0 + 1 < n && j <= i0 + 1) A[tri(i0 + 1) + j] = s1;
+      if (i0 + 2 < n && j <= i0 + 2) A[tri(i0 + 2) + j] = s2;
+      if (i0 + 3 < n && j <= i0 + 3) A[tri(i0 + 3) + j] = s3;
     }
   } else {
     for (int e = tid; e < n * n; e += NT) {
@@ -503,6 +544,47 @@ __global__ __launch_bounds__(NT) void k_beig_lds(const BatchMol* __restrict__ mo
   }
   __syncthreads();
   const int nblk = tri(npair);  // 2x2 blocks (k1 >= k2)
+  // the blocks of this thread, k1 | k2 << 8, kept in registers
+  constexpr int KB = NT >= 256 ? (kMaxBatchN / 2 * (kMaxBatchN / 2 + 1) / 2 + NT - 1) / NT : 1;
+  int blk[KB];
+  if constexpr (NT >= 256) {
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      const int b = tid + j * NT;
+      const int k1 = b < nblk ? tri_row(b) : 0;
+      blk[j] = b < nblk ? k1 | ((b - tri(k1)) << 8) : -1;
+    }
+  }
+  // A <- R A R^T on the block (pair k1 rows, pair k2 columns), R = [c -s; s c]
+  auto turn_block = [&](int k1, int k2) {
+    const double2 a1 = csn[k1], a2 = csn[k2];
+    if (a1.y == 0.0 && a2.y == 0.0) return;
+    const double c1 = a1.x, s1 = a1.y, c2 = a2.x, s2 = a2.y;
+    const int4 i1 = pq4[k1];
+    if (k1 == k2) {
+      const int ip = i1.z + i1.x, iq = i1.w + i1.y, ipq = i1.w + i1.x;
+      const double x = A[ip], y = A[ipq], v = A[iq];
+      // rows, then columns
+      const double x1 = c1 * x - s1 * y, y1 = s1 * x + c1 * y;
+      const double u1 = c1 * y - s1 * v, v1 = s1 * y + c1 * v;
+      A[ip] = c1 * x1 - s1 * u1;
+      A[iq] = s1 * y1 + c1 * v1;
+      A[ipq] = 0.0;
+    } else {
+      const int4 i2 = pq4[k2];
+      const int ixx = i1.x >= i2.x ? i1.z + i2.x : i2.z + i1.x;
+      const int iyx = i1.y >= i2.x ? i1.w + i2.x : i2.z + i1.y;
+      const int ixu = i1.x >= i2.y ? i1.z + i2.y : i2.w + i1.x;
+      const int iyu = i1.y >= i2.y ? i1.w + i2.y : i2.w + i1.y;
+      const double x = A[ixx], y = A[iyx], u = A[ixu], v = A[iyu];
+      const double x1 = c1 * x - s1 * y, y1 = s1 * x + c1 * y;
+      const double u1 = c1 * u - s1 * v, v1 = s1 * u + c1 * v;
+      A[ixx] = c2 * x1 - s2 * u1;
+      A[ixu] = s2 * x1 + c2 * u1;
+      A[iyx] = c2 * y1 - s2 * v1;
+      A[iyu] = s2 * y1 + c2 * v1;
+    }
+  };
   const int stepk = NT / n, stepr = NT % n;
   for (int sweep = 0; sweep < 60; ++sweep) {
     double off = 0.0, dg = 0.0;
@@ -524,17 +606,17 @@ __global__ __launch_bounds__(NT) void k_beig_lds(const BatchMol* __restrict__ mo
         double c = 1.0, s = 0.0;
         if (q < n) {
           const double app = A[tri(p) + p], aqq = A[tri(q) + q], apq = A[tri(q) + p];
-          if (fabs(apq) > skip && fabs(apq) > 1.0e-18 * sqrt(fabs(app * aqq))) {
+          // |a_pq| > 1e-18 sqrt|a_pp a_qq| compared as squares
+          if (fabs(apq) > skip && apq * apq > 1.0e-36 * fabs(app * aqq)) {
             const double theta = (aqq - app) / (2.0 * apq);
             const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-            c = 1.0 / sqrt(t * t + 1.0);
+            c = rsqrt(t * t + 1.0);
             s = t * c;
+            atomicAdd(&nrot, 1);
           }
         }
-        pp[k] = p;
-        qq[k] = q;
-        cs[k] = c;
-        sn[k] = s;
+        csn[k] = make_double2(c, s);
+        pq4[k] = make_int4(p, q, tri(p), tri(q));
         if (!CLDS) {
           lg[2 * (r * npair + k)] = c;
           lg[2 * (r * npair + k) + 1] = s;
@@ -542,45 +624,30 @@ __global__ __launch_bounds__(NT) void k_beig_lds(const BatchMol* __restrict__ mo
         }
       }
       __syncthreads();
-      // A <- R A R^T, R = [c -s; s c] on rows/columns (p, q) of every pair
-      for (int b = tid; b < nblk; b += NT) {
-        const int k1 = tri_row(b), k2 = b - tri(k1);
-        const double s1 = sn[k1], s2 = sn[k2];
-        if (s1 == 0.0 && s2 == 0.0) continue;
-        const double c1 = cs[k1], c2 = cs[k2];
-        const int p1 = pp[k1], q1 = qq[k1], p2 = pp[k2], q2 = qq[k2];
-        if (k1 == k2) {
-          const int ip = tri(p1) + p1, iq = tri(q1) + q1, ipq = tri(q1) + p1;
-          const double x = A[ip], y = A[ipq], v = A[iq];
-          // rows, then columns
-          const double x1 = c1 * x - s1 * y, y1 = s1 * x + c1 * y;
-          const double u1 = c1 * y - s1 * v, v1 = s1 * y + c1 * v;
-          A[ip] = c1 * x1 - s1 * u1;
-          A[iq] = s1 * y1 + c1 * v1;
-          A[ipq] = 0.0;
-        } else {
-          const int ixx = sym_at(p1, p2), iyx = sym_at(q1, p2), ixu = sym_at(p1, q2), iyu = sym_at(q1, q2);
-          const double x = A[ixx], y = A[iyx], u = A[ixu], v = A[iyu];
-          const double x1 = c1 * x - s1 * y, y1 = s1 * x + c1 * y;
-          const double u1 = c1 * u - s1 * v, v1 = s1 * u + c1 * v;
-          A[ixx] = c2 * x1 - s2 * u1;
-          A[ixu] = s2 * x1 + c2 * u1;
-          A[iyx] = c2 * y1 - s2 * v1;
-          A[iyu] = s2 * y1 + c2 * v1;
+      if constexpr (NT >= 256) {
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+          if (blk[j] < 0) break;
+          turn_block(blk[j] & 0xff, blk[j] >> 8);
+        }
+      } else {
+        for (int b = tid; b < nblk; b += NT) {
+          const int k1 = tri_row(b);
+          turn_block(k1, b - tri(k1));
         }
       }
       // C <- C R^T on columns (p, q)
       if (CLDS) {
         int k = tid / n, row = tid % n;
         for (int e = tid; e < npair * n; e += NT) {
-          const double s = sn[k];
-          if (s != 0.0) {
-            const double c = cs[k];
-            double* cp = C + (size_t)pp[k] * n + row;
-            double* cq = C + (size_t)qq[k] * n + row;
+          const double2 a = csn[k];
+          if (a.y != 0.0) {
+            const int4 iq = pq4[k];
+            double* cp = C + iq.x * n + row;
+            double* cq = C + iq.y * n + row;
             const double u = *cp, v = *cq;
-            *cp = c * u - s * v;
-            *cq = s * u + c * v;
+            *cp = a.x * u - a.y * v;
+            *cq = a.y * u + a.x * v;
           }
           k += stepk;
           row += stepr;
@@ -595,7 +662,7 @@ __global__ __launch_bounds__(NT) void k_beig_lds(const BatchMol* __restrict__ mo
     if (pending) {
       for (int e = tid; e < na; e += NT) park[e] = A[e];
       __syncthreads();
-      beig_flush_c<NT>(C, lg, n, np, dyn, room, cs2, sn2, pq2);
+      beig_flush_c<NT>(C, lg, n, np, dyn, room, csn2, off2);
       for (int e = tid; e < na; e += NT) A[e] = park[e];
       __syncthreads();
     }
@@ -621,15 +688,26 @@ __global__ __launch_bounds__(NT) void k_beig_lds(const BatchMol* __restrict__ mo
   if (!CLDS)
     for (int e = tid; e < n * n; e += NT) M.C[e] = Tt[e];
   __syncthreads();
-  // P = 2 C_occ C_occ^T, packed
+  // P = 2 C_occ C_occ^T, packed, with C_occ staged in LDS where it fits
+  const int nocc = M.nocc;
   const double* Cg = M.C;
+  if (n * nocc <= room) {
+    for (int e = tid; e < n * nocc; e += NT) dyn[e] = Cg[e];
+    Cg = dyn;
+    __syncthreads();
+  }
   for (int q = tid; q < tri(n); q += NT) {
     const int r = tri_row(q), c = q - tri(r);
     double s = 0.0;
-    for (int k = 0; k < M.nocc; ++k) s += Cg[r + k * n] * Cg[c + k * n];
+#pragma unroll 4
+    for (int k = 0; k < nocc; ++k) s += Cg[r + k * n] * Cg[c + k * n];
     M.p[q] = 2.0 * s;
   }
-  if (tid == 0) st[m].have_c = 1;
+  if (tid == 0) {
+    st[m].eig_rot += nrot;
+    st[m].eig_flops += beig_flops(n, M.nocc, nrot, warm);
+    st[m].have_c = 1;
+  }
 }
 
 // ---- control, part 3: PL, frontier levels, newdg (iter.F90:1030-1041) -------
@@ -694,6 +772,9 @@ struct mhip_batch {
   std::vector<BatchState> init_state;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
+  // eager timing of the eigensolver launches per size class
+  hipEvent_t eig_ev[4][2] = {};
+  bool eig_timing = false;
   int graph_iters = 0;
 };
 
@@ -799,7 +880,9 @@ void one_iteration(mhip_batch* b) {
   MHIP_LAUNCH_CHECK();
   int off = 0;
   for (int c = 0; c < 4; ++c) {
+    if (b->eig_timing) MHIP_CHECK(hipEventRecord(b->eig_ev[c][0], s));
     launch_eig_class(b, c, off, b->cls_count[c]);
+    if (b->eig_timing) MHIP_CHECK(hipEventRecord(b->eig_ev[c][1], s));
     off += b->cls_count[c];
   }
   cnvg_batch(s, b->cnv.as<BatchCnvg>(), nm, b->n_row_blk, b->row_mol.as<int>(), b->row_idx.as<int>(),
@@ -1083,20 +1166,68 @@ int mhip_batch_set_eig_lds(mhip_batch* b, int lds) {
   });
 }
 
+// the per-molecule state and the starting densities, before a run
+static void reset_batch(mhip_batch* b) {
+  hipStream_t s = b->s;
+  MHIP_CHECK(hipMemcpyAsync(b->st.ptr, b->init_state.data(), sizeof(BatchState) * b->nmol,
+                            hipMemcpyHostToDevice, s));
+  MHIP_CHECK(hipMemcpyAsync(b->p.ptr, b->p0.data(), sizeof(double) * b->p0.size(), hipMemcpyHostToDevice, s));
+  MHIP_CHECK(hipMemcpyAsync(b->pa.ptr, b->pa0.data(), sizeof(double) * b->pa0.size(), hipMemcpyHostToDevice, s));
+  MHIP_CHECK(hipMemcpyAsync(b->pold.ptr, b->pa0.data(), sizeof(double) * b->pa0.size(), hipMemcpyHostToDevice, s));
+  MHIP_CHECK(hipMemcpyAsync(b->p1.ptr, b->p1h.data(), sizeof(double) * b->p1h.size(), hipMemcpyHostToDevice, s));
+  MHIP_CHECK(hipStreamSynchronize(s));
+}
+
+int mhip_batch_eig_profile(mhip_batch* b, int iters, double* ms, double* flops, long long* rotations) {
+  return bguard([&] {
+    MHIP_REQUIRE(b && iters > 0 && ms && flops, "mhip_batch_eig_profile: bad arguments");
+    hipStream_t s = b->s;
+    reset_batch(b);
+    for (int c = 0; c < 4; ++c) {
+      ms[c] = 0.0;
+      flops[c] = 0.0;
+      if (rotations) rotations[c] = 0;
+      for (int e = 0; e < 2; ++e) MHIP_CHECK(hipEventCreate(&b->eig_ev[c][e]));
+    }
+    b->eig_timing = true;
+    struct Off {
+      mhip_batch* b;
+      ~Off() {
+        b->eig_timing = false;
+        for (auto& c : b->eig_ev)
+          for (auto& e : c)
+            if (e) {
+              (void)hipEventDestroy(e);
+              e = nullptr;
+            }
+      }
+    } off{b};
+    for (int it = 0; it < iters; ++it) {
+      one_iteration(b);
+      MHIP_CHECK(hipStreamSynchronize(s));
+      for (int c = 0; c < 4; ++c) {
+        float t = 0.0f;
+        MHIP_CHECK(hipEventElapsedTime(&t, b->eig_ev[c][0], b->eig_ev[c][1]));
+        ms[c] += t;
+      }
+    }
+    std::vector<BatchState> hs(b->nmol);
+    MHIP_CHECK(hipMemcpy(hs.data(), b->st.ptr, sizeof(BatchState) * b->nmol, hipMemcpyDeviceToHost));
+    for (int m = 0; m < b->nmol; ++m) {
+      const int c = size_class(b->n_h[m]);
+      flops[c] += hs[m].eig_flops;
+      if (rotations) rotations[c] += hs[m].eig_rot;
+    }
+  });
+}
+
 // Run the batched SCF from the starting densities: capture `iters`
 // iterations once into a graph, replay it until every molecule is done.
 int mhip_batch_run(mhip_batch* b, int iters_per_graph, int max_launches, double* seconds, int* n_converged) {
   return bguard([&] {
     MHIP_REQUIRE(b && iters_per_graph > 0 && max_launches > 0, "mhip_batch_run: bad arguments");
     hipStream_t s = b->s;
-    // reset the per-molecule state and the starting densities (outside the graph)
-    MHIP_CHECK(hipMemcpyAsync(b->st.ptr, b->init_state.data(), sizeof(BatchState) * b->nmol,
-                              hipMemcpyHostToDevice, s));
-    MHIP_CHECK(hipMemcpyAsync(b->p.ptr, b->p0.data(), sizeof(double) * b->p0.size(), hipMemcpyHostToDevice, s));
-    MHIP_CHECK(hipMemcpyAsync(b->pa.ptr, b->pa0.data(), sizeof(double) * b->pa0.size(), hipMemcpyHostToDevice, s));
-    MHIP_CHECK(hipMemcpyAsync(b->pold.ptr, b->pa0.data(), sizeof(double) * b->pa0.size(), hipMemcpyHostToDevice, s));
-    MHIP_CHECK(hipMemcpyAsync(b->p1.ptr, b->p1h.data(), sizeof(double) * b->p1h.size(), hipMemcpyHostToDevice, s));
-    MHIP_CHECK(hipStreamSynchronize(s));
+    reset_batch(b);
     if (!b->exec || b->graph_iters != iters_per_graph) {
       if (b->exec) {
         (void)hipGraphExecDestroy(b->exec);

@@ -143,6 +143,7 @@ SIGNATURES = {
     "mhip_batch_create": (c_int, [c_int, c_void_p, ctypes.POINTER(c_void_p)]),
     "mhip_batch_set_tiles": (c_int, [c_void_p, c_void_p]),
     "mhip_batch_set_eig_lds": (c_int, [c_void_p, c_int]),
+    "mhip_batch_eig_profile": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "mhip_batch_run": (c_int, [c_void_p, c_int, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_int)]),
     "mhip_batch_results": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhip_batch_density": (c_int, [c_void_p, c_int, c_void_p]),

@@ -66,3 +66,26 @@ def test_lds_and_global_eigensolver_forms_agree(small_batch):
     assert np.max(np.abs(eg - el)) < 1e-6
     for m in range(len(systems)):
         assert np.max(np.abs(b.density(m) - pg[m])) < 1e-7, (m, norbs[m])
+
+
+def test_eigensolver_profile_runs_the_same_iterations(small_batch):
+    """The eager, event-timed pass (mhip_batch_eig_profile) runs the same
+    iterations as the graph: the same heats and iteration counts, time and
+    flops in every populated size class, and no flops where no rotation ran."""
+    systems, b = small_batch
+    b.set_tiles([64, 128, 256, 256])
+    b.run(iters_per_graph=20, max_launches=20)
+    e1, n1, _ = b.results()
+    ms, fl, rot = b.eig_profile(20)
+    e2, n2, c2 = b.results()
+    assert c2.all() and np.array_equal(n1, n2)
+    assert np.max(np.abs(e1 - e2)) < 1e-9
+    norbs = np.array([s.norbs for s in systems])
+    for c, (lo, hi) in enumerate([(0, 32), (33, 64), (65, 128), (129, 256)]):
+        members = (norbs >= lo) & (norbs <= hi)
+        if members.any():
+            assert ms[c] > 0 and fl[c] > 0 and rot[c] > 0
+            # at least the density and 12 n per rotation of every molecule
+            assert fl[c] >= 12.0 * rot[c] * norbs[members].min()
+        else:
+            assert fl[c] == 0 and rot[c] == 0
