@@ -335,7 +335,12 @@ def run_helix(args):
     form's.  The conventional
     FP64 SCF of the same helix is run once beside them (--no-conventional
     skips it): the difference of the heats is MOZYME's cutoff approximation,
-    not an error of either solver."""
+    not an error of either solver.  Beside it the conventional fast form
+    (block pseudo-diagonalization, split-BF16 while far from convergence)
+    runs from the standard guess and from the learned H, C, N, O density
+    (guess_hcno; its construction timed with the SCF), and the plain form
+    from the learned density: iterations and time against the standard
+    guess."""
     import ctypes
     from mopac_gpu_amd import geometry, hamiltonian, mozyme
     from mopac_gpu_amd import lib as L
@@ -426,6 +431,8 @@ def run_helix(args):
         rc = c2.iter(scfcrt=1e-6)
         conv = {"seconds": round(rc["seconds"], 3), "iterations": rc["niter"], "escf": round(rc["escf"], 6),
                 "mozyme_minus_conventional_kcal": round(best["escf"] - rc["escf"], 6)}
+        progress(f"conventional plain FP64: {rc['niter']} iterations, {rc['seconds']:.3f} s")
+        conv["learned_guess"] = helix_learned_guess(c2, s, nat, xyz, rc)
         c2.close()
     out = {
         "metric": "MOZYME SCF time-to-converge (s), poly-glycine alpha helix",
@@ -538,6 +545,36 @@ def run_batch(args):
                 "profiles/batch4096_kernels.txt",
     }
     print(json.dumps(out), flush=True)
+
+
+def helix_learned_guess(ctx, s, nat, xyz, plain):
+    """The helix's conventional SCF from the standard guess and from the
+    learned H, C, N, O density, in the fast form and the plain form; the
+    learned guess's construction (host descriptors, device network, host
+    assembly) is counted in its time."""
+    from mopac_gpu_amd import geometry, guess_hcno
+    model = guess_hcno.load_default()
+    if model is None:
+        return None
+    frag = geometry.fragments(nat, xyz)
+    fast = dict(scfcrt=1e-6, adaptive_eigen=True, precision=1, pseudo_diag=2, block_start=2, fragment=frag)
+    rs = ctx.iter(**fast)
+    progress(f"conventional fast form, standard guess: {rs['niter']} iterations, {rs['seconds']:.3f} s")
+    t0 = time.perf_counter()
+    p0 = model.predict_density(s)
+    t_guess = time.perf_counter() - t0
+    rl = ctx.iter(p_init=p0, **fast)
+    progress(f"conventional fast form, learned guess: {rl['niter']} iterations, "
+             f"{rl['seconds'] + t_guess:.3f} s (guess {t_guess:.3f} s)")
+    rp = ctx.iter(scfcrt=1e-6, p_init=p0)
+    progress(f"conventional plain FP64, learned guess: {rp['niter']} iterations, {rp['seconds'] + t_guess:.3f} s")
+    return {"guess_seconds": round(t_guess, 3),
+            "fast_standard": {"seconds": round(rs["seconds"], 3), "iterations": rs["niter"]},
+            "fast_learned": {"seconds": round(rl["seconds"] + t_guess, 3), "iterations": rl["niter"],
+                             "dHf_vs_plain_kcal": round(abs(rl["escf"] - plain["escf"]), 7)},
+            "plain_standard": {"seconds": round(plain["seconds"], 3), "iterations": plain["niter"]},
+            "plain_learned": {"seconds": round(rp["seconds"] + t_guess, 3), "iterations": rp["niter"],
+                              "dHf_vs_plain_kcal": round(abs(rp["escf"] - plain["escf"]), 7)}}
 
 
 def helix_cpu_baseline(ms, niter, scfcrt):

@@ -131,8 +131,10 @@ typedef struct {
   const double* pdiag; /* starting diagonal (moldat.F90:731-790), n */
   const double* p_init;/* optional starting total density (mpack), overrides pdiag */
   int block_start;     /* with pseudo_diag == 2: 0 = block steps once newdg holds; 1 = whenever
-                          eigenvectors exist; 2 = also take the first eigenvectors from the
-                          fragment blocks of F (needs fragment and pdiag) */
+                          eigenvectors exist and came from a supplied starting density or the
+                          fragment blocks (from the standard diagonal guess alone, once newdg
+                          holds); 2 = also take the first eigenvectors from the fragment
+                          blocks of F (needs fragment and pdiag) */
   const int* fragment; /* optional atom -> fragment id (0-based, e.g. the molecules of a
                           cluster; <= 16 orbitals and an even electron count each) */
   int start_from_guess;/* 1 = start from the density mhip_guess_water left on the device

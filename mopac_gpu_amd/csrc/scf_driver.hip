@@ -606,8 +606,15 @@ void run_scf(mhip_ctx* c, const mhip_scf_opts* o, mhip_scf_result* r, double* p_
     // FP64 for the iterations that decide convergence
     const bool split_now = o->precision == 1 && pl > 100.0 * pltest;
     n_split += split_now ? 1 : 0;
-    const bool use_pseudo =
-        o->pseudo_diag && (newdg || (o->pseudo_diag == 2 && o->block_start >= 1)) && c->have_c && !okpuly;
+    // block steps before newdg only from good first eigenvectors: the
+    // fragment blocks' or those of a supplied starting density.  From the
+    // standard diagonal guess the first diagonalization is far from the
+    // answer and first-order occupied-virtual steps from it need not settle,
+    // so those wait for newdg (PL < TRANS), as iter's pseudo-diagonalization
+    // does (iter.F90:1039).
+    const bool early_blocks = o->pseudo_diag == 2 && o->block_start >= 1 &&
+                              (nfragdiag > 0 || o->start_from_guess || o->p_init != nullptr);
+    const bool use_pseudo = o->pseudo_diag && (newdg || early_blocks) && c->have_c && !okpuly;
     if (!c->have_c && nfrag > 0) {
       // fragment start: block-diagonal eigenvectors of F's fragment blocks
       c->ov.ensure(n, nclose, c->npad);

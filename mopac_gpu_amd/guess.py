@@ -267,14 +267,14 @@ def layer_dims():
     return [NIN] + [NHID] * NLAYERS + [NOUT]
 
 
-def n_params():
-    d = layer_dims()
+def n_params(dims=None):
+    d = dims or layer_dims()
     return sum(d[i] * d[i + 1] + d[i + 1] for i in range(len(d) - 1))
 
 
-def init_params(seed=0):
+def init_params(seed=0, dims=None):
     rng = np.random.default_rng(seed)
-    d = layer_dims()
+    d = dims or layer_dims()
     parts = []
     for i in range(len(d) - 1):
         w = rng.standard_normal((d[i], d[i + 1])) * np.sqrt(1.0 / d[i])
@@ -285,12 +285,13 @@ def init_params(seed=0):
 class MLP:
     """Handle on the device MLP (mhip_mlp_*)."""
 
-    def __init__(self, params=None, seed=0):
+    def __init__(self, params=None, seed=0, dims=None):
         self.lib = L.load()
         self.h = ctypes.c_void_p()
-        dims = np.array(layer_dims(), dtype=np.int32)
-        L.check(self.lib.mhip_mlp_create(len(dims) - 1, L.ptr(dims), ctypes.byref(self.h)))
-        self.set_params(init_params(seed) if params is None else params)
+        self.dims = list(dims or layer_dims())
+        d = np.array(self.dims, dtype=np.int32)
+        L.check(self.lib.mhip_mlp_create(len(d) - 1, L.ptr(d), ctypes.byref(self.h)))
+        self.set_params(init_params(seed, self.dims) if params is None else params)
 
     def close(self):
         if self.h:
@@ -299,23 +300,23 @@ class MLP:
 
     def set_params(self, params):
         params = np.ascontiguousarray(params, dtype=np.float32)
-        if params.size != n_params():
-            raise ValueError(f"MLP parameter vector has {params.size} values, the network {n_params()}")
+        if params.size != n_params(self.dims):
+            raise ValueError(f"MLP parameter vector has {params.size} values, the network {n_params(self.dims)}")
         L.check(self.lib.mhip_mlp_set_params(self.h, L.ptr(params)))
 
     def get_params(self):
-        out = np.zeros(n_params(), dtype=np.float32)
+        out = np.zeros(n_params(self.dims), dtype=np.float32)
         L.check(self.lib.mhip_mlp_get_params(self.h, L.ptr(out)))
         return out
 
     def _check_x(self, x):
-        if x.ndim != 2 or x.shape[1] != NIN or x.shape[0] == 0:
-            raise ValueError(f"MLP input must be (batch > 0, {NIN}), got {x.shape}")
+        if x.ndim != 2 or x.shape[1] != self.dims[0] or x.shape[0] == 0:
+            raise ValueError(f"MLP input must be (batch > 0, {self.dims[0]}), got {x.shape}")
 
     def forward(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)
         self._check_x(x)
-        y = np.zeros((x.shape[0], NOUT), dtype=np.float32)
+        y = np.zeros((x.shape[0], self.dims[-1]), dtype=np.float32)
         L.check(self.lib.mhip_mlp_forward(self.h, x.shape[0], L.ptr(x), L.ptr(y)))
         return y
 
@@ -323,9 +324,9 @@ class MLP:
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
         self._check_x(x)
-        if t.shape != (x.shape[0], NOUT):
-            raise ValueError(f"MLP targets must be ({x.shape[0]}, {NOUT}), got {t.shape}")
-        g = np.zeros(n_params(), dtype=np.float32)
+        if t.shape != (x.shape[0], self.dims[-1]):
+            raise ValueError(f"MLP targets must be ({x.shape[0]}, {self.dims[-1]}), got {t.shape}")
+        g = np.zeros(n_params(self.dims), dtype=np.float32)
         loss = ctypes.c_float()
         L.check(self.lib.mhip_mlp_gradients(self.h, x.shape[0], L.ptr(x), L.ptr(t), L.ptr(g),
                                             ctypes.byref(loss)))
@@ -335,8 +336,8 @@ class MLP:
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
         self._check_x(x)
-        if t.shape != (x.shape[0], NOUT):
-            raise ValueError(f"MLP targets must be ({x.shape[0]}, {NOUT}), got {t.shape}")
+        if t.shape != (x.shape[0], self.dims[-1]):
+            raise ValueError(f"MLP targets must be ({x.shape[0]}, {self.dims[-1]}), got {t.shape}")
         loss = ctypes.c_float()
         L.check(self.lib.mhip_mlp_train_step(self.h, x.shape[0], L.ptr(x), L.ptr(t), ctypes.c_float(lr),
                                              ctypes.byref(loss)))

@@ -119,3 +119,68 @@ def test_guess_falls_back_for_non_water():
     nat, xyz = geometry.alkane(4, 1)
     s = hamiltonian.build(nat, xyz)
     assert model.predict_density(s) is None
+
+
+def test_gradients_match_reference_hcno_dims():
+    """The same kernels at the H, C, N, O model's layer sizes (168 inputs)."""
+    from mopac_gpu_amd import guess_hcno as H
+    dims = H.layer_dims()
+    rng = np.random.default_rng(15)
+    params = guess.init_params(3, dims)
+    m = guess.MLP(params, dims=dims)
+    try:
+        x = rng.standard_normal((150, dims[0])).astype(np.float32)
+        t = rng.standard_normal((150, dims[-1])).astype(np.float32)
+        g, loss = m.gradients(x, t)
+        loss_ref, g_ref = mlp_ref.loss_and_grad(params, dims, x, t)
+        assert abs(loss - loss_ref) < 1e-3 * loss_ref
+        assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 1e-2
+        y = m.forward(x)
+        y_ref = mlp_ref.forward(params, dims, x)[-1]
+        assert np.sqrt(np.mean((y - y_ref) ** 2)) < 2e-3 * np.max(np.abs(y_ref))
+    finally:
+        m.close()
+
+
+@pytest.mark.parametrize("name,make", [("Gly12 helix", lambda: geometry.glycine_helix(12, 902)),
+                                       ("(H2O)20", lambda: geometry.water_cluster(20, 904))])
+def test_hcno_guess_saves_iterations(gpu_ctx, name, make):
+    """The H, C, N, O model on held-out systems (seeds outside its training
+    stream): the plain FP64 SCF from the learned density takes fewer
+    iterations than from the standard guess and reaches the same answer
+    (heat within 1e-3 kcal/mol at SCFCRT 1e-6; densities within 1e-6 once
+    both are converged to 1e-12: at 1e-10 each run stops once its density
+    changes by less than PLTEST = 5e-7, so the two stop up to ~1.5e-6 apart)."""
+    from mopac_gpu_amd import guess_hcno as H
+    model = H.load_default()
+    if model is None:
+        pytest.skip("no trained model at mopac_gpu_amd/models/hcno_guess.npz (python -m mopac_gpu_amd.train_guess hcno)")
+    nat, xyz = make()
+    s = hamiltonian.build(nat, xyz)
+    gpu_ctx.upload(s)
+    p0 = model.predict_density(s)
+    d = np.arange(s.norbs) * (np.arange(s.norbs) + 1) // 2 + np.arange(s.norbs)
+    assert abs(p0[d].sum() - 2 * s.nclose) < 1e-6
+    r0 = gpu_ctx.iter(scfcrt=1e-6)
+    r1 = gpu_ctx.iter(scfcrt=1e-6, p_init=p0)
+    print(f"{name}: standard guess {r0['niter']} iterations, learned {r1['niter']}", flush=True)
+    assert r0["converged"] and r1["converged"]
+    assert r1["niter"] < r0["niter"]
+    assert abs(r1["escf"] - r0["escf"]) < 1e-3
+    t0 = gpu_ctx.iter(scfcrt=1e-12)
+    t1 = gpu_ctx.iter(scfcrt=1e-12, p_init=p0)
+    assert t0["converged"] and t1["converged"]
+    assert np.max(np.abs(t1["p"] - t0["p"])) < 1e-6
+
+
+def test_hcno_guess_falls_back_for_other_elements():
+    from mopac_gpu_amd import guess_hcno as H
+    model = H.load_default()
+    if model is None:
+        pytest.skip("no trained model at mopac_gpu_amd/models/hcno_guess.npz")
+    assert not H.supported(np.array([8, 1, 1, 16]))
+
+    class S:  # a system with a sulfur atom
+        nat = np.array([16, 1, 1])
+        xyz = np.zeros((3, 3))
+    assert model.predict_density(S()) is None

@@ -269,3 +269,96 @@ def _dense(packed, n):
         m[i, :i + 1] = packed[k:k + i + 1]
         k += i + 1
     return m + np.tril(m, -1).T
+
+
+# ---- the H, C, N, O learned guess (guess_hcno): host logic ---------------------
+
+def _hcno_qtab():
+    from mopac_gpu_amd import guess_hcno as H
+    m = H.load_default()
+    if m is not None:
+        return m.qtab
+    return np.array([0, 1], dtype=np.int64), np.array([0.1, -0.2])
+
+
+@pytest.mark.parametrize("make", [lambda: geometry.glycine_helix(4, 5), lambda: geometry.water_cluster(6, 2),
+                                  lambda: geometry.alkane(5, 3), lambda: geometry.water_cluster(1, 2)])
+def test_hcno_descriptors_turn_with_the_molecule(make):
+    """The same pairs in the same order and the same descriptors (frames,
+    bonded shells and the long-range field are all taken in atom frames)
+    after a rotation and a shift, including symmetric groups with equal bonds."""
+    from scipy.spatial.transform import Rotation
+    from mopac_gpu_amd import guess_hcno as H
+    nat, xyz = make()
+    qtab = _hcno_qtab()
+    p1 = H.prepare(nat, xyz, qtab)
+    for seed in range(3):
+        Rm = Rotation.random(random_state=seed).as_matrix()
+        p2 = H.prepare(nat, (xyz - xyz.mean(0)) @ Rm.T + np.array([1.0, -2.0, 0.5]), qtab)
+        assert np.array_equal(p1["a"], p2["a"]) and np.array_equal(p1["b"], p2["b"])
+        assert np.max(np.abs(p1["X"] - p2["X"])) < 1e-9
+        assert np.array_equal(p1["typ"], p2["typ"])
+
+
+def test_hcno_blocks_round_trip():
+    """Blocks taken from any symmetric density in the atom frames assemble
+    back to it on every covered slot (pairs within 4 A and self pairs), and
+    to zero beyond."""
+    from mopac_gpu_amd import guess_hcno as H
+    nat, xyz = geometry.glycine_helix(3, 7)
+    s = hamiltonian.build(nat, xyz)
+    p = np.random.default_rng(0).standard_normal(s.mpack)
+    prep = H.prepare(nat, xyz)
+    back = H.assemble(s, H.blocks_from_density(s, p, prep), prep, renormalize=False)
+    covered = back != 0
+    assert np.max(np.abs(back[covered] - p[covered])) < 1e-12
+    # the covered slots are exactly the orbital pairs of atoms within 4 A
+    nf, nl = s.nfirst - 1, s.nlast
+    atom = np.concatenate([np.full(nl[i] - nf[i], i) for i in range(len(nat))])
+    r, c = np.tril_indices(s.norbs)
+    near = np.linalg.norm(xyz[atom[r]] - xyz[atom[c]], axis=1) < H.RCUT
+    assert np.array_equal(covered[r * (r + 1) // 2 + c], near)
+
+
+def test_hcno_renormalized_guess_holds_the_valence_electrons():
+    from mopac_gpu_amd import guess_hcno as H
+    nat, xyz = geometry.glycine_helix(3, 7)
+    s = hamiltonian.build(nat, xyz)
+    prep = H.prepare(nat, xyz)
+    y = np.abs(np.random.default_rng(1).standard_normal((len(prep["a"]), 16)))
+    p = H.assemble(s, y, prep)
+    d = np.arange(s.norbs) * (np.arange(s.norbs) + 1) // 2 + np.arange(s.norbs)
+    assert abs(p[d].sum() - 2 * s.nclose) < 1e-9
+
+
+def test_hcno_field_of_point_charges_closed_form():
+    """Potential q/r and field q (r_i - r_j) / r^3 of charges beyond 4 A,
+    expressed in each atom's frame; nothing from atoms within 4 A."""
+    from mopac_gpu_amd import guess_hcno as H
+    xyz = np.array([[0.0, 0.0, 0.0], [5.0, 0.0, 0.0], [0.0, 3.0, 0.0]])
+    q = np.array([0.3, -0.5, 0.7])
+    R = np.tile(np.eye(3), (3, 1, 1))
+    R[0] = np.array([[0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [1.0, 0.0, 0.0]])
+    f = H.field_descriptors(xyz, R, q)
+    assert abs(f[0, 0] - (-0.5 / 5.0)) < 1e-14
+    e0 = -0.5 * (xyz[0] - xyz[1]) / 125.0
+    assert np.allclose(f[0, 1:], R[0] @ e0, atol=1e-14)
+    r12 = np.linalg.norm(xyz[1] - xyz[2])
+    assert abs(f[1, 0] - (0.3 / 5.0 + 0.7 / r12)) < 1e-14
+    assert abs(f[2, 0] - (-0.5 / r12)) < 1e-14  # atom 0 is 3 A away: excluded
+
+
+def test_hcno_charges_sum_to_the_net_charge():
+    from mopac_gpu_amd import guess_hcno as H
+    nat, xyz = geometry.water_cluster(2, 1)
+    s = hamiltonian.build(nat, xyz)
+    p = np.zeros(s.mpack)
+    d = np.arange(s.norbs) * (np.arange(s.norbs) + 1) // 2 + np.arange(s.norbs)
+    p[d] = 2.0 * s.nclose / s.norbs
+    q = H.charges_from_density(s, p)
+    assert abs(q.sum()) < 1e-12
+    nbrs, _ = H.bonded_neighbours(nat, xyz)
+    t = H.atom_types(nat, nbrs)
+    # water: O with two bonded H; H with one bonded O
+    assert len(set(t[nat == 8])) == 1 and len(set(t[nat == 1])) == 1
+    assert t[nat == 8][0] == 3 + 4 * 2 and t[nat == 1][0] == 0 + 4 * 125
