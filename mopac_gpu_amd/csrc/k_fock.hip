@@ -43,8 +43,10 @@ __device__ __forceinline__ double finit(const FockArgs& a, long long q, bool dia
   }
 }
 
-// Row pass for atom i (one 256-thread workgroup); shared by the single-system
-// and the batched launches.
+// Row pass for atom i (one NT-thread workgroup); shared by the single-system
+// (256 threads) and the batched (64 threads: a small molecule's atom has few
+// partners) launches.
+template <int NT>
 __device__ void fock_rows_body(const FockArgs& a, const int i) {
   const int ni = a.sys.norb[i];
   const long long fi = a.sys.nf0[i];
@@ -73,7 +75,7 @@ __device__ void fock_rows_body(const FockArgs& a, const int i) {
   const int nlight = a.sys.nlight_before[i];
 
   // ---- heavy partners -------------------------------------------------------
-  for (int t = threadIdx.x; t < nheavy; t += blockDim.x) {
+  for (int t = threadIdx.x; t < nheavy; t += NT) {
     const int j = a.sys.heavy[t];
     const long long fj = a.sys.nf0[j];
     double pj2[10];
@@ -151,7 +153,7 @@ __device__ void fock_rows_body(const FockArgs& a, const int i) {
   }
 
   // ---- hydrogen partners ----------------------------------------------------
-  for (int t = threadIdx.x; t < nlight; t += blockDim.x) {
+  for (int t = threadIdx.x; t < nlight; t += NT) {
     const int j = a.sys.light[t];
     const long long fj = a.sys.nf0[j];
     const double pjj = P[pk(fj, fj)];
@@ -187,20 +189,33 @@ __device__ void fock_rows_body(const FockArgs& a, const int i) {
   }
 
   // ---- reduce the Coulomb field on atom i over the workgroup -----------------
-  __shared__ double red[10][256];
+  if constexpr (NT == 64) {
+    // one wave: lane exchange, fixed order
 #pragma unroll
-  for (int ab = 0; ab < 10; ++ab) red[ab][threadIdx.x] = ji[ab];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
+    for (int ab = 0; ab < 10; ++ab)
+      for (int o = 32; o > 0; o >>= 1) ji[ab] += __shfl_xor(ji[ab], o);
+    if (threadIdx.x == 0)
+      for (int ab = 0; ab < limi; ++ab) a.jrow[(long long)i * 10 + ab] = ji[ab];
+  } else {
+    __shared__ double red[10][NT];
 #pragma unroll
-      for (int ab = 0; ab < 10; ++ab) red[ab][threadIdx.x] += red[ab][threadIdx.x + w];
+    for (int ab = 0; ab < 10; ++ab) red[ab][threadIdx.x] = ji[ab];
     __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w)
+#pragma unroll
+        for (int ab = 0; ab < 10; ++ab) red[ab][threadIdx.x] += red[ab][threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x < limi) a.jrow[(long long)i * 10 + threadIdx.x] = red[threadIdx.x][0];
   }
-  if (threadIdx.x < limi) a.jrow[(long long)i * 10 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ __launch_bounds__(256) void k_fock_rows(FockArgs a) { fock_rows_body(a, blockIdx.x); }
+// atoms in descending order: atom i has i partners, so the longest rows
+// start first and the short ones fill in behind them
+__global__ __launch_bounds__(256) void k_fock_rows(FockArgs a) {
+  fock_rows_body<256>(a, a.sys.numat - 1 - (int)blockIdx.x);
+}
 
 // Column pass for atom j (one 256-thread workgroup).
 __device__ void fock_cols_body(const FockArgs& a, const int j) {
@@ -404,7 +419,7 @@ __global__ void k_cnvg_offdiag(int n, double* __restrict__ pnew, double* __restr
 // ---- batched forms: one launch for many molecules --------------------------
 // blk_mol / blk_idx map each workgroup to (molecule, atom or row); molecules
 // whose flag in `active` is 0 (converged) do nothing.
-__global__ __launch_bounds__(256) void k_bfock_rows(const FockArgs* __restrict__ args,
+__global__ __launch_bounds__(64) void k_bfock_rows(const FockArgs* __restrict__ args,
                                                     const int* __restrict__ blk_mol,
                                                     const int* __restrict__ blk_idx,
                                                     const int* __restrict__ active,
@@ -415,7 +430,7 @@ __global__ __launch_bounds__(256) void k_bfock_rows(const FockArgs* __restrict__
   FockArgs a = args[m];
   a.init_mode = mode[m];
   a.shift = shift[m];
-  fock_rows_body(a, blk_idx[blockIdx.x]);
+  fock_rows_body<64>(a, blk_idx[blockIdx.x]);
 }
 
 __global__ __launch_bounds__(256) void k_bfock_cols(const FockArgs* __restrict__ args,
@@ -548,7 +563,7 @@ void cnvg(hipStream_t s, int n, double* pnew, double* pold, double* p1, int nite
 void fock_build_batch(hipStream_t s, const FockArgs* args, int nblk, const int* blk_mol, const int* blk_atom,
                       const int* active, const int* mode, const double* shift) {
   if (nblk <= 0) return;
-  hipLaunchKernelGGL(k_bfock_rows, dim3(nblk), dim3(256), 0, s, args, blk_mol, blk_atom, active, mode, shift);
+  hipLaunchKernelGGL(k_bfock_rows, dim3(nblk), dim3(64), 0, s, args, blk_mol, blk_atom, active, mode, shift);
   MHIP_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_bfock_cols, dim3(nblk), dim3(256), 0, s, args, blk_mol, blk_atom, active, mode, shift);
   MHIP_LAUNCH_CHECK();
