@@ -57,43 +57,6 @@ __device__ __forceinline__ int nij_of(const LmoDev& L, int a1, int b1) {  // 1-b
 }
 
 // ---------------------------------------------------------------- density
-__global__ __launch_bounds__(256) void k_mz_density(LmoDev L, double* __restrict__ p) {
-  const int i = blockIdx.x;
-  __shared__ int atoms[512];
-  __shared__ int coff[512];
-  const int nat_i = L.ncf[i], base = L.nncf[i], loop = L.ncocc[i];
-  if (threadIdx.x == 0) {
-    int o = 0;
-    for (int k = 0; k < nat_i && k < 512; ++k) {
-      const int a = L.icocc[base + k];
-      atoms[k] = a;
-      coff[k] = o;
-      o += L.iorbs[a - 1];
-    }
-  }
-  __syncthreads();
-  const int npair = nat_i * nat_i;
-  for (int e = threadIdx.x; e < npair; e += blockDim.x) {
-    const int jj = e / nat_i, kk = e % nat_i;
-    const int j = atoms[jj], k = atoms[kk];
-    if (j < k) continue;
-    const int l0 = nij_of(L, j, k);
-    if (l0 < 0) continue;
-    const int nj = L.iorbs[j - 1], nk = L.iorbs[k - 1];
-    const double* cj = L.cocc + loop + coff[jj];
-    const double* ck = L.cocc + loop + coff[kk];
-    if (j == k) {
-      int l = l0;
-      for (int j1 = 0; j1 < nj; ++j1)
-        for (int k1 = 0; k1 <= j1; ++k1) atomicAdd(p + l++, cj[j1] * ck[k1]);
-    } else {
-      int l = l0;
-      for (int j1 = 0; j1 < nj; ++j1)
-        for (int k1 = 0; k1 < nk; ++k1) atomicAdd(p + l++, cj[j1] * ck[k1]);
-    }
-  }
-}
-
 __global__ void k_mz_density_init(long long n, int mode, const double* __restrict__ partp,
                                    double* __restrict__ p) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
@@ -105,6 +68,154 @@ __global__ void k_scale(long long n, double s, double* __restrict__ p) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     p[i] *= s;
+}
+
+// The density (density_for_MOZYME.F90:16-342) as a gather, in the reference's
+// order: per atom the occupied LMOs that hold it, in LMO order; then one
+// thread per stored atom pair adds c_i(a) c_i(b)^T over the LMOs the two
+// atoms share, in LMO order, onto the starting value -- the order in which
+// density_for_MOZYME.F90 adds them, LMO after LMO -- and applies the spin
+// factor.  No atomics on P, and P is the same from run to run.
+__global__ void k_mz_fill_u64(long long n, unsigned long long* __restrict__ x) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] = 0ull;
+}
+
+// bit i of bits[a * nw + i / 64]: occupied LMO i holds atom a
+__global__ void k_mz_atom_bits(LmoDev L, int nw, unsigned long long* __restrict__ bits) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.nocc) return;
+  const int b0 = L.nncf[i], n = L.ncf[i];
+  const unsigned long long bit = 1ull << (i & 63);
+  for (int k = 0; k < n; ++k) atomicOr(bits + (size_t)(L.icocc[b0 + k] - 1) * nw + (i >> 6), bit);
+}
+
+// per atom: LMOs before each 64-LMO word (cum) and in all (cnt)
+__global__ void k_mz_atom_cum(int numat, int nw, const unsigned long long* __restrict__ bits,
+                              int* __restrict__ cum, int* __restrict__ cnt) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= numat) return;
+  int c = 0;
+  for (int w = 0; w < nw; ++w) {
+    cum[(size_t)a * nw + w] = c;
+    c += __popcll(bits[(size_t)a * nw + w]);
+  }
+  cnt[a] = c;
+}
+
+// exclusive scan of cnt[0..n) into off[0..n], one 1024-thread workgroup
+__global__ __launch_bounds__(1024) void k_mz_scan_counts(int n, const int* __restrict__ cnt, int* __restrict__ off) {
+  __shared__ int part[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < n; c0 += 1024) {
+    const int i = c0 + threadIdx.x;
+    const int v = i < n ? cnt[i] : 0;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int t = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+      __syncthreads();
+      part[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < n) off[i] = carry + part[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += part[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) off[n] = carry;
+}
+
+// each (LMO, atom) entry straight to its place in the atom's list, in LMO
+// order: the rank of LMO i among the atom's LMOs
+__global__ void k_mz_atom_fill(LmoDev L, int nw, const unsigned long long* __restrict__ bits,
+                               const int* __restrict__ cum, const int* __restrict__ off,
+                               int* __restrict__ ent_lmo, int* __restrict__ ent_coff) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.nocc) return;
+  const int b0 = L.nncf[i], n = L.ncf[i];
+  const unsigned long long below = (1ull << (i & 63)) - 1ull;
+  int coff = L.ncocc[i];
+  for (int k = 0; k < n; ++k) {
+    const int a = L.icocc[b0 + k] - 1;
+    const size_t wa = (size_t)a * nw + (i >> 6);
+    const int slot = off[a] + cum[wa] + __popcll(bits[wa] & below);
+    ent_lmo[slot] = i;
+    ent_coff[slot] = coff;
+    coff += L.iorbs[a];
+  }
+}
+
+__global__ void k_mz_density_gather(LmoDev L, int npairs, const int2* __restrict__ pairs,
+                                    const int* __restrict__ off, const int* __restrict__ ent_lmo,
+                                    const int* __restrict__ ent_coff, int mode,
+                                    const double* __restrict__ partp, double spinfa, int scale,
+                                    double* __restrict__ p) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs) return;
+  const int a = pairs[t].x, b = pairs[t].y;  // 0-based, a >= b
+  const long long l0 = L.nijbo[(size_t)a + (size_t)b * L.numat];
+  const int na = L.iorbs[a], nb = L.iorbs[b];
+  const bool diag = a == b;
+  // element (x, y) of the block: packed lower for the diagonal block,
+  // row-major na x nb otherwise
+  auto idx = [&](int x, int y) { return diag ? x * (x + 1) / 2 + y : x * nb + y; };
+  double acc[16];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      double v = 0.0;
+      if (x < na && y < nb && (!diag || y <= x) && mode != 0) {
+        const double q = partp[l0 + idx(x, y)];
+        v = mode == -1 ? -0.5 * q : 0.5 * q;
+      }
+      acc[x * 4 + y] = v;
+    }
+  auto add = [&](const double* ca, const double* cb) {
+    double u[4], w[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      u[x] = x < na ? ca[x] : 0.0;
+      w[x] = x < nb ? cb[x] : 0.0;
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+        if (x < na && y < nb && (!diag || y <= x)) acc[x * 4 + y] += u[x] * w[y];
+  };
+  int ia = off[a];
+  const int ea = off[a + 1];
+  if (diag) {
+    for (; ia < ea; ++ia) {
+      const double* c = L.cocc + ent_coff[ia];
+      add(c, c);
+    }
+  } else {
+    int ib = off[b];
+    const int eb = off[b + 1];
+    while (ia < ea && ib < eb) {
+      const int la = ent_lmo[ia], lb = ent_lmo[ib];
+      if (la < lb) {
+        ++ia;
+      } else if (lb < la) {
+        ++ib;
+      } else {
+        add(L.cocc + ent_coff[ia], L.cocc + ent_coff[ib]);
+        ++ia;
+        ++ib;
+      }
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+      if (x < na && y < nb && (!diag || y <= x))
+        p[l0 + idx(x, y)] = scale ? acc[x * 4 + y] * spinfa : acc[x * 4 + y];
 }
 
 // ---------------------------------------------------------------- diagg1
@@ -400,46 +511,83 @@ __global__ __launch_bounds__(256) void k_mz_diagg1_write(LmoDev L, const double*
   if (tid == 0) vsum[i] = V.red[0];
 }
 
-// Occupied LMO energies (diagg1.F90:484-640); one thread per occupied LMO.
-__global__ void k_mz_occ_energy(LmoDev L, const double* __restrict__ fao, const double* __restrict__ p,
-                                const double* __restrict__ aocc, double cutoff,
-                                double* __restrict__ eigs) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Occupied LMO energies (diagg1.F90:484-640); one wave per occupied LMO.
+// The reference's sum runs over (atom j, atom k, orbital jx) in that order;
+// lanes take 64 atoms k at a time and form each product c_j(jx) s1(k, jx)
+// with the reference's own arithmetic (s1 summed over k's orbitals in its
+// order), and lane 0 adds the products in the reference's order, so the
+// energy rounds exactly as the sequential loop does.
+__device__ __forceinline__ void mz_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(64) void k_mz_occ_energy(LmoDev L, const double* __restrict__ fao,
+                                                      const double* __restrict__ p,
+                                                      const double* __restrict__ aocc, double cutoff,
+                                                      double* __restrict__ eigs) {
+  const int i = blockIdx.x, lane = threadIdx.x;
   if (i >= L.nocc) return;
+  __shared__ double term[64 * 4];
+  __shared__ int tok[64];
   const int loopi = L.ncocc[i], b0 = L.nncf[i], nat_i = L.ncf[i];
-  double sum = 0.0;
+  double sum = 0.0;  // lane 0's running sum
   int jl = loopi;
   for (int j = b0; j < b0 + nat_i; ++j) {
     const int j1 = L.icocc[j], nj = L.iorbs[j1 - 1];
-    int kl = loopi;
-    for (int k = b0; k < b0 + nat_i; ++k) {
-      const int k1 = L.icocc[k], nk = L.iorbs[k1 - 1];
-      const int k1j1 = nij_of(L, k1, j1);
-      // avir(k1) of the reference here is the squared weight of atom k1 in LMO i
-      if (k1j1 >= 0) {
-        double wk = 0.0;
-        for (int x = 0; x < nk; ++x) wk += L.cocc[kl + x] * L.cocc[kl + x];
-        if (wk * p[k1j1] * aocc[k] >= cutoff) {
-          for (int jx = 1; jx <= nj; ++jx) {
-            double s1 = 0.0;
-            if (k1 > j1) {
-              for (int i4 = 1; i4 <= nk; ++i4) s1 += fao[k1j1 + (i4 - 1) * nj + jx - 1] * L.cocc[kl + i4 - 1];
-            } else if (k1 < j1) {
-              for (int i4 = 1; i4 <= nk; ++i4) s1 += fao[k1j1 + (jx - 1) * nk + i4 - 1] * L.cocc[kl + i4 - 1];
-            } else {
-              for (int j4 = 1; j4 <= jx; ++j4) s1 += fao[k1j1 + (jx * (jx - 1)) / 2 + j4 - 1] * L.cocc[kl + j4 - 1];
-              for (int i4 = jx + 1; i4 <= nk; ++i4)
-                s1 += fao[k1j1 + (i4 * (i4 - 1)) / 2 + jx - 1] * L.cocc[kl + i4 - 1];
+    int carry = loopi;  // coefficient offset of the chunk's first atom
+    for (int c0 = 0; c0 < nat_i; c0 += 64) {
+      const bool in = c0 + lane < nat_i;
+      const int k = b0 + c0 + lane;
+      const int k1 = in ? L.icocc[k] : 1;
+      const int nk = in ? L.iorbs[k1 - 1] : 0;
+      // exclusive prefix of the orbital counts over the chunk
+      int incl = nk;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+      }
+      const int kl = carry + incl - nk;
+      carry += __shfl(incl, 63);
+      int ok = 0;
+      if (in) {
+        const int k1j1 = nij_of(L, k1, j1);
+        // avir(k1) of the reference here is the squared weight of atom k1 in LMO i
+        if (k1j1 >= 0) {
+          double wk = 0.0;
+          for (int x = 0; x < nk; ++x) wk += L.cocc[kl + x] * L.cocc[kl + x];
+          if (wk * p[k1j1] * aocc[k] >= cutoff) {
+            ok = 1;
+            for (int jx = 1; jx <= nj; ++jx) {
+              double s1 = 0.0;
+              if (k1 > j1) {
+                for (int i4 = 1; i4 <= nk; ++i4) s1 += fao[k1j1 + (i4 - 1) * nj + jx - 1] * L.cocc[kl + i4 - 1];
+              } else if (k1 < j1) {
+                for (int i4 = 1; i4 <= nk; ++i4) s1 += fao[k1j1 + (jx - 1) * nk + i4 - 1] * L.cocc[kl + i4 - 1];
+              } else {
+                for (int j4 = 1; j4 <= jx; ++j4) s1 += fao[k1j1 + (jx * (jx - 1)) / 2 + j4 - 1] * L.cocc[kl + j4 - 1];
+                for (int i4 = jx + 1; i4 <= nk; ++i4)
+                  s1 += fao[k1j1 + (i4 * (i4 - 1)) / 2 + jx - 1] * L.cocc[kl + i4 - 1];
+              }
+              term[lane * 4 + jx - 1] = L.cocc[jl + jx - 1] * s1;
             }
-            sum += L.cocc[jl + jx - 1] * s1;
           }
         }
       }
-      kl += nk;
+      tok[lane] = ok;
+      mz_wave_sync();
+      if (lane == 0) {
+        const int cnt = min(64, nat_i - c0);
+        for (int kk = 0; kk < cnt; ++kk)
+          if (tok[kk])
+            for (int jx = 0; jx < nj; ++jx) sum += term[kk * 4 + jx];
+      }
+      mz_wave_sync();
     }
     jl += nj;
   }
-  eigs[i] = sum;
+  if (lane == 0) eigs[i] = sum;
 }
 
 // ---------------------------------------------------------------- diagg2
@@ -1342,6 +1490,9 @@ struct MzWork {
   DevBuf dkeep, dkc, dkq, dnn_new, dnq_new, derr, deoff, delen;
   DevBuf dp, dpart, dfao, daocc, dcounts, doffs, dnfmo, dold, dfmo, difmo, dvmax, dvsum, deigv, deigs, dorder,
       dused, dlat, dsj, dsi, dsumb, dnrej, dh, df, dout, dpn, d1, d2, d3, di, dsc, dlevel, drec;
+  // the stored atom pairs (a >= b) of nijbo, and the density gather's lists
+  DevBuf dspairs, dacnt, daoff, dalmo, dacoff, dbits, dcum;
+  int nspairs = 0;
 };
 MzWork& mz_work() {
   static std::map<int, MzWork> work;
@@ -1379,6 +1530,15 @@ void upload_nijbo(hipStream_t s, MzWork& W, int numat, const int* nijbo, long lo
       MHIP_REQUIRE(nijbo[(size_t)b + (size_t)a * numat] == o, "LMO set: nijbo not symmetric");
     }
   W.nijbo_seen.clear();  // stays empty if an upload below fails
+  std::vector<int> sp;
+  for (int b = 0; b < numat; ++b)
+    for (int a = b; a < numat; ++a)
+      if (nijbo[(size_t)a + (size_t)b * numat] >= 0) {
+        sp.push_back(a);
+        sp.push_back(b);
+      }
+  W.nspairs = (int)(sp.size() / 2);
+  up(s, W.dspairs, sp.data(), sizeof(int) * sp.size());
   up(s, W.B.nijbo, nijbo, sizeof(int) * nn);
   up(s, W.B.iorbs, iorbs.data(), sizeof(int) * numat);
   MHIP_CHECK(hipStreamSynchronize(s));
@@ -1478,19 +1638,47 @@ hipStream_t mz_stream(mhip_ctx* ctx) {
 namespace mhip {
 
 void mz_density_dev(hipStream_t s, long long mpack, int nocc, int mode, const double* partp_d, double* p_d) {
-  const LmoBufs& B = mz_work().B;
+  MzWork& W = mz_work();
+  const LmoBufs& B = W.B;
   const int nb = std::min(4096, cdiv(mpack, 256));
+  const double spinfa = (mode == 0 || mode == 1) ? 2.0 : (mode == -1 ? -2.0 : 1.0);
+  const bool scale = std::fabs(spinfa - 1.0) > 1e-10;
+  // every element starts from its mode's value (elements of no stored pair
+  // keep it, scaled like the rest)
   hipLaunchKernelGGL(k_mz_density_init, dim3(nb), dim3(256), 0, s, mpack, mode, partp_d, p_d);
   MHIP_LAUNCH_CHECK();
-  if (nocc > 0) {
-    hipLaunchKernelGGL(k_mz_density, dim3(nocc), dim3(256), 0, s, B.dev, p_d);
-    MHIP_LAUNCH_CHECK();
-  }
-  const double spinfa = (mode == 0 || mode == 1) ? 2.0 : (mode == -1 ? -2.0 : 1.0);
-  if (std::fabs(spinfa - 1.0) > 1e-10) {
+  if (scale) {
     hipLaunchKernelGGL(k_scale, dim3(nb), dim3(256), 0, s, mpack, spinfa, p_d);
     MHIP_LAUNCH_CHECK();
   }
+  if (nocc <= 0 || W.nspairs == 0) return;
+  const int numat = B.dev.numat;
+  const int nw = cdiv(nocc, 64);
+  const long long nbits = (long long)numat * nw;
+  W.dbits.ensure(sizeof(unsigned long long) * nbits);
+  W.dcum.ensure(sizeof(int) * nbits);
+  W.dacnt.ensure(sizeof(int) * (numat + 1));
+  W.daoff.ensure(sizeof(int) * (numat + 1));
+  W.dalmo.ensure(sizeof(int) * std::max(B.dev.icocc_dim, 1));
+  W.dacoff.ensure(sizeof(int) * std::max(B.dev.icocc_dim, 1));
+  auto* bits = W.dbits.as<unsigned long long>();
+  hipLaunchKernelGGL(k_mz_fill_u64, dim3((int)std::min<long long>(4096, cdiv(nbits, 256))), dim3(256), 0, s, nbits,
+                     bits);
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mz_atom_bits, dim3(cdiv(nocc, 256)), dim3(256), 0, s, B.dev, nw, bits);
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mz_atom_cum, dim3(cdiv(numat, 256)), dim3(256), 0, s, numat, nw, bits, W.dcum.as<int>(),
+                     W.dacnt.as<int>());
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mz_scan_counts, dim3(1), dim3(1024), 0, s, numat, W.dacnt.as<int>(), W.daoff.as<int>());
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mz_atom_fill, dim3(cdiv(nocc, 256)), dim3(256), 0, s, B.dev, nw, bits, W.dcum.as<int>(),
+                     W.daoff.as<int>(), W.dalmo.as<int>(), W.dacoff.as<int>());
+  MHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mz_density_gather, dim3(cdiv(W.nspairs, 128)), dim3(128), 0, s, B.dev, W.nspairs,
+                     W.dspairs.as<int2>(), W.daoff.as<int>(), W.dalmo.as<int>(), W.dacoff.as<int>(), mode, partp_d,
+                     spinfa, scale ? 1 : 0, p_d);
+  MHIP_LAUNCH_CHECK();
 }
 
 double mz_helecz_dev(hipStream_t s, int numat, const double* p_d, const double* h_d, const double* f_d) {
@@ -1640,7 +1828,7 @@ void mz_diagg1_dev(hipStream_t s, const mhip_lmo_set* L, const double* fao_d, co
       st->fref = (tiny * tiny) * (tiny * tiny);  // tiny**4 as the reference's powi
     } else {
       if (L->nocc > 0) {
-        hipLaunchKernelGGL(k_mz_occ_energy, dim3(cdiv(L->nocc, 64)), dim3(64), 0, s, B.dev, fao_d,
+        hipLaunchKernelGGL(k_mz_occ_energy, dim3(L->nocc), dim3(64), 0, s, B.dev, fao_d,
                            p_d, daocc.as<double>(), P.cutoff, deigs.as<double>());
         MHIP_LAUNCH_CHECK();
         down(s, eigs, deigs, sizeof(double) * L->nocc);

@@ -238,15 +238,20 @@ def test_lewis_start_reaches_the_conventional_energy(gpu_ctx):
     assert abs(r["escf"] - conv["escf"]) < 0.05
 
 
-@pytest.mark.parametrize("nres,scfcrt", [(1, 1e-2), (12, 1e-2), (30, 1e-3)])
+@pytest.mark.parametrize("nres,scfcrt", [(1, 1e-2), (12, 1e-2), (30, 1e-3), (400, 1e-3)])
 def test_mozyme_loop_follows_the_reference_routines(gpu_ctx, nres, scfcrt):
     """The whole MOZYME SCF on the device against iter_for_MOZYME's loop
     driven over the reference's own routines (tidy, check, eimp, diagg1,
     diagg2, density_for_MOZYME, cnvgz, fz2n, helecz; oracle/ref.py
     mozyme_scf) from the same Lewis start at MOZYME's default cutoffs: Gly1
-    (under 30 atoms, every pair stored), Gly12 and Gly30 (stored, dipole and
-    point-charge pairs).  The heat after every iteration agrees to 1e-6
-    kcal/mol and both stop at the same iteration."""
+    (under 30 atoms, every pair stored), Gly12, Gly30 and the 400-residue
+    helix of the benchmark (stored, dipole and point-charge pairs; about 12 s
+    of the reference routines on one host core).  The heat after every
+    iteration agrees to 1e-6 kcal/mol, or 1e-10 of the heat where that is
+    larger (the 400-residue helix's heats reach 2.8e4 kcal/mol, and the
+    energy sums over its 882 259 stored elements round in another order
+    than the reference's sequential loops: 1.6e-6 kcal/mol at most, already
+    6e-7 at the first iteration), and both stop at the same iteration."""
     nat, xyz = geometry.glycine_helix(nres, 3)
     ms = mozyme.MozymeSystem.from_geometry(nat, xyz)
     ms.upload(gpu_ctx)
@@ -259,7 +264,8 @@ def test_mozyme_loop_follows_the_reference_routines(gpu_ctx, nres, scfcrt):
           flush=True)
     assert r["converged"] and ref["converged"]
     assert r["niter"] == ref["niter"]
-    assert np.max(np.abs(r["heats"] - ref["heats"])) < 1e-6
+    tol = max(1e-6, 1e-10 * np.max(np.abs(ref["heats"])))
+    assert np.max(np.abs(r["heats"] - ref["heats"])) < tol
 
 
 @pytest.mark.parametrize("nres", [12, 30])
