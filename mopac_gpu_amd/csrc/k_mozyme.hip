@@ -346,7 +346,7 @@ __device__ void ws_accumulate(const LmoDev& L, VirtLds& V, const double* __restr
 // not stored, 2 if stored; value in *sum.
 __device__ int occ_virt_element(const LmoDev& L, const VirtLds& V, const double* __restrict__ fao,
                                 const double* __restrict__ aocc, int j, const D1Params& P,
-                                bool fast_test, double* sum_out) {
+                                bool fast_test, double* sum_out, const short* amap = nullptr) {
   if (fast_test) {
     const int i1 = V.atoms[0], i2 = (V.nat > 1) ? V.atoms[1] : i1;
     const int j1 = L.icocc[L.nncf[j]], j2 = (L.ncf[j] > 1) ? L.icocc[L.nncf[j] + 1] : j1;
@@ -366,7 +366,7 @@ __device__ int occ_virt_element(const LmoDev& L, const VirtLds& V, const double*
   int kl = 0;
   for (int kk = L.nncf[j]; kk < L.nncf[j] + L.ncf[j]; ++kk) {
     const int k1 = L.icocc[kk], nk = L.iorbs[k1 - 1];
-    const int vp = vpos(V, k1);
+    const int vp = amap ? amap[k1 - 1] : vpos(V, k1);
     if (vp < 0 || aocc[kk] * V.aov[vp] < P.cutoff) {
       kl += nk;
     } else {
@@ -400,19 +400,104 @@ __global__ void k_mz_aocc(LmoDev L, double* __restrict__ aocc) {
   }
 }
 
+// Candidates of the full pass.  The fast test evaluates the pair (virtual
+// i, occupied j) only if one of the atom pairs (i1 or i2, j1 or j2) -- the
+// first two atoms of each LMO -- is stored.  So j can pass only if j1 or j2
+// is a stored neighbour of i1 or i2: the neighbours come from nijbo (fixed
+// per system), the occupied LMOs by first and second atom from per-atom
+// lists built per call, and every other j is skipped exactly where the fast
+// test would have returned 0.  The marks live in an LDS bitmap.
+constexpr int MZ_CANDW = 256;  // bitmap words: up to 16384 occupied LMOs
+struct CandDev {
+  const int* nbr_off;  // numat + 1
+  const int* nbr;      // stored neighbours (0-based), the atom itself included
+  const int* fl_off;   // numat + 1
+  const int* fl_ent;   // occupied LMOs whose first or second atom is this one
+  int on;
+};
+
+__global__ void k_mz_zero_int(int n, int* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0;
+}
+
+__global__ void k_mz_first_count(LmoDev L, int* __restrict__ cnt) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= L.nocc) return;
+  const int b0 = L.nncf[j];
+  const int j1 = L.icocc[b0] - 1;
+  atomicAdd(cnt + j1, 1);
+  if (L.ncf[j] > 1) {
+    const int j2 = L.icocc[b0 + 1] - 1;
+    if (j2 != j1) atomicAdd(cnt + j2, 1);
+  }
+}
+
+__global__ void k_mz_first_fill(LmoDev L, const int* __restrict__ off, int* __restrict__ fillc,
+                                int* __restrict__ ent) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= L.nocc) return;
+  const int b0 = L.nncf[j];
+  const int j1 = L.icocc[b0] - 1;
+  ent[off[j1] + atomicAdd(fillc + j1, 1)] = j;
+  if (L.ncf[j] > 1) {
+    const int j2 = L.icocc[b0 + 1] - 1;
+    if (j2 != j1) ent[off[j2] + atomicAdd(fillc + j2, 1)] = j;
+  }
+}
+
+__device__ void mark_candidates(const LmoDev& L, const VirtLds& V, const CandDev& C,
+                                unsigned long long* __restrict__ cand) {
+  for (int w = threadIdx.x; w < MZ_CANDW; w += blockDim.x) cand[w] = 0ull;
+  __syncthreads();
+  const int i1 = V.atoms[0] - 1, i2 = (V.nat > 1 ? V.atoms[1] : V.atoms[0]) - 1;
+  for (int h = 0; h < 2; ++h) {
+    const int x = h == 0 ? i1 : i2;
+    if (h == 1 && i2 == i1) break;
+    for (int e = C.nbr_off[x] + threadIdx.x; e < C.nbr_off[x + 1]; e += blockDim.x) {
+      const int b = C.nbr[e];
+      for (int f = C.fl_off[b]; f < C.fl_off[b + 1]; ++f) {
+        const int j = C.fl_ent[f];
+        atomicOr(cand + (j >> 6), 1ull << (j & 63));
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ bool is_candidate(const CandDev& C, const unsigned long long* cand, int j) {
+  return !C.on || ((cand[j >> 6] >> (j & 63)) & 1ull);
+}
+
+// atom (0-based) -> its position in the virtual LMO, or -1: replaces the
+// list search of vpos for systems of up to MZ_AMAP atoms
+constexpr int MZ_AMAP = 8192;
+__device__ const short* atom_map(const LmoDev& L, const VirtLds& V, short* amap) {
+  if (L.numat > MZ_AMAP) return nullptr;
+  for (int a = threadIdx.x; a < L.numat; a += blockDim.x) amap[a] = -1;
+  __syncthreads();
+  for (int k = threadIdx.x; k < V.nat; k += blockDim.x) amap[V.atoms[k] - 1] = (short)k;
+  __syncthreads();
+  return amap;
+}
+
 // Pass 1 (full lists): per virtual LMO, the number of stored elements.
 __global__ __launch_bounds__(256) void k_mz_diagg1_count(LmoDev L, const double* __restrict__ fao,
                                                          const double* __restrict__ p,
                                                          const double* __restrict__ aocc,
-                                                         D1Params P, int* __restrict__ counts,
+                                                         D1Params P, CandDev C, int* __restrict__ counts,
                                                          double* __restrict__ eigv) {
   __shared__ VirtLds V;
+  __shared__ unsigned long long cand[MZ_CANDW];
+  __shared__ short amap_s[MZ_AMAP];
   const int i = blockIdx.x;
   virt_setup(L, i, fao, p, P.cutoff, V, eigv, P.fp32 != 0);
+  if (C.on) mark_candidates(L, V, C, cand);
+  const short* amap = atom_map(L, V, amap_s);
   int cnt = 0;
   for (int j = threadIdx.x; j < L.nocc; j += blockDim.x) {
     double s;
-    if (occ_virt_element(L, V, fao, aocc, j, P, true, &s) == 2) ++cnt;
+    if (is_candidate(C, cand, j) && occ_virt_element(L, V, fao, aocc, j, P, true, &s, amap) == 2) ++cnt;
   }
   // block sum
   __shared__ int red[256];
@@ -431,7 +516,7 @@ __global__ __launch_bounds__(256) void k_mz_diagg1_count(LmoDev L, const double*
 __global__ __launch_bounds__(256) void k_mz_diagg1_write(LmoDev L, const double* __restrict__ fao,
                                                          const double* __restrict__ p,
                                                          const double* __restrict__ aocc,
-                                                         D1Params P, const int* __restrict__ offs,
+                                                         D1Params P, CandDev C, const int* __restrict__ offs,
                                                          int* __restrict__ nfmo,
                                                          const int* __restrict__ old_ifmo,
                                                          double* __restrict__ fmo, int* __restrict__ ifmo,
@@ -441,9 +526,13 @@ __global__ __launch_bounds__(256) void k_mz_diagg1_write(LmoDev L, const double*
   __shared__ VirtLds V;
   __shared__ int scan[256];
   __shared__ int base_s;
+  __shared__ unsigned long long cand[MZ_CANDW];
+  __shared__ short amap_s[MZ_AMAP];
   const int i = blockIdx.x, tid = threadIdx.x;
   virt_setup(L, i, fao, p, P.cutoff, V, eigv, P.fp32 != 0);
   double mx = 0.0, sm = 0.0;
+  if (P.full && C.on) mark_candidates(L, V, C, cand);
+  const short* amap = atom_map(L, V, amap_s);
   if (P.full) {
     if (tid == 0) base_s = offs[i];
     __syncthreads();
@@ -453,7 +542,7 @@ __global__ __launch_bounds__(256) void k_mz_diagg1_write(LmoDev L, const double*
       const int j = j0 + tid;
       double s = 0.0;
       int kind = 0;
-      if (j < L.nocc) kind = occ_virt_element(L, V, fao, aocc, j, P, true, &s);
+      if (j < L.nocc && is_candidate(C, cand, j)) kind = occ_virt_element(L, V, fao, aocc, j, P, true, &s, amap);
       // inclusive scan of stored flags in j order
       scan[tid] = (kind == 2);
       __syncthreads();
@@ -486,7 +575,7 @@ __global__ __launch_bounds__(256) void k_mz_diagg1_write(LmoDev L, const double*
       if (pos >= P.fmo_dim) break;
       const int j = old_ifmo[2 * pos + 1] - 1;
       double s;
-      occ_virt_element(L, V, fao, aocc, j, P, false, &s);
+      occ_virt_element(L, V, fao, aocc, j, P, false, &s, amap);
       fmo[pos] = s;
       ifmo[2 * pos] = i + 1;
       ifmo[2 * pos + 1] = j + 1;
@@ -1493,6 +1582,9 @@ struct MzWork {
   // the stored atom pairs (a >= b) of nijbo, and the density gather's lists
   DevBuf dspairs, dacnt, daoff, dalmo, dacoff, dbits, dcum;
   int nspairs = 0;
+  // diagg1's candidates: stored neighbours per atom (fixed per nijbo) and the
+  // occupied LMOs by first and second atom (per call)
+  DevBuf dnbr_off, dnbr, dfl_cnt, dfl_off, dfl_ent;
 };
 MzWork& mz_work() {
   static std::map<int, MzWork> work;
@@ -1539,6 +1631,15 @@ void upload_nijbo(hipStream_t s, MzWork& W, int numat, const int* nijbo, long lo
       }
   W.nspairs = (int)(sp.size() / 2);
   up(s, W.dspairs, sp.data(), sizeof(int) * sp.size());
+  std::vector<int> noff(numat + 1, 0), nb;
+  for (int a = 0; a < numat; ++a) {
+    noff[a] = (int)nb.size();
+    for (int b = 0; b < numat; ++b)
+      if (nijbo[(size_t)a + (size_t)b * numat] >= 0) nb.push_back(b);
+  }
+  noff[numat] = (int)nb.size();
+  up(s, W.dnbr_off, noff.data(), sizeof(int) * noff.size());
+  up(s, W.dnbr, nb.data(), sizeof(int) * std::max<size_t>(nb.size(), 1));
   up(s, W.B.nijbo, nijbo, sizeof(int) * nn);
   up(s, W.B.iorbs, iorbs.data(), sizeof(int) * numat);
   MHIP_CHECK(hipStreamSynchronize(s));
@@ -1766,10 +1867,33 @@ void mz_diagg1_dev(hipStream_t s, const mhip_lmo_set* L, const double* fao_d, co
       MHIP_LAUNCH_CHECK();
     }
     std::vector<int> offs(nv, 0), counts(nv, 0);
+    CandDev C{};
+    if (full && L->nocc > 0 && L->nocc <= 64 * MZ_CANDW) {
+      const int na = L->numat;
+      W.dfl_cnt.ensure(sizeof(int) * (na + 1));
+      W.dfl_off.ensure(sizeof(int) * (na + 1));
+      W.dfl_ent.ensure(sizeof(int) * 2 * L->nocc);
+      hipLaunchKernelGGL(k_mz_zero_int, dim3(cdiv(na + 1, 256)), dim3(256), 0, s, na + 1, W.dfl_cnt.as<int>());
+      MHIP_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_mz_first_count, dim3(cdiv(L->nocc, 256)), dim3(256), 0, s, B.dev, W.dfl_cnt.as<int>());
+      MHIP_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_mz_scan_counts, dim3(1), dim3(1024), 0, s, na, W.dfl_cnt.as<int>(), W.dfl_off.as<int>());
+      MHIP_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_mz_zero_int, dim3(cdiv(na + 1, 256)), dim3(256), 0, s, na + 1, W.dfl_cnt.as<int>());
+      MHIP_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_mz_first_fill, dim3(cdiv(L->nocc, 256)), dim3(256), 0, s, B.dev, W.dfl_off.as<int>(),
+                         W.dfl_cnt.as<int>(), W.dfl_ent.as<int>());
+      MHIP_LAUNCH_CHECK();
+      C.nbr_off = W.dnbr_off.as<int>();
+      C.nbr = W.dnbr.as<int>();
+      C.fl_off = W.dfl_off.as<int>();
+      C.fl_ent = W.dfl_ent.as<int>();
+      C.on = 1;
+    }
     if (full) {
       if (L->nvir > 0) {
         hipLaunchKernelGGL(k_mz_diagg1_count, dim3(L->nvir), dim3(256), 0, s, B.dev, fao_d,
-                           p_d, daocc.as<double>(), P, dcounts.as<int>(), deigv.as<double>());
+                           p_d, daocc.as<double>(), P, C, dcounts.as<int>(), deigv.as<double>());
         MHIP_LAUNCH_CHECK();
         down(s, counts.data(), dcounts, sizeof(int) * L->nvir);
         MHIP_CHECK(hipStreamSynchronize(s));
@@ -1796,7 +1920,7 @@ void mz_diagg1_dev(hipStream_t s, const mhip_lmo_set* L, const double* fao_d, co
     up(s, doffs, offs.data(), sizeof(int) * nv);
     if (L->nvir > 0) {
       hipLaunchKernelGGL(k_mz_diagg1_write, dim3(L->nvir), dim3(256), 0, s, B.dev, fao_d,
-                         p_d, daocc.as<double>(), P, doffs.as<int>(), dnfmo.as<int>(),
+                         p_d, daocc.as<double>(), P, C, doffs.as<int>(), dnfmo.as<int>(),
                          full ? nullptr : dold.as<int>(), dfmo.as<double>(), difmo.as<int>(),
                          dvmax.as<double>(), dvsum.as<double>(), deigv.as<double>());
       MHIP_LAUNCH_CHECK();
