@@ -497,27 +497,44 @@ def lewis_lmos(ms, fock=None, pad_atoms=6, scale=1.2):
             rest[a] = r
         elec[a] = VALENCE[int(nat[a])] - k
 
+    tril = {n: np.tril_indices(n) for n in (1, 4)}
+    diag_blk = {}
+
     def block(a, b):
         """F block rows on a, columns on b (dense, from the sparse storage)."""
         o = ms.nijbo[a, b]
         if a == b:
-            n = iorbs[a]
-            t = np.zeros((n, n))
-            t[np.tril_indices(n)] = fock[o:o + n * (n + 1) // 2]
-            return t + t.T - np.diag(np.diag(t))
+            t = diag_blk.get(a)
+            if t is None:
+                n = iorbs[a]
+                t = np.zeros((n, n))
+                t[tril[n]] = fock[o:o + n * (n + 1) // 2]
+                t = t + t.T - np.diag(np.diag(t))
+                diag_blk[a] = t
+            return t
         if a > b:
             return fock[o:o + iorbs[a] * iorbs[b]].reshape(iorbs[a], iorbs[b])
         return block(b, a).T
 
     def bond(a, ha, b, hb):
-        """(bonding, antibonding) as [(atom, coefficient vector), ...]."""
+        """(bonding, antibonding) as [(atom, coefficient vector), ...].  The
+        bonding pair (c1, c2) is the lower eigenvector of the 2 x 2 Fock
+        matrix of the two hybrids, in closed form, signed so that c1 + c2 > 0
+        (c1 >= 0 when they cancel)."""
         if fock is None:
             c1 = c2 = np.sqrt(0.5)
         else:
-            m = np.array([[ha @ block(a, a) @ ha, ha @ block(a, b) @ hb],
-                          [hb @ block(b, a) @ ha, hb @ block(b, b) @ hb]])
-            ev, v = np.linalg.eigh(m)
-            c1, c2 = v[0, 0], v[1, 0]
+            faa = ha @ block(a, a) @ ha
+            fbb = hb @ block(b, b) @ hb
+            fab = ha @ block(a, b) @ hb
+            # the rotation by t = atan2(2 fab, faa - fbb) / 2 diagonalizes the
+            # matrix with the upper eigenvalue on (cos t, sin t); the lower
+            # one is on (-sin t, cos t)
+            t = 0.5 * np.arctan2(2.0 * fab, faa - fbb)
+            u = np.array([-np.sin(t), np.cos(t)])
+            if u[0] + u[1] < 0 or (u[0] + u[1] == 0 and u[0] < 0):
+                u = -u
+            c1, c2 = float(u[0]), float(u[1])
         return [(a, c1 * ha), (b, c2 * hb)], [(a, -c2 * ha), (b, c1 * hb)]
 
     occ, vir = [], []
