@@ -434,23 +434,24 @@ VALENCE = {1: 1, 6: 4, 7: 5, 8: 6}
 
 
 def _hybrids(u):
-    """Orthonormal directed hybrids (s, px, py, pz) for the bond directions
-    u (k x 3 unit vectors), closest in the least-squares sense to the
-    sp3-like targets (1/2, sqrt(3)/2 u) (symmetric orthonormalization), and
-    an orthonormal basis of the rest of the valence space."""
-    k = len(u)
-    t = np.zeros((4, k))
-    t[0] = 0.5
-    t[1:] = 0.8660254037844386 * np.asarray(u).T
-    g = t.T @ t
-    ev, vec = np.linalg.eigh(g)
-    if ev.min() < 1e-6:
+    """Orthonormal directed hybrids (s, px, py, pz) for m atoms with k bonds
+    each, bond directions u (m x k x 3 unit vectors), closest in the
+    least-squares sense to the sp3-like targets (1/2, sqrt(3)/2 u)
+    (symmetric orthonormalization), and an orthonormal basis of the rest of
+    the valence space: (m x 4 x k, m x 4 x (4 - k))."""
+    u = np.asarray(u, dtype=np.float64)
+    m, k = u.shape[:2]
+    t = np.empty((m, 4, k))
+    t[:, 0] = 0.5
+    t[:, 1:] = 0.8660254037844386 * u.transpose(0, 2, 1)
+    ev, vec = np.linalg.eigh(t.transpose(0, 2, 1) @ t)
+    if m and ev.min() < 1e-6:
         raise ValueError("Lewis structure: linearly dependent bond directions")
-    h = t @ (vec @ np.diag(ev ** -0.5) @ vec.T)
-    proj = np.eye(4) - h @ h.T
-    ev2, vec2 = np.linalg.eigh(proj)
-    rest = vec2[:, ev2 > 0.5]
-    return h, rest
+    h = t @ ((vec * ev[:, None, :] ** -0.5) @ vec.transpose(0, 2, 1))
+    proj = np.eye(4) - h @ h.transpose(0, 2, 1)
+    # proj has eigenvalue 0 on the k hybrids and 1 on the rest (ascending)
+    _, vec2 = np.linalg.eigh(proj)
+    return h, vec2[:, :, k:]
 
 
 def lewis_lmos(ms, fock=None, pad_atoms=6, scale=1.2):
@@ -480,6 +481,7 @@ def lewis_lmos(ms, fock=None, pad_atoms=6, scale=1.2):
         nbr[b].append(int(a))
     # hybrids: hyb[a][b] = the orbital of atom a pointing at b; rest[a] = the others
     hyb, rest, elec = [dict() for _ in range(numat)], [None] * numat, np.zeros(numat, dtype=np.int64)
+    heavy = {k: [] for k in range(1, 5)}
     for a in range(numat):
         k = len(nbr[a])
         if iorbs[a] == 1:
@@ -490,12 +492,20 @@ def lewis_lmos(ms, fock=None, pad_atoms=6, scale=1.2):
         else:
             if not 1 <= k <= 4:
                 raise ValueError(f"Lewis structure: atom {a + 1} has {k} bonds")
-            u = np.array([(xyz[b] - xyz[a]) / np.linalg.norm(xyz[b] - xyz[a]) for b in nbr[a]])
-            h, r = _hybrids(u)
-            for i, b in enumerate(nbr[a]):
-                hyb[a][b] = h[:, i]
-            rest[a] = r
+            heavy[k].append(a)
         elec[a] = VALENCE[int(nat[a])] - k
+    for k, atoms in heavy.items():
+        if not atoms:
+            continue
+        # per-vector norms: the lone-pair basis spans a degenerate eigenspace,
+        # so the directions are kept bit-stable against the one-atom form
+        u = np.array([[(xyz[b] - xyz[a]) / np.linalg.norm(xyz[b] - xyz[a]) for b in nbr[a]]
+                      for a in atoms])
+        h, r = _hybrids(u)
+        for ai, a in enumerate(atoms):
+            for i, b in enumerate(nbr[a]):
+                hyb[a][b] = h[ai, :, i]
+            rest[a] = r[ai]
 
     tril = {n: np.tril_indices(n) for n in (1, 4)}
     diag_blk = {}
@@ -572,22 +582,21 @@ def lewis_lmos(ms, fock=None, pad_atoms=6, scale=1.2):
                          f"{nocc} / {ms.norbs - nocc}")
 
     def pack(lmos):
-        ncf, nncf, ncoc, idx, coef = [], [], [], [], []
-        ia = ic = 0
-        for parts in lmos:
+        """Atoms of each LMO by descending weight (stable), each LMO followed
+        by pad_atoms empty atom slots and 4 * pad_atoms zero coefficients."""
+        ncf = np.array([len(parts) for parts in lmos], np.int32)
+        idx, coef, ncc = [], [], np.empty(len(lmos), np.int64)
+        ipad, cpad = [0] * pad_atoms, np.zeros(4 * pad_atoms)
+        for k, parts in enumerate(lmos):
             parts = sorted(parts, key=lambda t: -float(t[1] @ t[1]))
-            nncf.append(ia)
-            ncoc.append(ic)
-            ncf.append(len(parts))
-            for a, c in parts:
-                idx.append(a + 1)
-                coef.extend(c)
-            idx.extend([0] * pad_atoms)
-            coef.extend([0.0] * (4 * pad_atoms))
-            ia += len(parts) + pad_atoms
-            ic += sum(len(c) for _, c in parts) + 4 * pad_atoms
-        return (np.array(ncf, np.int32), np.array(nncf, np.int32), np.array(ncoc, np.int32),
-                np.array(idx, np.int32), np.array(coef))
+            idx.extend(a + 1 for a, _ in parts)
+            idx.extend(ipad)
+            coef.extend(c for _, c in parts)
+            coef.append(cpad)
+            ncc[k] = sum(len(c) for _, c in parts) + 4 * pad_atoms
+        nncf = np.concatenate(([0], np.cumsum(ncf + pad_atoms)[:-1])).astype(np.int32)
+        ncoc = np.concatenate(([0], np.cumsum(ncc)[:-1])).astype(np.int32)
+        return ncf, nncf, ncoc, np.array(idx, np.int32), np.concatenate(coef)
 
     o = pack(occ)
     v = pack(vir)
