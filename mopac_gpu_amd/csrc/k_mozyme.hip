@@ -1062,6 +1062,21 @@ __device__ __forceinline__ D2Wave d2w_carve(unsigned char* lds, int numat, int w
   return v;
 }
 
+// dst[k] = src[k] for k < n by one wave, a pass of 64 * 8 elements at a
+// time with all of a lane's loads issued before its stores and no branch
+// between them (a lane past the end copies element n - 1 again: the same
+// value to the same address), so a pass costs one memory round trip.
+__device__ __forceinline__ void wave_copy(double* __restrict__ dst, const double* __restrict__ src, int n, int lane) {
+  constexpr int U = 8;
+  for (int k0 = 0; k0 < n; k0 += 64 * U) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = src[min(k0 + u * 64 + lane, n - 1)];
+#pragma unroll
+    for (int u = 0; u < U; ++u) dst[min(k0 + u * 64 + lane, n - 1)] = v[u];
+  }
+}
+
 // One rotation by one wave.  The dependent global reads are the record, the
 // two atom counts, the atom lists, their orbital counts and the
 // coefficients; everything the wave re-reads inside the rotation is in LDS
@@ -1088,15 +1103,28 @@ __device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P,
   auto atom_i = [&](int k) { return in_lds ? (int)M.li[k] : gi[k] - 1; };
   auto orbs_j = [&](int k, int a) { return in_lds ? (int)M.oj[k] : L.iorbs[a]; };
   auto orbs_i = [&](int k, int a) { return in_lds ? (int)M.oi[k] : L.iorbs[a]; };
-  if (in_lds) {
+  if (in_lds && ncfj > 0 && ncei > 0) {
+    // both lists at once, branch-free (a lane past the end of a list
+    // repeats its last entry), so the atom numbers cost one memory round
+    // trip and their orbital counts one more
+    for (int k = lane; k < max(ncfj, ncei); k += 64) {
+      const int kj = min(k, ncfj - 1), ki = min(k, ncei - 1);
+      const int aj = gj[kj] - 1, ai = gi[ki] - 1;
+      const int nj = L.iorbs[aj], ni = L.iorbs[ai];
+      M.lj[kj] = (unsigned short)aj;
+      M.oj[kj] = (unsigned char)nj;
+      M.li[ki] = (unsigned short)ai;
+      M.oi[ki] = (unsigned char)ni;
+    }
+  } else if (in_lds) {
     for (int k = lane; k < ncfj; k += 64) M.lj[k] = (unsigned short)(gj[k] - 1);
     for (int k = lane; k < ncei; k += 64) M.li[k] = (unsigned short)(gi[k] - 1);
     for (int k = lane; k < ncfj; k += 64) M.oj[k] = (unsigned char)L.iorbs[M.lj[k]];
     for (int k = lane; k < ncei; k += 64) M.oi[k] = (unsigned char)L.iorbs[M.li[k]];
   }
   const int bj = min(4 * ncfj, jur - jlr), bi = min(4 * ncei, iur - ilr);
-  for (int k = lane; k < bj; k += 64) storej[k] = L.cocc[jlr + k];
-  for (int k = lane; k < bi; k += 64) storei[k] = L.cvir[ilr + k];
+  wave_copy(storej, L.cocc + jlr, bj, lane);
+  wave_copy(storei, L.cvir + ilr, bi, lane);
   const double e = copysign(sqrt(4.0 * c * c + d * d), d);
   double alpha = sqrt(0.5 * (1.0 + d / e));
   wave_phase();
@@ -1138,12 +1166,23 @@ __device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P,
       if (le < ncei) {
         M.latoms[mie] = 1;
         const int u = M.iused[mie];
-        for (int x = 0; x < no; ++x) cv[x] = L.cvir[ilr + mle + x];
+        // an atom has 1 or 4 orbitals (checked when the LMO set is loaded).
+        // Orbital x is read and written at min(x, no - 1), so the loads and
+        // stores have no branch between them and every load of the lane is
+        // issued before its stores; an s atom repeats orbital 0 (the same
+        // value to the same address).
+        const int xl = no - 1;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) cv[x] = L.cvir[ilr + mle + min(x, xl)];
         if (u != D2W_NONE) {
-          for (int x = 0; x < no; ++x) {
-            const double a = L.cocc[jlr + u + x], b = cv[x];
-            L.cocc[jlr + u + x] = alpha * a + beta * b;
-            L.cvir[ilr + mle + x] = alpha * b - beta * a;
+          double co[4];
+#pragma unroll
+          for (int x = 0; x < 4; ++x) co[x] = L.cocc[jlr + u + min(x, xl)];
+#pragma unroll
+          for (int x = 0; x < 4; ++x) {
+            const double a = co[x], b = cv[x];
+            L.cocc[jlr + u + min(x, xl)] = alpha * a + beta * b;
+            L.cvir[ilr + mle + min(x, xl)] = alpha * b - beta * a;
           }
         } else {
           double s2 = 0.0;
@@ -1169,10 +1208,15 @@ __device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P,
             M.oj[ncf_run + rank] = (unsigned char)no;
           }
           M.iused[mie] = (unsigned short)obase;
-          for (int x = 0; x < no; ++x) {
-            storej[obase + x] = L.cocc[jlr + obase + x];
-            L.cocc[jlr + obase + x] = beta * cv[x];
-            L.cvir[ilr + mle + x] = alpha * cv[x];
+          const int xl = no - 1;
+          double old[4];
+#pragma unroll
+          for (int x = 0; x < 4; ++x) old[x] = L.cocc[jlr + obase + min(x, xl)];
+#pragma unroll
+          for (int x = 0; x < 4; ++x) {
+            storej[obase + min(x, xl)] = old[x];
+            L.cocc[jlr + obase + min(x, xl)] = beta * cv[x];
+            L.cvir[ilr + mle + min(x, xl)] = alpha * cv[x];
           }
         }
         if (fail) break;
@@ -1198,9 +1242,10 @@ __device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P,
           no = orbs_j(lf, ii);
           if (!M.latoms[ii]) {
             u = M.iused[ii];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) cj[x] = L.cocc[jlr + u + min(x, no - 1)];
             double s2 = 0.0;
             for (int x = 0; x < no; ++x) {
-              cj[x] = L.cocc[jlr + u + x];
               const double t = beta * cj[x];
               s2 += t * t;
             }
@@ -1218,10 +1263,15 @@ __device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P,
             L.icvir[slot] = ii + 1;
             if (in_lds) M.li[nce_run + rank] = (unsigned short)ii;
             M.latoms[ii] = 1;
-            for (int x = 0; x < no; ++x) {
-              storei[obase + x] = L.cvir[ilr + obase + x];
-              L.cvir[ilr + obase + x] = -beta * cj[x];
-              L.cocc[jlr + u + x] = alpha * cj[x];
+            const int xl = no - 1;
+            double old[4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) old[x] = L.cvir[ilr + obase + min(x, xl)];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+              storei[obase + min(x, xl)] = old[x];
+              L.cvir[ilr + obase + min(x, xl)] = -beta * cj[x];
+              L.cocc[jlr + u + min(x, xl)] = alpha * cj[x];
             }
           }
           if (fail) break;
