@@ -998,7 +998,7 @@ __global__ __launch_bounds__(D2_NT) void k_mz_diagg2(LmoDev L, const double* __r
 // an atom's orbitals in occupied LMO j, 0xffff when absent; latoms: atom in
 // virtual LMO i), entries set only for the atoms of the two LMOs and cleared
 // after the rotation; scans and counts are wave shuffles and ballots.
-constexpr int D2W_NW = 16;        // waves of the workgroup (at most)
+constexpr int D2W_NW = 8;        // waves of the workgroup (at most)
 constexpr unsigned short D2W_NONE = 0xffff;
 
 __device__ __forceinline__ int wave_scan_incl(int v, int lane) {
@@ -1027,14 +1027,19 @@ struct D2Rot {
   int pad;
 };
 
-// Per-wave LDS of the rotation kernels: the atom maps (iused, latoms) and
-// copies of the two LMOs' atom lists (0-based atom numbers) and orbital
-// counts when the two hold at most D2W_LCAP atoms together, which bounds both
-// lists after the rotation (each ends as a subset of the union of the two);
-// larger pairs read their lists from global memory.  Map entries are set for
-// the atoms of the two LMOs when a rotation starts, so no clearing is needed.
+// Per-wave LDS of the rotation kernels: the first D2W_PCAP coefficients of
+// each of the two LMOs as the rotation found them (cj, ci), the atom maps
+// (iused, latoms) and copies of the two LMOs' atom lists (0-based atom
+// numbers) and orbital counts when the two hold at most D2W_LCAP atoms
+// together, which bounds both lists after the rotation (each ends as a
+// subset of the union of the two); larger pairs read their lists from global
+// memory.  Map entries are set for the atoms of the two LMOs when a rotation
+// starts, so no clearing is needed.
 constexpr int D2W_LCAP = 1024;
+constexpr int D2W_PCAP = 1024;
 struct D2Wave {
+  double* cj;              // D2W_PCAP
+  double* ci;              // D2W_PCAP
   unsigned short* iused;   // numat
   unsigned char* latoms;   // numat
   unsigned short* lj;      // D2W_LCAP
@@ -1044,7 +1049,7 @@ struct D2Wave {
 };
 
 __host__ __device__ inline size_t d2w_wave_bytes(int numat) {
-  return (((size_t)3 * numat + 15) & ~(size_t)15) + 6 * (size_t)D2W_LCAP;
+  return 16 * (size_t)D2W_PCAP + (((size_t)3 * numat + 15) & ~(size_t)15) + 6 * (size_t)D2W_LCAP;
 }
 // LDS bytes of a workgroup of nw waves
 __host__ __device__ inline size_t d2w_lds_bytes(int numat, int nw) { return (size_t)nw * d2w_wave_bytes(numat); }
@@ -1052,6 +1057,9 @@ __host__ __device__ inline size_t d2w_lds_bytes(int numat, int nw) { return (siz
 __device__ __forceinline__ D2Wave d2w_carve(unsigned char* lds, int numat, int w) {
   unsigned char* base = lds + (size_t)w * d2w_wave_bytes(numat);
   D2Wave v;
+  v.cj = reinterpret_cast<double*>(base);
+  v.ci = v.cj + D2W_PCAP;
+  base += 16 * (size_t)D2W_PCAP;
   v.iused = reinterpret_cast<unsigned short*>(base);
   v.latoms = base + 2 * (size_t)numat;
   unsigned char* lists = base + (((size_t)3 * numat + 15) & ~(size_t)15);
@@ -1077,14 +1085,17 @@ __device__ __forceinline__ void wave_copy(double* __restrict__ dst, const double
   }
 }
 
-// One rotation by one wave.  The dependent global reads are the record, the
-// two atom counts, the atom lists, their orbital counts and the
-// coefficients; everything the wave re-reads inside the rotation is in LDS
-// or registers, except the undo log on a rejection (and the lists of an LMO
-// pair too large for the LDS copies).
-__device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P, const D2Wave& M,
-                             double* __restrict__ storej, double* __restrict__ storei, double* __restrict__ sumb_out,
-                             int* __restrict__ nrej_out, const int lane) {
+// The rotation proper (after its lists are in place), on the LMOs' current
+// coefficients read from the LDS copies (FAST: every position the rotation
+// can touch lies inside them; each position is read before the rotation
+// writes it, and a rejection restores the copies) or from global memory with
+// an undo log (storej, storei).
+template <bool FAST>
+__device__ __forceinline__ void d2w_rotate(const LmoDev& L, const D2Rot& R, const D2Params& P, const D2Wave& M,
+                                           double* __restrict__ storej, double* __restrict__ storei,
+                                           double* __restrict__ sumb_out, int* __restrict__ nrej_out,
+                                           const int lane, const int ncfj, const int ncei, const int bj,
+                                           const int bi, const bool in_lds) {
   const unsigned long long below = (1ull << lane) - 1ull;
   double sumb = 0.0;
   int nrej = 0;
@@ -1092,39 +1103,21 @@ __device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P,
   const double c = R.c, d = R.d;
   const int nnj = R.nnj, nni = R.nni, jlr = R.jlr, ilr = R.ilr;
   const int jur = R.jur, iur = R.iur, jncf = R.jncf, incv = R.incv;
-  const int ncfj = L.ncf[j], ncei = L.nce[i];
-  // atom lists (and orbital counts) into LDS when they fit (every list entry
-  // the rotation reads lies below ncfj + ncei), undo log of the coefficients
-  // the LMOs hold now
-  const bool in_lds = ncfj + ncei <= P.lcap;
   const int* __restrict__ gj = L.icocc + nnj;
   const int* __restrict__ gi = L.icvir + nni;
   auto atom_j = [&](int k) { return in_lds ? (int)M.lj[k] : gj[k] - 1; };
   auto atom_i = [&](int k) { return in_lds ? (int)M.li[k] : gi[k] - 1; };
   auto orbs_j = [&](int k, int a) { return in_lds ? (int)M.oj[k] : L.iorbs[a]; };
   auto orbs_i = [&](int k, int a) { return in_lds ? (int)M.oi[k] : L.iorbs[a]; };
-  if (in_lds && ncfj > 0 && ncei > 0) {
-    // both lists at once, branch-free (a lane past the end of a list
-    // repeats its last entry), so the atom numbers cost one memory round
-    // trip and their orbital counts one more
-    for (int k = lane; k < max(ncfj, ncei); k += 64) {
-      const int kj = min(k, ncfj - 1), ki = min(k, ncei - 1);
-      const int aj = gj[kj] - 1, ai = gi[ki] - 1;
-      const int nj = L.iorbs[aj], ni = L.iorbs[ai];
-      M.lj[kj] = (unsigned short)aj;
-      M.oj[kj] = (unsigned char)nj;
-      M.li[ki] = (unsigned short)ai;
-      M.oi[ki] = (unsigned char)ni;
-    }
-  } else if (in_lds) {
-    for (int k = lane; k < ncfj; k += 64) M.lj[k] = (unsigned short)(gj[k] - 1);
-    for (int k = lane; k < ncei; k += 64) M.li[k] = (unsigned short)(gi[k] - 1);
-    for (int k = lane; k < ncfj; k += 64) M.oj[k] = (unsigned char)L.iorbs[M.lj[k]];
-    for (int k = lane; k < ncei; k += 64) M.oi[k] = (unsigned char)L.iorbs[M.li[k]];
-  }
-  const int bj = min(4 * ncfj, jur - jlr), bi = min(4 * ncei, iur - ilr);
-  wave_copy(storej, L.cocc + jlr, bj, lane);
-  wave_copy(storei, L.cvir + ilr, bi, lane);
+  // coefficient at position k of LMO j / i as the rotation found it
+  auto old_j = [&](int k) -> double {
+    if constexpr (FAST) return M.cj[k];
+    else return L.cocc[jlr + k];
+  };
+  auto old_i = [&](int k) -> double {
+    if constexpr (FAST) return M.ci[k];
+    else return L.cvir[ilr + k];
+  };
   const double e = copysign(sqrt(4.0 * c * c + d * d), d);
   double alpha = sqrt(0.5 * (1.0 + d / e));
   wave_phase();
@@ -1173,11 +1166,11 @@ __device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P,
         // value to the same address).
         const int xl = no - 1;
 #pragma unroll
-        for (int x = 0; x < 4; ++x) cv[x] = L.cvir[ilr + mle + min(x, xl)];
+        for (int x = 0; x < 4; ++x) cv[x] = old_i(mle + min(x, xl));
         if (u != D2W_NONE) {
           double co[4];
 #pragma unroll
-          for (int x = 0; x < 4; ++x) co[x] = L.cocc[jlr + u + min(x, xl)];
+          for (int x = 0; x < 4; ++x) co[x] = old_j(u + min(x, xl));
 #pragma unroll
           for (int x = 0; x < 4; ++x) {
             const double a = co[x], b = cv[x];
@@ -1209,12 +1202,15 @@ __device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P,
           }
           M.iused[mie] = (unsigned short)obase;
           const int xl = no - 1;
-          double old[4];
+          if constexpr (!FAST) {
+            double old[4];
 #pragma unroll
-          for (int x = 0; x < 4; ++x) old[x] = L.cocc[jlr + obase + min(x, xl)];
+            for (int x = 0; x < 4; ++x) old[x] = L.cocc[jlr + obase + min(x, xl)];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) storej[obase + min(x, xl)] = old[x];
+          }
 #pragma unroll
           for (int x = 0; x < 4; ++x) {
-            storej[obase + min(x, xl)] = old[x];
             L.cocc[jlr + obase + min(x, xl)] = beta * cv[x];
             L.cvir[ilr + mle + min(x, xl)] = alpha * cv[x];
           }
@@ -1241,9 +1237,10 @@ __device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P,
           ii = atom_j(lf);
           no = orbs_j(lf, ii);
           if (!M.latoms[ii]) {
+            // an atom of LMO j as the rotation found it (not touched above)
             u = M.iused[ii];
 #pragma unroll
-            for (int x = 0; x < 4; ++x) cj[x] = L.cocc[jlr + u + min(x, no - 1)];
+            for (int x = 0; x < 4; ++x) cj[x] = old_j(u + min(x, no - 1));
             double s2 = 0.0;
             for (int x = 0; x < no; ++x) {
               const double t = beta * cj[x];
@@ -1264,12 +1261,15 @@ __device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P,
             if (in_lds) M.li[nce_run + rank] = (unsigned short)ii;
             M.latoms[ii] = 1;
             const int xl = no - 1;
-            double old[4];
+            if constexpr (!FAST) {
+              double old[4];
 #pragma unroll
-            for (int x = 0; x < 4; ++x) old[x] = L.cvir[ilr + obase + min(x, xl)];
+              for (int x = 0; x < 4; ++x) old[x] = L.cvir[ilr + obase + min(x, xl)];
+#pragma unroll
+              for (int x = 0; x < 4; ++x) storei[obase + min(x, xl)] = old[x];
+            }
 #pragma unroll
             for (int x = 0; x < 4; ++x) {
-              storei[obase + min(x, xl)] = old[x];
               L.cvir[ilr + obase + min(x, xl)] = -beta * cj[x];
               L.cocc[jlr + u + min(x, xl)] = alpha * cj[x];
             }
@@ -1286,14 +1286,19 @@ __device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P,
         break;
       }
     }
-    // rejection (label 1000): restore both LMOs from the undo log (written
-    // by other lanes: wait for the wave's stores first); the maps are set
-    // again at the retry
+    // rejection (label 1000): restore both LMOs as the rotation found them
+    // (the undo log is written by other lanes: wait for the wave's stores
+    // first); the maps are set again at the retry
     ++nrej;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     const int rj = max(bj, jext), ri = max(bi, iext);
-    for (int k = lane; k < rj; k += 64) L.cocc[jlr + k] = storej[k];
-    for (int k = lane; k < ri; k += 64) L.cvir[ilr + k] = storei[k];
+    if constexpr (FAST) {
+      for (int k = lane; k < rj; k += 64) L.cocc[jlr + k] = M.cj[k];
+      for (int k = lane; k < ri; k += 64) L.cvir[ilr + k] = M.ci[k];
+    } else {
+      for (int k = lane; k < rj; k += 64) L.cocc[jlr + k] = storej[k];
+      for (int k = lane; k < ri; k += 64) L.cvir[ilr + k] = storei[k];
+    }
     if (lane == 0) {
       L.ncf[j] = ncfj;
       L.nce[i] = ncei;
@@ -1309,6 +1314,91 @@ __device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P,
     sumb_out[R.ij] = sumb;
     nrej_out[R.ij] = nrej;
   }
+}
+
+// One rotation by one wave.  The dependent global reads are the record, then
+// in one round trip the two atom counts together with the heads of the two
+// atom lists and the LMOs' coefficients (read speculatively, clamped to each
+// LMO's storage), then the orbital counts of the listed atoms; when the
+// coefficient copies in LDS cover every position the rotation can touch
+// (FAST), the rotation itself reads nothing more from global memory.
+__device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P, const D2Wave& M,
+                             double* __restrict__ storej, double* __restrict__ storei, double* __restrict__ sumb_out,
+                             int* __restrict__ nrej_out, const int lane) {
+  const int i = R.i, j = R.j;
+  const int nnj = R.nnj, nni = R.nni, jlr = R.jlr, ilr = R.ilr;
+  const int jur = R.jur, iur = R.iur, jncf = R.jncf, incv = R.incv;
+  const int* __restrict__ gj = L.icocc + nnj;
+  const int* __restrict__ gi = L.icvir + nni;
+  const int ncap_j = jncf - nnj, ncap_i = incv - nni;
+  const int pj = min(jur - jlr, D2W_PCAP), pi = min(iur - ilr, D2W_PCAP);
+  const bool spec = ncap_j > 0 && ncap_i > 0 && pj > 0 && pi > 0;
+  int ncfj, ncei, sj = 0, si = 0;
+  if (spec) {
+    constexpr int U = D2W_PCAP / 64;
+    double uj[U], ui[U];
+    ncfj = L.ncf[j];
+    ncei = L.nce[i];
+    sj = gj[min(lane, ncap_j - 1)];
+    si = gi[min(lane, ncap_i - 1)];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      uj[u] = L.cocc[jlr + min(u * 64 + lane, pj - 1)];
+      ui[u] = L.cvir[ilr + min(u * 64 + lane, pi - 1)];
+    }
+    // a lane past the end copies the last position again (the same value)
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      M.cj[min(u * 64 + lane, pj - 1)] = uj[u];
+      M.ci[min(u * 64 + lane, pi - 1)] = ui[u];
+    }
+  } else {
+    ncfj = L.ncf[j];
+    ncei = L.nce[i];
+  }
+  const int bj = min(4 * ncfj, jur - jlr), bi = min(4 * ncei, iur - ilr);
+  // atom lists (and orbital counts) into LDS when they fit (every list entry
+  // the rotation reads lies below ncfj + ncei)
+  const bool in_lds = ncfj + ncei <= P.lcap;
+  // every position the rotation can touch: below the LMO's storage and below
+  // 4 orbitals for each atom of the two LMOs
+  const int reach = 4 * (ncfj + ncei);
+  const bool fast = spec && in_lds && min(jur - jlr, reach) <= pj && min(iur - ilr, reach) <= pi;
+  if (!fast) {
+    // undo log of the coefficients the LMOs hold now
+    wave_copy(storej, L.cocc + jlr, bj, lane);
+    wave_copy(storei, L.cvir + ilr, bi, lane);
+  }
+  if (in_lds && ncfj > 0 && ncei > 0) {
+    // the first 64 entries from the speculative reads, the rest read now,
+    // branch-free (a lane past the end of a list repeats its last entry:
+    // the same value to the same place)
+    int kj = min(lane, ncfj - 1), ki = min(lane, ncei - 1);
+    int aj = (spec ? __shfl(sj, kj, 64) : gj[kj]) - 1;
+    int ai = (spec ? __shfl(si, ki, 64) : gi[ki]) - 1;
+    for (int k0 = 0;;) {
+      const int nj = L.iorbs[aj], ni = L.iorbs[ai];
+      M.lj[kj] = (unsigned short)aj;
+      M.oj[kj] = (unsigned char)nj;
+      M.li[ki] = (unsigned short)ai;
+      M.oi[ki] = (unsigned char)ni;
+      k0 += 64;
+      if (k0 >= max(ncfj, ncei)) break;
+      kj = min(k0 + lane, ncfj - 1);
+      ki = min(k0 + lane, ncei - 1);
+      aj = gj[kj] - 1;
+      ai = gi[ki] - 1;
+    }
+  } else if (in_lds) {
+    for (int k = lane; k < ncfj; k += 64) M.lj[k] = (unsigned short)(gj[k] - 1);
+    for (int k = lane; k < ncei; k += 64) M.li[k] = (unsigned short)(gi[k] - 1);
+    for (int k = lane; k < ncfj; k += 64) M.oj[k] = (unsigned char)L.iorbs[M.lj[k]];
+    for (int k = lane; k < ncei; k += 64) M.oi[k] = (unsigned char)L.iorbs[M.li[k]];
+  }
+  if (fast)
+    d2w_rotate<true>(L, R, P, M, storej, storei, sumb_out, nrej_out, lane, ncfj, ncei, bj, bi, in_lds);
+  else
+    d2w_rotate<false>(L, R, P, M, storej, storei, sumb_out, nrej_out, lane, ncfj, ncei, bj, bi, in_lds);
 }
 
 // Every level of a run in one workgroup: the waves take a level's rotations
