@@ -410,8 +410,9 @@ def run_helix(args):
             "algorithmic_bytes_per_launch": round(st["bytes"] / max(st["launches"], 1)),
             "rotations_per_scf": int(st["rotations"]), "levels_per_scf": int(st["levels"]),
             "rounds_per_scf": int(st["rounds"]), "ms_per_scf": round(st["ms"], 2),
-            "note": "a wave per rotation, atom maps and lists in LDS; each rotation is a short chain of "
-                    "dependent gathers, so the kernels are latency-bound far below the HBM roof"}
+            "note": "a wave per rotation, atom maps, lists and the two LMOs' coefficients in LDS; each "
+                    "rotation is a short chain of dependent memory round trips, so the kernels are "
+                    "latency-bound far below the HBM roof"}
     for f in forms.values():
         f.pop("_stats")
         f.pop("_escf")
