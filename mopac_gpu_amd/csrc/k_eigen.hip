@@ -210,6 +210,35 @@ __device__ __forceinline__ double round_update(double (*S)[LDSW], double (*Q)[LD
   return npq;
 }
 
+// round_update for the cross rounds, with the elements that stay with the
+// thread through them in registers: S[k1][k2] (sll) and Q's rows 2 k1,
+// 2 k1 + 1 of column k2 (ql0, ql1).  Only the moving row q1 and column q2
+// go through LDS; the arithmetic is round_update's, in the same order.
+__device__ __forceinline__ double round_update_x(double (*S)[LDSW], double (*Q)[LDSW], int k1, int q1,
+                                                 double c1, double s1, int p2, int q2, double c2, double s2,
+                                                 double& sll, double& ql0, double& ql1) {
+  const int p1 = k1;
+  const double xpp = sll, xpq = S[p1][q2], xqp = S[q1][p2], xqq = S[q1][q2];
+  const double rpp = c1 * xpp - s1 * xqp, rpq = c1 * xpq - s1 * xqq;
+  const double rqp = s1 * xpp + c1 * xqp, rqq = s1 * xpq + c1 * xqq;
+  const double npq = s2 * rpp + c2 * rpq;
+  sll = c2 * rpp - s2 * rpq;
+  S[p1][q2] = npq;
+  S[q1][p2] = c2 * rqp - s2 * rqq;
+  S[q1][q2] = s2 * rqp + c2 * rqq;
+  {
+    const double u = ql0, v = Q[2 * k1][q2];
+    ql0 = c2 * u - s2 * v;
+    Q[2 * k1][q2] = s2 * u + c2 * v;
+  }
+  {
+    const double u = ql1, v = Q[2 * k1 + 1][q2];
+    ql1 = c2 * u - s2 * v;
+    Q[2 * k1 + 1][q2] = s2 * u + c2 * v;
+  }
+  return npq;
+}
+
 // The pair solve with cross rounds.  An inner sweep is, when intra[pair] is
 // set, 31 rounds inside each 32-wide half (circle order on 32 indices, both
 // halves at once), then 32 cross rounds: round r pairs local index k with
@@ -320,11 +349,15 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_jac_solve_x(
       dgl[0][q] = aqq + t * apq;
     }
     __syncthreads();
+    // row k1 and column k2 of this thread's block are the same in every
+    // cross round: their shared S element and Q's two elements of column k2
+    // stay in registers until the rounds end
+    double sll = S[k1][k2], ql0 = Q[2 * k1][k2], ql1 = Q[2 * k1 + 1][k2];
     for (int r = 0; r < 32; ++r) {
       const int b = r & 1;
       const double2 r1 = rot[b][k1], r2 = rot[b][k2];
       const int q1 = 32 + ((k1 + r) & 31), q2 = 32 + ((k2 + r) & 31);
-      const double npq = round_update(S, Q, k1, k1, q1, r1.x, r1.y, k2, q2, r2.x, r2.y);
+      const double npq = round_update_x(S, Q, k1, q1, r1.x, r1.y, k2, q2, r2.x, r2.y, sll, ql0, ql1);
       if (r < 31 && tid < 32) {  // thread (k1, k1 + 1)
         // rotation k1 of round r + 1 pairs k1 with q2
         const double dp = dgl[b][k1], dq = dgl[b][q2];
@@ -336,6 +369,10 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_jac_solve_x(
       }
       __syncthreads();
     }
+    S[k1][k2] = sll;
+    Q[2 * k1][k2] = ql0;
+    Q[2 * k1 + 1][k2] = ql1;
+    __syncthreads();
   }
   double* Qo = Qbuf + (size_t)pidx * SB * SB;
   for (int e = tid; e < SB * SB; e += SOLVE_THREADS) {
