@@ -1709,8 +1709,25 @@ int g_mz_lcap = 1024;  // D2W_LCAP
 
 // Device copies and scratch of the MOZYME entry points, one set per device,
 // grown as needed and kept until the process ends: no allocation per call.
+// Grow-only pinned host memory, kept for the life of the process.
+struct HostPin {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void* ensure(size_t need) {
+    if (need > bytes) {
+      if (ptr) MHIP_CHECK(hipHostFree(ptr));
+      ptr = nullptr;
+      bytes = 0;
+      MHIP_CHECK(hipHostMalloc(&ptr, need));
+      bytes = need;
+    }
+    return ptr;
+  }
+};
+
 struct MzWork {
   LmoBufs B;
+  HostPin hrec, hsumb, hnrej;  // diagg2's rotation records and per-rotation results
   std::vector<int> nijbo_seen, iorbs_seen;  // the nijbo (and sizes) last checked and uploaded
   long long nijbo_mpack = -1;
   bool resident = false;                 // a resident MOZYME loop owns B
@@ -2233,12 +2250,13 @@ void mz_diagg2_dev(hipStream_t s, const mhip_lmo_set* L, const double* eigs, con
     const LmoBufs& B = W.B;
     DevBuf &deigs = W.deigs, &deigv = W.deigv, &dfmo = W.dfmo, &difmo = W.difmo, &dorder = W.dorder,
            &dused = W.dused, &dlat = W.dlat, &dsj = W.dsj, &dsi = W.dsi, &dsumb = W.dsumb, &dnrej = W.dnrej;
-    up(s, deigs, eigs, sizeof(double) * std::max(L->nocc, 1));
-    up(s, deigv, eigv, sizeof(double) * std::max(L->nvir, 1));
-    up(s, dfmo, fmo, sizeof(double) * std::max(nij, 1));
-    up(s, difmo, ifmo, sizeof(int) * 2 * std::max(nij, 1));
-    up(s, dorder, order.data(), sizeof(int) * order.size());
     if (!lds_ok) {
+      // the global-map kernel reads the list itself
+      up(s, deigs, eigs, sizeof(double) * std::max(L->nocc, 1));
+      up(s, deigv, eigv, sizeof(double) * std::max(L->nvir, 1));
+      up(s, dfmo, fmo, sizeof(double) * std::max(nij, 1));
+      up(s, difmo, ifmo, sizeof(int) * 2 * std::max(nij, 1));
+      up(s, dorder, order.data(), sizeof(int) * order.size());
       dused.ensure(sizeof(int) * (size_t)nslots * L->numat);
       MHIP_CHECK(hipMemsetAsync(dused.ptr, 0xff, sizeof(int) * (size_t)nslots * L->numat, s));  // -1
       dlat.ensure((size_t)nslots * L->numat);
@@ -2246,7 +2264,9 @@ void mz_diagg2_dev(hipStream_t s, const mhip_lmo_set* L, const double* eigs, con
     } else {
       // level l (0-based) holds the records [level[l], level[l + 1])
       up(s, W.dlevel, start.data() + 1, sizeof(int) * (size_t)(nlev + 1));
-      std::vector<D2Rot> recs(order.size());
+      // built straight into pinned memory (the previous call's copy out of it
+      // ended at that call's final synchronization)
+      D2Rot* recs = static_cast<D2Rot*>(W.hrec.ensure(sizeof(D2Rot) * std::max<size_t>(order.size(), 1)));
       for (size_t k = 0; k < order.size(); ++k) {
         const int ij = order[k], i = ifmo[2 * ij] - 1, j = ifmo[2 * ij + 1] - 1;
         D2Rot& r = recs[k];
@@ -2265,7 +2285,7 @@ void mz_diagg2_dev(hipStream_t s, const mhip_lmo_set* L, const double* eigs, con
         r.incv = (i + 1 < L->nvir) ? L->nnce[i + 1] : L->icvir_dim;
         r.pad = 0;
       }
-      up(s, W.drec, recs.data(), sizeof(D2Rot) * recs.size());
+      up(s, W.drec, recs, sizeof(D2Rot) * order.size());
     }
     dsj.ensure(sizeof(double) * (size_t)nslots * span_j);
     dsi.ensure(sizeof(double) * (size_t)nslots * span_i);
@@ -2349,11 +2369,11 @@ void mz_diagg2_dev(hipStream_t s, const mhip_lmo_set* L, const double* eigs, con
         }
       }
     if (g_mz_stats.on) MHIP_CHECK(hipEventRecord(g_mz_stats.e1, s));
-    std::vector<double> sb(std::max(nij, 1));
-    std::vector<int> nr(std::max(nij, 1));
+    const double* sb = static_cast<const double*>(W.hsumb.ensure(sizeof(double) * std::max(nij, 1)));
+    const int* nr = static_cast<const int*>(W.hnrej.ensure(sizeof(int) * std::max(nij, 1)));
     if (!taken.empty()) {
-      down(s, sb.data(), dsumb, sizeof(double) * nij);
-      down(s, nr.data(), dnrej, sizeof(int) * nij);
+      down(s, W.hsumb.ptr, dsumb, sizeof(double) * nij);
+      down(s, W.hnrej.ptr, dnrej, sizeof(int) * nij);
     }
     MHIP_CHECK(hipStreamSynchronize(s));
     double sumb = 0.0;
