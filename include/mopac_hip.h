@@ -335,6 +335,13 @@ int mhip_mozyme_diagg2_wide_rounds(int rounds);
  * (default and maximum 1024) and read them from global memory otherwise;
  * results are identical.  cap = -1 only reads; returns the previous value. */
 int mhip_mozyme_diagg2_list_cap(int cap);
+/* No counterpart in the reference.  The LDS rotation kernels copy the first
+ * cap coefficients of each of the two LMOs into LDS (default and maximum
+ * 1024) and, when every position a rotation can touch lies in those copies,
+ * rotate from them; otherwise they read the coefficients from global memory
+ * with an undo log.  Results are identical.  cap = -1 only reads; returns the
+ * previous value. */
+int mhip_mozyme_diagg2_coef_cap(int cap);
 
 /* helecz() (src/MOZYME/helecz.F90:16-52): E = sum over stored atom pairs of
  * P (H + F), diagonal elements weighted 1/2, on the sparse nijbo storage. */

@@ -684,6 +684,7 @@ struct D2Params {
   double tiny, biglim, cnst, shift, thresh;
   int retry, nij;
   int lcap;  // two LMOs' atoms at most this many: lists in LDS (<= D2W_LCAP)
+  int pcap;  // coefficients of each LMO copied to LDS (<= D2W_PCAP; 0: global memory)
 };
 
 constexpr int D2_NT = 64;  // one wave: the per-rotation scans run at wave latency
@@ -1331,7 +1332,7 @@ __device__ void d2w_rotation(const LmoDev& L, const D2Rot& R, const D2Params& P,
   const int* __restrict__ gj = L.icocc + nnj;
   const int* __restrict__ gi = L.icvir + nni;
   const int ncap_j = jncf - nnj, ncap_i = incv - nni;
-  const int pj = min(jur - jlr, D2W_PCAP), pi = min(iur - ilr, D2W_PCAP);
+  const int pj = min(jur - jlr, P.pcap), pi = min(iur - ilr, P.pcap);
   const bool spec = ncap_j > 0 && ncap_i > 0 && pj > 0 && pi > 0;
   int ncfj, ncei, sj = 0, si = 0;
   if (spec) {
@@ -1706,6 +1707,7 @@ MzStats g_mz_stats;
 bool g_mz_one_wg = true;
 int g_mz_wide_rounds = 1;
 int g_mz_lcap = 1024;  // D2W_LCAP
+int g_mz_pcap = 1024;  // D2W_PCAP
 
 // Device copies and scratch of the MOZYME entry points, one set per device,
 // grown as needed and kept until the process ends: no allocation per call.
@@ -2154,6 +2156,7 @@ void mz_diagg2_dev(hipStream_t s, const mhip_lmo_set* L, const double* eigs, con
     P.retry = retry ? 1 : 0;
     P.nij = nij;
     P.lcap = std::min(g_mz_lcap, D2W_LCAP);
+    P.pcap = std::min(g_mz_pcap, D2W_PCAP);
     // the rotations diagg2.F90 takes (same tests, same arithmetic as the
     // reference's loop), each at one level past the last earlier rotation
     // on its virtual or its occupied LMO
@@ -2705,6 +2708,12 @@ int mhip_mozyme_diagg2_wide_rounds(int rounds) {
 int mhip_mozyme_diagg2_list_cap(int cap) {
   const int was = g_mz_lcap;
   if (cap >= 0) g_mz_lcap = std::min(cap, D2W_LCAP);
+  return was;
+}
+
+int mhip_mozyme_diagg2_coef_cap(int cap) {
+  const int was = g_mz_pcap;
+  if (cap >= 0) g_mz_pcap = std::min(cap, D2W_PCAP);
   return was;
 }
 

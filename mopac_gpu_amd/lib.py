@@ -130,6 +130,7 @@ SIGNATURES = {
     "mhip_mozyme_eimp": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p]),
     "mhip_mozyme_diagg2_wide_rounds": (c_int, [c_int]),
     "mhip_mozyme_diagg2_list_cap": (c_int, [c_int]),
+    "mhip_mozyme_diagg2_coef_cap": (c_int, [c_int]),
     "mhip_last_sweeps": (c_int, [c_void_p]),
     "mhip_mozyme_density": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "mhip_mozyme_diagg1": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,

@@ -72,23 +72,27 @@ def lewis_helix_lmos():
     return L
 
 
-@pytest.fixture(params=[(1, 1, 1024), (1, 1 << 30, 1024), (1, 0, 1024), (1, 1, 0), (0, 1, 1024)],
+@pytest.fixture(params=[(1, 1, 1024, 1024), (1, 1 << 30, 1024, 1024), (1, 0, 1024, 1024), (1, 1, 0, 1024),
+                        (1, 1, 1024, 0), (0, 1, 1024, 1024)],
                 ids=["lds_hybrid", "lds_one_workgroup", "lds_across_chip", "lds_lists_in_global",
-                     "global_per_level"])
+                     "lds_coefficients_in_global", "global_per_level"])
 def diagg2_path(request):
     """Every way of running diagg2's levels: the LDS kernels with the default
     split, with every level in the one workgroup, with every level across the
-    chip, with the atom lists read from global memory, and the global-map
-    kernel one launch per level."""
+    chip, with the atom lists read from global memory, with the coefficients
+    read from global memory (the lists in LDS, an undo log for rejections),
+    and the global-map kernel one launch per level."""
     from mopac_gpu_amd import lib as L_
     lib = L_.load()
     was = lib.mhip_mozyme_diagg2_one_workgroup(request.param[0])
     was_w = lib.mhip_mozyme_diagg2_wide_rounds(request.param[1])
     was_c = lib.mhip_mozyme_diagg2_list_cap(request.param[2])
+    was_p = lib.mhip_mozyme_diagg2_coef_cap(request.param[3])
     yield request.param
     lib.mhip_mozyme_diagg2_one_workgroup(was)
     lib.mhip_mozyme_diagg2_wide_rounds(was_w)
     lib.mhip_mozyme_diagg2_list_cap(was_c)
+    lib.mhip_mozyme_diagg2_coef_cap(was_p)
 
 
 def test_diagg_sequence_on_lewis_lmos_matches_reference(gpu_ctx, lewis_helix_lmos, diagg2_path):
