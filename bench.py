@@ -266,8 +266,8 @@ def main():
     if rank == 0 and not args.no_cpu and not args.fast_only:
         from oracle import oracle
         olib = oracle.load()
-        # every core this process may run on (its CPU affinity: on the GPU box
-        # the cores granted to the job), unless the thread count is set
+        # every core this process may run on (its CPU affinity: on a shared
+        # host the cores granted to the job), unless the thread count is set
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
         olib.o_set_threads(threads)
         t0 = time.perf_counter()
