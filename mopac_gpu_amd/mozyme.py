@@ -526,32 +526,35 @@ def lewis_lmos(ms, fock=None, pad_atoms=6, scale=1.2):
             return fock[o:o + iorbs[a] * iorbs[b]].reshape(iorbs[a], iorbs[b])
         return block(b, a).T
 
-    def bond(a, ha, b, hb):
-        """(bonding, antibonding) as [(atom, coefficient vector), ...].  The
-        bonding pair (c1, c2) is the lower eigenvector of the 2 x 2 Fock
-        matrix of the two hybrids, in closed form, signed so that c1 + c2 > 0
-        (c1 >= 0 when they cancel)."""
-        if fock is None:
-            c1 = c2 = np.sqrt(0.5)
-        else:
-            faa = ha @ block(a, a) @ ha
-            fbb = hb @ block(b, b) @ hb
-            fab = ha @ block(a, b) @ hb
-            # the rotation by t = atan2(2 fab, faa - fbb) / 2 diagonalizes the
-            # matrix with the upper eigenvalue on (cos t, sin t); the lower
-            # one is on (-sin t, cos t)
-            t = 0.5 * np.arctan2(2.0 * fab, faa - fbb)
-            u = np.array([-np.sin(t), np.cos(t)])
-            if u[0] + u[1] < 0 or (u[0] + u[1] == 0 and u[0] < 0):
-                u = -u
-            c1, c2 = float(u[0]), float(u[1])
-        return [(a, c1 * ha), (b, c2 * hb)], [(a, -c2 * ha), (b, c1 * hb)]
+    def polarize(reqs):
+        """(c1, c2) of every bond: the lower eigenvector of the 2 x 2 Fock
+        matrix of its two hybrids, [[ha F_aa ha, ha F_ab hb], [., hb F_bb hb]],
+        in closed form (the rotation by t = atan2(2 fab, faa - fbb) / 2 puts
+        the upper eigenvalue on (cos t, sin t), the lower on (-sin t, cos t)),
+        signed so that c1 + c2 > 0 (c1 >= 0 when they cancel); equal weights
+        without a Fock matrix.  Bonds with the same orbital counts are
+        evaluated together."""
+        m = len(reqs)
+        if fock is None or m == 0:
+            return np.full(m, np.sqrt(0.5)), np.full(m, np.sqrt(0.5))
+        faa, fbb, fab = np.empty(m), np.empty(m), np.empty(m)
+        groups = {}
+        for k, (a, ha, b, hb) in enumerate(reqs):
+            groups.setdefault((len(ha), len(hb)), []).append(k)
+        for (na, nb), ks in groups.items():
+            ha = np.array([reqs[k][1] for k in ks])
+            hb = np.array([reqs[k][3] for k in ks])
+            faa[ks] = np.einsum('ki,kij,kj->k', ha, np.array([block(reqs[k][0], reqs[k][0]) for k in ks]), ha)
+            fbb[ks] = np.einsum('ki,kij,kj->k', hb, np.array([block(reqs[k][2], reqs[k][2]) for k in ks]), hb)
+            fab[ks] = np.einsum('ki,kij,kj->k', ha, np.array([block(reqs[k][0], reqs[k][2]) for k in ks]), hb)
+        t = 0.5 * np.arctan2(2.0 * fab, faa - fbb)
+        u0, u1 = -np.sin(t), np.cos(t)
+        flip = (u0 + u1 < 0) | ((u0 + u1 == 0) & (u0 < 0))
+        return np.where(flip, -u0, u0), np.where(flip, -u1, u1)
 
-    occ, vir = [], []
-    for a, b in pairs:
-        o, v = bond(int(a), hyb[a][int(b)], int(b), hyb[b][int(a)])
-        occ.append(o)
-        vir.append(v)
+    # the bonds (sigma here, pi below) as (a, hybrid of a, b, hybrid of b),
+    # polarized together once all are known
+    reqs = [(int(a), hyb[a][int(b)], int(b), hyb[b][int(a)]) for a, b in pairs]
     # pi bonds between bonded atoms that each keep an odd electron
     for a in range(numat):
         if elec[a] % 2 == 0:
@@ -567,9 +570,11 @@ def lewis_lmos(ms, fock=None, pad_atoms=6, scale=1.2):
         rest[b] = rest[b] @ vt[1:].T
         elec[a] -= 1
         elec[b] -= 1
-        o, v = bond(a, pa, b, pb)
-        occ.append(o)
-        vir.append(v)
+        reqs.append((a, pa, b, pb))
+    occ, vir = [], []
+    for (a, ha, b, hb), c1, c2 in zip(reqs, *polarize(reqs)):
+        occ.append([(a, c1 * ha), (b, c2 * hb)])
+        vir.append([(a, -c2 * ha), (b, c1 * hb)])
     for a in range(numat):
         n = rest[a].shape[1]
         if elec[a] != 2 * n:
